@@ -1,0 +1,341 @@
+"""aloha_amd -- MI355X-native per-brick render-and-composite path of the Aloha
+sort-last renderer (vtvamr/aloha).
+
+The compute lives in libaloha_amd.so (hand-written HIP for gfx950 behind the C
+ABI of include/aloha_amd.h).  This module is a thin Python view of that ABI
+for tests, the benchmark and the autograd wrapper: host numpy arrays in,
+device torch tensors for images.  It has no compute path of its own.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from ._lib import (F32, FLAG_CROSSHAIR, FLAG_HEATMAP, JITTER_FIXED, JITTER_RANDOM, MAX_RANKS,
+                   MAX_RANK_BRICKS, MAX_TF, U8, U16, AlohaError, Box3f, Box3i, BrickDesc, Images,
+                   KDNode, Range1f, RenderParams, check, lib)
+
+__all__ = [
+    "AlohaError", "BrickDesc", "Box3i", "Box3f", "RenderParams", "Context", "Partitioner",
+    "split_volume", "volume_extract", "synth_blobs_u8", "synth_blobs_f32", "synth_fbm_f32",
+    "synth_tf", "synth_camera", "max_depth", "U8", "U16", "F32", "JITTER_RANDOM",
+    "JITTER_FIXED", "FLAG_CROSSHAIR", "FLAG_HEATMAP", "MAX_RANKS", "MAX_RANK_BRICKS", "MAX_TF",
+    "vol_write", "vol_read",
+]
+
+
+def _i3(v) -> C.Array:
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+def _f3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+# ----------------------------------------------------------------------------
+# host-side bricking and partitioning
+# ----------------------------------------------------------------------------
+def split_volume(dims: Sequence[int], num_bricks: int) -> list[BrickDesc]:
+    """chopSuey VolumeSplitter (chopSuey.cpp:245-323)."""
+    cap = max(4 * num_bricks + 8, 16)
+    out = (BrickDesc * cap)()
+    n = C.c_int32()
+    check(lib().aloha_split_volume(_i3(dims), int(num_bricks), out, cap, C.byref(n)))
+    return [out[i] for i in range(n.value)]
+
+
+def volume_extract(raw: np.ndarray, bpc: int, voxel_range: Box3i):
+    """Volume::get (volume.h:46-98): raw is (z, y, x) u8/u16/f32; returns
+    (float32 array (z,y,x) of the sub-box, (lo, hi))."""
+    raw = np.ascontiguousarray(raw)
+    dims = raw.shape[::-1]
+    ext = [voxel_range.upper[a] - voxel_range.lower[a] for a in range(3)]
+    out = np.empty(ext[::-1], dtype=np.float32)
+    vr = Range1f(np.finfo(np.float32).max, -np.finfo(np.float32).max)
+    check(lib().aloha_volume_extract(raw.ctypes.data, _i3(dims), int(bpc), C.byref(voxel_range),
+                                     out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(vr)))
+    return out, (vr.lower, vr.upper)
+
+
+class Partitioner:
+    """Partitioner (waikikiRenderer.cpp:126-338)."""
+
+    def __init__(self, handle: C.c_void_p, num_ranks: int):
+        self._h = handle
+        self.num_ranks = num_ranks
+
+    @classmethod
+    def kd(cls, domains: Sequence[Box3f], num_ranks: int) -> "Partitioner":
+        arr = (Box3f * max(1, len(domains)))(*domains)
+        h = C.c_void_p()
+        check(lib().aloha_partitioner_create_kd(arr, len(domains), int(num_ranks), C.byref(h)))
+        return cls(h, num_ranks)
+
+    @classmethod
+    def round_robin(cls, n: int, num_ranks: int) -> "Partitioner":
+        h = C.c_void_p()
+        check(lib().aloha_partitioner_create_rr(int(n), int(num_ranks), C.byref(h)))
+        return cls(h, num_ranks)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().aloha_partitioner_destroy(self._h)
+            self._h = None
+
+    def rank_bricks(self, rank: int) -> list[int]:
+        cnt = C.c_int32()
+        cap = 4096
+        ids = (C.c_int32 * cap)()
+        check(lib().aloha_partitioner_rank_bricks(self._h, int(rank), ids, cap, C.byref(cnt)))
+        return list(ids[: cnt.value])
+
+    def composite_order(self, reference: Sequence[float]) -> list[int]:
+        cnt = C.c_int32()
+        cap = 4096
+        out = (C.c_int32 * cap)()
+        check(lib().aloha_partitioner_composite_order(self._h, _f3(reference), out, cap,
+                                                      C.byref(cnt)))
+        return list(out[: cnt.value])
+
+    def kd_nodes(self) -> list[tuple[int, float, int, int]]:
+        cnt = C.c_int32()
+        cap = 8192
+        out = (KDNode * cap)()
+        check(lib().aloha_partitioner_kd_nodes(self._h, out, cap, C.byref(cnt)))
+        return [(n.axis, n.plane, n.child1, n.child2) for n in out[: cnt.value]]
+
+
+# ----------------------------------------------------------------------------
+# .vol files
+# ----------------------------------------------------------------------------
+def vol_write(path: str, cell_range: Box3i, voxel_range: Box3i, space_range: Box3f,
+              bricks: Sequence[BrickDesc], voxels: Sequence[np.ndarray]) -> None:
+    arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in voxels]
+    ptrs = (C.POINTER(C.c_float) * max(1, len(arrs)))(
+        *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in arrs])
+    barr = (BrickDesc * max(1, len(bricks)))(*bricks)
+    check(lib().aloha_vol_write(path.encode(), C.byref(cell_range), C.byref(voxel_range),
+                                C.byref(space_range), len(bricks), barr, ptrs))
+
+
+def vol_read(path: str):
+    """Returns (cell_range, voxel_range, space_range, [BrickDesc], [voxels (z,y,x)])."""
+    cr, vr, sr = Box3i(), Box3i(), Box3f()
+    n = C.c_int32()
+    cap = 65536
+    descs = (BrickDesc * cap)()
+    check(lib().aloha_vol_read_header(path.encode(), C.byref(cr), C.byref(vr), C.byref(sr), descs,
+                                      cap, C.byref(n)))
+    bricks, vox = [], []
+    for i in range(n.value):
+        d = descs[i]
+        dims = d.voxel_dims()
+        a = np.empty(dims[::-1], dtype=np.float32)
+        check(lib().aloha_vol_read_brick(path.encode(), i,
+                                          a.ctypes.data_as(C.POINTER(C.c_float)), a.size))
+        bricks.append(d)
+        vox.append(a)
+    return cr, vr, sr, bricks, vox
+
+
+# ----------------------------------------------------------------------------
+# synthetic inputs
+# ----------------------------------------------------------------------------
+def synth_blobs_u8(dims, num_blobs=16, seed=0x5EED0, smin=4.0, smax=16.0, amin=0.3, amax=1.0):
+    out = np.empty(tuple(dims)[::-1], dtype=np.uint8)
+    check(lib().aloha_synth_blobs_u8(_i3(dims), num_blobs, seed, smin, smax, amin, amax,
+                                     out.ctypes.data))
+    return out
+
+
+def synth_blobs_f32(dims, num_blobs=16, seed=0x5EED0, smin=4.0, smax=16.0, amin=0.3, amax=1.0):
+    out = np.empty(tuple(dims)[::-1], dtype=np.float32)
+    check(lib().aloha_synth_blobs_f32(_i3(dims), num_blobs, seed, smin, smax, amin, amax,
+                                      out.ctypes.data))
+    return out
+
+
+def synth_fbm_f32(dims, octaves=5, base_period=64.0, seed=0x5EED1):
+    out = np.empty(tuple(dims)[::-1], dtype=np.float32)
+    check(lib().aloha_synth_fbm_f32(_i3(dims), octaves, base_period, seed, out.ctypes.data))
+    return out
+
+
+def synth_tf(n=256, num_points=6, seed=0x5EED2) -> np.ndarray:
+    out = np.empty((n, 4), dtype=np.float32)
+    check(lib().aloha_synth_tf(n, num_points, seed, out.ctypes.data))
+    return out
+
+
+def synth_camera(bounds: Box3f, W: int, H: int, fovy_deg=45.0, dist_factor=2.5, seed=0x5EED3):
+    out = (C.c_float * 12)()
+    check(lib().aloha_synth_camera(C.byref(bounds), W, H, fovy_deg, dist_factor, seed, out))
+    v = np.frombuffer(out, dtype=np.float32).copy()
+    return {"org": v[0:3], "dir_00": v[3:6], "dir_du": v[6:9], "dir_dv": v[9:12]}
+
+
+def max_depth(bounds: Box3f, org) -> float:
+    """handleCameraEvent's maxDepth (waikikiRenderer.cpp:858-860), in float32:
+    length(org + center) + length(center + upper)."""
+    f = np.float32
+    lo = np.array(bounds.lower, dtype=f)
+    hi = np.array(bounds.upper, dtype=f)
+    center = (lo + hi) * f(0.5)
+
+    def length(v):
+        return np.sqrt(f(v[0] * v[0] + v[1] * v[1]) + f(v[2] * v[2]), dtype=f)
+
+    radius = length((center + hi).astype(f))
+    return float(f(length((np.asarray(org, dtype=f) + center).astype(f)) + radius))
+
+
+# ----------------------------------------------------------------------------
+# device context
+# ----------------------------------------------------------------------------
+@dataclass
+class RankImages:
+    """Device tensors of one render launch (P rank slots); any may be None."""
+    rgba: object = None
+    rgba8: object = None
+    depth: object = None
+    accum: object = None
+    nsamples: object = None
+
+    def as_c(self) -> Images:
+        def p(t):
+            return C.c_void_p(t.data_ptr()) if t is not None else None
+        return Images(p(self.rgba), p(self.rgba8), p(self.depth), p(self.accum), p(self.nsamples))
+
+    @classmethod
+    def alloc(cls, P: int, W: int, H: int, device="cuda", rgba=True, rgba8=True, depth=True,
+              accum=False, nsamples=False):
+        import torch
+        dev = torch.device(device)
+        return cls(
+            rgba=torch.empty((P, H, W, 4), dtype=torch.float32, device=dev) if rgba else None,
+            rgba8=torch.empty((P, H, W), dtype=torch.int32, device=dev) if rgba8 else None,
+            depth=torch.empty((P, H, W), dtype=torch.float32, device=dev) if depth else None,
+            accum=torch.zeros((P, H, W, 4), dtype=torch.float32, device=dev) if accum else None,
+            nsamples=torch.empty((P, H, W), dtype=torch.int32, device=dev) if nsamples else None,
+        )
+
+
+def render_params(dt=0.5, max_depth=1.0, spp=1, accum_id=0, island_id=0, num_islands=1,
+                  jitter_mode=JITTER_RANDOM, jitter_value=0.0, flags=0, heat_map_scale=1e-5):
+    return RenderParams(dt, max_depth, spp, accum_id, island_id, num_islands, jitter_mode,
+                        jitter_value, flags, heat_map_scale)
+
+
+def _rank_table(ranks: Sequence[Sequence[int]]):
+    offs = [0]
+    ids: list[int] = []
+    for r in ranks:
+        ids.extend(int(b) for b in r)
+        offs.append(len(ids))
+    return (C.c_int32 * len(offs))(*offs), (C.c_int32 * max(1, len(ids)))(*ids)
+
+
+class Context:
+    """Device context (bricks in HBM, TF, camera) on one GPU."""
+
+    def __init__(self, device: int = 0, stream=None):
+        h = C.c_void_p()
+        check(lib().aloha_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+        if stream is not None:
+            self.set_stream(stream)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().aloha_ctx_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        check(rc, self._h)
+
+    def set_stream(self, stream) -> None:
+        """stream: a torch.cuda.Stream or a raw hipStream_t int (0: the null stream)."""
+        handle = int(getattr(stream, "cuda_stream", stream) or 0)
+        self._check(lib().aloha_set_stream(self._h, C.c_void_p(handle) if handle else None))
+
+    def sync(self) -> None:
+        self._check(lib().aloha_sync(self._h))
+
+    def upload_brick(self, desc: BrickDesc, voxels, dtype: int | None = None) -> int:
+        """voxels: numpy (z,y,x) u8/u16/f32 on the host, or a cuda torch tensor."""
+        bid = C.c_int32()
+        if hasattr(voxels, "data_ptr") and getattr(voxels, "is_cuda", False):
+            import torch
+            t = voxels.contiguous()
+            code = {torch.uint8: U8, torch.int16: U16, torch.float32: F32}[t.dtype]
+            self._check(lib().aloha_brick_upload(self._h, C.byref(desc), C.c_void_p(t.data_ptr()),
+                                                 dtype or code, 1, C.byref(bid)))
+            return bid.value
+        a = np.ascontiguousarray(voxels)
+        code = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
+                np.dtype(np.float32): F32}[a.dtype]
+        self._check(lib().aloha_brick_upload(self._h, C.byref(desc), a.ctypes.data, dtype or code,
+                                             0, C.byref(bid)))
+        return bid.value
+
+    def brick_bytes(self) -> int:
+        v = C.c_uint64()
+        self._check(lib().aloha_brick_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def set_tf(self, tf: np.ndarray) -> None:
+        tf = np.ascontiguousarray(tf, dtype=np.float32)
+        self._check(lib().aloha_tf_set(self._h, tf.ctypes.data, tf.shape[0]))
+
+    def set_camera(self, org, dir_00, dir_du, dir_dv) -> None:
+        self._check(lib().aloha_camera_set(self._h, _f3(org), _f3(dir_00), _f3(dir_du),
+                                           _f3(dir_dv)))
+
+    def render_ranks(self, params: RenderParams, W: int, H: int,
+                     ranks: Sequence[Sequence[int]], out: RankImages) -> None:
+        offs, ids = _rank_table(ranks)
+        imgs = out.as_c()
+        self._check(lib().aloha_render_ranks(self._h, C.byref(params), W, H, len(ranks), offs, ids,
+                                             C.byref(imgs)))
+
+    def footprint(self, params: RenderParams, W: int, H: int,
+                  ranks: Sequence[Sequence[int]]) -> int:
+        offs, ids = _rank_table(ranks)
+        v = C.c_uint64()
+        self._check(lib().aloha_render_footprint(self._h, C.byref(params), W, H, len(ranks), offs,
+                                                 ids, C.byref(v)))
+        return v.value
+
+    def composite_over(self, rank_rgba, order: Sequence[int], W: int, H: int, out_rgba=None,
+                       out_rgba8=None) -> None:
+        o = (C.c_int32 * len(order))(*order)
+        self._check(lib().aloha_composite_over(
+            self._h, C.c_void_p(rank_rgba.data_ptr()), len(order), o, W, H,
+            C.c_void_p(out_rgba.data_ptr()) if out_rgba is not None else None,
+            C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None))
+
+    def composite_over_rgba8(self, rank_rgba8, order: Sequence[int], W: int, H: int,
+                             out_rgba8) -> None:
+        o = (C.c_int32 * len(order))(*order)
+        self._check(lib().aloha_composite_over_rgba8(
+            self._h, C.c_void_p(rank_rgba8.data_ptr()), len(order), o, W, H,
+            C.c_void_p(out_rgba8.data_ptr())))
+
+    def composite_zmin(self, rank_rgba8, rank_depth, P: int, W: int, H: int, out_rgba8=None,
+                       out_depth=None) -> None:
+        self._check(lib().aloha_composite_zmin(
+            self._h, C.c_void_p(rank_rgba8.data_ptr()), C.c_void_p(rank_depth.data_ptr()), P, W, H,
+            C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None,
+            C.c_void_p(out_depth.data_ptr()) if out_depth is not None else None))

@@ -1,0 +1,435 @@
+// context.hip -- the device context behind the C ABI: brick storage in HBM,
+// transfer function, camera, and the render / composite entry points.
+// Replaces the OWL context, launch params and CUDA texture objects that
+// Renderer builds (waikikiRenderer.cpp:377-384, :697-797, :853-1013) and the
+// IceT composite it runs per frame (:1030-1079).
+//
+// Every entry point validates its arguments on the host and returns an
+// ALOHA_ERR_* code with a message instead of launching anything it cannot
+// prove in bounds.  Render and composite calls only enqueue kernels on the
+// context's stream (no allocation, no synchronisation), so a caller may
+// capture them in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_types.h"
+#include "internal.h"
+
+using namespace aloha;
+
+struct aloha_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<DevBrick> bricks;
+  std::vector<float *> brick_mem;
+  std::vector<aloha_brick_desc> descs;
+  DevBrick *d_bricks = nullptr;
+  size_t d_bricks_cap = 0;
+  bool bricks_dirty = true;
+  uint64_t brick_bytes = 0;
+  float4 *d_tf = nullptr;
+  int tf_n = 0, tf_cap = 0;
+  float cam[12] = {};
+  bool has_cam = false;
+  std::string err;
+};
+
+namespace {
+
+int fail(aloha_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  global_error() = buf;
+  return code;
+}
+
+#define HIP_CHECK(ctx, call)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ctx, ALOHA_ERR_HIP, "%s: %s (%s)", __func__, #call, hipGetErrorString(e_)); \
+  } while (0)
+
+bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int sync_bricks(aloha_ctx *ctx) {
+  if (!ctx->bricks_dirty) return ALOHA_OK;
+  if (ctx->bricks.size() > ctx->d_bricks_cap) {
+    if (ctx->d_bricks) HIP_CHECK(ctx, hipFree(ctx->d_bricks));
+    ctx->d_bricks = nullptr;
+    const size_t cap = std::max<size_t>(16, 2 * ctx->bricks.size());
+    HIP_CHECK(ctx, hipMalloc(&ctx->d_bricks, cap * sizeof(DevBrick)));
+    ctx->d_bricks_cap = cap;
+  }
+  if (!ctx->bricks.empty())
+    HIP_CHECK(ctx, hipMemcpy(ctx->d_bricks, ctx->bricks.data(),
+                             ctx->bricks.size() * sizeof(DevBrick), hipMemcpyHostToDevice));
+  ctx->bricks_dirty = false;
+  return ALOHA_OK;
+}
+
+int check_params(aloha_ctx *ctx, const aloha_render_params *p) {
+  if (!p) return fail(ctx, ALOHA_ERR_ARG, "render: params is NULL");
+  if (!(p->dt > 0.f) || !std::isfinite(p->dt))
+    return fail(ctx, ALOHA_ERR_ARG, "render: dt must be finite and > 0 (got %g)", p->dt);
+  if (!std::isfinite(p->max_depth) || p->max_depth == 0.f)
+    return fail(ctx, ALOHA_ERR_ARG, "render: max_depth must be finite and non-zero");
+  if (p->spp < 1) return fail(ctx, ALOHA_ERR_ARG, "render: spp must be >= 1");
+  if (p->accum_id < 0) return fail(ctx, ALOHA_ERR_ARG, "render: accum_id must be >= 0");
+  if (p->num_islands < 1 || p->island_id < 0 || p->island_id >= p->num_islands)
+    return fail(ctx, ALOHA_ERR_ARG, "render: island %d of %d invalid", p->island_id,
+                p->num_islands);
+  if (p->jitter_mode != ALOHA_JITTER_RANDOM && p->jitter_mode != ALOHA_JITTER_FIXED)
+    return fail(ctx, ALOHA_ERR_ARG, "render: unknown jitter_mode %d", p->jitter_mode);
+  if (p->jitter_mode == ALOHA_JITTER_FIXED && !(p->jitter_value >= 0.f && p->jitter_value < 1.f))
+    return fail(ctx, ALOHA_ERR_ARG, "render: jitter_value must be in [0,1)");
+  return ALOHA_OK;
+}
+
+// Fill MarchArgs from the context and a validated rank table.
+int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t H, int32_t P,
+                const int32_t *rank_offsets, const int32_t *brick_ids, MarchArgs &a) {
+  if (int rc = check_params(ctx, p)) return rc;
+  if (W < 1 || H < 1 || (int64_t)W * H > (int64_t)1 << 30)
+    return fail(ctx, ALOHA_ERR_ARG, "render: image %dx%d out of range", W, H);
+  if (P < 1 || P > ALOHA_MAX_RANKS)
+    return fail(ctx, ALOHA_ERR_LIMIT, "render: P=%d outside [1,%d]", P, ALOHA_MAX_RANKS);
+  if (!rank_offsets || rank_offsets[0] != 0)
+    return fail(ctx, ALOHA_ERR_ARG, "render: rank_offsets must start at 0");
+  for (int s = 0; s < P; ++s)
+    if (rank_offsets[s + 1] < rank_offsets[s])
+      return fail(ctx, ALOHA_ERR_ARG, "render: rank_offsets not monotone at %d", s);
+  const int nb = rank_offsets[P];
+  if (nb > ALOHA_MAX_RANK_BRICKS)
+    return fail(ctx, ALOHA_ERR_LIMIT, "render: %d brick slots exceed %d", nb,
+                ALOHA_MAX_RANK_BRICKS);
+  if (nb > 0 && !brick_ids) return fail(ctx, ALOHA_ERR_ARG, "render: brick_ids is NULL");
+  for (int k = 0; k < nb; ++k)
+    if (brick_ids[k] < 0 || brick_ids[k] >= (int)ctx->bricks.size())
+      return fail(ctx, ALOHA_ERR_ARG, "render: brick id %d not uploaded", brick_ids[k]);
+  if (!ctx->d_tf || ctx->tf_n < 1) return fail(ctx, ALOHA_ERR_STATE, "render: no transfer function set");
+  if (!ctx->has_cam) return fail(ctx, ALOHA_ERR_STATE, "render: no camera set");
+  if (int rc = sync_bricks(ctx)) return rc;
+  std::memset(&a, 0, sizeof a);
+  std::memcpy(a.org, ctx->cam, sizeof a.org);
+  std::memcpy(a.dir00, ctx->cam + 3, sizeof a.dir00);
+  std::memcpy(a.du, ctx->cam + 6, sizeof a.du);
+  std::memcpy(a.dv, ctx->cam + 9, sizeof a.dv);
+  a.dt = p->dt;
+  a.max_depth = p->max_depth;
+  a.jitter_value = p->jitter_value;
+  a.heat_scale = p->heat_map_scale;
+  a.W = W;
+  a.H = H;
+  a.spp = p->spp;
+  a.accum_id = p->accum_id;
+  a.island_id = p->island_id;
+  a.num_islands = p->num_islands;
+  a.jitter_mode = p->jitter_mode;
+  a.flags = p->flags;
+  a.P = P;
+  a.tf_n = ctx->tf_n;
+  a.bricks = ctx->d_bricks;
+  a.tf = ctx->d_tf;
+  for (int s = 0; s <= P; ++s) a.rank_off[s] = rank_offsets[s];
+  for (int k = 0; k < nb; ++k) a.rank_bricks[k] = brick_ids[k];
+  return ALOHA_OK;
+}
+
+}  // namespace
+
+extern "C" const char *aloha_last_error(const aloha_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : global_error().c_str();
+}
+
+extern "C" int aloha_ctx_create(int32_t device, aloha_ctx **out) {
+  if (!out) return fail(nullptr, ALOHA_ERR_ARG, "aloha_ctx_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1)
+    return fail(nullptr, ALOHA_ERR_HIP, "aloha_ctx_create: no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n)
+    return fail(nullptr, ALOHA_ERR_ARG, "aloha_ctx_create: device %d of %d", device, n);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(nullptr, ALOHA_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  auto *ctx = new aloha_ctx;
+  ctx->device = device;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, ALOHA_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  ctx->own_stream = true;
+  *out = ctx;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
+  if (!ctx) return ALOHA_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (float *m : ctx->brick_mem) hipFree(m);
+  if (ctx->d_bricks) hipFree(ctx->d_bricks);
+  if (ctx->d_tf) hipFree(ctx->d_tf);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_set_stream(aloha_ctx *ctx, void *hip_stream) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_set_stream: ctx is NULL");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (ctx->own_stream) {
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_CHECK(ctx, hipStreamDestroy(ctx->stream));
+    ctx->own_stream = false;
+    ctx->stream = nullptr;
+  }
+  // NULL is the device's default (null) stream, as hipStream_t 0 is in HIP
+  ctx->stream = (hipStream_t)hip_stream;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_sync(aloha_ctx *ctx) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_sync: ctx is NULL");
+  HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
+                                  const void *voxels, int32_t dtype, int32_t src_on_device,
+                                  int32_t *brick_id) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_brick_upload: ctx is NULL");
+  if (!desc || !voxels) return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_upload: NULL desc/voxels");
+  if (dtype != ALOHA_U8 && dtype != ALOHA_U16 && dtype != ALOHA_F32)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_upload: unknown dtype %d", dtype);
+  int32_t n[3];
+  for (int a = 0; a < 3; ++a) {
+    n[a] = desc->voxel_range.upper[a] - desc->voxel_range.lower[a];
+    if (n[a] < 1)
+      return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_upload: empty voxel range on axis %d", a);
+    if (!(desc->space_range.lower[a] <= desc->space_range.upper[a]))
+      return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_upload: inverted space range on axis %d", a);
+  }
+  if (!finite3(desc->space_range.lower) || !finite3(desc->space_range.upper))
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_upload: non-finite space range");
+  const int64_t px = (int64_t)n[0] + kPadLo + kPadHi;
+  const int64_t py = (int64_t)n[1] + kPadLo + kPadHi;
+  const int64_t pz = (int64_t)n[2] + kPadLo + kPadHi;
+  const int64_t padded = px * py * pz;
+  if (padded >= ((int64_t)1 << 31))
+    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_brick_upload: brick of %lld padded voxels exceeds 2^31",
+                (long long)padded);
+  const size_t src_bytes = (size_t)n[0] * n[1] * n[2] * (size_t)dtype;
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  float *mem = nullptr;
+  HIP_CHECK(ctx, hipMalloc(&mem, (size_t)padded * sizeof(float)));
+  const void *src = voxels;
+  void *staging = nullptr;
+  if (!src_on_device) {
+    hipError_t e = hipMalloc(&staging, src_bytes);
+    if (e == hipSuccess) e = hipMemcpy(staging, voxels, src_bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(mem);
+      if (staging) hipFree(staging);
+      return fail(ctx, ALOHA_ERR_HIP, "aloha_brick_upload: staging copy: %s", hipGetErrorString(e));
+    }
+    src = staging;
+  }
+  hipError_t e = launch_pad_brick(src, dtype, n, mem, (int32_t)px, (int32_t)py, (int32_t)pz, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (staging) hipFree(staging);
+  if (e != hipSuccess) {
+    hipFree(mem);
+    return fail(ctx, ALOHA_ERR_HIP, "aloha_brick_upload: pad kernel: %s", hipGetErrorString(e));
+  }
+  DevBrick b{};
+  for (int a = 0; a < 3; ++a) {
+    b.space_lo[a] = desc->space_range.lower[a];
+    b.space_hi[a] = desc->space_range.upper[a];
+    b.vox_lo[a] = (float)desc->voxel_range.lower[a];
+    b.n[a] = n[a];
+  }
+  b.px = (int32_t)px;
+  b.pxy = (int32_t)(px * py);
+  b.dtype = ALOHA_F32;
+  b.origin = mem + (kPadLo + kPadLo * px + kPadLo * px * py);
+  ctx->bricks.push_back(b);
+  ctx->brick_mem.push_back(mem);
+  ctx->descs.push_back(*desc);
+  ctx->brick_bytes += (uint64_t)padded * sizeof(float);
+  ctx->bricks_dirty = true;
+  if (brick_id) *brick_id = (int32_t)ctx->bricks.size() - 1;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_brick_count(const aloha_ctx *ctx, int32_t *count) {
+  if (!ctx || !count) return fail(nullptr, ALOHA_ERR_ARG, "aloha_brick_count: NULL");
+  *count = (int32_t)ctx->bricks.size();
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_brick_bytes(const aloha_ctx *ctx, uint64_t *bytes) {
+  if (!ctx || !bytes) return fail(nullptr, ALOHA_ERR_ARG, "aloha_brick_bytes: NULL");
+  *bytes = ctx->brick_bytes;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_tf_set(aloha_ctx *ctx, const float *rgba, int32_t n) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_tf_set: ctx is NULL");
+  if (!rgba || n < 1 || n > ALOHA_MAX_TF)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_tf_set: need 1..%d entries", ALOHA_MAX_TF);
+  for (int i = 0; i < 4 * n; ++i)
+    if (!std::isfinite(rgba[i]))
+      return fail(ctx, ALOHA_ERR_ARG, "aloha_tf_set: entry %d is not finite", i / 4);
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (n > ctx->tf_cap) {
+    if (ctx->d_tf) HIP_CHECK(ctx, hipFree(ctx->d_tf));
+    ctx->d_tf = nullptr;
+    HIP_CHECK(ctx, hipMalloc(&ctx->d_tf, sizeof(float4) * (size_t)n));
+    ctx->tf_cap = n;
+  }
+  // ordered after earlier launches on the stream that read the old table
+  HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_CHECK(ctx, hipMemcpy(ctx->d_tf, rgba, sizeof(float4) * (size_t)n, hipMemcpyHostToDevice));
+  ctx->tf_n = n;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_camera_set(aloha_ctx *ctx, const float org[3], const float dir_00[3],
+                                const float dir_du[3], const float dir_dv[3]) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_camera_set: ctx is NULL");
+  if (!org || !dir_00 || !dir_du || !dir_dv) return fail(ctx, ALOHA_ERR_ARG, "aloha_camera_set: NULL");
+  if (!finite3(org) || !finite3(dir_00) || !finite3(dir_du) || !finite3(dir_dv))
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_camera_set: non-finite camera");
+  std::memcpy(ctx->cam, org, 12);
+  std::memcpy(ctx->cam + 3, dir_00, 12);
+  std::memcpy(ctx->cam + 6, dir_du, 12);
+  std::memcpy(ctx->cam + 9, dir_dv, 12);
+  ctx->has_cam = true;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                  int32_t H, int32_t P, const int32_t *rank_offsets,
+                                  const int32_t *brick_ids, const aloha_images *out) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_ranks: ctx is NULL");
+  if (!out) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_ranks: out is NULL");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  if (params->accum_id > 0 && !out->accum)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_render_ranks: accum_id > 0 needs the accum buffer");
+  a.rgba = (float4 *)out->rgba;
+  a.rgba8 = out->rgba8;
+  a.depth = out->depth;
+  a.accum = (float4 *)out->accum;
+  a.nsamples = out->nsamples;
+  HIP_CHECK(ctx, launch_march(a, false, ctx->stream));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params,
+                                      int32_t W, int32_t H, int32_t P,
+                                      const int32_t *rank_offsets, const int32_t *brick_ids,
+                                      uint64_t *touched_voxels) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_footprint: ctx is NULL");
+  if (!touched_voxels) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_footprint: NULL output");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  // one bitset per brick, word-aligned
+  uint64_t words = 0;
+  for (auto &b : ctx->bricks) {
+    const uint64_t nv = (uint64_t)b.n[0] * b.n[1] * b.n[2];
+    if (words + (nv + 31) / 32 > 0x7fffffffull)
+      return fail(ctx, ALOHA_ERR_LIMIT, "aloha_render_footprint: bitset too large");
+    b.touched_word = (int32_t)words;
+    words += (nv + 31) / 32;
+  }
+  ctx->bricks_dirty = true;
+  if (int rc = sync_bricks(ctx)) return rc;
+  uint32_t *bits = nullptr;
+  HIP_CHECK(ctx, hipMalloc(&bits, words * 4));
+  hipError_t e = hipMemsetAsync(bits, 0, words * 4, ctx->stream);
+  a.touched = bits;
+  a.bricks = ctx->d_bricks;
+  if (e == hipSuccess) e = launch_march(a, true, ctx->stream);
+  std::vector<uint32_t> host(words);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(host.data(), bits, words * 4, hipMemcpyDeviceToHost);
+  hipFree(bits);
+  if (e != hipSuccess) return fail(ctx, ALOHA_ERR_HIP, "aloha_render_footprint: %s", hipGetErrorString(e));
+  uint64_t cnt = 0;
+  for (uint32_t w : host) cnt += (uint64_t)__builtin_popcount(w);
+  *touched_voxels = cnt;
+  return ALOHA_OK;
+}
+
+namespace {
+int check_order(aloha_ctx *ctx, const int32_t *order, int32_t P, CompositeArgs &c) {
+  if (P < 1 || P > ALOHA_MAX_RANKS)
+    return fail(ctx, ALOHA_ERR_LIMIT, "composite: P=%d outside [1,%d]", P, ALOHA_MAX_RANKS);
+  if (!order) return fail(ctx, ALOHA_ERR_ARG, "composite: order is NULL");
+  std::vector<char> seen(P, 0);
+  for (int k = 0; k < P; ++k) {
+    if (order[k] < 0 || order[k] >= P || seen[order[k]])
+      return fail(ctx, ALOHA_ERR_ARG, "composite: order is not a permutation of 0..%d", P - 1);
+    seen[order[k]] = 1;
+    c.order[k] = order[k];
+  }
+  c.P = P;
+  return ALOHA_OK;
+}
+}  // namespace
+
+extern "C" int aloha_composite_over(aloha_ctx *ctx, const float *d_rank_rgba, int32_t P,
+                                    const int32_t *order, int32_t W, int32_t H,
+                                    float *d_out_rgba, uint32_t *d_out_rgba8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_composite_over: ctx is NULL");
+  if (!d_rank_rgba || W < 1 || H < 1) return fail(ctx, ALOHA_ERR_ARG, "aloha_composite_over: bad argument");
+  CompositeArgs c;
+  if (int rc = check_order(ctx, order, P, c)) return rc;
+  c.npix = (int64_t)W * H;
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  HIP_CHECK(ctx, launch_composite_over((const float4 *)d_rank_rgba, c, (float4 *)d_out_rgba,
+                                       d_out_rgba8, ctx->stream));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_composite_over_rgba8(aloha_ctx *ctx, const uint32_t *d_rank_rgba8,
+                                          int32_t P, const int32_t *order, int32_t W, int32_t H,
+                                          uint32_t *d_out_rgba8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_composite_over_rgba8: ctx is NULL");
+  if (!d_rank_rgba8 || !d_out_rgba8 || W < 1 || H < 1)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_composite_over_rgba8: bad argument");
+  CompositeArgs c;
+  if (int rc = check_order(ctx, order, P, c)) return rc;
+  c.npix = (int64_t)W * H;
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  HIP_CHECK(ctx, launch_composite_over_rgba8(d_rank_rgba8, c, d_out_rgba8, ctx->stream));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_composite_zmin(aloha_ctx *ctx, const uint32_t *d_rank_rgba8,
+                                    const float *d_rank_depth, int32_t P, int32_t W, int32_t H,
+                                    uint32_t *d_out_rgba8, float *d_out_depth) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_composite_zmin: ctx is NULL");
+  if (!d_rank_rgba8 || !d_rank_depth || W < 1 || H < 1 || P < 1)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_composite_zmin: bad argument");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  HIP_CHECK(ctx, launch_composite_zmin(d_rank_rgba8, d_rank_depth, P, (int64_t)W * H,
+                                       d_out_rgba8, d_out_depth, ctx->stream));
+  return ALOHA_OK;
+}

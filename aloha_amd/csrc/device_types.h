@@ -1,0 +1,66 @@
+// device_types.h -- POD records shared by the host launchers and the HIP
+// kernels of the render-and-composite path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aloha_amd.h"
+
+namespace aloha {
+
+// Brick storage in HBM: f32 voxels with a replicated border of 1 voxel below
+// and 3 above on every axis, so that every index a trilinear tap or the
+// 3-wide gradient footprint can form (-1 .. n+2) reads the clamp-to-edge
+// value without a per-load clamp.  `origin` points at voxel (0,0,0).
+constexpr int kPadLo = 1;
+constexpr int kPadHi = 3;
+
+struct DevBrick {
+  float space_lo[3], space_hi[3];  // spaceRange (ray clipping)
+  float vox_lo[3];                 // (float)voxelRange.lower (texel origin)
+  int32_t n[3];                    // voxel extent voxelRange.size()
+  int32_t px;                      // padded x pitch, elements
+  int32_t pxy;                     // padded slice pitch, elements
+  int32_t dtype;                   // storage encoding (ALOHA_F32)
+  int32_t touched_word;            // footprint pass: first 32-bit word of this brick's bitset
+  const float *origin;
+};
+
+// Kernel arguments of one render launch (all virtual ranks at once).
+struct MarchArgs {
+  float org[3], dir00[3], du[3], dv[3];
+  float dt, max_depth, jitter_value, heat_scale;
+  int32_t W, H, spp, accum_id;
+  int32_t island_id, num_islands, jitter_mode, flags;
+  int32_t P, tf_n;
+  const DevBrick *bricks;
+  const float4 *tf;
+  float4 *rgba;
+  uint32_t *rgba8;
+  float *depth;
+  float4 *accum;
+  uint32_t *nsamples;
+  uint32_t *touched;  // footprint pass only
+  int32_t rank_off[ALOHA_MAX_RANKS + 1];
+  int32_t rank_bricks[ALOHA_MAX_RANK_BRICKS];
+};
+
+struct CompositeArgs {
+  int32_t P;
+  int64_t npix;
+  int32_t order[ALOHA_MAX_RANKS];
+};
+
+// launchers (march.hip, composite.hip)
+hipError_t launch_march(const MarchArgs &a, bool footprint, hipStream_t s);
+hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
+                            int32_t px, int32_t py, int32_t pz, hipStream_t s);
+hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, float4 *out,
+                                 uint32_t *out8, hipStream_t s);
+hipError_t launch_composite_over_rgba8(const uint32_t *imgs, const CompositeArgs &c,
+                                       uint32_t *out8, hipStream_t s);
+hipError_t launch_composite_zmin(const uint32_t *imgs, const float *depth, int32_t P,
+                                 int64_t npix, uint32_t *out8, float *outz, hipStream_t s);
+
+}  // namespace aloha

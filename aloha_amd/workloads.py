@@ -1,0 +1,90 @@
+"""Seeded synthetic volume workloads in the reference's data layout.
+
+Each workload is a raw volume bricked by chopSuey's VolumeSplitter, bricks
+assigned to virtual ranks by partitionKD, a piecewise-linear TF and an orbit
+camera -- everything Renderer's constructor and event handlers would hold
+(waikikiRenderer.cpp:344-846, :853-1013) after loading a .vol file.
+
+  blobs128   128^3 u8 Gaussian-blob volume, 2 bricks, 512^2 (the CPU-mode case)
+  fbm512     512^3 f32 fBm volume, 8 bricks (2x2x2), 1024^2 (the GPU headline)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (Box3f, BrickDesc, Partitioner, max_depth, render_params, split_volume,
+               synth_blobs_u8, synth_camera, synth_fbm_f32, synth_tf)
+
+
+@dataclass
+class VolumeWorkload:
+    name: str
+    dims: tuple
+    raw: np.ndarray                      # (z, y, x) u8 or f32
+    bricks: list                         # BrickDesc per brick (chopSuey order)
+    voxels: list                         # raw-dtype (z,y,x) array per brick
+    tf: np.ndarray                       # (n, 4) float32
+    cam: dict                            # org, dir_00, dir_du, dir_dv
+    W: int
+    H: int
+    dt: float
+    num_ranks: int
+    rank_bricks: list = field(default_factory=list)   # per rank: brick ids
+    order: list = field(default_factory=list)         # composite order (rank ids)
+    max_depth: float = 1.0
+
+    @property
+    def bounds(self) -> Box3f:
+        # the .vol header spaceRange = [0, dims] is Renderer::modelBounds
+        b = Box3f()
+        for a in range(3):
+            b.lower[a] = 0.0
+            b.upper[a] = float(self.dims[a])
+        return b
+
+    def params(self, **kw):
+        base = dict(dt=self.dt, max_depth=self.max_depth)
+        base.update(kw)
+        return render_params(**base)
+
+    def brick_voxels_total(self) -> int:
+        return int(sum(v.size for v in self.voxels))
+
+
+def brick_subvolumes(raw: np.ndarray, bricks) -> list:
+    out = []
+    for d in bricks:
+        lo, hi = d.voxel_range.lower, d.voxel_range.upper
+        out.append(np.ascontiguousarray(raw[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]))
+    return out
+
+
+def make_volume_workload(name: str, raw: np.ndarray, num_bricks: int, num_ranks: int, W: int,
+                         H: int, dt: float = 0.5, tf_seed: int = 0x5EED2,
+                         cam_seed: int = 0x5EED3, tf_n: int = 256) -> VolumeWorkload:
+    dims = tuple(raw.shape[::-1])
+    bricks = split_volume(dims, num_bricks)
+    voxels = brick_subvolumes(raw, bricks)
+    wl = VolumeWorkload(name=name, dims=dims, raw=raw, bricks=bricks, voxels=voxels,
+                        tf=synth_tf(tf_n, 6, tf_seed), cam={}, W=W, H=H, dt=dt,
+                        num_ranks=num_ranks)
+    wl.cam = synth_camera(wl.bounds, W, H, 45.0, 2.5, cam_seed)
+    part = Partitioner.kd([d.space_range for d in bricks], num_ranks)
+    wl.rank_bricks = [part.rank_bricks(r) for r in range(num_ranks)]
+    wl.order = part.composite_order(wl.cam["org"])
+    wl.max_depth = max_depth(wl.bounds, wl.cam["org"])
+    return wl
+
+
+def blobs128(num_ranks: int = 2, W: int = 512, H: int = 512, seed: int = 0x5EED0,
+             size: int = 128) -> VolumeWorkload:
+    raw = synth_blobs_u8((size, size, size), 16, seed, 4.0, 16.0, 0.3, 1.0)
+    return make_volume_workload("blobs128", raw, 2, num_ranks, W, H)
+
+
+def fbm512(num_ranks: int = 8, W: int = 1024, H: int = 1024, seed: int = 0x5EED1,
+           size: int = 512) -> VolumeWorkload:
+    raw = synth_fbm_f32((size, size, size), 5, size / 8.0, seed)
+    return make_volume_workload("fbm512", raw, 8, num_ranks, W, H)

@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-brick render-and-composite path on one MI355X.
+
+One step = one frame: every virtual rank ray-marches its bricks into its
+float RGBA + RGBA8 + depth image (one launch for all ranks), then the ranks'
+images are 'over'-composited in kd visibility order (one launch).  Inputs
+(bricks, TF, camera) are resident in HBM before timing starts.
+
+Default workload: 512^3 float32 fBm volume, chopSuey-split into 8 bricks
+(2x2x2, ghost voxels), one brick per virtual rank, 1024x1024, dt = 0.5.
+
+Prints one JSON line (metric = Gsamples/s; ms_per_step = ms/frame).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Gsamples/s & ms/frame, 1024² DVR of 512³ 8-brick vol; "
+          "fwd+adjoint vs fwd ms")
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="fbm512", choices=["fbm512", "blobs128"])
+    ap.add_argument("--size", type=int, default=None, help="volume edge (default per workload)")
+    ap.add_argument("--res", type=int, default=None, help="image edge (default per workload)")
+    ap.add_argument("--ranks", type=int, default=None, help="virtual ranks (default: bricks)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU work of the baseline sample")
+    ap.add_argument("--profile", action="store_true",
+                    help="few frames, no CPU baseline (for rocprofv3 runs)")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU: every virtual rank's bricks "
+                          "live on a single MI355X; run with --gpus 1"}))
+        sys.exit(2)
+    if args.profile:
+        args.no_cpu = True
+
+    import torch
+    import aloha_amd as A
+    from aloha_amd import RankImages
+    from aloha_amd import workloads as WL
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    t0 = time.time()
+    if args.workload == "fbm512":
+        size, res = args.size or 512, args.res or 1024
+        wl = WL.fbm512(num_ranks=args.ranks or 8, W=res, H=res, size=size)
+        desc = f"fbm{size} f32 8 bricks (2x2x2, ghost voxels), {res}x{res}, dt 0.5"
+    else:
+        size, res = args.size or 128, args.res or 512
+        wl = WL.blobs128(num_ranks=args.ranks or 2, W=res, H=res, size=size)
+        desc = f"blobs{size} u8 2 bricks, {res}x{res}, dt 0.5"
+    log(f"[bench] workload {desc} built in {time.time() - t0:.1f}s")
+    P, W, H = wl.num_ranks, wl.W, wl.H
+
+    ctx = A.Context(0, stream=stream)
+    t_up = time.time()
+    ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
+    ctx.sync()
+    upload_s = time.time() - t_up
+    ctx.set_tf(wl.tf)
+    ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+    ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
+    params = wl.params()
+    imgs = RankImages.alloc(P, W, H, rgba=True, rgba8=True, depth=True)
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    out8 = torch.empty((H, W), dtype=torch.int32, device=dev)
+    log(f"[bench] {len(ids)} bricks uploaded ({ctx.brick_bytes() / 1e6:.0f} MB padded) "
+        f"in {upload_s:.2f}s incl. PCIe")
+
+    def frame():
+        ctx.render_ranks(params, W, H, ranks, imgs)
+        ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8)
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    for _ in range(args.warmup):
+        frame()
+    torch.cuda.synchronize()
+    ms_frame = timed(frame, args.steps)
+    log(f"[bench] {args.steps} frames: {ms_frame:.4f} ms/frame")
+
+    # per-kernel averages over the same number of launches
+    ms_render = timed(lambda: ctx.render_ranks(params, W, H, ranks, imgs), args.steps)
+    ms_comp = timed(lambda: ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8), args.steps)
+    log(f"[bench] render {ms_render:.4f} ms, composite {ms_comp:.4f} ms")
+
+    # exact sample count and compulsory voxel footprint of one frame (untimed)
+    cnt = RankImages.alloc(P, W, H, rgba=False, rgba8=False, depth=False, nsamples=True)
+    ctx.render_ranks(params, W, H, ranks, cnt)
+    torch.cuda.synchronize()
+    samples = int(cnt.nsamples.to(torch.int64).sum().item())
+    touched = ctx.footprint(params, W, H, ranks)
+    gsamples = samples / (ms_frame * 1e-3) / 1e9
+
+    # algorithmic bytes per launch
+    npix = W * H
+    render_bytes = touched * 4 + wl.tf.nbytes + P * npix * (16 + 4 + 4)
+    comp_bytes = P * npix * 16 + npix * (16 + 4)
+    render_gbs = render_bytes / (ms_render * 1e-3) / 1e9
+    comp_gbs = comp_bytes / (ms_comp * 1e-3) / 1e9
+    dominant = "render" if ms_render >= ms_comp else "composite"
+    roof = {
+        "kernel": "march_kernel" if dominant == "render" else "over_kernel",
+        "bound": "hbm",
+        "achieved": round(render_gbs if dominant == "render" else comp_gbs, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round((render_gbs if dominant == "render" else comp_gbs) / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
+        "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
+    }
+    kernels = {
+        "march_kernel": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
+                         "GB/s": round(render_gbs, 1), "frac_hbm": round(render_gbs / HBM_PEAK_GBS, 4),
+                         "Gsamples/s": round(samples / (ms_render * 1e-3) / 1e9, 3)},
+        "over_kernel": {"ms": round(ms_comp, 5), "bytes": int(comp_bytes),
+                        "GB/s": round(comp_gbs, 1), "frac_hbm": round(comp_gbs / HBM_PEAK_GBS, 4)},
+    }
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(wl, imgs, args.cpu_seconds)
+
+    res = {
+        "metric": METRIC,
+        "value": round(gsamples, 4),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_frame, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": desc, "virtual_ranks": P, "bricks": len(ids),
+                   "samples_per_frame": samples, "touched_voxels": touched,
+                   "brick_bytes_padded": ctx.brick_bytes(), "upload_s_incl_pcie": round(upload_s, 3)},
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "parity_sample": parity,
+    }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(wl, imgs, seconds):
+    """The oracle (FP32 CPU restatement, OpenMP) on a bounded band of rows of
+    every rank, on this host's cores; also the FP64 error of both paths on
+    that band."""
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    H = wl.H
+    gpu_rgba = imgs.rgba.cpu().numpy()
+    # band through the image centre, grown until it holds ~`seconds` of work
+    band = 8
+    per_rank = lambda r, rows: O.render_rank(
+        wl.cam, wl.tf, [(wl.bricks[i], wl.voxels[i]) for i in wl.rank_bricks[r]], wl.W, wl.H,
+        dt=wl.dt, max_depth=wl.max_depth, rows=rows, threads=threads)
+    while True:
+        rows = (H // 2 - band // 2, H // 2 + band // 2)
+        t = time.time()
+        ns = 0
+        for r in range(wl.num_ranks):
+            ns += int(per_rank(r, rows)["nsamples"].sum())
+        el = time.time() - t
+        if el * 2 > seconds or band >= H:
+            break
+        band = min(H, band * max(2, int(seconds / max(el, 1e-3) / 2)))
+    cpu_gs = ns / el / 1e9
+    # FP64 error statement on a small band
+    r0 = (H // 2 - 4, H // 2 + 4)
+    err_gpu, err_cpu, max_ulp = 0.0, 0.0, 0.0
+    for r in range(wl.num_ranks):
+        o = O.render_rank(wl.cam, wl.tf, [(wl.bricks[i], wl.voxels[i]) for i in wl.rank_bricks[r]],
+                          wl.W, wl.H, dt=wl.dt, max_depth=wl.max_depth, rows=r0, want64=True,
+                          threads=threads)
+        sl = slice(r0[0], r0[1])
+        err_gpu = max(err_gpu, float(np.abs(gpu_rgba[r, sl] - o["rgba64"][sl]).max()))
+        err_cpu = max(err_cpu, float(np.abs(o["rgba"][sl] - o["rgba64"][sl]).max()))
+        max_ulp = max(max_ulp, float(np.abs(gpu_rgba[r, sl] - o["rgba"][sl]).max()))
+    cpu = {"value": round(cpu_gs, 5), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+           "sample": f"rows {rows[0]}..{rows[1]} of all {wl.num_ranks} ranks ({ns} samples, "
+                     f"{el:.1f}s), FP32 oracle, OpenMP"}
+    parity = {"rows": f"{r0[0]}..{r0[1]}", "max_abs_gpu_vs_cpu_fp32": max_ulp,
+              "max_abs_gpu_vs_fp64": err_gpu, "max_abs_cpu_fp32_vs_fp64": err_cpu}
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,234 @@
+/*
+ * aloha_amd.h -- C ABI of the MI355X-native per-brick render-and-composite
+ * path.  Plain pointers and sizes only; no exceptions cross this boundary.
+ *
+ * Every entry point returns an int status: ALOHA_OK (0) or a negative
+ * ALOHA_ERR_* code; aloha_last_error() gives the message.  Pointers named
+ * d_* are device pointers (HBM); all others are host pointers.
+ *
+ * Each declaration names the reference interface it replaces
+ * (vtvamr/aloha, file:line).
+ */
+#ifndef ALOHA_AMD_H
+#define ALOHA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALOHA_OK 0
+#define ALOHA_ERR_ARG (-1)
+#define ALOHA_ERR_HIP (-2)
+#define ALOHA_ERR_ALLOC (-3)
+#define ALOHA_ERR_STATE (-4)
+#define ALOHA_ERR_IO (-5)
+#define ALOHA_ERR_LIMIT (-6)
+
+/* voxel encodings (bytes per voxel, volume.h:76-86 / chopSuey -bpc) */
+#define ALOHA_U8 1
+#define ALOHA_U16 2
+#define ALOHA_F32 4
+
+/* jitter of the per-brick sample lattice */
+#define ALOHA_JITTER_RANDOM 0 /* one RNG draw per brick visit (deviceCode.cpp:386) */
+#define ALOHA_JITTER_FIXED 1  /* constant jitter_value (rank-invariant images) */
+
+/* render flags */
+#define ALOHA_FLAG_CROSSHAIR 1 /* invert row H/2 and column W/2 in rgba8 (deviceCode.cpp:410-414) */
+#define ALOHA_FLAG_HEATMAP 2   /* blend a per-pixel clock heat map (deviceCode.cpp:399-403) */
+
+/* limits of one render launch (kernel-argument tables) */
+#define ALOHA_MAX_RANKS 64
+#define ALOHA_MAX_RANK_BRICKS 256
+#define ALOHA_MAX_TF 4096
+
+/* owl box3i / box3f / interval<float> layouts (2 x vec3 of 4-byte scalars) */
+typedef struct { int32_t lower[3], upper[3]; } aloha_box3i;
+typedef struct { float lower[3], upper[3]; } aloha_box3f;
+typedef struct { float lower, upper; } aloha_range1f;
+
+/* One brick record: the per-brick header of the .vol format
+ * (chopSuey.cpp:342-345) and the VolBrick device record minus its texture
+ * (deviceCode.h:35-50; brick.h:32-41).  80 bytes. */
+typedef struct {
+  aloha_box3i cell_range;
+  aloha_box3i voxel_range;
+  aloha_box3f space_range;
+  aloha_range1f value_range;
+} aloha_brick_desc;
+
+/* Per-frame render settings: LaunchParams.render + accumID/island fields
+ * (deviceCode.h:66-96), set by Renderer::render (waikikiRenderer.cpp:1021-1026).
+ * dt is exposed here; the reference hard-codes 2.0 (waikikiRenderer.cpp:1023). */
+typedef struct {
+  float dt;
+  float max_depth;
+  int32_t spp;
+  int32_t accum_id;
+  int32_t island_id;
+  int32_t num_islands;
+  int32_t jitter_mode;
+  float jitter_value;
+  int32_t flags;
+  float heat_map_scale;
+} aloha_render_params;
+
+/* Per-rank outputs (device pointers, each may be NULL).  Rank slot s of a
+ * launch writes pixel p at index s*W*H + p.
+ *   rgba     float4, accum/(accumID+1)/spp before the crosshair (premultiplied)
+ *   rgba8    make_rgba pack of the same (LaunchParams.fbPointer)
+ *   depth    first brick entry / maxDepth, 1 if none (LaunchParams.fbDepth)
+ *   accum    float4 running sum over frames (LaunchParams.accumBuffer)
+ *   nsamples per-pixel executed sample count (measurement) */
+typedef struct {
+  float *rgba;
+  uint32_t *rgba8;
+  float *depth;
+  float *accum;
+  uint32_t *nsamples;
+} aloha_images;
+
+typedef struct aloha_ctx aloha_ctx;
+typedef struct aloha_partitioner aloha_partitioner;
+
+/* ---------------------------------------------------------------------- */
+/* errors                                                                  */
+/* ---------------------------------------------------------------------- */
+/* message of the last failing call on ctx (ctx may be NULL: last failure of
+ * a call that had no context). */
+const char *aloha_last_error(const aloha_ctx *ctx);
+/* library version string */
+const char *aloha_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* host-side bricking and partitioning (no GPU needed)                     */
+/* ---------------------------------------------------------------------- */
+/* VolumeSplitter (chopSuey.cpp:245-323): middle-split a dims volume into at
+ * least num_bricks bricks (always splits once).  value_range is set to the
+ * empty interval {FLT_MAX,-FLT_MAX}.  *count receives the brick count. */
+int aloha_split_volume(const int32_t dims[3], int32_t num_bricks, aloha_brick_desc *out,
+                       int32_t cap, int32_t *count);
+
+/* Volume::get (volume.h:46-98): copy voxel sub-box [range.lower,range.upper)
+ * of an x-fastest raw volume (bpc 1/2/4) to float and extend value_range. */
+int aloha_volume_extract(const void *raw, const int32_t dims[3], int32_t bpc,
+                         const aloha_box3i *range, float *out, aloha_range1f *value_range);
+
+/* Partitioner (waikikiRenderer.cpp:126-338). */
+int aloha_partitioner_create_kd(const aloha_box3f *domains, int32_t n, int32_t num_ranks,
+                                aloha_partitioner **out); /* partitionKD :150-270 */
+int aloha_partitioner_create_rr(int32_t n, int32_t num_ranks,
+                                aloha_partitioner **out); /* partitionRoundRobin :137-148 */
+void aloha_partitioner_destroy(aloha_partitioner *p);
+/* perRank[rank] (:334) */
+int aloha_partitioner_rank_bricks(const aloha_partitioner *p, int32_t rank, int32_t *ids,
+                                  int32_t cap, int32_t *count);
+/* computeCompositeOrder(reference) (:281-316) */
+int aloha_partitioner_composite_order(const aloha_partitioner *p, const float reference[3],
+                                      int32_t *order, int32_t cap, int32_t *count);
+/* kd nodes as {axis, plane, child1, child2} (KDNode :318-322), 16 bytes each */
+int aloha_partitioner_kd_nodes(const aloha_partitioner *p, void *nodes, int32_t cap,
+                               int32_t *count);
+
+/* ---------------------------------------------------------------------- */
+/* .vol brick files (chopSuey.cpp:325-349; waikikiRenderer.cpp:596-665)    */
+/* ---------------------------------------------------------------------- */
+/* Write header {u64 n; box3i cellRange; box3i voxelRange; box3f spaceRange}
+ * and per brick {desc; float voxels[]} (voxels[i] has the brick's voxel count). */
+int aloha_vol_write(const char *path, const aloha_box3i *cell_range,
+                    const aloha_box3i *voxel_range, const aloha_box3f *space_range,
+                    int32_t n, const aloha_brick_desc *bricks, const float *const *voxels);
+/* Read the header and brick records (no voxels); *n receives the brick count. */
+int aloha_vol_read_header(const char *path, aloha_box3i *cell_range, aloha_box3i *voxel_range,
+                          aloha_box3f *space_range, aloha_brick_desc *bricks, int32_t cap,
+                          int32_t *n);
+/* Read the voxels of brick i into out (float, voxel count of brick i). */
+int aloha_vol_read_brick(const char *path, int32_t i, float *out, size_t out_count);
+
+/* ---------------------------------------------------------------------- */
+/* seeded synthetic inputs (bench / tests)                                 */
+/* ---------------------------------------------------------------------- */
+/* Sum of Gaussian blobs, centres U[0,dims), sigma U[smin,smax], amplitude
+ * U[amin,amax]; normalised round(255*f/max f) to u8. */
+int aloha_synth_blobs_u8(const int32_t dims[3], int32_t num_blobs, uint64_t seed, float smin,
+                         float smax, float amin, float amax, uint8_t *out);
+/* Same field normalised to [0,1] float. */
+int aloha_synth_blobs_f32(const int32_t dims[3], int32_t num_blobs, uint64_t seed, float smin,
+                          float smax, float amin, float amax, float *out);
+/* Fractal Brownian motion of seeded gradient noise, rescaled to [0,1]. */
+int aloha_synth_fbm_f32(const int32_t dims[3], int32_t octaves, float base_period,
+                        uint64_t seed, float *out);
+/* Piecewise-linear RGBA TF through num_points seeded control points
+ * (first at 0, last at 1, sorted interior, RGBA U[0,1]); n entries. */
+int aloha_synth_tf(int32_t n, int32_t num_points, uint64_t seed, float *out_rgba);
+/* Orbit pinhole camera looking at the centre of bounds from dist_factor x
+ * its diagonal, seeded azimuth/elevation; out = org, dir_00, dir_du, dir_dv
+ * in the viewer's convention (viewer.cpp:148-158). */
+int aloha_synth_camera(const aloha_box3f *bounds, int32_t W, int32_t H, float fovy_deg,
+                       float dist_factor, uint64_t seed, float out[12]);
+
+/* ---------------------------------------------------------------------- */
+/* device context (replaces the OWL context/params/raygen, waikikiRenderer */
+/* .cpp:377-384, and the CUDA texture objects, :697-777, :945-1013)        */
+/* ---------------------------------------------------------------------- */
+int aloha_ctx_create(int32_t device, aloha_ctx **out);
+int aloha_ctx_destroy(aloha_ctx *ctx);
+/* Launch all work of ctx on this hipStream_t from now on (NULL: the null
+ * stream).  A new context owns a non-blocking stream of its own. */
+int aloha_set_stream(aloha_ctx *ctx, void *hip_stream);
+int aloha_sync(aloha_ctx *ctx);
+
+/* Brick upload (cudaMalloc3DArray+cudaMemcpy3D+texture, :697-777).  voxels
+ * is x-fastest with the desc's voxel_range extent, encoding dtype
+ * (ALOHA_U8/U16/F32), on the device if src_on_device.  Values are decoded
+ * as Volume::get does (u8/255.f, u16/65535.f).  *brick_id receives its id. */
+int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc, const void *voxels,
+                       int32_t dtype, int32_t src_on_device, int32_t *brick_id);
+int aloha_brick_count(const aloha_ctx *ctx, int32_t *count);
+/* device bytes held by bricks (padded layout) */
+int aloha_brick_bytes(const aloha_ctx *ctx, uint64_t *bytes);
+
+/* Transfer function (handleColorMapEvent, :945-1013): n RGBA float entries. */
+int aloha_tf_set(aloha_ctx *ctx, const float *rgba, int32_t n);
+
+/* Camera (handleCameraEvent, :853-871): org, dir_00, dir_du, dir_dv. */
+int aloha_camera_set(aloha_ctx *ctx, const float org[3], const float dir_00[3],
+                     const float dir_du[3], const float dir_dv[3]);
+
+/* The raygen launch (owlLaunch2D(rayGen,W,H), :1028; renderFrame,
+ * deviceCode.cpp:340-416) for P virtual ranks at once.  Rank slot s
+ * (0<=s<P) marches bricks brick_ids[rank_offsets[s] .. rank_offsets[s+1]).
+ * Asynchronous on the ctx stream. */
+int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                       int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
+                       const aloha_images *out);
+
+/* Count distinct voxels read by the march of the same launch (measurement:
+ * compulsory bytes).  Synchronous. */
+int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                           int32_t H, int32_t P, const int32_t *rank_offsets,
+                           const int32_t *brick_ids, uint64_t *touched_voxels);
+
+/* Sort-last ordered 'over' of P premultiplied float RGBA rank images, front
+ * to back in order[] (IceT ORDERED BLEND, :911-916, :1043-1077), onto
+ * transparent black.  d_rank_rgba: P*W*H float4.  Outputs may be NULL. */
+int aloha_composite_over(aloha_ctx *ctx, const float *d_rank_rgba, int32_t P,
+                         const int32_t *order, int32_t W, int32_t H, float *d_out_rgba,
+                         uint32_t *d_out_rgba8);
+/* Same over P RGBA8 rank images (the reference's ubyte path, :908, :915). */
+int aloha_composite_over_rgba8(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, int32_t P,
+                               const int32_t *order, int32_t W, int32_t H,
+                               uint32_t *d_out_rgba8);
+/* Depth composite (IceT Z_BUFFER/REDUCE, :903-909): per pixel the rank with
+ * the smallest depth wins, ties to the lower rank.  Outputs may be NULL. */
+int aloha_composite_zmin(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, const float *d_rank_depth,
+                         int32_t P, int32_t W, int32_t H, uint32_t *d_out_rgba8,
+                         float *d_out_depth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALOHA_AMD_H */

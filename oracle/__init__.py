@@ -1,0 +1,290 @@
+"""ctypes view of the CPU oracle (oracle/aloha_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  Build it with
+`make -C oracle` (or `python -m aloha_amd.build`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libaloha_oracle.so"
+
+
+class OrcBrick(C.Structure):
+    _fields_ = [("cell_lo", C.c_int32 * 3), ("cell_hi", C.c_int32 * 3),
+                ("vox_lo", C.c_int32 * 3), ("vox_hi", C.c_int32 * 3),
+                ("space_lo", C.c_float * 3), ("space_hi", C.c_float * 3),
+                ("dtype", C.c_int32), ("pad_", C.c_int32), ("voxels", C.c_void_p)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("dir00", C.c_float * 3), ("du", C.c_float * 3),
+                ("dv", C.c_float * 3), ("dt", C.c_float), ("max_depth", C.c_float),
+                ("W", C.c_int32), ("H", C.c_int32), ("spp", C.c_int32), ("accum_id", C.c_int32),
+                ("island_id", C.c_int32), ("num_islands", C.c_int32),
+                ("jitter_mode", C.c_int32), ("jitter_value", C.c_float),
+                ("crosshair", C.c_int32), ("tf_n", C.c_int32), ("tf", C.POINTER(C.c_float))]
+
+
+class OrcDomain(C.Structure):
+    _fields_ = [("cell_lo", C.c_int32 * 3), ("cell_hi", C.c_int32 * 3),
+                ("vox_lo", C.c_int32 * 3), ("vox_hi", C.c_int32 * 3),
+                ("space_lo", C.c_float * 3), ("space_hi", C.c_float * 3)]
+
+
+class OrcKD(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("plane", C.c_float), ("child1", C.c_int32),
+                ("child2", C.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} not built: run `make -C oracle`")
+        _lib = C.CDLL(str(LIB_PATH))
+        _lib.orc_make_rgba.restype = C.c_uint32
+        _lib.orc_rng_seed.restype = C.c_uint32
+        _lib.orc_rng_draw.restype = C.c_float
+        _lib.orc_rng_seed.argtypes = [C.c_uint32, C.c_uint32]
+        _lib.orc_rng_draw.argtypes = [C.POINTER(C.c_uint32)]
+        _lib.orc_composite_over.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong,
+                                            C.c_void_p, C.c_void_p]
+        _lib.orc_composite_over_d.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong,
+                                              C.c_void_p]
+        _lib.orc_composite_over_u8.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong,
+                                               C.c_void_p]
+        _lib.orc_composite_zmin.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong,
+                                            C.c_void_p, C.c_void_p]
+    return _lib
+
+
+def _i3(v):
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def split_volume(dims, n):
+    cap = 4 * n + 8
+    out = (OrcDomain * cap)()
+    k = lib().orc_split_volume(_i3(dims), int(n), out, cap)
+    if k < 0:
+        raise ValueError("orc_split_volume failed")
+    return [{"cell": (tuple(d.cell_lo), tuple(d.cell_hi)), "voxel": (tuple(d.vox_lo), tuple(d.vox_hi)),
+             "space": (tuple(d.space_lo), tuple(d.space_hi))} for d in out[:k]]
+
+
+def volume_get(raw: np.ndarray, bpc: int, first, last):
+    raw = np.ascontiguousarray(raw)
+    ext = [int(last[a]) - int(first[a]) for a in range(3)]
+    out = np.empty(ext[::-1], dtype=np.float32)
+    vr = (C.c_float * 2)(np.finfo(np.float32).max, -np.finfo(np.float32).max)
+    if lib().orc_volume_get(C.c_void_p(raw.ctypes.data), _i3(raw.shape[::-1]), bpc, _i3(first),
+                            _i3(last), C.c_void_p(out.ctypes.data), vr) != 0:
+        raise ValueError("bad bpc")
+    return out, (vr[0], vr[1])
+
+
+def partition_kd(domains: np.ndarray, P: int):
+    """domains: (n,6) float32 (lower xyz, upper xyz).  Returns (per_rank, kd)."""
+    d = np.ascontiguousarray(domains, dtype=np.float32)
+    n = d.shape[0]
+    counts = (C.c_int * P)()
+    ids = (C.c_int * max(1, n))()
+    cap = 4 * n + 8
+    kd = (OrcKD * cap)()
+    nk = lib().orc_partition_kd(C.c_void_p(d.ctypes.data), n, P, counts, ids, kd, cap)
+    if nk < 0:
+        raise ValueError("kd cap too small")
+    per, m = [], 0
+    for r in range(P):
+        per.append(list(ids[m:m + counts[r]]))
+        m += counts[r]
+    return per, [(k.axis, k.plane, k.child1, k.child2) for k in kd[:nk]]
+
+
+def partition_rr(n, P):
+    out = (C.c_int * max(1, n))()
+    lib().orc_partition_rr(n, P, out)
+    return list(out[:n])
+
+
+def composite_order(kd, ref):
+    arr = (OrcKD * max(1, len(kd)))(*[OrcKD(a, p, c1, c2) for a, p, c1, c2 in kd])
+    cap = 4096
+    out = (C.c_int * cap)()
+    m = lib().orc_composite_order(arr, len(kd), _f3(ref), out, cap)
+    return list(out[:m])
+
+
+def _orc_brick(desc, voxels: np.ndarray) -> OrcBrick:
+    b = OrcBrick()
+    if isinstance(desc, dict):
+        cell, vox, space = desc["cell"], desc["voxel"], desc["space"]
+    else:
+        cell = (tuple(desc.cell_range.lower), tuple(desc.cell_range.upper))
+        vox = (tuple(desc.voxel_range.lower), tuple(desc.voxel_range.upper))
+        space = (tuple(desc.space_range.lower), tuple(desc.space_range.upper))
+    b.cell_lo, b.cell_hi = _i3(cell[0]), _i3(cell[1])
+    b.vox_lo, b.vox_hi = _i3(vox[0]), _i3(vox[1])
+    b.space_lo, b.space_hi = _f3(space[0]), _f3(space[1])
+    b.dtype = _DT[voxels.dtype]
+    b.voxels = voxels.ctypes.data
+    assert voxels.shape[::-1] == tuple(vox[1][a] - vox[0][a] for a in range(3))
+    return b
+
+
+def sample_probe(desc, voxels: np.ndarray, x: float, y: float, z: float):
+    """(value, gx, gy, gz) in float32 and float64 at texel-space (x,y,z)."""
+    voxels = np.ascontiguousarray(voxels)
+    b = _orc_brick(desc, voxels)
+    out = (C.c_float * 4)()
+    out64 = (C.c_double * 4)()
+    lib().orc_sample_probe(C.byref(b), C.c_float(x), C.c_float(y), C.c_float(z), out, out64)
+    return np.array(out[:], np.float32), np.array(out64[:], np.float64)
+
+
+def tf_probe(tf: np.ndarray, v: float):
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    out = (C.c_float * 4)()
+    lib().orc_tf_probe(tf.ctypes.data_as(C.POINTER(C.c_float)), tf.shape[0], C.c_float(v), out)
+    return np.array(out[:], np.float32)
+
+
+def first_sample_t(t0: float, dt: float, jitter: float) -> float:
+    f = lib().orc_first_sample_t
+    f.restype = C.c_float
+    f.argtypes = [C.c_float, C.c_float, C.c_float]
+    return f(t0, dt, jitter)
+
+
+def slab(o, d, tmin, tmax, lo, hi):
+    t = (C.c_float * 2)()
+    f = lib().orc_slab_probe
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
+    hit = f(_f3(o), _f3(d), tmin, tmax, _f3(lo), _f3(hi), t)
+    return bool(hit), t[0], t[1]
+
+
+def rng_stream(pixel: int, frame: int, count: int):
+    s = C.c_uint32(lib().orc_rng_seed(pixel, frame))
+    return [lib().orc_rng_draw(C.byref(s)) for _ in range(count)]
+
+
+def make_rgba(c) -> int:
+    return lib().orc_make_rgba((C.c_float * 4)(*[float(x) for x in c]))
+
+
+_DT = {np.dtype(np.uint8): 1, np.dtype(np.uint16): 2, np.dtype(np.float32): 4}
+
+
+def render_rank(cam: dict, tf: np.ndarray, bricks, W: int, H: int, dt=0.5, max_depth=1.0,
+                spp=1, accum_id=0, island_id=0, num_islands=1, jitter_mode=0, jitter_value=0.0,
+                crosshair=False, rows=None, accum=None, want64=False, touched=False,
+                threads=0):
+    """bricks: list of (desc, voxels) where desc has cell_range/voxel_range/
+    space_range (aloha_amd.BrickDesc or dict) and voxels is (z,y,x) u8/u16/f32.
+    Returns dict with rgba (H,W,4), rgba8 (H,W) u32, depth, nsamples, accum,
+    and rgba64 / touched (list of per-brick u8 masks) when requested."""
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    keep = [tf]
+    br = (OrcBrick * max(1, len(bricks)))()
+    for i, (d, v) in enumerate(bricks):
+        v = np.ascontiguousarray(v)
+        keep.append(v)
+        if isinstance(d, dict):
+            cell, vox, space = d["cell"], d["voxel"], d["space"]
+        else:
+            cell = (tuple(d.cell_range.lower), tuple(d.cell_range.upper))
+            vox = (tuple(d.voxel_range.lower), tuple(d.voxel_range.upper))
+            space = (tuple(d.space_range.lower), tuple(d.space_range.upper))
+        br[i].cell_lo, br[i].cell_hi = _i3(cell[0]), _i3(cell[1])
+        br[i].vox_lo, br[i].vox_hi = _i3(vox[0]), _i3(vox[1])
+        br[i].space_lo, br[i].space_hi = _f3(space[0]), _f3(space[1])
+        br[i].dtype = _DT[v.dtype]
+        br[i].voxels = v.ctypes.data
+        assert v.shape[::-1] == tuple(vox[1][a] - vox[0][a] for a in range(3))
+    p = OrcParams()
+    p.org, p.dir00, p.du, p.dv = (_f3(cam[k]) for k in ("org", "dir_00", "dir_du", "dir_dv"))
+    p.dt, p.max_depth, p.W, p.H = dt, max_depth, W, H
+    p.spp, p.accum_id, p.island_id, p.num_islands = spp, accum_id, island_id, num_islands
+    p.jitter_mode, p.jitter_value, p.crosshair = jitter_mode, jitter_value, int(crosshair)
+    p.tf_n = tf.shape[0]
+    p.tf = tf.ctypes.data_as(C.POINTER(C.c_float))
+    row0, row1 = (0, H) if rows is None else rows
+    out = {
+        "rgba": np.zeros((H, W, 4), np.float32),
+        "rgba8": np.zeros((H, W), np.uint32),
+        "depth": np.zeros((H, W), np.float32),
+        "nsamples": np.zeros((H, W), np.uint32),
+        "accum": np.zeros((H, W, 4), np.float32) if accum is None else np.ascontiguousarray(accum, np.float32),
+    }
+    if want64:
+        out["rgba64"] = np.zeros((H, W, 4), np.float64)
+    tptr = None
+    if touched:
+        masks = [np.zeros(np.prod(v.shape), np.uint8) for _, v in bricks]
+        out["touched"] = masks
+        tptr = (C.c_void_p * len(masks))(*[m.ctypes.data for m in masks])
+    rc = lib().orc_render_rank(
+        C.byref(p), br, len(bricks), row0, row1, C.c_void_p(out["rgba"].ctypes.data),
+        C.c_void_p(out["rgba8"].ctypes.data), C.c_void_p(out["depth"].ctypes.data),
+        C.c_void_p(out["accum"].ctypes.data), C.c_void_p(out["nsamples"].ctypes.data),
+        C.c_void_p(out["rgba64"].ctypes.data) if want64 else None, tptr, int(threads))
+    if rc != 0:
+        raise ValueError(f"orc_render_rank failed ({rc})")
+    return out
+
+
+def composite_over(imgs: np.ndarray, order):
+    """imgs: (P,H,W,4) float32 -> (rgba (H,W,4), rgba8 (H,W))."""
+    imgs = np.ascontiguousarray(imgs, dtype=np.float32)
+    P = imgs.shape[0]
+    npix = int(np.prod(imgs.shape[1:3]))
+    o = (C.c_int * P)(*order)
+    out = np.zeros(imgs.shape[1:], np.float32)
+    out8 = np.zeros(imgs.shape[1:3], np.uint32)
+    lib().orc_composite_over(imgs.ctypes.data, o, P, npix, out.ctypes.data, out8.ctypes.data)
+    return out, out8
+
+
+def composite_over_d(imgs: np.ndarray, order):
+    imgs = np.ascontiguousarray(imgs, dtype=np.float64)
+    P = imgs.shape[0]
+    npix = int(np.prod(imgs.shape[1:3]))
+    o = (C.c_int * P)(*order)
+    out = np.zeros(imgs.shape[1:], np.float64)
+    lib().orc_composite_over_d(imgs.ctypes.data, o, P, npix, out.ctypes.data)
+    return out
+
+
+def composite_over_u8(imgs8: np.ndarray, order):
+    imgs8 = np.ascontiguousarray(imgs8, dtype=np.uint32)
+    P = imgs8.shape[0]
+    npix = int(np.prod(imgs8.shape[1:]))
+    o = (C.c_int * P)(*order)
+    out8 = np.zeros(imgs8.shape[1:], np.uint32)
+    lib().orc_composite_over_u8(imgs8.ctypes.data, o, P, npix, out8.ctypes.data)
+    return out8
+
+
+def composite_zmin(imgs8: np.ndarray, depths: np.ndarray):
+    imgs8 = np.ascontiguousarray(imgs8, dtype=np.uint32)
+    depths = np.ascontiguousarray(depths, dtype=np.float32)
+    P = imgs8.shape[0]
+    npix = int(np.prod(imgs8.shape[1:]))
+    out8 = np.zeros(imgs8.shape[1:], np.uint32)
+    outz = np.zeros(imgs8.shape[1:], np.float32)
+    lib().orc_composite_zmin(imgs8.ctypes.data, depths.ctypes.data, P, npix, out8.ctypes.data,
+                             outz.ctypes.data)
+    return out8, outz
