@@ -96,6 +96,64 @@ int check_params(aloha_ctx *ctx, const aloha_render_params *p) {
   return ALOHA_OK;
 }
 
+// Conservative pixel rectangle of the bricks bricks[ids[0..n)] as seen from
+// the camera: a primary ray through pixel (x,y) has direction
+// dir_00 + (x+.5)*du + (y+.5)*dv, so solving P - org = c0*dir_00 + c1*du +
+// c2*dv for each box corner P gives its pixel coordinate (c1/c0 - .5,
+// c2/c0 - .5).  With every corner in front (c0 > 0) the box's image is the
+// convex hull of these points; a 2-pixel margin absorbs float rounding of the
+// rays (relative 1e-6) and the 1e-5 direction clamp.  Otherwise (a box
+// reaching the camera plane) the rectangle is the whole image.
+void rank_rect(const float cam[12], const std::vector<DevBrick> &bricks, const int32_t *ids,
+               int n, int W, int H, int32_t out[4]) {
+  out[0] = 0, out[1] = 0, out[2] = W, out[3] = H;
+  if (n == 0) {
+    out[2] = out[3] = 0;
+    return;
+  }
+  const double O[3] = {cam[0], cam[1], cam[2]};
+  const double D[3] = {cam[3], cam[4], cam[5]};
+  const double U[3] = {cam[6], cam[7], cam[8]};
+  const double V[3] = {cam[9], cam[10], cam[11]};
+  auto cross = [](const double a[3], const double b[3], double r[3]) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto dot = [](const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  double uv[3], vd[3], du[3];
+  cross(U, V, uv);
+  cross(V, D, vd);
+  cross(D, U, du);
+  const double det = dot(D, uv);
+  if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) return;
+  double smin = 1e300, smax = -1e300, tmin = 1e300, tmax = -1e300;
+  for (int k = 0; k < n; ++k) {
+    const DevBrick &b = bricks[ids[k]];
+    for (int c = 0; c < 8; ++c) {
+      double Q[3], q2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        Q[a] = ((c >> a) & 1 ? b.space_hi[a] : b.space_lo[a]) - O[a];
+        q2 += Q[a] * Q[a];
+      }
+      const double c0 = dot(uv, Q) / det, c1 = dot(vd, Q) / det, c2 = dot(du, Q) / det;
+      // corners on or behind the plane through org spanned by du, dv
+      if (!(c0 > 1e-6 * std::sqrt(q2) / std::sqrt(dot(D, D)))) return;
+      const double s = c1 / c0 - 0.5, t = c2 / c0 - 0.5;
+      if (!std::isfinite(s) || !std::isfinite(t)) return;
+      smin = std::min(smin, s), smax = std::max(smax, s);
+      tmin = std::min(tmin, t), tmax = std::max(tmax, t);
+    }
+  }
+  const double m = 2.0;
+  const double x0 = std::floor(smin - m), x1 = std::ceil(smax + m) + 1.0;
+  const double y0 = std::floor(tmin - m), y1 = std::ceil(tmax + m) + 1.0;
+  out[0] = (int32_t)std::min<double>(W, std::max(0.0, x0));
+  out[2] = (int32_t)std::min<double>(W, std::max(0.0, x1));
+  out[1] = (int32_t)std::min<double>(H, std::max(0.0, y0));
+  out[3] = (int32_t)std::min<double>(H, std::max(0.0, y1));
+}
+
 // Fill MarchArgs from the context and a validated rank table.
 int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t H, int32_t P,
                 const int32_t *rank_offsets, const int32_t *brick_ids, MarchArgs &a) {
@@ -143,6 +201,9 @@ int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t
   a.tf = ctx->d_tf;
   for (int s = 0; s <= P; ++s) a.rank_off[s] = rank_offsets[s];
   for (int k = 0; k < nb; ++k) a.rank_bricks[k] = brick_ids[k];
+  for (int s = 0; s < P; ++s)
+    rank_rect(ctx->cam, ctx->bricks, brick_ids + rank_offsets[s],
+              rank_offsets[s + 1] - rank_offsets[s], W, H, a.rect[s]);
   return ALOHA_OK;
 }
 
@@ -228,9 +289,11 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
   const int64_t py = (int64_t)n[1] + kPadLo + kPadHi;
   const int64_t pz = (int64_t)n[2] + kPadLo + kPadHi;
   const int64_t padded = px * py * pz;
-  if (padded >= ((int64_t)1 << 31))
-    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_brick_upload: brick of %lld padded voxels exceeds 2^31",
-                (long long)padded);
+  // offsets are 32-bit and the kernels form them with 24-bit multiplies
+  if (padded >= ((int64_t)1 << 31) || px * py >= ((int64_t)1 << 24))
+    return fail(ctx, ALOHA_ERR_LIMIT,
+                "aloha_brick_upload: brick of %lld padded voxels (slice %lld) exceeds 2^31 (2^24)",
+                (long long)padded, (long long)(px * py));
   const size_t src_bytes = (size_t)n[0] * n[1] * n[2] * (size_t)dtype;
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
   float *mem = nullptr;
@@ -264,7 +327,7 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
   b.px = (int32_t)px;
   b.pxy = (int32_t)(px * py);
   b.dtype = ALOHA_F32;
-  b.origin = mem + (kPadLo + kPadLo * px + kPadLo * px * py);
+  b.origin = mem;  // padded element (-1,-1,-1)
   ctx->bricks.push_back(b);
   ctx->brick_mem.push_back(mem);
   ctx->descs.push_back(*desc);

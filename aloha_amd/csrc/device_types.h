@@ -12,7 +12,8 @@ namespace aloha {
 // Brick storage in HBM: f32 voxels with a replicated border of 1 voxel below
 // and 3 above on every axis, so that every index a trilinear tap or the
 // 3-wide gradient footprint can form (-1 .. n+2) reads the clamp-to-edge
-// value without a per-load clamp.  `origin` points at voxel (0,0,0).
+// value without a per-load clamp.  `origin` points at padded voxel (-1,-1,-1)
+// (the allocation start), so tap offsets are non-negative 24-bit products.
 constexpr int kPadLo = 1;
 constexpr int kPadHi = 3;
 
@@ -44,6 +45,10 @@ struct MarchArgs {
   uint32_t *touched;  // footprint pass only
   int32_t rank_off[ALOHA_MAX_RANKS + 1];
   int32_t rank_bricks[ALOHA_MAX_RANK_BRICKS];
+  // per rank slot: pixel rectangle [x0,x1) x [y0,y1) outside of which no
+  // primary ray can reach one of the slot's bricks (host-projected, with a
+  // margin); pixels outside skip the march and keep the miss result
+  int32_t rect[ALOHA_MAX_RANKS][4];
 };
 
 struct CompositeArgs {

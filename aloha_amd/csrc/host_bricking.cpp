@@ -114,7 +114,7 @@ extern "C" int aloha_split_volume(const int32_t dims[3], int32_t num_bricks,
     if (cell_volume(R.cell) > 0) cl.push_back(R);
     // The reference recurses forever when a split yields a single non-empty
     // half (a one-cell brick); report it instead.
-    if (cl.size() == before)
+    if (cl.size() < before + 2)
       return ALOHA_FAIL_GLOBAL(ALOHA_ERR_ARG, "aloha_split_volume: cannot split a one-cell brick");
   } while ((int)cl.size() < num_bricks);
   if ((int)cl.size() > cap)
@@ -253,6 +253,15 @@ extern "C" int aloha_partitioner_create_kd(const aloha_box3f *domains, int32_t n
         extend(R.bounds, d);
         R.ids.push_back(id);
       }
+    }
+    // The reference picks the same set again forever when the middle plane
+    // separates nothing (e.g. fewer bricks than ranks); report it instead.
+    if (L.ids.empty() || R.ids.empty()) {
+      delete p;
+      return ALOHA_FAIL_GLOBAL(ALOHA_ERR_ARG,
+                               "aloha_partitioner_create_kd: %zu bricks cannot be separated "
+                               "for %d ranks",
+                               cs.ids.size(), num_ranks);
     }
     p->kd[cs.kd].axis = axis;
     p->kd[cs.kd].plane = plane;

@@ -49,11 +49,17 @@ __device__ __forceinline__ float rng_next(uint32_t &s) {
   return (float)(s >> 8) * (1.0f / 16777216.0f);
 }
 
+// o points at padded voxel (-1,-1,-1), so every index the taps form is
+// non-negative after +1 and all offsets are 24-bit products.
 struct BrickView {
   const float *o;
-  int px, pxy;
+  uint32_t px, pxy;
   int n0, n1, n2;
 };
+
+__device__ __forceinline__ uint32_t voff(const BrickView &b, int i, int j, int k) {
+  return (uint32_t)(i + 1) + __umul24((uint32_t)(j + 1), b.px) + __umul24((uint32_t)(k + 1), b.pxy);
+}
 
 // Footprint pass: mark the clamp-to-edge voxel that a corner read hits.
 __device__ __forceinline__ void mark(const MarchArgs &a, const DevBrick &db, int x, int y,
@@ -81,7 +87,7 @@ __device__ __forceinline__ float tap_exact(const BrickView &b, const MarchArgs &
       for (int j = 0; j < 2; ++j)
         for (int i = 0; i < 2; ++i) mark(a, db, ix + i, iy + j, iz + k);
   }
-  const float *p = b.o + (ix + iy * b.px + iz * b.pxy);
+  const float *p = b.o + voff(b, ix, iy, iz);
   const float c00 = lerp_(p[0], p[1], wx);
   const float c10 = lerp_(p[b.px], p[b.px + 1], wx);
   const float c01 = lerp_(p[b.pxy], p[b.pxy + 1], wx);
@@ -127,10 +133,10 @@ __device__ __forceinline__ bool sample_fast(const BrickView &b, float x, float y
   const float wxm = xm - fxm, wxc = x - fxc, wxp = xp - fxp;
   const float wym = ym - fym, wyc = y - fyc, wyp = yp - fyp;
   const float wzm = zm - fzm, wzc = z - fzc, wzp = zp - fzp;
-  const int px = b.px, pxy = b.pxy;
+  const uint32_t px = b.px, pxy = b.pxy;
 
   // R[jj][kk][i] = V(ax+i, cy+jj, cz+kk)
-  const float *r = b.o + (ax + cy * px + cz * pxy);
+  const float *r = b.o + voff(b, ax, cy, cz);
   float R[2][2][3];
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
@@ -141,8 +147,8 @@ __device__ __forceinline__ bool sample_fast(const BrickView &b, float x, float y
   // the y row outside the value tap's pair, and the z row likewise
   const int yn = scy ? ay : ay + 2;
   const int zn = scz ? az : az + 2;
-  const float *ry = b.o + (cx + yn * px + cz * pxy);
-  const float *rz = b.o + (cx + cy * px + zn * pxy);
+  const float *ry = b.o + voff(b, cx, yn, cz);
+  const float *rz = b.o + voff(b, cx, cy, zn);
   float YN[2][2], ZN[2][2];
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk) {
@@ -234,19 +240,19 @@ __device__ __forceinline__ bool slab(const float o[3], const float d[3], float t
   return t0 < t1;
 }
 
-// integrateBrick: march one brick from the ray's current tmin.
+// integrateBrick: march one brick over [t0, t1), the interval its own
+// intersect() computes from the ray's current tmin -- the same operands as
+// the brick selection's slab test, so the caller passes that result.
 template <bool kFoot>
 __device__ __forceinline__ void integrate(const MarchArgs &a, const float4 *s_tf,
                                           const DevBrick &db, const float o[3],
-                                          const float d[3], float tmin, float jitter,
+                                          const float d[3], float t0, float t1, float jitter,
                                           float4 &c, uint32_t &ns) {
-  float t0, t1;
-  if (!slab(o, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) return;
   const float dt = a.dt;
   const float segs = floorf((t0 - dt * jitter) / dt);
   float t = dt * (jitter + segs);
   if (t < t0) t += dt;
-  const BrickView b{db.origin, db.px, db.pxy, db.n[0], db.n[1], db.n[2]};
+  const BrickView b{db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1], db.n[2]};
   for (; t < t1 && c.w < .99f; t += dt) {
     ++ns;
     const float x = (o[0] + t * d[0]) - db.vox_lo[0];
@@ -312,7 +318,9 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   float z = 1.f;
   uint32_t ns = 0;
-  if (block_id % a.num_islands == a.island_id) {
+  const int32_t *rc = a.rect[slot];
+  const bool may_hit = px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
+  if (may_hit && block_id % a.num_islands == a.island_id) {
     for (int sid = 0; sid < a.spp; ++sid) {
       const float su = (float)px + .5f, sv = (float)py + .5f;
       float d[3];
@@ -330,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
       do {
         // closest reported brick hit (VolBrickIsect), ties to the lower index
         hit = -1;
-        float best = 0.f, bnear = 0.f, bfar = 0.f;
+        float best = 0.f, bnear = 0.f, bt0 = 0.f, bfar = 0.f;
         for (int k = b0; k < b1; ++k) {
           const int bid = a.rank_bricks[k];  // k is wave-uniform: scalar loads
           const DevBrick &db = a.bricks[bid];
@@ -349,6 +357,7 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
             hit = bid;
             best = rep;
             bnear = tn;
+            bt0 = t0;
             bfar = t1;
           }
         }
@@ -356,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
           bnear = bnear + tmin;
           const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
           const DevBrick db = a.bricks[hit];
-          integrate<kFoot>(a, s_tf, db, a.org, d, tmin, j, c, ns);
+          integrate<kFoot>(a, s_tf, db, a.org, d, bt0, bfar, j, c, ns);
           z = fminf(z, bnear / a.max_depth);
           tmin = bfar + 1e-20f;
         }

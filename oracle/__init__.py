@@ -105,7 +105,7 @@ def partition_kd(domains: np.ndarray, P: int):
     kd = (OrcKD * cap)()
     nk = lib().orc_partition_kd(C.c_void_p(d.ctypes.data), n, P, counts, ids, kd, cap)
     if nk < 0:
-        raise ValueError("kd cap too small")
+        raise ValueError("kd cap too small" if nk == -1 else "bricks cannot be separated")
     per, m = [], 0
     for r in range(P):
         per.append(list(ids[m:m + counts[r]]))

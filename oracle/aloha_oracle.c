@@ -198,7 +198,7 @@ int orc_split_volume(const int32_t dims[3], int n, orc_domain *out, int cap) {
     int before = count;
     if (ivolume(L.cell_lo, L.cell_hi) > 0) cl[count++] = L;
     if (ivolume(R.cell_lo, R.cell_hi) > 0) cl[count++] = R;
-    if (count > cap || count == before) { free(cl); return -1; }
+    if (count > cap || count < before + 2) { free(cl); return -1; }
   } while (count < n);
   memcpy(out, cl, sizeof(orc_domain) * (size_t)count);
   free(cl);
@@ -294,6 +294,8 @@ int orc_partition_kd(const float *domains, int n, int P, int *rank_count, int *r
       else { box_extend(R.lo, R.hi, d, d + 3); R.ids[R.n++] = id; }
     }
     free(cs.ids);
+    /* a split that separates nothing repeats forever in the reference */
+    if (L.n == 0 || R.n == 0) { free(L.ids); free(R.ids); return -2; }
     kd[cs.kd].axis = axis;
     kd[cs.kd].plane = plane;
     if (nkd + 2 > kd_cap) return -1;
