@@ -18,6 +18,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libaloha_amd.so"
+TOOLS = PKG / "tools"
+BIN = PKG / "bin"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ALOHA_OFFLOAD_ARCH", "gfx950")
 
@@ -63,7 +65,29 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB.relative_to(ROOT)} from {len(srcs)} sources")
+    build_tools(verbose)
     return LIB
+
+
+def build_tools(verbose: bool = True) -> None:
+    """Command-line tools over the C ABI (aloha_amd/bin/), linked to the
+    in-tree library with an $ORIGIN rpath."""
+    BIN.mkdir(exist_ok=True)
+    srcs = sorted(TOOLS.glob("*.cpp"))
+
+    def one(src: Path):
+        exe = BIN / src.stem
+        cmd = [HIPCC, "-x", "hip", *COMMON, f"--offload-arch={ARCH}", str(src), "-o", str(exe),
+               f"-L{PKG}", "-laloha_amd", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"tool build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return exe
+
+    with ThreadPoolExecutor(max_workers=max(1, min(8, len(srcs)))) as ex:
+        exes = list(ex.map(one, srcs))
+    if verbose:
+        print(f"built {len(exes)} tools in {BIN.relative_to(ROOT)}")
 
 
 def build_oracle(verbose: bool = True) -> Path:
