@@ -310,6 +310,16 @@ class Context:
         self._check(lib().aloha_render_ranks(self._h, C.byref(params), W, H, len(ranks), offs, ids,
                                              C.byref(imgs)))
 
+    def render_composite(self, params: RenderParams, W: int, H: int,
+                         ranks: Sequence[Sequence[int]], order: Sequence[int], out_rgba=None,
+                         out_rgba8=None, nsamples=None) -> None:
+        """Fused raygen + ordered 'over' composite (one launch)."""
+        offs, ids = _rank_table(ranks)
+        o = (C.c_int32 * len(order))(*order)
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        self._check(lib().aloha_render_composite(self._h, C.byref(params), W, H, len(ranks), offs,
+                                                 ids, o, p(out_rgba), p(out_rgba8), p(nsamples)))
+
     def footprint(self, params: RenderParams, W: int, H: int,
                   ranks: Sequence[Sequence[int]]) -> int:
         offs, ids = _rank_table(ranks)

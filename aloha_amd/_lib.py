@@ -102,6 +102,13 @@ _SIGS = [
     ("aloha_tf_set", C.c_int, [_P, _P, _I]),
     ("aloha_camera_set", C.c_int, [_P, _FP, _FP, _FP, _FP]),
     ("aloha_render_ranks", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, C.POINTER(Images)]),
+    ("aloha_render_composite", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP,
+                                         _P, _P, _P]),
+    ("aloha_tf_set_device", C.c_int, [_P, _P, _I]),
+    ("aloha_brick_update", C.c_int, [_P, _I, _P, _I]),
+    ("aloha_diff_forward", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _FP, _I, _IP, _P, _P]),
+    ("aloha_diff_backward", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _FP, _I, _IP, _P, _P,
+                                      _P, _P, C.POINTER(C.c_void_p)]),
     ("aloha_render_footprint", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP,
                                          C.POINTER(C.c_uint64)]),
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),

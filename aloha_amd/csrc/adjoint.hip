@@ -1,0 +1,361 @@
+// adjoint.hip -- differentiable form of the per-brick ray march: a forward
+// kernel and its hand-written adjoint, giving dLoss/dTF and dLoss/dDensity
+// for an image loss (no reference counterpart; the forward restates
+// integrateBrick, deviceCode.cpp:256-298, for one rank marching its bricks).
+//
+// Two deliberate differences from the plain raygen kernel make the march
+// reversible:
+//   * samples sit on the closed-form lattice t_k = dt*(j + (k0+k)) (the
+//     lattice firstSampleT starts, evaluated per index) instead of repeated
+//     t += dt, so the adjoint can visit them in reverse order exactly;
+//   * the per-sample opacity is clamped to 0.9999 so transmittance can be
+//     recovered backward by division.
+// The adjoint stores nothing per sample: from the forward's final colour
+// and alpha (and per-pixel sample count) it re-derives each brick visit,
+// walks its samples from last to first, recovers T_k = T_{k+1}/(1-a_k)
+// (Weiss & Westermann, "Differentiable Direct Volume Rendering", IEEE VIS
+// 2021, the inversion method) and back-propagates through compositing,
+// shading (N = g/|g+1e-4|), the TF lookup and all seven trilinear taps.
+// dTF is accumulated in LDS per workgroup and flushed with one global atomic
+// per touched entry; dDensity with global float atomics on the brick's
+// unpadded voxel grid (clamped corners receive the weight of the clamped-away
+// neighbour, exactly as the forward reads them).
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+#include "march_device.h"
+
+namespace aloha {
+namespace {
+
+using namespace dev;
+
+constexpr int kBlock = 256;
+constexpr int kTile = 16;
+
+struct Visit {
+  int bid;        // brick id
+  float *grad;    // its density-gradient grid (may be null)
+  float t0, t1;   // clipped interval
+  float j;        // lattice jitter
+  float k0;       // first lattice index
+};
+
+// The brick visit sequence of renderFrame's do/while for one ray: closest
+// reported hit among the rank's bricks, ties to the lower slot.  Returns the
+// number of visits (<= max_visits) and fills vis[] if non-null.
+__device__ int visits(const DiffArgs &a, const float org[3], const float d[3], uint32_t pixel,
+                      int max_visits, Visit *vis, int want) {
+  uint32_t rng = rng_seed(pixel, 0u);
+  float tmin = 0.f;
+  int nv = 0;
+  while (nv < max_visits) {
+    int hit = -1;
+    float *hgrad = nullptr;
+    float best = 0.f, bt0 = 0.f, bt1 = 0.f;
+    for (int k = 0; k < a.nb; ++k) {
+      const int bid = a.brick_ids[k];  // k is wave-uniform: scalar loads
+      const DevBrick &db = a.bricks[bid];
+      float t0, t1, rep;
+      if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
+      if (t0 > tmin) rep = t0;
+      else if (t1 > tmin) rep = t1;
+      else continue;
+      if (hit < 0 || rep < best) {
+        hit = bid;
+        hgrad = a.dvox[k];
+        best = rep;
+        bt0 = t0;
+        bt1 = t1;
+      }
+    }
+    if (hit < 0) break;
+    const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
+    if (nv == want || want < 0) {
+      const float dt = a.dt;
+      const float segs = floorf((bt0 - dt * j) / dt);
+      const float tf = dt * (j + segs);
+      Visit v{hit, hgrad, bt0, bt1, j, tf < bt0 ? segs + 1.f : segs};
+      if (vis) vis[want < 0 ? nv : 0] = v;
+    }
+    ++nv;
+    tmin = bt1 + 1e-20f;
+  }
+  return nv;
+}
+
+__device__ __forceinline__ float lattice_t(const DiffArgs &a, const Visit &v, int k) {
+  return a.dt * (v.j + (v.k0 + (float)k));
+}
+
+__device__ __forceinline__ void ray_dir(const DiffArgs &a, int view, int px, int py, float d[3]) {
+  const float *c = a.cam[view];
+  const float su = (float)px + .5f, sv = (float)py + .5f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = (c[3 + k] + su * c[6 + k]) + sv * c[9 + k];
+  const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    d[k] = d[k] / len;
+    if (fabsf(d[k]) < 1e-5f) d[k] = 1e-5f;
+  }
+}
+
+__device__ __forceinline__ BrickView view_of(const DevBrick &db) {
+  return BrickView{db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1], db.n[2]};
+}
+
+struct Shaded {
+  Sample s;
+  float4 tf;  // TF colour, opacity clamped
+  float shade;
+  float len;
+};
+
+__device__ __forceinline__ Shaded shade_sample(const DiffArgs &a, const float4 *s_tf,
+                                               const DevBrick &db, const float org[3],
+                                               const float d[3], float t) {
+  const BrickView b = view_of(db);
+  const float x = (org[0] + t * d[0]) - db.vox_lo[0];
+  const float y = (org[1] + t * d[1]) - db.vox_lo[1];
+  const float z = (org[2] + t * d[2]) - db.vox_lo[2];
+  Shaded r;
+  if (!sample_fast(b, x, y, z, r.s)) r.s = sample_exact<false>(b, nullptr, db, x, y, z);
+  const float ex = r.s.gx + 1e-4f, ey = r.s.gy + 1e-4f, ez = r.s.gz + 1e-4f;
+  r.len = sqrtf(ex * ex + ey * ey + ez * ez);
+  const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
+  r.tf = tf_lookup(s_tf, a.tf_n, r.s.val);
+  r.tf.w = fminf(r.tf.w, a.alpha_max);
+  r.shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
+  return r;
+}
+
+__device__ void load_tf(const DiffArgs &a, float4 *s_tf) {
+  for (int i = threadIdx.x; i < a.tf_n + 4; i += kBlock)
+    s_tf[i] = (i >= 2 && i < a.tf_n + 2) ? a.tf[i - 2] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void diff_forward_kernel(const DiffArgs a) {
+  extern __shared__ float4 s_tf[];
+  load_tf(a, s_tf);
+  __syncthreads();
+  const int view = blockIdx.z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+  const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+  if (px >= a.W || py >= a.H) return;
+  const int pixel = px + a.W * py;
+  const size_t out = (size_t)view * a.W * a.H + pixel;
+  const float *org = a.cam[view];
+  float d[3];
+  ray_dir(a, view, px, py, d);
+  float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t ns = 0;
+  uint32_t rng = rng_seed((uint32_t)pixel, 0u);
+  float tmin = 0.f;
+  for (;;) {
+    int hit = -1;
+    float best = 0.f, bt0 = 0.f, bt1 = 0.f;
+    for (int k = 0; k < a.nb; ++k) {
+      const int bid = a.brick_ids[k];
+      const DevBrick &db = a.bricks[bid];
+      float t0, t1, rep;
+      if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
+      if (t0 > tmin) rep = t0;
+      else if (t1 > tmin) rep = t1;
+      else continue;
+      if (hit < 0 || rep < best) {
+        hit = bid;
+        best = rep;
+        bt0 = t0;
+        bt1 = t1;
+      }
+    }
+    if (hit < 0) break;
+    const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
+    const float dt = a.dt;
+    const float segs = floorf((bt0 - dt * j) / dt);
+    const float tfirst = dt * (j + segs);
+    const Visit v{hit, nullptr, bt0, bt1, j, tfirst < bt0 ? segs + 1.f : segs};
+    const DevBrick db = a.bricks[hit];
+    for (int k = 0;; ++k) {
+      const float t = lattice_t(a, v, k);
+      if (!(t < bt1 && c.w < .99f)) break;
+      ++ns;
+      const Shaded r = shade_sample(a, s_tf, db, org, d, t);
+      const float w = (1.f - c.w) * r.tf.w;
+      c.x = c.x + (w * r.tf.x) * r.shade;
+      c.y = c.y + (w * r.tf.y) * r.shade;
+      c.z = c.z + (w * r.tf.z) * r.shade;
+      c.w = c.w + w;
+    }
+    tmin = bt1 + 1e-20f;
+    if (!(c.w < .99f)) break;
+  }
+  a.rgba[out] = c;
+  a.nsamples[out] = ns;
+}
+
+// Scatter d(loss)/d(tap value) onto the 8 corners of one trilinear tap.
+__device__ __forceinline__ void scatter_tap(float *grad, const DevBrick &db, float x, float y,
+                                            float z, float g) {
+  if (g == 0.f || grad == nullptr) return;
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const float wx = x - fx, wy = y - fy, wz = z - fz;
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const int nx = db.n[0], ny = db.n[1], nz = db.n[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int zz = clampi(iz + k, 0, nz - 1);
+    const float w_z = k ? wz : 1.f - wz;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int yy = clampi(iy + j, 0, ny - 1);
+      const float w_yz = (j ? wy : 1.f - wy) * w_z;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int xx = clampi(ix + i, 0, nx - 1);
+        const float w = (i ? wx : 1.f - wx) * w_yz;
+        atomicAdd(&grad[((size_t)zz * ny + yy) * nx + xx], g * w);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a) {
+  extern __shared__ float4 s_mem[];
+  float4 *s_tf = s_mem;
+  float *s_dtf = (float *)(s_mem + a.tf_n + 4);  // tf_n x 4
+  load_tf(a, s_tf);
+  for (int i = threadIdx.x; i < 4 * a.tf_n; i += kBlock) s_dtf[i] = 0.f;
+  __syncthreads();
+
+  const int view = blockIdx.z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+  const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+  const bool live = px < a.W && py < a.H;
+  uint32_t N = 0;
+  size_t out = 0;
+  if (live) {
+    out = (size_t)view * a.W * a.H + (px + a.W * py);
+    N = a.nsamples[out];
+  }
+  if (N > 0) {
+    const int pixel = px + a.W * py;
+    const float *org = a.cam[view];
+    float d[3];
+    ray_dir(a, view, px, py, d);
+    const float4 C = a.rgba[out];
+    const float4 G = a.dldc[out];
+    // visits and their sample counts: all but the last run to t1
+    const int nv = visits(a, org, d, (uint32_t)pixel, kMaxVisits, nullptr, -1);
+    int last = -1;
+    uint32_t before_last = 0, acc = 0;
+    for (int vi = 0; vi < nv && last < 0; ++vi) {
+      Visit v;
+      visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
+      uint32_t cnt = 0;
+      while (lattice_t(a, v, (int)cnt) < v.t1) ++cnt;
+      if (acc + cnt >= N) {
+        last = vi;
+        before_last = acc;
+      }
+      acc += cnt;
+    }
+    float T = 1.f - C.w;  // transmittance after the last sample
+    const float TN = T;
+    float Sr = 0.f, Sg = 0.f, Sb = 0.f;  // suffix sums of later contributions
+    for (int vi = last; vi >= 0; --vi) {
+      Visit v;
+      visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
+      uint32_t cnt;
+      if (vi == last) {
+        cnt = N - before_last;
+      } else {
+        cnt = 0;
+        while (lattice_t(a, v, (int)cnt) < v.t1) ++cnt;
+      }
+      const DevBrick db = a.bricks[v.bid];
+      float *grad = v.grad;
+      for (int k = (int)cnt - 1; k >= 0; --k) {
+        const float t = lattice_t(a, v, k);
+        const Shaded r = shade_sample(a, s_tf, db, org, d, t);
+        const float am = r.tf.w;
+        const float one_m = 1.f - am;
+        const float Tk = T / one_m;
+        const float cs_r = r.tf.x * r.shade, cs_g = r.tf.y * r.shade, cs_b = r.tf.z * r.shade;
+        const float Gc = G.x * r.tf.x + G.y * r.tf.y + G.z * r.tf.z;
+        // d/d(colour), d/d(shade), d/d(opacity)
+        const float Ta = Tk * am;
+        const float dcr = G.x * Ta * r.shade, dcg = G.y * Ta * r.shade, dcb = G.z * Ta * r.shade;
+        const float dshade = Ta * Gc;
+        const float dalpha = Tk * (G.x * cs_r + G.y * cs_g + G.z * cs_b) -
+                             ((G.x * Sr + G.y * Sg + G.z * Sb) - G.w * TN) / one_m;
+        Sr += Ta * cs_r;
+        Sg += Ta * cs_g;
+        Sb += Ta * cs_b;
+        T = Tk;
+        // TF entries (border entries are constants)
+        const int n = a.tf_n;
+        const float xt = r.s.val * (float)n - 0.5f;
+        const float fi = floorf(xt);
+        const float w = xt - fi;
+        const int i0 = (int)fminf(fmaxf(fi, -2.f), (float)n);
+        const float4 raw = tf_lookup(s_tf, n, r.s.val);
+        const float da = raw.w < a.alpha_max ? dalpha : 0.f;
+        const float g4[4] = {dcr, dcg, dcb, da};
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+          if (i0 >= 0 && i0 < n) atomicAdd(&s_dtf[4 * i0 + ch], (1.f - w) * g4[ch]);
+          if (i0 + 1 >= 0 && i0 + 1 < n) atomicAdd(&s_dtf[4 * (i0 + 1) + ch], w * g4[ch]);
+        }
+        if (grad) {
+          // value tap through the TF slope
+          const float4 e0 = s_tf[i0 + 2], e1 = s_tf[i0 + 3];
+          const float dv = (dcr * (e1.x - e0.x) + dcg * (e1.y - e0.y) + dcb * (e1.z - e0.z) +
+                            da * (e1.w - e0.w)) * (float)n;
+          // shading: s = .1 + .9|N.d|, N = g/L, L = |g + 1e-4|
+          const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
+          const float nd = nx * d[0] + ny * d[1] + nz * d[2];
+          const float sg = nd > 0.f ? .9f : (nd < 0.f ? -.9f : 0.f);
+          const float dN[3] = {dshade * sg * d[0], dshade * sg * d[1], dshade * sg * d[2]};
+          const float gv[3] = {r.s.gx, r.s.gy, r.s.gz};
+          const float dNg = dN[0] * gv[0] + dN[1] * gv[1] + dN[2] * gv[2];
+          const float L3 = r.len * r.len * r.len;
+          float dg[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) dg[q] = dN[q] / r.len - dNg * (gv[q] + 1e-4f) / L3;
+          const float x = (org[0] + t * d[0]) - db.vox_lo[0];
+          const float y = (org[1] + t * d[1]) - db.vox_lo[1];
+          const float z = (org[2] + t * d[2]) - db.vox_lo[2];
+          scatter_tap(grad, db, x, y, z, dv);
+          scatter_tap(grad, db, x + .5f, y, z, dg[0]);
+          scatter_tap(grad, db, x - .5f, y, z, -dg[0]);
+          scatter_tap(grad, db, x, y + .5f, z, dg[1]);
+          scatter_tap(grad, db, x, y - .5f, z, -dg[1]);
+          scatter_tap(grad, db, x, y, z + .5f, dg[2]);
+          scatter_tap(grad, db, x, y, z - .5f, -dg[2]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (a.dtf)
+    for (int i = threadIdx.x; i < 4 * a.tf_n; i += kBlock)
+      if (s_dtf[i] != 0.f) atomicAdd(&a.dtf[i], s_dtf[i]);
+}
+
+}  // namespace
+
+hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s) {
+  const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
+  const size_t tf_lds = sizeof(float4) * (size_t)(a.tf_n + 4);
+  if (backward)
+    diff_backward_kernel<<<grid, kBlock, tf_lds + sizeof(float) * 4 * (size_t)a.tf_n, s>>>(a);
+  else
+    diff_forward_kernel<<<grid, kBlock, tf_lds, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace aloha

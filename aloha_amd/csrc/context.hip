@@ -403,6 +403,33 @@ extern "C" int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *par
   return ALOHA_OK;
 }
 
+extern "C" int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params *params,
+                                      int32_t W, int32_t H, int32_t P,
+                                      const int32_t *rank_offsets, const int32_t *brick_ids,
+                                      const int32_t *order, float *d_out_rgba,
+                                      uint32_t *d_out_rgba8, uint32_t *d_nsamples) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_composite: ctx is NULL");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  if (params->accum_id != 0 || (params->flags & (ALOHA_FLAG_HEATMAP | ALOHA_FLAG_CROSSHAIR)))
+    return fail(ctx, ALOHA_ERR_ARG,
+                "aloha_render_composite: accum_id 0 and no heat map / crosshair only");
+  if (!order) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_composite: order is NULL");
+  std::vector<char> seen(P, 0);
+  for (int k = 0; k < P; ++k) {
+    if (order[k] < 0 || order[k] >= P || seen[order[k]])
+      return fail(ctx, ALOHA_ERR_ARG, "aloha_render_composite: order is not a permutation");
+    seen[order[k]] = 1;
+    a.order[k] = order[k];
+  }
+  a.rgba = (float4 *)d_out_rgba;
+  a.rgba8 = d_out_rgba8;
+  a.nsamples = d_nsamples;
+  HIP_CHECK(ctx, launch_march_composite(a, ctx->stream));
+  return ALOHA_OK;
+}
+
 extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params,
                                       int32_t W, int32_t H, int32_t P,
                                       const int32_t *rank_offsets, const int32_t *brick_ids,
@@ -494,5 +521,121 @@ extern "C" int aloha_composite_zmin(aloha_ctx *ctx, const uint32_t *d_rank_rgba8
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
   HIP_CHECK(ctx, launch_composite_zmin(d_rank_rgba8, d_rank_depth, P, (int64_t)W * H,
                                        d_out_rgba8, d_out_depth, ctx->stream));
+  return ALOHA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident inputs
+// ---------------------------------------------------------------------------
+extern "C" int aloha_tf_set_device(aloha_ctx *ctx, const float *d_rgba, int32_t n) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_tf_set_device: ctx is NULL");
+  if (!d_rgba || n < 1 || n > ALOHA_MAX_TF)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_tf_set_device: need 1..%d entries", ALOHA_MAX_TF);
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (n > ctx->tf_cap) {
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->d_tf) HIP_CHECK(ctx, hipFree(ctx->d_tf));
+    ctx->d_tf = nullptr;
+    HIP_CHECK(ctx, hipMalloc(&ctx->d_tf, sizeof(float4) * (size_t)n));
+    ctx->tf_cap = n;
+  }
+  HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_tf, d_rgba, sizeof(float4) * (size_t)n,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  ctx->tf_n = n;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_brick_update(aloha_ctx *ctx, int32_t brick_id, const void *d_voxels,
+                                  int32_t dtype) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_brick_update: ctx is NULL");
+  if (brick_id < 0 || brick_id >= (int)ctx->bricks.size() || !d_voxels)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_update: bad brick id %d or NULL voxels", brick_id);
+  if (dtype != ALOHA_U8 && dtype != ALOHA_U16 && dtype != ALOHA_F32)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_update: unknown dtype %d", dtype);
+  const DevBrick &b = ctx->bricks[brick_id];
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  HIP_CHECK(ctx, launch_pad_brick(d_voxels, dtype, b.n, ctx->brick_mem[brick_id], b.px,
+                                  b.pxy / b.px, b.n[2] + kPadLo + kPadHi, ctx->stream));
+  return ALOHA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// differentiable form
+// ---------------------------------------------------------------------------
+namespace {
+int build_diff(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t H, int32_t V,
+               const float *cams, int32_t nb, const int32_t *brick_ids, DiffArgs &a) {
+  if (!p) return fail(ctx, ALOHA_ERR_ARG, "diff: params is NULL");
+  if (!(p->dt > 0.f) || !std::isfinite(p->dt))
+    return fail(ctx, ALOHA_ERR_ARG, "diff: dt must be finite and > 0");
+  if (p->jitter_mode != ALOHA_JITTER_RANDOM && p->jitter_mode != ALOHA_JITTER_FIXED)
+    return fail(ctx, ALOHA_ERR_ARG, "diff: unknown jitter_mode %d", p->jitter_mode);
+  if (p->jitter_mode == ALOHA_JITTER_FIXED && !(p->jitter_value >= 0.f && p->jitter_value < 1.f))
+    return fail(ctx, ALOHA_ERR_ARG, "diff: jitter_value must be in [0,1)");
+  if (W < 1 || H < 1 || (int64_t)W * H * V > (int64_t)1 << 30)
+    return fail(ctx, ALOHA_ERR_ARG, "diff: image %dx%dx%d out of range", W, H, V);
+  if (V < 1 || V > kMaxViews || !cams)
+    return fail(ctx, ALOHA_ERR_LIMIT, "diff: V=%d outside [1,%d]", V, kMaxViews);
+  if (nb < 1 || nb > kMaxDiffBricks || !brick_ids)
+    return fail(ctx, ALOHA_ERR_LIMIT, "diff: nb=%d outside [1,%d]", nb, kMaxDiffBricks);
+  for (int k = 0; k < nb; ++k)
+    if (brick_ids[k] < 0 || brick_ids[k] >= (int)ctx->bricks.size())
+      return fail(ctx, ALOHA_ERR_ARG, "diff: brick id %d not uploaded", brick_ids[k]);
+  for (int i = 0; i < 12 * V; ++i)
+    if (!std::isfinite(cams[i])) return fail(ctx, ALOHA_ERR_ARG, "diff: non-finite camera");
+  if (!ctx->d_tf || ctx->tf_n < 1) return fail(ctx, ALOHA_ERR_STATE, "diff: no transfer function set");
+  if (int rc = sync_bricks(ctx)) return rc;
+  std::memset(&a, 0, sizeof a);
+  std::memcpy(a.cam, cams, sizeof(float) * 12 * (size_t)V);
+  a.dt = p->dt;
+  a.jitter_value = p->jitter_value;
+  a.jitter_mode = p->jitter_mode;
+  a.alpha_max = 0.9999f;
+  a.W = W;
+  a.H = H;
+  a.V = V;
+  a.tf_n = ctx->tf_n;
+  a.nb = nb;
+  for (int k = 0; k < nb; ++k) a.brick_ids[k] = brick_ids[k];
+  a.bricks = ctx->d_bricks;
+  a.tf = ctx->d_tf;
+  return ALOHA_OK;
+}
+}  // namespace
+
+extern "C" int aloha_diff_forward(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                  int32_t H, int32_t V, const float *cams, int32_t nb,
+                                  const int32_t *brick_ids, float *d_rgba,
+                                  uint32_t *d_nsamples) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_diff_forward: ctx is NULL");
+  if (!d_rgba || !d_nsamples) return fail(ctx, ALOHA_ERR_ARG, "aloha_diff_forward: NULL output");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  DiffArgs a;
+  if (int rc = build_diff(ctx, params, W, H, V, cams, nb, brick_ids, a)) return rc;
+  a.rgba = (float4 *)d_rgba;
+  a.nsamples = d_nsamples;
+  HIP_CHECK(ctx, launch_diff(a, false, ctx->stream));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                   int32_t H, int32_t V, const float *cams, int32_t nb,
+                                   const int32_t *brick_ids, const float *d_rgba,
+                                   const uint32_t *d_nsamples, const float *d_dl_drgba,
+                                   float *d_dtf, float *const *d_dvoxels) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_diff_backward: ctx is NULL");
+  if (!d_rgba || !d_nsamples || !d_dl_drgba)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_diff_backward: NULL input");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  DiffArgs a;
+  if (int rc = build_diff(ctx, params, W, H, V, cams, nb, brick_ids, a)) return rc;
+  if ((size_t)sizeof(float4) * (a.tf_n + 4) + sizeof(float) * 4 * (size_t)a.tf_n > 64 * 1024)
+    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: TF of %d entries exceeds LDS", a.tf_n);
+  a.rgba = (float4 *)d_rgba;
+  a.nsamples = (uint32_t *)d_nsamples;
+  a.dldc = (const float4 *)d_dl_drgba;
+  a.dtf = d_dtf;
+  for (int k = 0; k < nb; ++k) a.dvox[k] = d_dvoxels ? d_dvoxels[k] : nullptr;
+  HIP_CHECK(ctx, launch_diff(a, true, ctx->stream));
   return ALOHA_OK;
 }

@@ -49,6 +49,26 @@ struct MarchArgs {
   // primary ray can reach one of the slot's bricks (host-projected, with a
   // margin); pixels outside skip the march and keep the miss result
   int32_t rect[ALOHA_MAX_RANKS][4];
+  int32_t order[ALOHA_MAX_RANKS];  // fused composite: rank slots front to back
+};
+
+// Differentiable form: one rank marching its bricks, seen from V views.
+constexpr int kMaxViews = 16;
+constexpr int kMaxDiffBricks = 64;
+constexpr int kMaxVisits = 64;
+
+struct DiffArgs {
+  float cam[kMaxViews][12];  // org, dir_00, dir_du, dir_dv per view
+  float dt, jitter_value, alpha_max;
+  int32_t jitter_mode, W, H, V, tf_n, nb;
+  int32_t brick_ids[kMaxDiffBricks];
+  float *dvox[kMaxDiffBricks];  // per brick slot: dLoss/dvoxel grid (unpadded) or null
+  const DevBrick *bricks;
+  const float4 *tf;
+  float4 *rgba;                 // V x H x W
+  uint32_t *nsamples;           // V x H x W
+  const float4 *dldc;           // backward: dLoss/dRGBA, V x H x W
+  float *dtf;                   // backward: dLoss/dTF, tf_n x 4 (accumulated)
 };
 
 struct CompositeArgs {
@@ -59,8 +79,10 @@ struct CompositeArgs {
 
 // launchers (march.hip, composite.hip)
 hipError_t launch_march(const MarchArgs &a, bool footprint, hipStream_t s);
+hipError_t launch_march_composite(const MarchArgs &a, hipStream_t s);
 hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
                             int32_t px, int32_t py, int32_t pz, hipStream_t s);
+hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
 hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, float4 *out,
                                  uint32_t *out8, hipStream_t s);
 hipError_t launch_composite_over_rgba8(const uint32_t *imgs, const CompositeArgs &c,

@@ -1,0 +1,230 @@
+// march_device.h -- device building blocks of the ray march shared by the
+// plain raygen kernel (march.hip) and the differentiable forward/adjoint
+// kernels (adjoint.hip): RNG, brick addressing, the exact trilinear tap, the
+// shared 20-voxel gradient footprint, the LDS transfer-function lookup and
+// the slab test.  All arithmetic is written in the CPU restatement's order
+// and must be compiled without contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_types.h"
+
+namespace aloha {
+namespace dev {
+
+__device__ __forceinline__ float lerp_(float a, float b, float w) {
+  return __builtin_fmaf(w, b - a, a);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// --- RNG: murmur3 fmix32 seed, Numerical-Recipes LCG, 24-bit draws -------
+__device__ __forceinline__ uint32_t rng_seed(uint32_t pixel, uint32_t frame) {
+  uint32_t h = (pixel * 0x9E3779B1u) ^ (frame * 0x85EBCA77u + 0x165667B1u);
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ float rng_next(uint32_t &s) {
+  s = s * 1664525u + 1013904223u;
+  return (float)(s >> 8) * (1.0f / 16777216.0f);
+}
+
+// o points at padded voxel (-1,-1,-1), so every index the taps form is
+// non-negative after +1 and all offsets are 24-bit products.
+struct BrickView {
+  const float *o;
+  uint32_t px, pxy;
+  int n0, n1, n2;
+};
+
+__device__ __forceinline__ uint32_t voff(const BrickView &b, int i, int j, int k) {
+  return (uint32_t)(i + 1) + __umul24((uint32_t)(j + 1), b.px) + __umul24((uint32_t)(k + 1), b.pxy);
+}
+
+// Footprint pass: mark the clamp-to-edge voxel that a corner read hits.
+__device__ __forceinline__ void mark(uint32_t *touched, const DevBrick &db, int x, int y,
+                                     int z) {
+  x = clampi(x, 0, db.n[0] - 1);
+  y = clampi(y, 0, db.n[1] - 1);
+  z = clampi(z, 0, db.n[2] - 1);
+  const uint64_t idx = ((uint64_t)z * db.n[1] + y) * db.n[0] + x;
+  atomicOr(&touched[db.touched_word + (idx >> 5)], 1u << (idx & 31));
+}
+
+// One trilinear fetch, exactly as the restated texture unit does it.
+template <bool kFoot>
+__device__ __forceinline__ float tap_exact(const BrickView &b, uint32_t *touched,
+                                           const DevBrick &db, float x, float y, float z) {
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const float wx = x - fx, wy = y - fy, wz = z - fz;
+  // indices stay inside the padded border; the border replicates the edge so
+  // this equals per-axis clamping to [0, n-1]
+  const int ix = clampi((int)fx, -1, b.n0 + 1);
+  const int iy = clampi((int)fy, -1, b.n1 + 1);
+  const int iz = clampi((int)fz, -1, b.n2 + 1);
+  if (kFoot) {
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) mark(touched, db, ix + i, iy + j, iz + k);
+  }
+  const float *p = b.o + voff(b, ix, iy, iz);
+  const float c00 = lerp_(p[0], p[1], wx);
+  const float c10 = lerp_(p[b.px], p[b.px + 1], wx);
+  const float c01 = lerp_(p[b.pxy], p[b.pxy + 1], wx);
+  const float c11 = lerp_(p[b.px + b.pxy], p[b.px + b.pxy + 1], wx);
+  const float c0 = lerp_(c00, c10, wy);
+  const float c1 = lerp_(c01, c11, wy);
+  return lerp_(c0, c1, wz);
+}
+
+struct Sample {
+  float val, gx, gy, gz;
+};
+
+template <bool kFoot>
+__device__ __forceinline__ Sample sample_exact(const BrickView &b, uint32_t *touched,
+                                               const DevBrick &db, float x, float y, float z) {
+  Sample s;
+  s.gx = tap_exact<kFoot>(b, touched, db, x + .5f, y, z) - tap_exact<kFoot>(b, touched, db, x - .5f, y, z);
+  s.gy = tap_exact<kFoot>(b, touched, db, x, y + .5f, z) - tap_exact<kFoot>(b, touched, db, x, y - .5f, z);
+  s.gz = tap_exact<kFoot>(b, touched, db, x, y, z + .5f) - tap_exact<kFoot>(b, touched, db, x, y, z - .5f);
+  s.val = tap_exact<kFoot>(b, touched, db, x, y, z);
+  return s;
+}
+
+// Shared 20-voxel footprint.  Returns false (and computes nothing) when the
+// lane's taps do not fit the window; the caller then uses sample_exact.
+__device__ __forceinline__ bool sample_fast(const BrickView &b, float x, float y, float z,
+                                            Sample &s) {
+  const float xm = x - .5f, xp = x + .5f;
+  const float ym = y - .5f, yp = y + .5f;
+  const float zm = z - .5f, zp = z + .5f;
+  const float fxm = floorf(xm), fxc = floorf(x), fxp = floorf(xp);
+  const float fym = floorf(ym), fyc = floorf(y), fyp = floorf(yp);
+  const float fzm = floorf(zm), fzc = floorf(z), fzp = floorf(zp);
+  const int ax = (int)fxm, cx = (int)fxc, bx = (int)fxp;
+  const int ay = (int)fym, cy = (int)fyc, by = (int)fyp;
+  const int az = (int)fzm, cz = (int)fzc, bz = (int)fzp;
+  const int scx = cx - ax, scy = cy - ay, scz = cz - az;
+  const bool ok = ((unsigned)scx <= 1u) & ((unsigned)scy <= 1u) & ((unsigned)scz <= 1u) &
+                  (bx == ax + 1) & (by == ay + 1) & (bz == az + 1) & (ax >= -1) & (ax <= b.n0) &
+                  (ay >= -1) & (ay <= b.n1) & (az >= -1) & (az <= b.n2);
+  if (!ok) return false;
+  const float wxm = xm - fxm, wxc = x - fxc, wxp = xp - fxp;
+  const float wym = ym - fym, wyc = y - fyc, wyp = yp - fyp;
+  const float wzm = zm - fzm, wzc = z - fzc, wzp = zp - fzp;
+  const uint32_t px = b.px, pxy = b.pxy;
+
+  // R[jj][kk][i] = V(ax+i, cy+jj, cz+kk)
+  const float *r = b.o + voff(b, ax, cy, cz);
+  float R[2][2][3];
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) R[jj][kk][i] = r[i + jj * px + kk * pxy];
+  // the y row outside the value tap's pair, and the z row likewise
+  const int yn = scy ? ay : ay + 2;
+  const int zn = scz ? az : az + 2;
+  const float *ry = b.o + voff(b, cx, yn, cz);
+  const float *rz = b.o + voff(b, cx, cy, zn);
+  float YN[2][2], ZN[2][2];
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    YN[kk][0] = ry[kk * pxy];
+    YN[kk][1] = ry[kk * pxy + 1];
+  }
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj) {
+    ZN[jj][0] = rz[jj * px];
+    ZN[jj][1] = rz[jj * px + 1];
+  }
+
+  // value tap: x-lerps on the (cx, cx+1) pair
+  float cX[2][2];
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const float x0 = scx ? R[jj][kk][1] : R[jj][kk][0];
+      const float x1 = scx ? R[jj][kk][2] : R[jj][kk][1];
+      cX[jj][kk] = lerp_(x0, x1, wxc);
+    }
+  const float cY0 = lerp_(cX[0][0], cX[1][0], wyc);
+  const float cY1 = lerp_(cX[0][1], cX[1][1], wyc);
+  s.val = lerp_(cY0, cY1, wzc);
+
+  // x taps: pairs (ax, ax+1) and (ax+1, ax+2)
+  float mX[2][2], pX[2][2];
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      mX[jj][kk] = lerp_(R[jj][kk][0], R[jj][kk][1], wxm);
+      pX[jj][kk] = lerp_(R[jj][kk][1], R[jj][kk][2], wxp);
+    }
+  const float xm_v =
+      lerp_(lerp_(mX[0][0], mX[1][0], wyc), lerp_(mX[0][1], mX[1][1], wyc), wzc);
+  const float xp_v =
+      lerp_(lerp_(pX[0][0], pX[1][0], wyc), lerp_(pX[0][1], pX[1][1], wyc), wzc);
+  s.gx = xp_v - xm_v;
+
+  // y taps: rows ay, ay+1, ay+2 after x-lerp with the value tap's weight
+  const float yX0 = lerp_(YN[0][0], YN[0][1], wxc);  // row yn, z = cz
+  const float yX1 = lerp_(YN[1][0], YN[1][1], wxc);  // row yn, z = cz+1
+  // ym pair (ay, ay+1); yp pair (ay+1, ay+2)
+  const float ymLo0 = scy ? yX0 : cX[0][0], ymHi0 = scy ? cX[0][0] : cX[1][0];
+  const float ymLo1 = scy ? yX1 : cX[0][1], ymHi1 = scy ? cX[0][1] : cX[1][1];
+  const float ypLo0 = scy ? cX[0][0] : cX[1][0], ypHi0 = scy ? cX[1][0] : yX0;
+  const float ypLo1 = scy ? cX[0][1] : cX[1][1], ypHi1 = scy ? cX[1][1] : yX1;
+  const float ym_v = lerp_(lerp_(ymLo0, ymHi0, wym), lerp_(ymLo1, ymHi1, wym), wzc);
+  const float yp_v = lerp_(lerp_(ypLo0, ypHi0, wyp), lerp_(ypLo1, ypHi1, wyp), wzc);
+  s.gy = yp_v - ym_v;
+
+  // z taps: planes az, az+1, az+2 after x- and y-lerps with the value weights
+  const float zX0 = lerp_(ZN[0][0], ZN[0][1], wxc);  // plane zn, y = cy
+  const float zX1 = lerp_(ZN[1][0], ZN[1][1], wxc);  // plane zn, y = cy+1
+  const float zY = lerp_(zX0, zX1, wyc);
+  const float zmLo = scz ? zY : cY0, zmHi = scz ? cY0 : cY1;
+  const float zpLo = scz ? cY0 : cY1, zpHi = scz ? cY1 : zY;
+  s.gz = lerp_(zpLo, zpHi, wzp) - lerp_(zmLo, zmHi, wzm);
+  return true;
+}
+
+// TF lookup from LDS: s_tf[k+2] = T[k], two zero entries on each side.
+__device__ __forceinline__ float4 tf_lookup(const float4 *s_tf, int n, float v) {
+  const float xt = v * (float)n - 0.5f;
+  const float fi = floorf(xt);
+  const float w = xt - fi;
+  const int i = (int)fminf(fmaxf(fi, -2.f), (float)n);  // NaN -> -2 (zeros)
+  const float4 e0 = s_tf[i + 2];
+  const float4 e1 = s_tf[i + 3];
+  return make_float4(lerp_(e0.x, e1.x, w), lerp_(e0.y, e1.y, w), lerp_(e0.z, e1.z, w),
+                     lerp_(e0.w, e1.w, w));
+}
+
+__device__ __forceinline__ bool slab(const float o[3], const float d[3], float tmin, float tmax,
+                                     const float lo[3], const float hi[3], float &t0,
+                                     float &t1) {
+  float nr[3], fr[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float l = (lo[k] - o[k]) / d[k];
+    const float h = (hi[k] - o[k]) / d[k];
+    nr[k] = fminf(l, h);
+    fr[k] = fmaxf(l, h);
+  }
+  t0 = fmaxf(tmin, fmaxf(fmaxf(nr[0], nr[1]), nr[2]));
+  t1 = fminf(tmax, fminf(fminf(fr[0], fr[1]), fr[2]));
+  return t0 < t1;
+}
+
+}  // namespace dev
+}  // namespace aloha

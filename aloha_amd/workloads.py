@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import (Box3f, BrickDesc, Partitioner, max_depth, render_params, split_volume,
-               synth_blobs_u8, synth_camera, synth_fbm_f32, synth_tf)
+               synth_blobs_f32, synth_blobs_u8, synth_camera, synth_fbm_f32, synth_tf)
 
 
 @dataclass
@@ -88,3 +88,35 @@ def fbm512(num_ranks: int = 8, W: int = 1024, H: int = 1024, seed: int = 0x5EED1
            size: int = 512) -> VolumeWorkload:
     raw = synth_fbm_f32((size, size, size), 5, size / 8.0, seed)
     return make_volume_workload("fbm512", raw, 8, num_ranks, W, H)
+
+
+@dataclass
+class DiffWorkload:
+    """Differentiable-form workload: a density grid (blob field), a TF,
+    V orbit views, and target images made from a perturbed TF (+-20% per
+    entry) and perturbed density (+5% seeded noise)."""
+    density: np.ndarray   # (z, y, x) f32
+    tf: np.ndarray        # (n, 4) f32
+    cams: list
+    W: int
+    H: int
+    target_tf: np.ndarray
+    target_density: np.ndarray
+    dt: float = 0.5
+
+
+def diff256(size: int = 256, views: int = 8, W: int = 512, H: int = 512, seed: int = 0x5EED4,
+            tf_alpha_scale: float = 0.25) -> DiffWorkload:
+    dims = (size, size, size)
+    density = synth_blobs_f32(dims, 16, seed, 4.0 * size / 128, 16.0 * size / 128, 0.3, 1.0)
+    tf = synth_tf(256, 6, seed + 1)
+    # a semi-transparent TF keeps gradients flowing through many samples
+    tf[:, 3] *= tf_alpha_scale
+    b = Box3f()
+    for a in range(3):
+        b.lower[a], b.upper[a] = 0.0, float(size)
+    cams = [synth_camera(b, W, H, 45.0, 2.5, seed + 10 + v) for v in range(views)]
+    rng = np.random.default_rng(seed + 2)
+    target_tf = np.clip(tf * (1.0 + 0.2 * rng.uniform(-1, 1, tf.shape)), 0, 1).astype(np.float32)
+    target_density = (density * (1.0 + 0.05 * rng.standard_normal(density.shape))).astype(np.float32)
+    return DiffWorkload(density, tf, cams, W, H, target_tf, target_density)

@@ -43,6 +43,8 @@ def parse():
                     help="target CPU work of the baseline sample")
     ap.add_argument("--profile", action="store_true",
                     help="few frames, no CPU baseline (for rocprofv3 runs)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable form")
+    ap.add_argument("--adam-steps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -121,6 +123,16 @@ def main():
     ms_render = timed(lambda: ctx.render_ranks(params, W, H, ranks, imgs), args.steps)
     ms_comp = timed(lambda: ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8), args.steps)
     log(f"[bench] render {ms_render:.4f} ms, composite {ms_comp:.4f} ms")
+    # fused form: the same frame (bit-identical result) in one launch
+    fout = torch.empty_like(out)
+    fout8 = torch.empty_like(out8)
+    fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
+    for _ in range(args.warmup):
+        fused()
+    ms_fused = timed(fused, args.steps)
+    torch.cuda.synchronize()
+    fused_equal = bool(torch.equal(fout, out) and torch.equal(fout8, out8))
+    log(f"[bench] fused render+composite {ms_fused:.4f} ms (equal to plain: {fused_equal})")
 
     # exact sample count and compulsory voxel footprint of one frame (untimed)
     cnt = RankImages.alloc(P, W, H, rgba=False, rgba8=False, depth=False, nsamples=True)
@@ -160,6 +172,9 @@ def main():
     parity = None
     if not args.no_cpu:
         cpu, parity = cpu_baseline(wl, imgs, args.cpu_seconds)
+    adjoint = None
+    if not args.no_adjoint and not args.profile:
+        adjoint = adjoint_bench(args, timed)
 
     res = {
         "metric": METRIC,
@@ -179,11 +194,75 @@ def main():
                    "brick_bytes_padded": ctx.brick_bytes(), "upload_s_incl_pcie": round(upload_s, 3)},
         "roofline": roof,
         "kernels": kernels,
+        "forms": {
+            "plain_ranks_then_composite": {"ms_per_frame": round(ms_frame, 5),
+                                           "Gsamples/s": round(gsamples, 4)},
+            "fused_render_composite": {"ms_per_frame": round(ms_fused, 5),
+                                       "Gsamples/s": round(samples / (ms_fused * 1e-3) / 1e9, 4),
+                                       "bit_identical_to_plain": fused_equal},
+        },
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
         "parity_sample": parity,
     }
     print(json.dumps(res), flush=True)
     ctx.close()
+
+
+def adjoint_bench(args, timed):
+    """Differentiable form at 256^3, 8 views of 512^2: forward-only vs
+    forward+adjoint ms, and an Adam fit of TF + density to targets rendered
+    from a perturbed TF and density."""
+    import torch
+    from aloha_amd import workloads as WL
+    from aloha_amd.autograd import DifferentiableVolume
+    t0 = time.time()
+    dw = WL.diff256()
+    vol = DifferentiableVolume(dw.density.shape, dw.cams, dw.W, dw.H, dt=dw.dt, jitter_value=0.5)
+    tf = torch.from_numpy(dw.tf).cuda()
+    den = torch.from_numpy(dw.density).cuda()
+    with torch.no_grad():
+        target, _ = vol.forward_only(torch.from_numpy(dw.target_tf).cuda(),
+                                     torch.from_numpy(dw.target_density).cuda())
+        target = target.clone()
+    log(f"[bench] adjoint workload built in {time.time() - t0:.1f}s")
+    tfp = tf.clone().requires_grad_(True)
+    denp = den.clone().requires_grad_(True)
+
+    def fwd():
+        vol.forward_only(tfp, denp)
+
+    def fwd_bwd():
+        tfp.grad = None
+        denp.grad = None
+        ((vol.render(tfp, denp) - target) ** 2).sum().backward()
+
+    n = max(3, min(args.steps, 20))
+    for _ in range(2):
+        fwd_bwd()
+    ms_f = timed(fwd, n)
+    ms_fb = timed(fwd_bwd, n)
+    _, ns = vol.forward_only(tfp, denp)
+    samples = int(ns.to(torch.int64).sum())
+    log(f"[bench] adjoint: forward {ms_f:.3f} ms, forward+adjoint {ms_fb:.3f} ms")
+    opt = torch.optim.Adam([tfp, denp], lr=2e-3)
+    losses = []
+    t0 = time.time()
+    for _ in range(args.adam_steps):
+        opt.zero_grad()
+        loss = ((vol.render(tfp, denp) - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            tfp.clamp_(0, 1)
+        losses.append(float(loss))
+    log(f"[bench] Adam {args.adam_steps} steps: loss {losses[0]:.3f} -> {losses[-1]:.3f} "
+        f"in {time.time() - t0:.1f}s")
+    return {"workload": "256^3 f32 blob density, 256-entry TF, 8 views 512x512, dt 0.5",
+            "samples_per_forward": samples, "fwd_ms": round(ms_f, 4),
+            "fwd_adjoint_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
+            "adam_steps": args.adam_steps, "loss_first": round(losses[0], 4),
+            "loss_last": round(losses[-1], 4)}
 
 
 def cpu_baseline(wl, imgs, seconds):
@@ -209,6 +288,15 @@ def cpu_baseline(wl, imgs, seconds):
         if el * 2 > seconds or band >= H:
             break
         band = min(H, band * max(2, int(seconds / max(el, 1e-3) / 2)))
+    reps = 1
+    if el < seconds / 2:  # whole frames are cheap: repeat them up to the budget
+        reps = max(1, int(seconds / max(el, 1e-3)))
+        t = time.time()
+        ns = 0
+        for _ in range(reps):
+            for r in range(wl.num_ranks):
+                ns += int(per_rank(r, rows)["nsamples"].sum())
+        el = time.time() - t
     cpu_gs = ns / el / 1e9
     # FP64 error statement on a small band
     r0 = (H // 2 - 4, H // 2 + 4)
@@ -222,8 +310,8 @@ def cpu_baseline(wl, imgs, seconds):
         err_cpu = max(err_cpu, float(np.abs(o["rgba"][sl] - o["rgba64"][sl]).max()))
         max_ulp = max(max_ulp, float(np.abs(gpu_rgba[r, sl] - o["rgba"][sl]).max()))
     cpu = {"value": round(cpu_gs, 5), "unit": "Gsamples/s", "cores": threads, "kind": "port",
-           "sample": f"rows {rows[0]}..{rows[1]} of all {wl.num_ranks} ranks ({ns} samples, "
-                     f"{el:.1f}s), FP32 oracle, OpenMP"}
+           "sample": f"{reps} x rows {rows[0]}..{rows[1]} of all {wl.num_ranks} ranks ({ns} "
+                     f"samples, {el:.1f}s), FP32 oracle, OpenMP"}
     parity = {"rows": f"{r0[0]}..{r0[1]}", "max_abs_gpu_vs_cpu_fp32": max_ulp,
               "max_abs_gpu_vs_fp64": err_gpu, "max_abs_cpu_fp32_vs_fp64": err_cpu}
     return cpu, parity

@@ -206,6 +206,18 @@ int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *params, int32_
                        int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
                        const aloha_images *out);
 
+/* Fused raygen + sort-last composite: for every pixel, each rank slot of
+ * the same table is marched from transparent and blended 'over' the running
+ * colour in order[] -- the same result, bit for bit, as aloha_render_ranks
+ * (spp>=1, accum_id 0, no heat map / crosshair) followed by
+ * aloha_composite_over, without the ranks' images in HBM.  Outputs (device,
+ * W*H; each may be NULL): float RGBA, RGBA8, per-pixel sample count over all
+ * ranks. */
+int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                           int32_t H, int32_t P, const int32_t *rank_offsets,
+                           const int32_t *brick_ids, const int32_t *order, float *d_out_rgba,
+                           uint32_t *d_out_rgba8, uint32_t *d_nsamples);
+
 /* Count distinct voxels read by the march of the same launch (measurement:
  * compulsory bytes).  Synchronous. */
 int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
@@ -227,6 +239,38 @@ int aloha_composite_over_rgba8(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, int
 int aloha_composite_zmin(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, const float *d_rank_depth,
                          int32_t P, int32_t W, int32_t H, uint32_t *d_out_rgba8,
                          float *d_out_depth);
+
+/* ---------------------------------------------------------------------- */
+/* inputs from device memory (training loops update them every step)      */
+/* ---------------------------------------------------------------------- */
+/* TF from n device-resident RGBA float entries; asynchronous on the stream. */
+int aloha_tf_set_device(aloha_ctx *ctx, const float *d_rgba, int32_t n);
+/* Replace the voxels of an uploaded brick (same extent) from d_voxels
+ * (x-fastest, encoding dtype); asynchronous on the stream. */
+int aloha_brick_update(aloha_ctx *ctx, int32_t brick_id, const void *d_voxels, int32_t dtype);
+
+/* ---------------------------------------------------------------------- */
+/* differentiable form (no reference counterpart: the adjoint of           */
+/* integrateBrick, deviceCode.cpp:256-298, for one rank's bricks)          */
+/* ---------------------------------------------------------------------- */
+#define ALOHA_MAX_VIEWS 16
+/* Forward: V views (cams: V x {org, dir_00, dir_du, dir_dv} = 12 floats, host)
+ * of one rank marching bricks brick_ids[0..nb).  Samples on the lattice
+ * t_k = dt*(j+k0+k) with opacity clamped to 0.9999 (so the march can be
+ * reversed).  d_rgba: V*H*W float4; d_nsamples: V*H*W.  Uses params->dt,
+ * jitter_mode, jitter_value; spp 1, accumID 0. */
+int aloha_diff_forward(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                       int32_t V, const float *cams, int32_t nb, const int32_t *brick_ids,
+                       float *d_rgba, uint32_t *d_nsamples);
+/* Adjoint of aloha_diff_forward: given its outputs and d_dl_drgba (V*H*W
+ * float4), ADD dLoss/dTF into d_dtf (n x 4 floats) and dLoss/dvoxel into
+ * d_dvoxels[i] (device, brick i's unpadded voxel count, x-fastest; a NULL
+ * entry skips that brick).  Transmittance is recovered backward from the
+ * stored final alpha; nothing per sample is stored. */
+int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                        int32_t V, const float *cams, int32_t nb, const int32_t *brick_ids,
+                        const float *d_rgba, const uint32_t *d_nsamples,
+                        const float *d_dl_drgba, float *d_dtf, float *const *d_dvoxels);
 
 #ifdef __cplusplus
 }

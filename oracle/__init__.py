@@ -288,3 +288,61 @@ def composite_zmin(imgs8: np.ndarray, depths: np.ndarray):
     lib().orc_composite_zmin(imgs8.ctypes.data, depths.ctypes.data, P, npix, out8.ctypes.data,
                              outz.ctypes.data)
     return out8, outz
+
+
+def _params(cam, tf, W, H, dt, jitter_mode, jitter_value):
+    p = OrcParams()
+    p.org, p.dir00, p.du, p.dv = (_f3(cam[k]) for k in ("org", "dir_00", "dir_du", "dir_dv"))
+    p.dt, p.max_depth, p.W, p.H = dt, 1.0, W, H
+    p.spp, p.accum_id, p.island_id, p.num_islands = 1, 0, 0, 1
+    p.jitter_mode, p.jitter_value, p.crosshair = jitter_mode, jitter_value, 0
+    p.tf_n = tf.shape[0]
+    p.tf = tf.ctypes.data_as(C.POINTER(C.c_float))
+    return p
+
+
+def diff_render(cam, tf, bricks, W, H, dt=0.5, jitter_mode=0, jitter_value=0.0, threads=0):
+    """Float restatement of the differentiable forward (closed-form lattice,
+    opacity clamped to 0.9999).  Returns (rgba (H,W,4), nsamples (H,W))."""
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    vox = [np.ascontiguousarray(v) for _, v in bricks]
+    br = (OrcBrick * len(bricks))(*[_orc_brick(d, v) for (d, _), v in zip(bricks, vox)])
+    p = _params(cam, tf, W, H, dt, jitter_mode, jitter_value)
+    rgba = np.zeros((H, W, 4), np.float32)
+    ns = np.zeros((H, W), np.uint32)
+    rc = lib().orc_diff_render(C.byref(p), br, len(bricks), C.c_void_p(rgba.ctypes.data),
+                               C.c_void_p(ns.ctypes.data), int(threads))
+    if rc:
+        raise ValueError("orc_diff_render failed")
+    return rgba, ns
+
+
+def diff_adjoint(cam, tf, bricks, W, H, dl_drgba, dt=0.5, jitter_mode=0, jitter_value=0.0,
+                 threads=0):
+    """float64 adjoint of diff_render: (dtf (n,4) f64, [dvox per brick (z,y,x) f64])."""
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    vox = [np.ascontiguousarray(v) for _, v in bricks]
+    br = (OrcBrick * len(bricks))(*[_orc_brick(d, v) for (d, _), v in zip(bricks, vox)])
+    p = _params(cam, tf, W, H, dt, jitter_mode, jitter_value)
+    g = np.ascontiguousarray(dl_drgba, dtype=np.float32)
+    dtf = np.zeros(tf.shape, np.float64)
+    dv = [np.zeros(v.shape, np.float64) for v in vox]
+    ptrs = (C.c_void_p * len(dv))(*[a.ctypes.data for a in dv])
+    rc = lib().orc_diff_adjoint(C.byref(p), br, len(bricks), C.c_void_p(g.ctypes.data),
+                                C.c_void_p(dtf.ctypes.data), ptrs, int(threads))
+    if rc:
+        raise ValueError("orc_diff_adjoint failed")
+    return dtf, dv
+
+
+def diff_render64(cam, tf, bricks, W, H, dt=0.5, jitter_mode=0, jitter_value=0.0, threads=0):
+    """float64 colour of the differentiable forward on its float control."""
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    vox = [np.ascontiguousarray(v) for _, v in bricks]
+    br = (OrcBrick * len(bricks))(*[_orc_brick(d, v) for (d, _), v in zip(bricks, vox)])
+    p = _params(cam, tf, W, H, dt, jitter_mode, jitter_value)
+    out = np.zeros((H, W, 4), np.float64)
+    if lib().orc_diff_render64(C.byref(p), br, len(bricks), C.c_void_p(out.ctypes.data),
+                               int(threads)):
+        raise ValueError("orc_diff_render64 failed")
+    return out

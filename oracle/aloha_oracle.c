@@ -557,7 +557,7 @@ int orc_slab_probe(const float o[3], const float d[3], float tmin, float tmax, c
 /* One pixel of renderFrame for one rank whose bricks are b[0..nb). */
 static void render_pixel(const orc_params *p, const bview *bv, int nb, int px, int py,
                          float acc[4], double acc64[4], float *zout, uint32_t *ns, int want64) {
-  const int W = p->W, H = p->H;
+  const int W = p->W;
   const int pixel = px + W * py;
   /* island mask with the reference's gridSize formula (deviceCode.cpp:346-361) */
   /* gridSize.x = iDivUp(pixel.x, W) is 1 for every column but the first */
@@ -749,6 +749,315 @@ int orc_composite_zmin(const uint32_t *imgs, const float *depths, int P, long lo
     }
     out8[i] = imgs[(size_t)best * (size_t)npix + (size_t)i];
     if (outz) outz[i] = bz;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* differentiable form: forward (float) and adjoint (float64)          */
+/* ------------------------------------------------------------------ */
+/* The HIP differentiable forward marches the closed-form lattice
+ * t_k = dt*(j + (k0+k)) with opacity clamped to DIFF_AMAX; restated here in
+ * float.  The adjoint is restated in float64 data arithmetic on the same
+ * float control (same samples): it stores the samples of a ray and computes
+ * transmittance forward as a product, then back-propagates with suffix
+ * sums -- a different method from the HIP kernel's backward inversion. */
+#define DIFF_AMAX 0.9999f
+
+typedef struct {
+  int brick;
+  float t;
+} diff_sample;
+
+/* float control of one pixel: appends its samples (brick, t) to out (cap),
+ * returns the count; fills the float colour. */
+static int diff_march_pixel(const orc_params *p, const bview *bv, int nb, const float o[3],
+                            const float d[3], uint32_t pixel, diff_sample *out, int cap,
+                            float c[4]) {
+  uint32_t rng = rng_seed(pixel, 0u);
+  float tmin = 0.f;
+  int n = 0;
+  c[0] = c[1] = c[2] = c[3] = 0.f;
+  for (;;) {
+    int hit = -1;
+    float best = 0.f, bt0 = 0.f, bt1 = 0.f;
+    for (int i = 0; i < nb; ++i) {
+      float t0, t1, rep;
+      if (!slab(o, d, tmin, 1e10f, bv[i].b->space_lo, bv[i].b->space_hi, &t0, &t1)) continue;
+      if (t0 > tmin) rep = t0;
+      else if (t1 > tmin) rep = t1;
+      else continue;
+      if (hit < 0 || rep < best) { hit = i; best = rep; bt0 = t0; bt1 = t1; }
+    }
+    if (hit < 0) break;
+    const float j = p->jitter_mode == 1 ? p->jitter_value : rng_next(&rng);
+    const float dt = p->dt;
+    const float segs = floorf((bt0 - dt * j) / dt);
+    const float tf = dt * (j + segs);
+    const float k0 = tf < bt0 ? segs + 1.f : segs;
+    const orc_brick *b = bv[hit].b;
+    const float vl[3] = {(float)b->vox_lo[0], (float)b->vox_lo[1], (float)b->vox_lo[2]};
+    for (int k = 0;; ++k) {
+      const float t = dt * (j + (k0 + (float)k));
+      if (!(t < bt1 && c[3] < .99f)) break;
+      if (n < cap) { out[n].brick = hit; out[n].t = t; }
+      ++n;
+      float x = (o[0] + t * d[0]) - vl[0];
+      float y = (o[1] + t * d[1]) - vl[1];
+      float z = (o[2] + t * d[2]) - vl[2];
+      float gx = tri_f(&bv[hit], x + .5f, y, z) - tri_f(&bv[hit], x - .5f, y, z);
+      float gy = tri_f(&bv[hit], x, y + .5f, z) - tri_f(&bv[hit], x, y - .5f, z);
+      float gz = tri_f(&bv[hit], x, y, z + .5f) - tri_f(&bv[hit], x, y, z - .5f);
+      float ex = gx + 1e-4f, ey = gy + 1e-4f, ez = gz + 1e-4f;
+      float len = sqrtf(ex * ex + ey * ey + ez * ez);
+      float nx = gx / len, ny = gy / len, nz = gz / len;
+      float s[4];
+      tf_f(p, tri_f(&bv[hit], x, y, z), s);
+      s[3] = fminf(s[3], DIFF_AMAX);
+      float shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
+      float w = (1.f - c[3]) * s[3];
+      c[0] = c[0] + (w * s[0]) * shade;
+      c[1] = c[1] + (w * s[1]) * shade;
+      c[2] = c[2] + (w * s[2]) * shade;
+      c[3] = c[3] + w;
+    }
+    tmin = bt1 + 1e-20f;
+    if (!(c[3] < .99f)) break;
+  }
+  return n;
+}
+
+static void ray_of(const orc_params *p, int px, int py, float d[3]) {
+  const float su = (float)px + .5f, sv = (float)py + .5f;
+  for (int a = 0; a < 3; ++a) d[a] = (p->dir00[a] + su * p->du[a]) + sv * p->dv[a];
+  float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (int a = 0; a < 3; ++a) d[a] = d[a] / len;
+  for (int a = 0; a < 3; ++a)
+    if (fabsf(d[a]) < 1e-5f) d[a] = 1e-5f;
+}
+
+static int init_views(const orc_brick *bricks, int nb, bview *bv) {
+  if (nb > 256) return -2;
+  for (int i = 0; i < nb; ++i) {
+    bv[i].b = &bricks[i];
+    for (int a = 0; a < 3; ++a) bv[i].n[a] = bricks[i].vox_hi[a] - bricks[i].vox_lo[a];
+    bv[i].touched = NULL;
+  }
+  return 0;
+}
+
+int orc_diff_render(const orc_params *p, const orc_brick *bricks, int nb, float *rgba,
+                    uint32_t *nsamples, int nthreads) {
+  bview bv[256];
+  if (init_views(bricks, nb, bv)) return -2;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int py = 0; py < p->H; ++py)
+    for (int px = 0; px < p->W; ++px) {
+      float d[3], c[4];
+      ray_of(p, px, py, d);
+      const size_t pix = (size_t)py * p->W + px;
+      int n = diff_march_pixel(p, bv, nb, p->org, d, (uint32_t)pix, NULL, 0, c);
+      memcpy(rgba + 4 * pix, c, sizeof c);
+      nsamples[pix] = (uint32_t)n;
+    }
+  return 0;
+}
+
+static void scatter_d(double *grad, const bview *v, double x, double y, double z, double g) {
+  if (g == 0.0 || !grad) return;
+  double fx = floor(x), fy = floor(y), fz = floor(z);
+  double wx = x - fx, wy = y - fy, wz = z - fz;
+  int ix = floor_to_int((float)fx), iy = floor_to_int((float)fy), iz = floor_to_int((float)fz);
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int xx = clampi(ix + i, 0, v->n[0] - 1), yy = clampi(iy + j, 0, v->n[1] - 1),
+            zz = clampi(iz + k, 0, v->n[2] - 1);
+        double w = (i ? wx : 1 - wx) * (j ? wy : 1 - wy) * (k ? wz : 1 - wz);
+        double *dst = &grad[((size_t)zz * v->n[1] + yy) * v->n[0] + xx];
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        *dst += g * w;
+      }
+}
+
+/* dl_drgba: H*W*4 floats; dtf: tf_n*4 doubles (accumulated); dvox[i]:
+ * brick i's voxel count doubles (accumulated, may be NULL). */
+int orc_diff_adjoint(const orc_params *p, const orc_brick *bricks, int nb, const float *dl_drgba,
+                     double *dtf, double **dvox, int nthreads) {
+  bview bv[256];
+  if (init_views(bricks, nb, bv)) return -2;
+  const int n_tf = p->tf_n;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    int cap = 1 << 16;
+    diff_sample *smp = (diff_sample *)malloc(sizeof(diff_sample) * (size_t)cap);
+    double *A = (double *)malloc(sizeof(double) * (size_t)cap * 12);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int py = 0; py < p->H; ++py)
+      for (int px = 0; px < p->W; ++px) {
+        float d[3], c[4];
+        ray_of(p, px, py, d);
+        const size_t pix = (size_t)py * p->W + px;
+        int n = diff_march_pixel(p, bv, nb, p->org, d, (uint32_t)pix, smp, cap, c);
+        if (n == 0) continue;
+        if (n > cap) n = cap; /* not reached for the sizes tested */
+        const double G[4] = {dl_drgba[4 * pix], dl_drgba[4 * pix + 1], dl_drgba[4 * pix + 2],
+                             dl_drgba[4 * pix + 3]};
+        /* forward in double: per sample a, c(3), shade, T, plus data for backprop */
+        double T = 1.0;
+        for (int k = 0; k < n; ++k) {
+          const bview *v = &bv[smp[k].brick];
+          const double t = smp[k].t;
+          double X = ((double)p->org[0] + t * d[0]) - v->b->vox_lo[0];
+          double Y = ((double)p->org[1] + t * d[1]) - v->b->vox_lo[1];
+          double Z = ((double)p->org[2] + t * d[2]) - v->b->vox_lo[2];
+          double gx = tri_d(v, X + .5, Y, Z) - tri_d(v, X - .5, Y, Z);
+          double gy = tri_d(v, X, Y + .5, Z) - tri_d(v, X, Y - .5, Z);
+          double gz = tri_d(v, X, Y, Z + .5) - tri_d(v, X, Y, Z - .5);
+          double L = sqrt((gx + 1e-4) * (gx + 1e-4) + (gy + 1e-4) * (gy + 1e-4) +
+                          (gz + 1e-4) * (gz + 1e-4));
+          double val = tri_d(v, X, Y, Z);
+          double s[4];
+          tf_d(p, val, s);
+          double araw = s[3];
+          double a = araw < DIFF_AMAX ? araw : DIFF_AMAX;
+          double nd = (gx / L) * d[0] + (gy / L) * d[1] + (gz / L) * d[2];
+          double sh = .1 + .9 * fabs(nd);
+          double *r = A + 12 * (size_t)k;
+          r[0] = a; r[1] = s[0]; r[2] = s[1]; r[3] = s[2]; r[4] = sh; r[5] = T;
+          r[6] = val; r[7] = gx; r[8] = gy; r[9] = gz; r[10] = L; r[11] = araw;
+          T *= (1.0 - a);
+        }
+        const double TN = T;
+        double S[3] = {0, 0, 0};
+        for (int k = n - 1; k >= 0; --k) {
+          const bview *v = &bv[smp[k].brick];
+          const double *r = A + 12 * (size_t)k;
+          const double a = r[0], sh = r[4], Tk = r[5], val = r[6], L = r[10];
+          const double cc[3] = {r[1], r[2], r[3]};
+          const double g3[3] = {r[7], r[8], r[9]};
+          const double Ta = Tk * a;
+          double Gcs = 0, GS = 0, Gc = 0;
+          for (int q = 0; q < 3; ++q) { Gcs += G[q] * cc[q] * sh; GS += G[q] * S[q]; Gc += G[q] * cc[q]; }
+          double dc[3];
+          for (int q = 0; q < 3; ++q) dc[q] = G[q] * Ta * sh;
+          const double dshade = Ta * Gc;
+          double dalpha = Tk * Gcs - (GS - G[3] * TN) / (1.0 - a);
+          for (int q = 0; q < 3; ++q) S[q] += Ta * cc[q] * sh;
+          if (!(r[11] < DIFF_AMAX)) dalpha = 0.0;
+          /* TF entries */
+          const double xt = val * n_tf - 0.5;
+          const double fi = floor(xt);
+          const double w = xt - fi;
+          const int i0 = (int)(fi < -2 ? -2 : (fi > n_tf ? n_tf : fi));
+          const double g4[4] = {dc[0], dc[1], dc[2], dalpha};
+          for (int ch = 0; ch < 4; ++ch) {
+            if (i0 >= 0 && i0 < n_tf) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+              dtf[4 * i0 + ch] += (1 - w) * g4[ch];
+            }
+            if (i0 + 1 >= 0 && i0 + 1 < n_tf) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+              dtf[4 * (i0 + 1) + ch] += w * g4[ch];
+            }
+          }
+          double *grad = dvox ? dvox[smp[k].brick] : NULL;
+          if (!grad) continue;
+          double e0[4], e1[4];
+          for (int ch = 0; ch < 4; ++ch) {
+            e0[ch] = (i0 >= 0 && i0 < n_tf) ? p->tf[4 * i0 + ch] : 0.0;
+            e1[ch] = (i0 + 1 >= 0 && i0 + 1 < n_tf) ? p->tf[4 * (i0 + 1) + ch] : 0.0;
+          }
+          double dv = 0;
+          for (int ch = 0; ch < 4; ++ch) dv += g4[ch] * (e1[ch] - e0[ch]);
+          dv *= n_tf;
+          const double nd = (g3[0] / L) * d[0] + (g3[1] / L) * d[1] + (g3[2] / L) * d[2];
+          const double sg = nd > 0 ? .9 : (nd < 0 ? -.9 : 0.0);
+          double dN[3], dNg = 0;
+          for (int q = 0; q < 3; ++q) { dN[q] = dshade * sg * d[q]; dNg += dN[q] * g3[q]; }
+          double dg[3];
+          for (int q = 0; q < 3; ++q) dg[q] = dN[q] / L - dNg * (g3[q] + 1e-4) / (L * L * L);
+          const double t = smp[k].t;
+          double X = ((double)p->org[0] + t * d[0]) - v->b->vox_lo[0];
+          double Y = ((double)p->org[1] + t * d[1]) - v->b->vox_lo[1];
+          double Z = ((double)p->org[2] + t * d[2]) - v->b->vox_lo[2];
+          scatter_d(grad, v, X, Y, Z, dv);
+          scatter_d(grad, v, X + .5, Y, Z, dg[0]);
+          scatter_d(grad, v, X - .5, Y, Z, -dg[0]);
+          scatter_d(grad, v, X, Y + .5, Z, dg[1]);
+          scatter_d(grad, v, X, Y - .5, Z, -dg[1]);
+          scatter_d(grad, v, X, Y, Z + .5, dg[2]);
+          scatter_d(grad, v, X, Y, Z - .5, -dg[2]);
+        }
+      }
+    free(smp);
+    free(A);
+  }
+  return 0;
+}
+
+/* float64 colour of the differentiable forward on the float control (the
+ * function whose derivatives orc_diff_adjoint computes); out: H*W*4. */
+int orc_diff_render64(const orc_params *p, const orc_brick *bricks, int nb, double *out,
+                      int nthreads) {
+  bview bv[256];
+  if (init_views(bricks, nb, bv)) return -2;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    int cap = 1 << 16;
+    diff_sample *smp = (diff_sample *)malloc(sizeof(diff_sample) * (size_t)cap);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int py = 0; py < p->H; ++py)
+      for (int px = 0; px < p->W; ++px) {
+        float d[3], c[4];
+        ray_of(p, px, py, d);
+        const size_t pix = (size_t)py * p->W + px;
+        int n = diff_march_pixel(p, bv, nb, p->org, d, (uint32_t)pix, smp, cap, c);
+        if (n > cap) n = cap;
+        double T = 1.0, C[3] = {0, 0, 0};
+        for (int k = 0; k < n; ++k) {
+          const bview *v = &bv[smp[k].brick];
+          const double t = smp[k].t;
+          double X = ((double)p->org[0] + t * d[0]) - v->b->vox_lo[0];
+          double Y = ((double)p->org[1] + t * d[1]) - v->b->vox_lo[1];
+          double Z = ((double)p->org[2] + t * d[2]) - v->b->vox_lo[2];
+          double gx = tri_d(v, X + .5, Y, Z) - tri_d(v, X - .5, Y, Z);
+          double gy = tri_d(v, X, Y + .5, Z) - tri_d(v, X, Y - .5, Z);
+          double gz = tri_d(v, X, Y, Z + .5) - tri_d(v, X, Y, Z - .5);
+          double L = sqrt((gx + 1e-4) * (gx + 1e-4) + (gy + 1e-4) * (gy + 1e-4) +
+                          (gz + 1e-4) * (gz + 1e-4));
+          double s[4];
+          tf_d(p, tri_d(v, X, Y, Z), s);
+          double a = s[3] < DIFF_AMAX ? s[3] : DIFF_AMAX;
+          double sh = .1 + .9 * fabs((gx / L) * d[0] + (gy / L) * d[1] + (gz / L) * d[2]);
+          for (int q = 0; q < 3; ++q) C[q] += T * a * s[q] * sh;
+          T *= 1.0 - a;
+        }
+        out[4 * pix] = C[0];
+        out[4 * pix + 1] = C[1];
+        out[4 * pix + 2] = C[2];
+        out[4 * pix + 3] = 1.0 - T;
+      }
+    free(smp);
   }
   return 0;
 }

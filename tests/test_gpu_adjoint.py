@@ -1,0 +1,145 @@
+"""GPU checks of the differentiable form (aloha_diff_forward / _backward):
+the forward against the oracle's float restatement, the hand-written adjoint
+against the oracle's float64 adjoint and against central finite differences
+through the torch.autograd wrapper, and a short Adam fit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from aloha_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small():
+    return WL.diff256(size=48, views=2, W=80, H=72)
+
+
+def _vol(wl):
+    from aloha_amd.autograd import DifferentiableVolume
+    return DifferentiableVolume(wl.density.shape, wl.cams, wl.W, wl.H, dt=wl.dt, jitter_value=0.5)
+
+
+def _brick(wl):
+    n = wl.density.shape[::-1]
+    return {"cell": ((0, 0, 0), n), "voxel": ((0, 0, 0), n),
+            "space": ((0.0, 0.0, 0.0), tuple(float(v) for v in n))}
+
+
+def test_forward_matches_oracle(small):
+    import torch
+    print("\n[case] diff forward 48^3, 2 views 80x72", flush=True)
+    vol = _vol(small)
+    tf = torch.from_numpy(small.tf).cuda()
+    den = torch.from_numpy(small.density).cuda()
+    rgba, ns = vol.forward_only(tf, den)
+    torch.cuda.synchronize()
+    rgba, ns = rgba.cpu().numpy(), ns.cpu().numpy()
+    for v, cam in enumerate(small.cams):
+        ref, rns = O.diff_render(cam, small.tf, [(_brick(small), small.density)], small.W, small.H,
+                                 dt=small.dt, jitter_mode=1, jitter_value=0.5)
+        assert np.array_equal(ns[v], rns.astype(np.int32))
+        assert np.abs(rgba[v] - ref).max() <= 1e-4
+    assert ns.sum() > 10000
+
+
+def test_adjoint_matches_fp64_oracle(small):
+    import torch
+    print("\n[case] diff backward vs fp64 oracle", flush=True)
+    vol = _vol(small)
+    tf = torch.from_numpy(small.tf).cuda().requires_grad_(True)
+    den = torch.from_numpy(small.density).cuda().requires_grad_(True)
+    rgba = vol.render(tf, den)
+    w = torch.from_numpy(np.random.default_rng(0).standard_normal(tuple(rgba.shape)).astype(np.float32)).cuda()
+    (rgba * w).sum().backward()
+    torch.cuda.synchronize()
+    g_tf, g_den = tf.grad.cpu().numpy(), den.grad.cpu().numpy()
+    ref_tf = np.zeros_like(g_tf, dtype=np.float64)
+    ref_den = np.zeros_like(g_den, dtype=np.float64)
+    wn = w.cpu().numpy()
+    for v, cam in enumerate(small.cams):
+        dtf, (dvox,) = O.diff_adjoint(cam, small.tf, [(_brick(small), small.density)], small.W,
+                                      small.H, wn[v], dt=small.dt, jitter_mode=1, jitter_value=0.5)
+        ref_tf += dtf
+        ref_den += dvox
+    e_tf = np.linalg.norm(g_tf - ref_tf) / np.linalg.norm(ref_tf)
+    e_den = np.linalg.norm(g_den - ref_den) / np.linalg.norm(ref_den)
+    print(f"relative L2 error dTF {e_tf:.2e} dDensity {e_den:.2e}")
+    assert e_tf < 1e-3 and e_den < 1e-3
+
+
+def test_adjoint_matches_finite_differences(small):
+    import torch
+    print("\n[case] diff backward vs central differences", flush=True)
+    vol = _vol(small)
+    tf0 = torch.from_numpy(small.tf).cuda()
+    den0 = torch.from_numpy(small.density).cuda()
+    tgt_tf = torch.from_numpy(small.target_tf).cuda()
+    with torch.no_grad():
+        target, _ = vol.forward_only(tgt_tf, den0)
+        target = target.clone()
+
+    def loss(rgba):
+        return ((rgba.double() - target.double()) ** 2).sum()
+
+    tf = tf0.clone().requires_grad_(True)
+    den = den0.clone().requires_grad_(True)
+    loss(vol.render(tf, den)).backward()
+    g_tf, g_den = tf.grad.clone(), den.grad.clone()
+    eps = 1e-3
+    rng = np.random.default_rng(5)
+
+    def fd(kind, idx):
+        outs, counts = [], []
+        for sgn in (1, -1):
+            t, d = tf0.clone(), den0.clone()
+            (t if kind == "tf" else d)[idx] += sgn * eps
+            rgba, ns = vol.forward_only(t, d)
+            outs.append(float(loss(rgba)))
+            counts.append(ns.clone())
+        flipped = not torch.equal(counts[0], counts[1])
+        return (outs[0] - outs[1]) / (2 * eps), flipped
+
+    ok = flagged = 0
+    nz = (g_tf.abs() > 0).nonzero().cpu().numpy()
+    picks = [("tf", tuple(nz[i])) for i in rng.choice(len(nz), 32, replace=False)]
+    dz = (g_den.abs() > 1e-3 * float(g_den.abs().max())).nonzero().cpu().numpy()
+    picks += [("den", tuple(dz[i])) for i in rng.choice(len(dz), 32, replace=False)]
+    worst = 0.0
+    for kind, idx in picks:
+        f, flipped = fd(kind, idx)
+        g = float((g_tf if kind == "tf" else g_den)[idx])
+        if flipped:  # the perturbation moved an alpha<.99 / t<t1 branch
+            flagged += 1
+            continue
+        err = abs(f - g) / (abs(f) + 1e-2 * float((g_tf if kind == "tf" else g_den).abs().max()))
+        worst = max(worst, err)
+        ok += 1
+    print(f"FD: {ok} compared, {flagged} flagged (branch flips), worst rel err {worst:.3e}")
+    assert ok >= 48 and worst < 5e-2
+
+
+def test_adam_fit_reduces_loss(small):
+    import torch
+    print("\n[case] 40 Adam steps", flush=True)
+    vol = _vol(small)
+    den0 = torch.from_numpy(small.density).cuda()
+    with torch.no_grad():
+        target, _ = vol.forward_only(torch.from_numpy(small.target_tf).cuda(),
+                                     torch.from_numpy(small.target_density).cuda())
+        target = target.clone()
+    tf = torch.from_numpy(small.tf).cuda().requires_grad_(True)
+    den = den0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([tf, den], lr=2e-3)
+    losses = []
+    for _ in range(40):
+        opt.zero_grad()
+        l = ((vol.render(tf, den) - target) ** 2).sum()
+        l.backward()
+        opt.step()
+        with torch.no_grad():
+            tf.clamp_(0, 1)
+        losses.append(float(l))
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.5 * losses[0]

@@ -89,3 +89,28 @@ def test_small_bricks_and_u16():
               "dir_dv": np.array([0.0, 0.02, 0.0], np.float32)}
     wl.max_depth = A.max_depth(wl.bounds, wl.cam["org"])
     print(compare(gpu_render(wl, dt=0.25), oracle_render(wl, dt=0.25)))
+
+
+@pytest.mark.parametrize("which", ["c0", "c1"])
+def test_fused_render_composite_bit_exact(which, c0, c1):
+    import torch
+    wl = c0 if which == "c0" else c1
+    _case(f"{wl.name} fused render+composite vs ranks+composite")
+    out = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
+    out8 = torch.empty((wl.H, wl.W), dtype=torch.int32, device="cuda")
+    ns = torch.empty((wl.H, wl.W), dtype=torch.int32, device="cuda")
+    ref = torch.empty_like(out)
+    ref8 = torch.empty_like(out8)
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
+        ctx.set_tf(wl.tf)
+        ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+        ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
+        imgs = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, nsamples=True)
+        ctx.render_ranks(wl.params(), wl.W, wl.H, ranks, imgs)
+        ctx.composite_over(imgs.rgba, wl.order, wl.W, wl.H, ref, ref8)
+        ctx.render_composite(wl.params(), wl.W, wl.H, ranks, wl.order, out, out8, ns)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+        assert torch.equal(out8, ref8)
+        assert int(ns.sum()) == int(imgs.nsamples.sum()) > 0
