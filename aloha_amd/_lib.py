@@ -130,6 +130,14 @@ def load(path: Path = LIB_PATH) -> C.CDLL:
     if not path.exists():
         raise ImportError(
             f"{path} is not built; run `python -m aloha_amd.build` (there is no CPU fallback)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (NEEDED as
+    # "libamdhip64.so", soname libamdhip64.so.7).  Loaded first, it satisfies
+    # this library's libamdhip64.so.7 dependency; loaded after, a second HIP
+    # and HSA runtime would come up next to ours and fail to find the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(path))
     for name, res, args in _SIGS:
         fn = getattr(lib, name)

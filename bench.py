@@ -113,6 +113,13 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / n
 
+    if args.profile:
+        # calibration for HBM-traffic counters: one copy of exactly 256 MiB
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
