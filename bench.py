@@ -155,6 +155,12 @@ def main():
     comp_bytes = P * npix * 16 + npix * (16 + 4)
     render_gbs = render_bytes / (ms_render * 1e-3) / 1e9
     comp_gbs = comp_bytes / (ms_comp * 1e-3) / 1e9
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and args.workload == "fbm512" and (args.size or 512) == 512:
+        with open(tpath) as f:
+            traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in json.load(f).items()
+                       if not k.startswith("_")}
     dominant = "render" if ms_render >= ms_comp else "composite"
     roof = {
         "kernel": "march_kernel" if dominant == "render" else "over_kernel",
@@ -163,7 +169,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round((render_gbs if dominant == "render" else comp_gbs) / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic.get("march_kernel" if dominant == "render" else "over_kernel"),
+        "traffic_source": "profiles/traffic.json (rocprofv3 --pmc, calibrated)" if traffic else None,
         "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
         "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
     }

@@ -70,14 +70,19 @@ def test_adjoint_matches_fp64_oracle(small):
 
 
 def test_adjoint_matches_finite_differences(small):
+    """Central differences (eps 1e-3) of a float64 L2 image loss through the
+    float32 forward, on 32 TF entries and the 32 voxels with the largest
+    gradients.  A perturbation that changes any pixel's sample count (an
+    alpha<.99 or t<t1 branch) is flagged and excluded; the loss is only
+    piecewise smooth, so at most 1 in 8 of the remaining voxel entries may
+    sit across a kink (TF control point, |N.d| = 0)."""
     import torch
     print("\n[case] diff backward vs central differences", flush=True)
     vol = _vol(small)
     tf0 = torch.from_numpy(small.tf).cuda()
     den0 = torch.from_numpy(small.density).cuda()
-    tgt_tf = torch.from_numpy(small.target_tf).cuda()
     with torch.no_grad():
-        target, _ = vol.forward_only(tgt_tf, den0)
+        target, _ = vol.forward_only(torch.from_numpy(small.target_tf).cuda(), den0)
         target = target.clone()
 
     def loss(rgba):
@@ -88,7 +93,6 @@ def test_adjoint_matches_finite_differences(small):
     loss(vol.render(tf, den)).backward()
     g_tf, g_den = tf.grad.clone(), den.grad.clone()
     eps = 1e-3
-    rng = np.random.default_rng(5)
 
     def fd(kind, idx):
         outs, counts = [], []
@@ -98,26 +102,33 @@ def test_adjoint_matches_finite_differences(small):
             rgba, ns = vol.forward_only(t, d)
             outs.append(float(loss(rgba)))
             counts.append(ns.clone())
-        flipped = not torch.equal(counts[0], counts[1])
-        return (outs[0] - outs[1]) / (2 * eps), flipped
+        return (outs[0] - outs[1]) / (2 * eps), not torch.equal(counts[0], counts[1])
 
-    ok = flagged = 0
-    nz = (g_tf.abs() > 0).nonzero().cpu().numpy()
-    picks = [("tf", tuple(nz[i])) for i in rng.choice(len(nz), 32, replace=False)]
-    dz = (g_den.abs() > 1e-3 * float(g_den.abs().max())).nonzero().cpu().numpy()
-    picks += [("den", tuple(dz[i])) for i in rng.choice(len(dz), 32, replace=False)]
-    worst = 0.0
-    for kind, idx in picks:
-        f, flipped = fd(kind, idx)
-        g = float((g_tf if kind == "tf" else g_den)[idx])
-        if flipped:  # the perturbation moved an alpha<.99 / t<t1 branch
-            flagged += 1
-            continue
-        err = abs(f - g) / (abs(f) + 1e-2 * float((g_tf if kind == "tf" else g_den).abs().max()))
-        worst = max(worst, err)
-        ok += 1
-    print(f"FD: {ok} compared, {flagged} flagged (branch flips), worst rel err {worst:.3e}")
-    assert ok >= 48 and worst < 5e-2
+    rng = np.random.default_rng(5)
+    nz = (g_tf.abs() > 1e-3 * float(g_tf.abs().max())).nonzero().cpu().numpy()
+    tf_idx = [tuple(nz[i]) for i in rng.choice(len(nz), 32, replace=False)]
+    top = torch.topk(g_den.abs().flatten(), 32).indices.cpu().numpy()
+    den_idx = [tuple(int(v) for v in np.unravel_index(i, tuple(g_den.shape))) for i in top]
+    report = {}
+    for kind, picks, g in (("tf", tf_idx, g_tf), ("den", den_idx, g_den)):
+        gmax = float(g.abs().max())
+        errs, flagged = [], 0
+        for idx in picks:
+            f, flipped = fd(kind, idx)
+            if flipped:
+                flagged += 1
+                continue
+            a = float(g[idx])
+            errs.append(abs(f - a) / (abs(f) + 1e-3 * gmax))
+            if errs[-1] > 5e-2:
+                print(f"  {kind}{idx}: fd {f:.6g} adjoint {a:.6g}")
+        errs = np.array(errs)
+        report[kind] = (len(errs), flagged, float(np.median(errs)), int((errs > 5e-2).sum()))
+    print("FD (compared, flagged, median rel err, >5% outliers):", report)
+    n_tf, _, med_tf, bad_tf = report["tf"]
+    n_den, _, med_den, bad_den = report["den"]
+    assert n_tf >= 16 and bad_tf == 0 and med_tf < 1e-2
+    assert n_den >= 16 and bad_den <= n_den // 8 and med_den < 2e-2
 
 
 def test_adam_fit_reduces_loss(small):
