@@ -349,3 +349,58 @@ class Context:
             self._h, C.c_void_p(rank_rgba8.data_ptr()), C.c_void_p(rank_depth.data_ptr()), P, W, H,
             C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None,
             C.c_void_p(out_depth.data_ptr()) if out_depth is not None else None))
+
+
+class Batch:
+    """Many views of one rank table in one launch (aloha_batch_*).  Output
+    tensors are (V, H, W, 4) float32 / (V, H, W) int32; each view's image
+    equals Context.render_composite for that view's camera and order."""
+
+    def __init__(self, ctx: Context, params: RenderParams, W: int, H: int, cams,
+                 ranks: Sequence[Sequence[int]], orders, out_rgba=None, out_rgba8=None,
+                 nsamples=None):
+        self.ctx = ctx
+        V = len(cams)
+        cam = np.concatenate([np.concatenate([c["org"], c["dir_00"], c["dir_du"], c["dir_dv"]])
+                              for c in cams]).astype(np.float32)
+        offs, ids = _rank_table(ranks)
+        ords = np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(V, len(ranks)))
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        h = C.c_void_p()
+        ctx._check(lib().aloha_batch_create(ctx._h, C.byref(params), W, H, V,
+                                            cam.ctypes.data_as(C.POINTER(C.c_float)), len(ranks),
+                                            offs, ids, ords.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            p(out_rgba), p(out_rgba8), p(nsamples), C.byref(h)))
+        self._h = h
+        self._keep = (out_rgba, out_rgba8, nsamples)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().aloha_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_schedule(self, tile_shape: int, persistent_blocks: int) -> None:
+        self.ctx._check(lib().aloha_batch_set_schedule(self._h, tile_shape, persistent_blocks))
+
+    def schedule(self):
+        t, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx._check(lib().aloha_batch_get_schedule(self._h, C.byref(t), C.byref(b), C.byref(c)))
+        return {"tile_shape": t.value, "persistent_blocks": b.value, "captured": bool(c.value)}
+
+    def autotune(self, reps: int = 3):
+        best = C.c_float()
+        table = (C.c_float * (3 * 64))()
+        n = C.c_int32()
+        self.ctx._check(lib().aloha_batch_autotune(self._h, reps, C.byref(best), table, 64,
+                                                   C.byref(n)))
+        rows = [(int(table[3 * i]), int(table[3 * i + 1]), float(table[3 * i + 2]))
+                for i in range(n.value)]
+        return best.value, rows
+
+    def capture(self) -> None:
+        self.ctx._check(lib().aloha_batch_capture(self._h))
+
+    def launch(self) -> None:
+        self.ctx._check(lib().aloha_batch_launch(self._h))

@@ -109,6 +109,14 @@ _SIGS = [
     ("aloha_diff_forward", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _FP, _I, _IP, _P, _P]),
     ("aloha_diff_backward", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _FP, _I, _IP, _P, _P,
                                       _P, _P, C.POINTER(C.c_void_p)]),
+    ("aloha_batch_create", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _FP, _I, _IP, _IP, _IP,
+                                     _P, _P, _P, C.POINTER(_P)]),
+    ("aloha_batch_destroy", C.c_int, [_P]),
+    ("aloha_batch_set_schedule", C.c_int, [_P, _I, _I]),
+    ("aloha_batch_get_schedule", C.c_int, [_P, _IP, _IP, _IP]),
+    ("aloha_batch_autotune", C.c_int, [_P, _I, _FP, _FP, _I, _IP]),
+    ("aloha_batch_capture", C.c_int, [_P]),
+    ("aloha_batch_launch", C.c_int, [_P]),
     ("aloha_render_footprint", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP,
                                          C.POINTER(C.c_uint64)]),
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),

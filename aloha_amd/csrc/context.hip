@@ -639,3 +639,245 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
   HIP_CHECK(ctx, launch_diff(a, true, ctx->stream));
   return ALOHA_OK;
 }
+
+// ---------------------------------------------------------------------------
+// batched form: many views of one rank table in one (graph-captured) launch
+// ---------------------------------------------------------------------------
+struct aloha_batch {
+  aloha_ctx *ctx = nullptr;
+  BatchArgs b;
+  float *d_cams = nullptr;
+  int32_t *d_orders = nullptr, *d_rects = nullptr;
+  uint32_t *d_counter = nullptr;
+  int tile_shape = 0;
+  int persistent_blocks = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;  // capture / replay stream (not the null stream)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+void batch_drop_graph(aloha_batch *bt) {
+  if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
+  bt->exec = nullptr;
+  bt->graph = nullptr;
+}
+
+void batch_tiles(aloha_batch *bt) {
+  const int bw = 2 * kBatchTileW[bt->tile_shape], bh = 2 * kBatchTileH[bt->tile_shape];
+  bt->b.tiles_x = (bt->b.m.W + bw - 1) / bw;
+  bt->b.tiles_y = (bt->b.m.H + bh - 1) / bh;
+  bt->b.total_items = bt->b.tiles_x * bt->b.tiles_y * bt->b.V;
+}
+}  // namespace
+
+extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                  int32_t H, int32_t V, const float *cams, int32_t P,
+                                  const int32_t *rank_offsets, const int32_t *brick_ids,
+                                  const int32_t *orders, float *d_out_rgba,
+                                  uint32_t *d_out_rgba8, uint32_t *d_nsamples,
+                                  aloha_batch **out) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_create: ctx is NULL");
+  if (!out || !cams || !orders || V < 1)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_create: NULL argument or V < 1");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // camera of view 0 stands in for build_march's camera check; views are validated below
+  const float saved[12] = {ctx->cam[0], ctx->cam[1], ctx->cam[2], ctx->cam[3], ctx->cam[4], ctx->cam[5],
+                           ctx->cam[6], ctx->cam[7], ctx->cam[8], ctx->cam[9], ctx->cam[10], ctx->cam[11]};
+  const bool had = ctx->has_cam;
+  for (int i = 0; i < 12 * V; ++i)
+    if (!std::isfinite(cams[i])) return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_create: non-finite camera");
+  std::memcpy(ctx->cam, cams, sizeof saved);
+  ctx->has_cam = true;
+  MarchArgs m;
+  int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, m);
+  std::memcpy(ctx->cam, saved, sizeof saved);
+  ctx->has_cam = had;
+  if (rc) return rc;
+  if (params->accum_id != 0 || (params->flags & (ALOHA_FLAG_HEATMAP | ALOHA_FLAG_CROSSHAIR)))
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_create: accum_id 0 and no heat map / crosshair only");
+  if ((int64_t)V * W * H > (int64_t)1 << 31)
+    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: %d views of %dx%d too large", V, W, H);
+  std::vector<int32_t> rects((size_t)V * P * 4);
+  for (int v = 0; v < V; ++v) {
+    std::vector<char> seen(P, 0);
+    for (int k = 0; k < P; ++k) {
+      const int o = orders[(size_t)v * P + k];
+      if (o < 0 || o >= P || seen[o])
+        return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_create: order of view %d is not a permutation", v);
+      seen[o] = 1;
+    }
+    for (int s = 0; s < P; ++s)
+      rank_rect(cams + 12 * v, ctx->bricks, brick_ids + rank_offsets[s],
+                rank_offsets[s + 1] - rank_offsets[s], W, H, &rects[((size_t)v * P + s) * 4]);
+  }
+  auto *bt = new aloha_batch;
+  bt->ctx = ctx;
+  std::memset(&bt->b, 0, sizeof bt->b);
+  bt->b.m = m;
+  bt->b.m.rgba = (float4 *)d_out_rgba;
+  bt->b.m.rgba8 = d_out_rgba8;
+  bt->b.m.nsamples = d_nsamples;
+  bt->b.V = V;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, ctx->device);
+  bt->cus = e == hipSuccess ? prop.multiProcessorCount : 256;
+  if (e == hipSuccess) e = hipMalloc(&bt->d_cams, sizeof(float) * 12 * (size_t)V);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_orders, sizeof(int32_t) * (size_t)V * P);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_rects, sizeof(int32_t) * rects.size());
+  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(bt->d_cams, cams, sizeof(float) * 12 * (size_t)V, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(bt->d_orders, orders, sizeof(int32_t) * (size_t)V * P, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(bt->d_rects, rects.data(), sizeof(int32_t) * rects.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    aloha_batch_destroy(bt);
+    return fail(ctx, ALOHA_ERR_HIP, "aloha_batch_create: %s", hipGetErrorString(e));
+  }
+  bt->b.cams = bt->d_cams;
+  bt->b.orders = bt->d_orders;
+  bt->b.rects = bt->d_rects;
+  bt->b.counter = bt->d_counter;
+  batch_tiles(bt);
+  *out = bt;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_batch_destroy(aloha_batch *bt) {
+  if (!bt) return ALOHA_OK;
+  (void)hipSetDevice(bt->ctx->device);
+  if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+  batch_drop_graph(bt);
+  for (void *p : {(void *)bt->d_cams, (void *)bt->d_orders, (void *)bt->d_rects, (void *)bt->d_counter})
+    if (p) (void)hipFree(p);
+  if (bt->ev_in) (void)hipEventDestroy(bt->ev_in);
+  if (bt->ev_out) (void)hipEventDestroy(bt->ev_out);
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  delete bt;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_batch_set_schedule(aloha_batch *bt, int32_t tile_shape,
+                                        int32_t persistent_blocks) {
+  if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_set_schedule: NULL");
+  if (tile_shape < 0 || tile_shape > 2 || persistent_blocks < 0 || persistent_blocks > (1 << 20))
+    return fail(bt->ctx, ALOHA_ERR_ARG, "aloha_batch_set_schedule: tile_shape 0..2, blocks >= 0");
+  if (tile_shape != bt->tile_shape || persistent_blocks != bt->persistent_blocks) batch_drop_graph(bt);
+  bt->tile_shape = tile_shape;
+  bt->persistent_blocks = persistent_blocks;
+  batch_tiles(bt);
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_batch_get_schedule(const aloha_batch *bt, int32_t *tile_shape,
+                                        int32_t *persistent_blocks, int32_t *captured) {
+  if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_get_schedule: NULL");
+  if (tile_shape) *tile_shape = bt->tile_shape;
+  if (persistent_blocks) *persistent_blocks = bt->persistent_blocks;
+  if (captured) *captured = bt->exec != nullptr;
+  return ALOHA_OK;
+}
+
+// Launch on the batch stream, ordered after earlier work on the ctx stream
+// and before later work on it.
+extern "C" int aloha_batch_launch(aloha_batch *bt) {
+  if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_launch: NULL");
+  aloha_ctx *ctx = bt->ctx;
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  HIP_CHECK(ctx, hipEventRecord(bt->ev_in, ctx->stream));
+  HIP_CHECK(ctx, hipStreamWaitEvent(bt->stream, bt->ev_in, 0));
+  if (bt->exec)
+    HIP_CHECK(ctx, hipGraphLaunch(bt->exec, bt->stream));
+  else
+    HIP_CHECK(ctx, launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream));
+  HIP_CHECK(ctx, hipEventRecord(bt->ev_out, bt->stream));
+  HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, bt->ev_out, 0));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_batch_capture(aloha_batch *bt) {
+  if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_capture: NULL");
+  aloha_ctx *ctx = bt->ctx;
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  batch_drop_graph(bt);
+  HIP_CHECK(ctx, hipStreamSynchronize(bt->stream));
+  HIP_CHECK(ctx, hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(bt->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(ctx, ALOHA_ERR_HIP, "aloha_batch_capture: %s",
+                hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  bt->graph = g;
+  e = hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    batch_drop_graph(bt);
+    return fail(ctx, ALOHA_ERR_HIP, "aloha_batch_capture: instantiate: %s", hipGetErrorString(e));
+  }
+  return ALOHA_OK;
+}
+
+// Times every schedule (3 tile shapes x {one block per item, persistent with
+// 2/4/8 blocks per CU}) over `reps` launches each and keeps the fastest.  The
+// schedule never changes the samples or their arithmetic, so the images do
+// not depend on the choice.  Synchronous.
+extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_ms,
+                                    float *table, int32_t table_cap, int32_t *table_n) {
+  if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_autotune: NULL");
+  aloha_ctx *ctx = bt->ctx;
+  if (reps < 1) return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_autotune: reps < 1");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  batch_drop_graph(bt);
+  hipEvent_t e0, e1;
+  HIP_CHECK(ctx, hipEventCreate(&e0));
+  HIP_CHECK(ctx, hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_shape = 0, best_blocks = 0, n = 0;
+  const int blocks_opts[4] = {0, 2, 4, 8};
+  for (int shape = 0; shape < 3; ++shape)
+    for (int bo : blocks_opts) {
+      bt->tile_shape = shape;
+      bt->persistent_blocks = bo ? bo * bt->cus : 0;
+      batch_tiles(bt);
+      hipError_t e = launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream);  // warm
+      if (e == hipSuccess) e = hipEventRecord(e0, bt->stream);
+      for (int r = 0; e == hipSuccess && r < reps; ++r)
+        e = launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, bt->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(ctx, ALOHA_ERR_HIP, "aloha_batch_autotune: %s", hipGetErrorString(e));
+      }
+      ms /= (float)reps;
+      if (table && n < table_cap) {
+        table[3 * n] = (float)shape;
+        table[3 * n + 1] = (float)bt->persistent_blocks;
+        table[3 * n + 2] = ms;
+      }
+      ++n;
+      if (ms < best) {
+        best = ms;
+        best_shape = shape;
+        best_blocks = bt->persistent_blocks;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  bt->tile_shape = best_shape;
+  bt->persistent_blocks = best_blocks;
+  batch_tiles(bt);
+  if (best_ms) *best_ms = best;
+  if (table_n) *table_n = n;
+  return ALOHA_OK;
+}

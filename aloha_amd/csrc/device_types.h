@@ -52,6 +52,19 @@ struct MarchArgs {
   int32_t order[ALOHA_MAX_RANKS];  // fused composite: rank slots front to back
 };
 
+// Batched form: V views of one rank table in one launch.
+struct BatchArgs {
+  MarchArgs m;              // ranks, bricks, TF, params; m.rgba/rgba8/nsamples are V x H x W
+  const float *cams;        // V x 12 (org, dir_00, dir_du, dir_dv)
+  const int32_t *orders;    // V x P rank slots, front to back
+  const int32_t *rects;     // V x P x 4 pixel rectangles
+  uint32_t *counter;        // persistent schedule: next work item
+  int32_t V, tiles_x, tiles_y, total_items;
+};
+// tile shapes: 0 = 8x8 pixels per wave, 1 = 16x4, 2 = 4x16 (blocks are 2x2 waves)
+constexpr int kBatchTileW[3] = {8, 16, 4};
+constexpr int kBatchTileH[3] = {8, 4, 16};
+
 // Differentiable form: one rank marching its bricks, seen from V views.
 constexpr int kMaxViews = 16;
 constexpr int kMaxDiffBricks = 64;
@@ -80,6 +93,7 @@ struct CompositeArgs {
 // launchers (march.hip, composite.hip)
 hipError_t launch_march(const MarchArgs &a, bool footprint, hipStream_t s);
 hipError_t launch_march_composite(const MarchArgs &a, hipStream_t s);
+hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_blocks, hipStream_t s);
 hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
                             int32_t px, int32_t py, int32_t pz, hipStream_t s);
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);

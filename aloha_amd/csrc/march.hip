@@ -85,10 +85,11 @@ __device__ __forceinline__ uint32_t pack8(float f) {
 
 // Primary ray of pixel (px,py): generateRay (deviceCode.cpp:94-107) at the
 // pixel centre (:365).
-__device__ __forceinline__ void pixel_ray(const MarchArgs &a, int px, int py, float d[3]) {
+__device__ __forceinline__ void pixel_ray(const float *dir00, const float *du, const float *dv,
+                                          int px, int py, float d[3]) {
   const float su = (float)px + .5f, sv = (float)py + .5f;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) d[k] = (a.dir00[k] + su * a.du[k]) + sv * a.dv[k];
+  for (int k = 0; k < 3; ++k) d[k] = (dir00[k] + su * du[k]) + sv * dv[k];
   const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -102,8 +103,8 @@ __device__ __forceinline__ void pixel_ray(const MarchArgs &a, int px, int py, fl
 // The RNG is seeded per (pixel, accumID) as the reference seeds it per launch.
 template <bool kFoot>
 __device__ __forceinline__ float4 rank_pixel(const MarchArgs &a, const float4 *s_tf, int slot,
-                                             const float d[3], uint32_t pixel, float &z,
-                                             uint32_t &ns) {
+                                             const float org[3], const float d[3],
+                                             uint32_t pixel, float &z, uint32_t &ns) {
   const int b0 = a.rank_off[slot], b1 = a.rank_off[slot + 1];
   uint32_t rng = rng_seed(pixel, (uint32_t)a.accum_id);
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -119,7 +120,7 @@ __device__ __forceinline__ float4 rank_pixel(const MarchArgs &a, const float4 *s
         const int bid = a.rank_bricks[k];  // k is wave-uniform: scalar loads
         const DevBrick &db = a.bricks[bid];
         float t0, t1, rep, tn;
-        if (!slab(a.org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
+        if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
         if (t0 > tmin) {
           rep = t0;
           tn = t0;
@@ -141,7 +142,7 @@ __device__ __forceinline__ float4 rank_pixel(const MarchArgs &a, const float4 *s
         bnear = bnear + tmin;
         const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
         const DevBrick db = a.bricks[hit];
-        integrate<kFoot>(a, s_tf, db, a.org, d, bt0, bfar, j, c, ns);
+        integrate<kFoot>(a, s_tf, db, org, d, bt0, bfar, j, c, ns);
         z = fminf(z, bnear / a.max_depth);
         tmin = bfar + 1e-20f;
       }
@@ -206,8 +207,8 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   uint32_t ns = 0;
   if (active && in_rect(a, slot, px, py) && island_pixel(a, px, py)) {
     float d[3];
-    pixel_ray(a, px, py, d);
-    acc = rank_pixel<kFoot>(a, s_tf, slot, d, (uint32_t)pixel, z, ns);
+    pixel_ray(a.dir00, a.du, a.dv, px, py, d);
+    acc = rank_pixel<kFoot>(a, s_tf, slot, a.org, d, (uint32_t)pixel, z, ns);
   }
   if (kFoot) return;
   if (a.depth) a.depth[out_idx] = z;
@@ -267,13 +268,13 @@ __global__ __launch_bounds__(kBlock) void march_composite_kernel(const MarchArgs
   uint32_t ns = 0;
   if (active && island_pixel(a, px, py)) {
     float d[3];
-    pixel_ray(a, px, py, d);
+    pixel_ray(a.dir00, a.du, a.dv, px, py, d);
     const float inv_spp = 1.f / (float)a.spp;
     for (int k = 0; k < a.P; ++k) {
       const int slot = a.order[k];  // wave-uniform
       if (!in_rect(a, slot, px, py)) continue;
       float z = 1.f;
-      const float4 acc = rank_pixel<false>(a, s_tf, slot, d, (uint32_t)pixel, z, ns);
+      const float4 acc = rank_pixel<false>(a, s_tf, slot, a.org, d, (uint32_t)pixel, z, ns);
       // rank image value (accum_id 0): (acc * 1) * (1/spp)
       const float4 c = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
                                    (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
@@ -289,6 +290,79 @@ __global__ __launch_bounds__(kBlock) void march_composite_kernel(const MarchArgs
     a.rgba8[pixel] = pack8(C.x) | (pack8(C.y) << 8) | (pack8(C.z) << 16) | (pack8(C.w) << 24);
   if (a.nsamples) a.nsamples[pixel] = ns;
 }
+
+// Batched form: V views of the same ranks in one launch.  A work item is
+// (view, block tile); each pixel marches the ranks in that view's visibility
+// order and blends them as march_composite_kernel does, so every view's
+// image equals its own fused (and plain) frame bit for bit.  The wave tile
+// shape (TW x TH pixels, TW*TH = 64) and persistence (a work counter that
+// blocks pull from until it runs out) are schedule-only choices.
+template <int TW, int TH>
+__device__ __forceinline__ void batch_item(const BatchArgs &b, const float4 *s_tf, int item) {
+  const MarchArgs &a = b.m;
+  constexpr int BW = 2 * TW, BH = 2 * TH;
+  const int per_view = b.tiles_x * b.tiles_y;
+  const int view = item / per_view;
+  const int tile = item - view * per_view;
+  const int bx = (tile % b.tiles_x) * BW, by = (tile / b.tiles_x) * BH;
+  const float *cam = b.cams + 12 * view;
+  const int32_t *order = b.orders + a.P * view;
+  const int32_t *rects = b.rects + 4 * a.P * view;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = bx + (wave & 1) * TW + (lane % TW);
+  const int py = by + (wave >> 1) * TH + (lane / TW);
+  if (px >= a.W || py >= a.H) return;
+  const int pixel = px + a.W * py;
+  float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t ns = 0;
+  if (island_pixel(a, px, py)) {
+    float d[3];
+    pixel_ray(cam + 3, cam + 6, cam + 9, px, py, d);
+    const float inv_spp = 1.f / (float)a.spp;
+    for (int k = 0; k < a.P; ++k) {
+      const int slot = order[k];  // wave-uniform
+      const int32_t *rc = rects + 4 * slot;
+      if (!(px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3])) continue;
+      float z = 1.f;
+      const float4 acc = rank_pixel<false>(a, s_tf, slot, cam, d, (uint32_t)pixel, z, ns);
+      const float4 c = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
+                                   (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
+      const float w = 1.f - C.w;
+      C.x = C.x + w * c.x;
+      C.y = C.y + w * c.y;
+      C.z = C.z + w * c.z;
+      C.w = C.w + w * c.w;
+    }
+  }
+  const size_t o = (size_t)view * a.W * a.H + pixel;
+  if (a.rgba) a.rgba[o] = C;
+  if (a.rgba8)
+    a.rgba8[o] = pack8(C.x) | (pack8(C.y) << 8) | (pack8(C.z) << 16) | (pack8(C.w) << 24);
+  if (a.nsamples) a.nsamples[o] = ns;
+}
+
+template <int TW, int TH, bool kPersistent>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
+  extern __shared__ float4 s_tf[];
+  load_tf(b.m, s_tf);
+  __syncthreads();
+  if (!kPersistent) {
+    batch_item<TW, TH>(b, s_tf, blockIdx.x);
+    return;
+  }
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter, 1u);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();  // every wave has read s_item before it is replaced
+    if (item >= b.total_items) break;
+    batch_item<TW, TH>(b, s_tf, item);
+  }
+}
+
+// Work-counter reset as a kernel node (graph-capturable without a memset node).
+__global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
 
 // Decode + pad one brick: dst element (i,j,k), i in [-1, n+2], reads the
 // clamp-to-edge source voxel.  u8/u16 decode as Volume::get (u/255.f).
@@ -315,6 +389,28 @@ __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dty
 hipError_t launch_march_composite(const MarchArgs &a, hipStream_t s) {
   const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, 1);
   march_composite_kernel<<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_blocks,
+                        hipStream_t s) {
+  const size_t lds = sizeof(float4) * (size_t)(b.m.tf_n + 4);
+  if (persistent_blocks > 0) {
+    reset_counter_kernel<<<1, 1, 0, s>>>(b.counter);
+    const dim3 grid(persistent_blocks);
+    switch (tile_shape) {
+      case 1: batch_kernel<16, 4, true><<<grid, kBlock, lds, s>>>(b); break;
+      case 2: batch_kernel<4, 16, true><<<grid, kBlock, lds, s>>>(b); break;
+      default: batch_kernel<8, 8, true><<<grid, kBlock, lds, s>>>(b); break;
+    }
+  } else {
+    const dim3 grid(b.total_items);
+    switch (tile_shape) {
+      case 1: batch_kernel<16, 4, false><<<grid, kBlock, lds, s>>>(b); break;
+      case 2: batch_kernel<4, 16, false><<<grid, kBlock, lds, s>>>(b); break;
+      default: batch_kernel<8, 8, false><<<grid, kBlock, lds, s>>>(b); break;
+    }
+  }
   return hipGetLastError();
 }
 

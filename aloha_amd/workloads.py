@@ -120,3 +120,61 @@ def diff256(size: int = 256, views: int = 8, W: int = 512, H: int = 512, seed: i
     target_tf = np.clip(tf * (1.0 + 0.2 * rng.uniform(-1, 1, tf.shape)), 0, 1).astype(np.float32)
     target_density = (density * (1.0 + 0.05 * rng.standard_normal(density.shape))).astype(np.float32)
     return DiffWorkload(density, tf, cams, W, H, target_tf, target_density)
+
+
+@dataclass
+class BatchWorkload:
+    """Uneven bricks on a lattice of cells (one brick per cell, at the cell's
+    lower corner) seen from V views; one virtual rank per brick, kd-assigned
+    over the cell boxes so middle planes never cut a brick."""
+    bricks: list
+    voxels: list
+    tf: np.ndarray
+    cams: list
+    orders: list
+    rank_bricks: list
+    W: int
+    H: int
+    dt: float = 0.5
+
+    def params(self, **kw):
+        base = dict(dt=self.dt, max_depth=1.0)
+        base.update(kw)
+        return render_params(**base)
+
+
+def batched(n_side: int = 4, cell: int = 256, smin: int = 32, views: int = 16, W: int = 768,
+            H: int = 768, seed: int = 0x5EED5) -> BatchWorkload:
+    rng = np.random.default_rng(seed)
+    bricks, voxels, cells = [], [], []
+    for i in range(n_side ** 3):
+        c = (i % n_side, (i // n_side) % n_side, i // (n_side * n_side))
+        dims = tuple(int(v) for v in rng.integers(smin, cell + 1, 3))
+        lo = tuple(c[a] * cell for a in range(3))
+        d = BrickDesc()
+        for a in range(3):
+            d.cell_range.lower[a] = d.voxel_range.lower[a] = lo[a]
+            d.cell_range.upper[a] = d.voxel_range.upper[a] = lo[a] + dims[a]
+            d.space_range.lower[a] = float(lo[a])
+            d.space_range.upper[a] = float(lo[a] + dims[a])
+        if rng.random() < 0.5:
+            v = synth_blobs_u8(dims, 16, seed + 100 + i, 4.0, 16.0, 0.3, 1.0)
+        else:
+            v = synth_blobs_f32(dims, 16, seed + 100 + i, 4.0, 16.0, 0.3, 1.0)
+        d.value_range.lower, d.value_range.upper = 0.0, 1.0
+        bricks.append(d)
+        voxels.append(v)
+        cb = Box3f()
+        for a in range(3):
+            cb.lower[a], cb.upper[a] = float(lo[a]), float(lo[a] + cell)
+        cells.append(cb)
+    P = len(bricks)
+    part = Partitioner.kd(cells, P)
+    rank_bricks = [part.rank_bricks(r) for r in range(P)]
+    bounds = Box3f()
+    for a in range(3):
+        bounds.lower[a], bounds.upper[a] = 0.0, float(n_side * cell)
+    cams = [synth_camera(bounds, W, H, 45.0, 2.5, seed + 10 + v) for v in range(views)]
+    orders = [part.composite_order(c["org"]) for c in cams]
+    return BatchWorkload(bricks, voxels, synth_tf(256, 6, seed + 1), cams, orders, rank_bricks,
+                         W, H)

@@ -45,6 +45,7 @@ def parse():
                     help="few frames, no CPU baseline (for rocprofv3 runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable form")
     ap.add_argument("--adam-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     return ap.parse_args()
 
 
@@ -189,6 +190,9 @@ def main():
     adjoint = None
     if not args.no_adjoint and not args.profile:
         adjoint = adjoint_bench(args, timed)
+    batched = None
+    if not args.no_batch and not args.profile:
+        batched = batch_bench(args, timed)
 
     res = {
         "metric": METRIC,
@@ -216,6 +220,7 @@ def main():
                                        "bit_identical_to_plain": fused_equal},
         },
         "adjoint": adjoint,
+        "batched": batched,
         "cpu_baseline": cpu,
         "parity_sample": parity,
     }
@@ -277,6 +282,81 @@ def adjoint_bench(args, timed):
             "fwd_adjoint_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
             "adam_steps": args.adam_steps, "loss_first": round(losses[0], 4),
             "loss_last": round(losses[-1], 4)}
+
+
+def batch_bench(args, timed):
+    """Batched form: 64 uneven u8/f32 bricks (axes 32..256) on a 4x4x4 lattice,
+    one virtual rank each, 16 views of 768x768 with per-view sort-last
+    compositing, in one autotuned, hipGraph-captured launch -- against the
+    plain per-view sequence (render_ranks over 64 slots + composite per view)
+    and the per-brick launch sequence (one render launch per brick per view)."""
+    import torch
+    import aloha_amd as A
+    from aloha_amd import RankImages
+    from aloha_amd import workloads as WL
+    t0 = time.time()
+    bw = WL.batched()
+    ctx = A.Context(0, stream=torch.cuda.current_stream())
+    ids = [ctx.upload_brick(d, v) for d, v in zip(bw.bricks, bw.voxels)]
+    ctx.set_tf(bw.tf)
+    ranks = [[ids[i] for i in r] for r in bw.rank_bricks]
+    V, W, H, P = len(bw.cams), bw.W, bw.H, len(ranks)
+    log(f"[bench] batched workload built in {time.time() - t0:.1f}s "
+        f"({sum(v.size for v in bw.voxels) / 1e6:.0f} M voxels)")
+    out = torch.empty((V, H, W, 4), dtype=torch.float32, device="cuda")
+    out8 = torch.empty((V, H, W), dtype=torch.int32, device="cuda")
+    ns = torch.empty((V, H, W), dtype=torch.int32, device="cuda")
+    b = A.Batch(ctx, bw.params(), W, H, bw.cams, ranks, bw.orders, out, out8, ns)
+    b.launch()
+    torch.cuda.synchronize()
+    samples = int(ns.to(torch.int64).sum())
+    best, table = b.autotune(reps=3)
+    sched = b.schedule()
+    b.capture()
+    n = max(3, min(args.steps, 10))
+    ms_batch = timed(b.launch, n)
+    # plain per-view sequence
+    imgs = RankImages.alloc(P, W, H, rgba=True, rgba8=False, depth=False)
+    ref = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    p = bw.params()
+
+    def per_view():
+        for cam, order in zip(bw.cams, bw.orders):
+            ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+            ctx.render_ranks(p, W, H, ranks, imgs)
+            ctx.composite_over(imgs.rgba, order, W, H, ref, None)
+
+    per_view()
+    ms_views = timed(per_view, max(2, n // 3))
+    equal = bool(torch.equal(ref, out[V - 1]))
+    # per-brick launch sequence: one render launch per (view, brick)
+    one = RankImages.alloc(1, W, H, rgba=True, rgba8=False, depth=False)
+
+    def per_brick():
+        for cam, order in zip(bw.cams, bw.orders):
+            ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+            for s in range(P):
+                sub = RankImages(rgba=imgs.rgba[s:s + 1])
+                ctx.render_ranks(p, W, H, [ranks[s]], sub)
+            ctx.composite_over(imgs.rgba, order, W, H, ref, None)
+
+    per_brick()
+    ms_bricks = timed(per_brick, 2)
+    del one
+    log(f"[bench] batched: graph {ms_batch:.3f} ms ({sched}), per-view sequence {ms_views:.3f} ms, "
+        f"per-brick sequence {ms_bricks:.3f} ms, equal {equal}")
+    b.close()
+    ctx.close()
+    return {"workload": "64 bricks (axes U[32,256], u8/f32), 16 views 768x768, per-view kd order",
+            "samples": samples, "ms_graph_launch": round(ms_batch, 4),
+            "Gsamples/s": round(samples / (ms_batch * 1e-3) / 1e9, 3),
+            "ms_per_view_sequence": round(ms_views, 4), "ms_per_brick_sequence": round(ms_bricks, 4),
+            "speedup_vs_per_view": round(ms_views / ms_batch, 2),
+            "speedup_vs_per_brick": round(ms_bricks / ms_batch, 2),
+            "autotune": {"chosen": sched, "best_ms": round(best, 4),
+                         "table": [{"tile_shape": t, "persistent_blocks": pb, "ms": round(m, 4)}
+                                   for t, pb, m in table]},
+            "last_view_equal_to_per_view_sequence": equal}
 
 
 def cpu_baseline(wl, imgs, seconds):

@@ -241,6 +241,35 @@ int aloha_composite_zmin(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, const flo
                          float *d_out_depth);
 
 /* ---------------------------------------------------------------------- */
+/* batched form (no reference counterpart: many views of the same virtual  */
+/* ranks -- renderFrame + the IceT blend per view -- in one launch)         */
+/* ---------------------------------------------------------------------- */
+typedef struct aloha_batch aloha_batch;
+/* V views (cams: V x 12 floats, host), one rank table, per-view visibility
+ * orders (V x P, host).  Outputs (device, V*H*W; each may be NULL): float
+ * RGBA, RGBA8, sample counts -- each view's image equals its
+ * aloha_render_composite frame bit for bit.  Same params limits as that call. */
+int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                       int32_t V, const float *cams, int32_t P, const int32_t *rank_offsets,
+                       const int32_t *brick_ids, const int32_t *orders, float *d_out_rgba,
+                       uint32_t *d_out_rgba8, uint32_t *d_nsamples, aloha_batch **out);
+int aloha_batch_destroy(aloha_batch *batch);
+/* Schedule: tile_shape 0 = 8x8 pixels per wave, 1 = 16x4, 2 = 4x16;
+ * persistent_blocks 0 = one workgroup per work item, else that many
+ * workgroups pulling (view, tile) items from a counter.  Drops a captured graph. */
+int aloha_batch_set_schedule(aloha_batch *batch, int32_t tile_shape, int32_t persistent_blocks);
+int aloha_batch_get_schedule(const aloha_batch *batch, int32_t *tile_shape,
+                             int32_t *persistent_blocks, int32_t *captured);
+/* Time every schedule over reps launches, keep the fastest; table (optional)
+ * receives {tile_shape, persistent_blocks, ms} per candidate.  Synchronous. */
+int aloha_batch_autotune(aloha_batch *batch, int32_t reps, float *best_ms, float *table,
+                         int32_t table_cap, int32_t *table_n);
+/* Capture one launch (counter reset + batch kernel) into a hipGraph. */
+int aloha_batch_capture(aloha_batch *batch);
+/* Run the batch (the graph if captured), ordered on the context stream. */
+int aloha_batch_launch(aloha_batch *batch);
+
+/* ---------------------------------------------------------------------- */
 /* inputs from device memory (training loops update them every step)      */
 /* ---------------------------------------------------------------------- */
 /* TF from n device-resident RGBA float entries; asynchronous on the stream. */
