@@ -23,7 +23,7 @@ __all__ = [
     "split_volume", "volume_extract", "synth_blobs_u8", "synth_blobs_f32", "synth_fbm_f32",
     "synth_tf", "synth_camera", "max_depth", "U8", "U16", "F32", "JITTER_RANDOM",
     "JITTER_FIXED", "FLAG_CROSSHAIR", "FLAG_HEATMAP", "MAX_RANKS", "MAX_RANK_BRICKS", "MAX_TF",
-    "vol_write", "vol_read",
+    "vol_write", "vol_read", "Mesh", "mesh_color_ids",
 ]
 
 
@@ -349,6 +349,162 @@ class Context:
             self._h, C.c_void_p(rank_rgba8.data_ptr()), C.c_void_p(rank_depth.data_ptr()), P, W, H,
             C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None,
             C.c_void_p(out_depth.data_ptr()) if out_depth is not None else None))
+
+
+    # ---- surface path ---------------------------------------------------
+    def upload_tris(self, mesh: "Mesh", ranks: Sequence[Sequence[int]],
+                    colors: Sequence[Sequence[int]] | None = None) -> list[int]:
+        """Upload each rank's clusters (flattened, with a BVH); colors: per
+        rank, the colour id of each listed cluster (default mesh_color_ids)."""
+        P = len(ranks)
+        if colors is None:
+            colors = mesh_color_ids(ranks, mesh.num_clusters)
+        offs, ids = _rank_table(ranks)
+        col = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.uint32) for c in colors])
+                                   if sum(len(c) for c in colors) else np.zeros(1, np.uint32))
+        out = (C.c_int32 * P)()
+        self._check(lib().aloha_tris_upload(self._h, mesh._h, P, offs, ids, col.ctypes.data, out))
+        return list(out)
+
+    def tris_info(self, geom: int) -> dict:
+        nt, nn, dp, nb = C.c_int64(), C.c_int64(), C.c_int32(), C.c_uint64()
+        self._check(lib().aloha_tris_info(self._h, geom, C.byref(nt), C.byref(nn), C.byref(dp),
+                                          C.byref(nb)))
+        return {"ntris": nt.value, "nnodes": nn.value, "depth": dp.value, "bytes": nb.value}
+
+    def tris_export(self, geom: int):
+        """(nodes (n,16) raw 4-byte words as uint32, tris (ntris,12) float32, root)."""
+        info = self.tris_info(geom)
+        nodes = np.zeros((max(info["nnodes"], 1), 16), dtype=np.uint32)
+        tris = np.zeros((max(info["ntris"], 1), 12), dtype=np.float32)
+        root = C.c_int32()
+        self._check(lib().aloha_tris_export(self._h, geom, nodes.ctypes.data, tris.ctypes.data,
+                                            C.byref(root)))
+        return nodes[:info["nnodes"]], tris[:info["ntris"]], root.value
+
+    def render_tris(self, params: RenderParams, W: int, H: int, geoms: Sequence[int],
+                    out: RankImages, stats=None) -> None:
+        g = (C.c_int32 * len(geoms))(*geoms)
+        imgs = out.as_c()
+        self._check(lib().aloha_render_tris(self._h, C.byref(params), W, H, len(geoms), g,
+                                            C.byref(imgs),
+                                            C.c_void_p(stats.data_ptr()) if stats is not None else None))
+
+    def render_tris_composite(self, params: RenderParams, W: int, H: int, geoms: Sequence[int],
+                              out_rgba8=None, out_depth=None, stats=None) -> None:
+        g = (C.c_int32 * len(geoms))(*geoms)
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        self._check(lib().aloha_render_tris_composite(self._h, C.byref(params), W, H, len(geoms), g,
+                                                      p(out_rgba8), p(out_depth), p(stats)))
+
+
+# ----------------------------------------------------------------------------
+# triangle meshes (host)
+# ----------------------------------------------------------------------------
+def mesh_color_ids(ranks: Sequence[Sequence[int]], num_clusters: int) -> list[list[int]]:
+    """ClusterID each rank gives its i-th geometry for randomColor
+    (waikikiRenderer.cpp:554-557): rank * (numClusters / numRanks) + i, with
+    integer division (the increment at :588 changes a dead loop-local copy)."""
+    per = num_clusters // len(ranks)
+    return [[r * per + i for i in range(len(cl))] for r, cl in enumerate(ranks)]
+
+
+class Mesh:
+    """Host triangle mesh (aloha_mesh): vertices, triangles, clusters."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def from_arrays(cls, verts: np.ndarray, tris: np.ndarray) -> "Mesh":
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        check(lib().aloha_mesh_create(v.ctypes.data, len(v), t.ctypes.data, len(t), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def spheres(cls, num_spheres: int, rings: int, segments: int, domain: Box3f,
+                rmin_frac=0.005, rmax_frac=0.03, disp_frac=0.1, seed=0x5EED4) -> "Mesh":
+        h = C.c_void_p()
+        check(lib().aloha_mesh_synth_spheres(num_spheres, rings, segments, C.byref(domain),
+                                             rmin_frac, rmax_frac, disp_frac, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read_tri(cls, path: str) -> "Mesh":
+        h = C.c_void_p()
+        check(lib().aloha_tri_read(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    def write_tri(self, path: str) -> None:
+        check(lib().aloha_tri_write(str(path).encode(), self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().aloha_mesh_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def info(self) -> dict:
+        nv, nt, nc = C.c_int64(), C.c_int64(), C.c_int32()
+        b = Box3f()
+        check(lib().aloha_mesh_info(self._h, C.byref(nv), C.byref(nt), C.byref(b), C.byref(nc)))
+        return {"nverts": nv.value, "ntris": nt.value, "bounds": b, "nclusters": nc.value}
+
+    @property
+    def num_clusters(self) -> int:
+        return max(1, self.info()["nclusters"])
+
+    def arrays(self):
+        i = self.info()
+        v = np.empty((i["nverts"], 3), dtype=np.float32)
+        t = np.empty((i["ntris"], 3), dtype=np.int32)
+        check(lib().aloha_mesh_get(self._h, v.ctypes.data, t.ctypes.data))
+        return v, t
+
+    def split(self, num_clusters: int) -> int:
+        n = C.c_int32()
+        check(lib().aloha_mesh_split(self._h, num_clusters, C.byref(n)))
+        return n.value
+
+    def clusters(self):
+        """[(first, last, Box3f)] triangle ranges and split domains."""
+        n = C.c_int32()
+        cap = max(1, self.info()["nclusters"])
+        r = np.zeros((cap, 2), dtype=np.int64)
+        b = (Box3f * cap)()
+        check(lib().aloha_mesh_clusters(self._h, r.ctypes.data, b, cap, C.byref(n)))
+        return [(int(r[i, 0]), int(r[i, 1]), b[i]) for i in range(n.value)]
+
+    def build_bvh(self, cluster_ids: Sequence[int], colors: Sequence[int] | None = None):
+        """Host-only build of one rank's device layout (aloha_tris_build_host):
+        (nodes (n,16) uint32 words, tris (m,12) float32, root, depth)."""
+        ids = (C.c_int32 * max(1, len(cluster_ids)))(*cluster_ids)
+        col = np.ascontiguousarray(colors if colors is not None else np.zeros(len(cluster_ids)),
+                                   dtype=np.uint32)
+        if col.size == 0:
+            col = np.zeros(1, np.uint32)
+        nt, nn, dp, root = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        check(lib().aloha_tris_build_host(self._h, len(cluster_ids), ids, col.ctypes.data,
+                                          C.byref(nt), C.byref(nn), C.byref(dp), None, None,
+                                          C.byref(root)))
+        nodes = np.zeros((max(nn.value, 1), 16), np.uint32)
+        tris = np.zeros((max(nt.value, 1), 12), np.float32)
+        check(lib().aloha_tris_build_host(self._h, len(cluster_ids), ids, col.ctypes.data,
+                                          C.byref(nt), C.byref(nn), C.byref(dp), nodes.ctypes.data,
+                                          tris.ctypes.data, C.byref(root)))
+        return nodes[:nn.value], tris[:nt.value], root.value, dp.value
+
+    def rank_tri9(self, cluster_ids: Sequence[int]) -> np.ndarray:
+        """A rank's flattened triangles (ntris, 9) in list order, as the
+        upload flattens them (primitive id = row)."""
+        v, t = self.arrays()
+        cl = self.clusters() or [(0, len(t), None)]
+        rows = [t[cl[c][0]:cl[c][1]] for c in cluster_ids]
+        idx = np.concatenate(rows) if rows else np.zeros((0, 3), np.int32)
+        return v[idx].reshape(-1, 9)
 
 
 class Batch:

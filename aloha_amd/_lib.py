@@ -122,6 +122,24 @@ _SIGS = [
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),
     ("aloha_composite_over_rgba8", C.c_int, [_P, _P, _I, _IP, _I, _I, _P]),
     ("aloha_composite_zmin", C.c_int, [_P, _P, _P, _I, _I, _I, _P, _P]),
+    ("aloha_mesh_create", C.c_int, [_P, C.c_int64, _P, C.c_int64, C.POINTER(_P)]),
+    ("aloha_mesh_destroy", None, [_P]),
+    ("aloha_mesh_info", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(Box3f), _IP]),
+    ("aloha_mesh_get", C.c_int, [_P, _P, _P]),
+    ("aloha_mesh_clusters", C.c_int, [_P, _P, _P, _I, _IP]),
+    ("aloha_mesh_synth_spheres", C.c_int, [_I, _I, _I, C.POINTER(Box3f), C.c_float, C.c_float, C.c_float,
+                                           C.c_uint64, C.POINTER(_P)]),
+    ("aloha_mesh_split", C.c_int, [_P, _I, _IP]),
+    ("aloha_tri_write", C.c_int, [C.c_char_p, _P]),
+    ("aloha_tri_read", C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    ("aloha_tris_upload", C.c_int, [_P, _P, _I, _IP, _IP, _P, _IP]),
+    ("aloha_tris_build_host", C.c_int, [_P, _I, _IP, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _IP,
+                                        _P, _P, _IP]),
+    ("aloha_tris_info", C.c_int, [_P, _I, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _IP,
+                                  C.POINTER(C.c_uint64)]),
+    ("aloha_tris_export", C.c_int, [_P, _I, _P, _P, _IP]),
+    ("aloha_render_tris", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, C.POINTER(Images), _P]),
+    ("aloha_render_tris_composite", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _P, _P, _P]),
 ]
 
 # every symbol include/aloha_amd.h declares

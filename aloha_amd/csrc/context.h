@@ -1,0 +1,58 @@
+// context.h -- the device context record behind the C ABI, shared by the
+// volume entry points (context.hip) and the surface entry points
+// (context_tris.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device_types.h"
+#include "internal.h"
+
+// One rank's triangle geometry in HBM: BVH nodes (64 B each) and triangles
+// in BVH leaf order (three float4 vertices, 48 B each).
+struct TriGeom {
+  float4 *d_nodes = nullptr;
+  float4 *d_tris = nullptr;
+  int32_t root = 0, depth = 0;
+  int64_t ntris = 0, nnodes = 0;
+  uint64_t bytes = 0;
+};
+
+struct aloha_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<aloha::DevBrick> bricks;
+  std::vector<float *> brick_mem;
+  std::vector<aloha_brick_desc> descs;
+  aloha::DevBrick *d_bricks = nullptr;
+  size_t d_bricks_cap = 0;
+  bool bricks_dirty = true;
+  uint64_t brick_bytes = 0;
+  float4 *d_tf = nullptr;
+  int tf_n = 0, tf_cap = 0;
+  float cam[12] = {};
+  bool has_cam = false;
+  std::vector<TriGeom> geoms;
+  std::string err;
+};
+
+namespace aloha {
+
+// Record a message on ctx (and as the thread's global error); returns code.
+int fail(aloha_ctx *ctx, int code, const char *fmt, ...);
+// Validate the render parameters every render entry point shares.
+int check_params(aloha_ctx *ctx, const aloha_render_params *p);
+
+}  // namespace aloha
+
+#define HIP_CHECK(ctx, call)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ::aloha::fail(ctx, ALOHA_ERR_HIP, "%s: %s (%s)", __func__, #call,            \
+                           hipGetErrorString(e_));                                        \
+  } while (0)

@@ -17,30 +17,11 @@
 #include <string>
 #include <vector>
 
-#include "device_types.h"
-#include "internal.h"
+#include "context.h"
 
 using namespace aloha;
 
-struct aloha_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  std::vector<DevBrick> bricks;
-  std::vector<float *> brick_mem;
-  std::vector<aloha_brick_desc> descs;
-  DevBrick *d_bricks = nullptr;
-  size_t d_bricks_cap = 0;
-  bool bricks_dirty = true;
-  uint64_t brick_bytes = 0;
-  float4 *d_tf = nullptr;
-  int tf_n = 0, tf_cap = 0;
-  float cam[12] = {};
-  bool has_cam = false;
-  std::string err;
-};
-
-namespace {
+namespace aloha {
 
 int fail(aloha_ctx *ctx, int code, const char *fmt, ...) {
   char buf[512];
@@ -53,12 +34,9 @@ int fail(aloha_ctx *ctx, int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIP_CHECK(ctx, call)                                                              \
-  do {                                                                                    \
-    hipError_t e_ = (call);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(ctx, ALOHA_ERR_HIP, "%s: %s (%s)", __func__, #call, hipGetErrorString(e_)); \
-  } while (0)
+}  // namespace aloha
+
+namespace {
 
 bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
@@ -78,7 +56,9 @@ int sync_bricks(aloha_ctx *ctx) {
   return ALOHA_OK;
 }
 
-int check_params(aloha_ctx *ctx, const aloha_render_params *p) {
+}  // namespace
+
+int aloha::check_params(aloha_ctx *ctx, const aloha_render_params *p) {
   if (!p) return fail(ctx, ALOHA_ERR_ARG, "render: params is NULL");
   if (!(p->dt > 0.f) || !std::isfinite(p->dt))
     return fail(ctx, ALOHA_ERR_ARG, "render: dt must be finite and > 0 (got %g)", p->dt);
@@ -95,6 +75,8 @@ int check_params(aloha_ctx *ctx, const aloha_render_params *p) {
     return fail(ctx, ALOHA_ERR_ARG, "render: jitter_value must be in [0,1)");
   return ALOHA_OK;
 }
+
+namespace {
 
 // Conservative pixel rectangle of the bricks bricks[ids[0..n)] as seen from
 // the camera: a primary ray through pixel (x,y) has direction
@@ -243,6 +225,10 @@ extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
   for (float *m : ctx->brick_mem) hipFree(m);
   if (ctx->d_bricks) hipFree(ctx->d_bricks);
   if (ctx->d_tf) hipFree(ctx->d_tf);
+  for (TriGeom &g : ctx->geoms) {
+    if (g.d_nodes) hipFree(g.d_nodes);
+    if (g.d_tris) hipFree(g.d_tris);
+  }
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return ALOHA_OK;

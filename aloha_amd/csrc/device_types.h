@@ -90,6 +90,29 @@ struct CompositeArgs {
   int32_t order[ALOHA_MAX_RANKS];
 };
 
+// Surface path: one rank's triangle geometry (BVH nodes of 4 float4, see
+// TriNode; triangles of 3 float4 in leaf order, w = primitive id / colour id
+// bits) and the launch arguments of all ranks.
+constexpr int kTriStack = 32;  // = kMaxBvhDepth: deepest leaf bounds the stack
+struct DevTris {
+  const float4 *nodes;
+  const float4 *tris;
+  int32_t root, pad;
+};
+
+struct TrisArgs {
+  float org[3], dir00[3], du[3], dv[3];
+  float max_depth, heat_scale;
+  int32_t W, H, spp, accum_id;
+  int32_t island_id, num_islands, flags, P;
+  float4 *rgba;
+  uint32_t *rgba8;
+  float *depth;
+  float4 *accum;
+  uint32_t *stats;  // 2 per pixel: nodes visited, triangles tested
+  DevTris geo[ALOHA_MAX_RANKS];
+};
+
 // launchers (march.hip, composite.hip)
 hipError_t launch_march(const MarchArgs &a, bool footprint, hipStream_t s);
 hipError_t launch_march_composite(const MarchArgs &a, hipStream_t s);
@@ -103,5 +126,7 @@ hipError_t launch_composite_over_rgba8(const uint32_t *imgs, const CompositeArgs
                                        uint32_t *out8, hipStream_t s);
 hipError_t launch_composite_zmin(const uint32_t *imgs, const float *depth, int32_t P,
                                  int64_t npix, uint32_t *out8, float *outz, hipStream_t s);
+// tris.hip: per-rank images (composite = false) or the fused depth composite
+hipError_t launch_tris(const TrisArgs &a, bool composite, hipStream_t s);
 
 }  // namespace aloha

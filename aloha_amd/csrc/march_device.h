@@ -226,5 +226,40 @@ __device__ __forceinline__ bool slab(const float o[3], const float d[3], float t
   return t0 < t1;
 }
 
+// ---- per-pixel helpers shared by the volume and surface raygens ----
+
+__device__ __forceinline__ float sat(float f) { return fminf(fmaxf(f, 0.f), 1.f); }
+
+// heat-map ramp (deviceCode.cpp:300-320)
+__device__ inline float3 heat_map(float t) {
+  const float K = 4.0f / 6.0f;
+  const float h = K - K * t;
+  const float v = .5f + 0.5f * t;
+  const float s = sat(h) * 6.0f;
+  const float r = sat(fabsf(s - 3.f) - 1.0f);
+  const float g = sat(2.0f - fabsf(s - 2.0f));
+  const float b = sat(2.0f - fabsf(s - 4.0f));
+  return make_float3(v * r, v * g, v * b);
+}
+
+__device__ __forceinline__ uint32_t pack8(float f) {
+  return (uint32_t)fminf(fmaxf(f * 256.f, 0.f), 255.f);
+}
+
+// Primary ray of pixel (px,py): generateRay (deviceCode.cpp:94-107) at the
+// pixel centre (:365).
+__device__ __forceinline__ void pixel_ray(const float *dir00, const float *du, const float *dv,
+                                          int px, int py, float d[3]) {
+  const float su = (float)px + .5f, sv = (float)py + .5f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = (dir00[k] + su * du[k]) + sv * dv[k];
+  const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    d[k] = d[k] / len;
+    if (fabsf(d[k]) < 1e-5f) d[k] = 1e-5f;
+  }
+}
+
 }  // namespace dev
 }  // namespace aloha

@@ -7,6 +7,7 @@ camera -- everything Renderer's constructor and event handlers would hold
 
   blobs128   128^3 u8 Gaussian-blob volume, 2 bricks, 512^2 (the CPU-mode case)
   fbm512     512^3 f32 fBm volume, 8 bricks (2x2x2), 1024^2 (the GPU headline)
+  surface    4096 displaced spheres (9.75 M triangles), 8 clusters / ranks, 1920x1080
 """
 from __future__ import annotations
 
@@ -178,3 +179,44 @@ def batched(n_side: int = 4, cell: int = 256, smin: int = 32, views: int = 16, W
     orders = [part.composite_order(c["org"]) for c in cams]
     return BatchWorkload(bricks, voxels, synth_tf(256, 6, seed + 1), cams, orders, rank_bricks,
                          W, H)
+
+
+@dataclass
+class SurfaceWorkload:
+    """Triangle-mesh scene as Renderer holds it after loading a .tri file in
+    TriMesh mode (waikikiRenderer.cpp:390-593): clusters from MeshSplitter,
+    blocks of clusters per rank (partitionRoundRobin), colour ids per
+    rank-local geometry, camera and maxDepth."""
+    name: str
+    mesh: object                          # aloha_amd.Mesh (split)
+    ranks: list                           # cluster ids per rank
+    colors: list                          # colour id per listed cluster
+    cam: dict
+    W: int
+    H: int
+    max_depth: float
+    ntris: int
+
+    def params(self, **kw):
+        return render_params(max_depth=self.max_depth, **kw)
+
+
+def surface(num_spheres: int = 4096, rings: int = 35, segments: int = 35, num_clusters: int = 8,
+            num_ranks: int = 8, W: int = 1920, H: int = 1080, seed: int = 0x5EED6,
+            dist_factor: float = 1.2, name: str = "spheres", rmin: float = 0.005,
+            rmax: float = 0.03) -> SurfaceWorkload:
+    """Seeded displaced spheres in the unit cube (radius 0.005-0.03, radial
+    noise 10 % of the radius), split into clusters, one block per rank.
+    Defaults: 4096 spheres x 2380 triangles = 9.75 M triangles, 8 clusters,
+    8 ranks, 1920x1080, camera 1.2 diagonals from the centre."""
+    from . import Mesh, mesh_color_ids
+    dom = Box3f((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    mesh = Mesh.spheres(num_spheres, rings, segments, dom, rmin, rmax, 0.1, seed=seed)
+    n = mesh.split(num_clusters)
+    rr = Partitioner.round_robin(n, num_ranks)
+    ranks = [rr.rank_bricks(r) for r in range(num_ranks)]
+    info = mesh.info()
+    b = info["bounds"]
+    cam = synth_camera(b, W, H, 45.0, dist_factor, seed=seed + 1)
+    return SurfaceWorkload(name, mesh, ranks, mesh_color_ids(ranks, n), cam, W, H,
+                           max_depth(b, cam["org"]), info["ntris"])

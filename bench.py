@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable form")
     ap.add_argument("--adam-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
+    ap.add_argument("--no-surface", action="store_true", help="skip the triangle surface path")
     return ap.parse_args()
 
 
@@ -193,6 +194,9 @@ def main():
     batched = None
     if not args.no_batch and not args.profile:
         batched = batch_bench(args, timed)
+    surface = None
+    if not args.no_surface:
+        surface = surface_bench(args, timed)
 
     res = {
         "metric": METRIC,
@@ -221,6 +225,7 @@ def main():
         },
         "adjoint": adjoint,
         "batched": batched,
+        "surface": surface,
         "cpu_baseline": cpu,
         "parity_sample": parity,
     }
@@ -274,7 +279,7 @@ def adjoint_bench(args, timed):
         opt.step()
         with torch.no_grad():
             tfp.clamp_(0, 1)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     log(f"[bench] Adam {args.adam_steps} steps: loss {losses[0]:.3f} -> {losses[-1]:.3f} "
         f"in {time.time() - t0:.1f}s")
     return {"workload": "256^3 f32 blob density, 256-entry TF, 8 views 512x512, dt 0.5",
@@ -357,6 +362,114 @@ def batch_bench(args, timed):
                          "table": [{"tile_shape": t, "persistent_blocks": pb, "ms": round(m, 4)}
                                    for t, pb, m in table]},
             "last_view_equal_to_per_view_sequence": equal}
+
+
+def surface_bench(args, timed):
+    """Triangle surface path: 4096 displaced spheres (9.75 M triangles),
+    MeshSplitter into 8 clusters, one per virtual rank, 1920x1080.  One frame
+    = per-rank RGBA8 + depth images (one launch, BVH traversal) + depth
+    composite (one launch); the fused form does both in one launch.  Rays
+    count one primary ray per pixel per rank."""
+    import torch
+    import aloha_amd as A
+    from aloha_amd import RankImages
+    from aloha_amd import workloads as WL
+    t0 = time.time()
+    sw = WL.surface()
+    t_host = time.time() - t0
+    stream = torch.cuda.current_stream()
+    ctx = A.Context(0, stream=stream)
+    ctx.set_camera(sw.cam["org"], sw.cam["dir_00"], sw.cam["dir_du"], sw.cam["dir_dv"])
+    t0 = time.time()
+    geoms = ctx.upload_tris(sw.mesh, sw.ranks, sw.colors)
+    t_up = time.time() - t0
+    info = [ctx.tris_info(g) for g in geoms]
+    log(f"[bench] surface: {sw.ntris} triangles, scene {t_host:.1f}s, BVH build + upload "
+        f"{t_up:.1f}s, depth {max(i['depth'] for i in info)}")
+    P, W, H = len(geoms), sw.W, sw.H
+    p = sw.params()
+    imgs = RankImages.alloc(P, W, H, rgba=False, rgba8=True, depth=True)
+    out8 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    outz = torch.empty((H, W), dtype=torch.float32, device="cuda")
+
+    def frame():
+        ctx.render_tris(p, W, H, geoms, imgs)
+        ctx.composite_zmin(imgs.rgba8, imgs.depth, P, W, H, out8, outz)
+
+    for _ in range(args.warmup):
+        frame()
+    n = max(3, min(args.steps, 20))
+    ms_frame = timed(frame, n)
+    ms_render = timed(lambda: ctx.render_tris(p, W, H, geoms, imgs), n)
+    ms_zmin = timed(lambda: ctx.composite_zmin(imgs.rgba8, imgs.depth, P, W, H, out8, outz), n)
+    f8 = torch.empty_like(out8)
+    fz = torch.empty_like(outz)
+    fused = lambda: ctx.render_tris_composite(p, W, H, geoms, f8, fz)
+    fused()
+    ms_fused = timed(fused, n)
+    torch.cuda.synchronize()
+    equal = bool(torch.equal(f8, out8) and torch.equal(fz, outz))
+    # traversal work of one frame (untimed): node fetches and triangle tests
+    st = torch.zeros((P, H, W, 2), dtype=torch.int32, device="cuda")
+    ctx.render_tris(p, W, H, geoms, imgs, stats=st)
+    fst = torch.zeros((H, W, 2), dtype=torch.int32, device="cuda")
+    ctx.render_tris_composite(p, W, H, geoms, f8, fz, stats=fst)
+    torch.cuda.synchronize()
+    tot = st.to(torch.int64).sum(dim=(0, 1, 2)).tolist()
+    ftot = fst.to(torch.int64).sum(dim=(0, 1)).tolist()
+    rays = P * W * H
+    hit_frac = float((outz < 1).float().mean())
+    # algorithmic bytes of one per-rank launch: nodes x 64 + triangle tests x 48
+    # + the RGBA8 and depth images written
+    bytes_render = tot[0] * 64 + tot[1] * 48 + rays * 8
+    log(f"[bench] surface: frame {ms_frame:.3f} ms (render {ms_render:.3f}, zmin {ms_zmin:.3f}), "
+        f"fused {ms_fused:.3f} ms (equal {equal}), {tot[0] / rays:.1f} nodes + {tot[1] / rays:.1f} "
+        f"tris per ray")
+    res = {"workload": f"{sw.ntris} triangles (4096 displaced spheres), 8 clusters = 8 ranks, "
+                       f"{W}x{H}, headlight shading",
+           "rays_per_frame": rays, "hit_fraction": round(hit_frac, 4),
+           "bvh": {"nodes": sum(i["nnodes"] for i in info), "max_depth": max(i["depth"] for i in info),
+                   "bytes": sum(i["bytes"] for i in info), "build_upload_s": round(t_up, 2)},
+           "ms_per_frame": round(ms_frame, 4), "Mrays/s": round(rays / (ms_frame * 1e-3) / 1e6, 1),
+           "render_ms": round(ms_render, 4), "zmin_ms": round(ms_zmin, 4),
+           "render_Mrays/s": round(rays / (ms_render * 1e-3) / 1e6, 1),
+           "nodes_per_ray": round(tot[0] / rays, 2), "tris_per_ray": round(tot[1] / rays, 2),
+           "render_algorithmic_GB/s": round(bytes_render / (ms_render * 1e-3) / 1e9, 1),
+           "fused_ms": round(ms_fused, 4), "fused_Mrays/s": round(rays / (ms_fused * 1e-3) / 1e6, 1),
+           "fused_nodes_per_pixel": round(ftot[0] / (W * H), 2),
+           "fused_bit_identical": equal}
+    if not args.no_cpu:
+        res["cpu_baseline"] = surface_cpu_baseline(ctx, geoms, sw, imgs, args.cpu_seconds / 3)
+    ctx.close()
+    return res
+
+
+def surface_cpu_baseline(ctx, geoms, sw, imgs, seconds):
+    """The oracle's CPU walk of the same BVHs (OpenMP) on a band of rows of
+    every rank; also checks those rows against the GPU images."""
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    exported = [ctx.tris_export(g) for g in geoms]
+    d8 = imgs.rgba8.cpu().numpy().view(np.uint32)
+    dz = imgs.depth.cpu().numpy()
+    band, H = 4, sw.H
+    while True:
+        rows = (H // 2 - band // 2, H // 2 + band // 2)
+        t = time.time()
+        outs = [O.tri_render_bvh(sw.cam, nd, tr, rt, sw.W, sw.H, max_depth=sw.max_depth,
+                                 rows=rows, threads=threads) for nd, tr, rt in exported]
+        el = time.time() - t
+        if el * 2 > seconds or band >= H:
+            break
+        band = min(H, band * max(2, int(seconds / max(el, 1e-3) / 2)))
+    rays = len(geoms) * sw.W * (rows[1] - rows[0])
+    sl = slice(rows[0], rows[1])
+    mism = sum(int((o["rgba8"][sl] != d8[r, sl]).sum() + (o["depth"][sl] != dz[r, sl]).sum())
+               for r, o in enumerate(outs))
+    return {"value": round(rays / el / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"rows {rows[0]}..{rows[1]} of all {len(geoms)} ranks ({rays} rays, {el:.1f}s), "
+                      f"oracle BVH walk, OpenMP",
+            "gpu_mismatched_values_in_sample": mism}
 
 
 def cpu_baseline(wl, imgs, seconds):

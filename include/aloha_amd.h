@@ -301,6 +301,77 @@ int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *params, int32
                         const float *d_rgba, const uint32_t *d_nsamples,
                         const float *d_dl_drgba, float *d_dtf, float *const *d_dvoxels);
 
+/* ---------------------------------------------------------------------- */
+/* surface path: triangle meshes (Renderer::Mode::TriMesh)                */
+/* ---------------------------------------------------------------------- */
+typedef struct aloha_mesh aloha_mesh;
+/* Indexed triangle mesh on the host: verts nverts x 3 floats, tris ntris x
+ * 3 vertex ids (Mesh/Geometry, mesh.h:30-45).  Copies its inputs. */
+int aloha_mesh_create(const float *verts, int64_t nverts, const int32_t *tris, int64_t ntris,
+                      aloha_mesh **out);
+void aloha_mesh_destroy(aloha_mesh *mesh);
+int aloha_mesh_info(const aloha_mesh *mesh, int64_t *nverts, int64_t *ntris, aloha_box3f *bounds,
+                    int32_t *nclusters);
+/* Copy vertices / triangles out (triangles in cluster order once split). */
+int aloha_mesh_get(const aloha_mesh *mesh, float *verts, int32_t *tris);
+/* ranges: count x {first, last} triangle ranges; bounds: split domains. */
+int aloha_mesh_clusters(const aloha_mesh *mesh, int64_t *ranges, aloha_box3f *bounds, int32_t cap,
+                        int32_t *count);
+/* Seeded test scene: num_spheres UV spheres (rings x segments, 2*segments*
+ * (rings-1) triangles each), radius uniform [rmin_frac, rmax_frac] of the
+ * largest domain extent, radially displaced by smooth noise of amplitude
+ * disp_frac x radius, centres uniform inside the domain. */
+int aloha_mesh_synth_spheres(int32_t num_spheres, int32_t rings, int32_t segments,
+                             const aloha_box3f *domain, float rmin_frac, float rmax_frac,
+                             float disp_frac, uint64_t seed, aloha_mesh **out);
+/* MeshSplitter, median strategy (chopSuey.cpp:48-198): repeatedly split the
+ * cluster with most triangles at the median primitive centroid of its
+ * domain's longest axis, triangles with min vertex < plane to the left.
+ * Reorders the mesh's triangles cluster by cluster. */
+int aloha_mesh_split(aloha_mesh *mesh, int32_t num_clusters, int32_t *count);
+/* .tri files (MeshSplitter::saveTris chopSuey.cpp:200-218; reader
+ * waikikiRenderer.cpp:394-451).  An unsplit mesh is written as one cluster. */
+int aloha_tri_write(const char *path, const aloha_mesh *mesh);
+int aloha_tri_read(const char *path, aloha_mesh **out);
+
+/* Upload P ranks' triangle geometry (waikikiRenderer.cpp:508-593): rank r
+ * owns clusters cluster_ids[rank_offsets[r] .. rank_offsets[r+1]), each
+ * drawn in colour color_ids[k] (randomColor input, ModelCH).  Triangles are
+ * flattened to three private vertices each and given a binned-SAH BVH (the
+ * OptiX BLAS/TLAS build); primitive ids count the rank's triangles in list
+ * order.  geom_ids receives P ids.  An unsplit mesh has one cluster, 0. */
+int aloha_tris_upload(aloha_ctx *ctx, const aloha_mesh *mesh, int32_t P,
+                      const int32_t *rank_offsets, const int32_t *cluster_ids,
+                      const uint32_t *color_ids, int32_t *geom_ids);
+/* The same flatten + BVH build on the host only (no context, no device):
+ * one rank's clusters in the device layout aloha_tris_export returns.
+ * nodes / tris may be NULL to query the sizes; root is INT32_MIN for an
+ * empty rank. */
+int aloha_tris_build_host(const aloha_mesh *mesh, int32_t n, const int32_t *cluster_ids,
+                          const uint32_t *color_ids, int64_t *ntris, int64_t *nnodes,
+                          int32_t *depth, void *nodes, float *tris, int32_t *root);
+/* Triangle count, BVH internal nodes, BVH depth and device bytes of one geometry. */
+int aloha_tris_info(const aloha_ctx *ctx, int32_t geom, int64_t *ntris, int64_t *nnodes,
+                    int32_t *depth, uint64_t *bytes);
+/* Host copy of one geometry's device layout (for checking): nodes nnodes x
+ * 16 x 4 bytes, tris ntris x 12 floats (three float4 vertices whose w hold
+ * the primitive id and colour id as integer bits), root reference. */
+int aloha_tris_export(const aloha_ctx *ctx, int32_t geom, void *nodes, float *tris, int32_t *root);
+/* Per-rank surface images (renderFrame with traceRay(world)+ModelCH,
+ * deviceCode.cpp:340-416): rank r traces geometry geom_ids[r].  Uses the
+ * context camera, params->max_depth/spp/accum_id/island_id/num_islands/
+ * flags/heat_map_scale; images->rgba / rgba8 / depth / accum (device, P x H
+ * x W) may each be NULL.  d_stats (may be NULL): P x H x W x 2 uint32 of
+ * BVH nodes visited and triangles tested per pixel. */
+int aloha_render_tris(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                      int32_t P, const int32_t *geom_ids, const aloha_images *images,
+                      uint32_t *d_stats);
+/* Render + depth composite in one launch: equals aloha_render_tris (rgba8,
+ * depth, accum_id 0) followed by aloha_composite_zmin bit for bit. */
+int aloha_render_tris_composite(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                int32_t H, int32_t P, const int32_t *geom_ids,
+                                uint32_t *d_out_rgba8, float *d_out_depth, uint32_t *d_stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         _lib.orc_rng_seed.restype = C.c_uint32
         _lib.orc_rng_draw.restype = C.c_float
         _lib.orc_rng_seed.argtypes = [C.c_uint32, C.c_uint32]
+        _lib.orc_tri_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong,
+                                        C.c_int, C.c_int] + [C.c_void_p] * 5 + [C.c_int]
         _lib.orc_rng_draw.argtypes = [C.POINTER(C.c_uint32)]
         _lib.orc_composite_over.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong,
                                             C.c_void_p, C.c_void_p]
@@ -346,3 +348,136 @@ def diff_render64(cam, tf, bricks, W, H, dt=0.5, jitter_mode=0, jitter_value=0.0
                                int(threads)):
         raise ValueError("orc_diff_render64 failed")
     return out
+
+
+# ----------------------------------------------------------------------------
+# surface path
+# ----------------------------------------------------------------------------
+def _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair):
+    p = OrcParams()
+    p.org, p.dir00, p.du, p.dv = (_f3(cam[k]) for k in ("org", "dir_00", "dir_du", "dir_dv"))
+    p.dt, p.max_depth, p.W, p.H = 0.5, max_depth, W, H
+    p.spp, p.accum_id, p.island_id, p.num_islands = spp, 0, island_id, num_islands
+    p.jitter_mode, p.jitter_value, p.crosshair = 0, 0.0, int(crosshair)
+    p.tf_n, p.tf = 0, None
+    return p
+
+
+def _tri_out(H, W):
+    return {"rgba": np.zeros((H, W, 4), np.float32), "rgba8": np.zeros((H, W), np.uint32),
+            "depth": np.zeros((H, W), np.float32), "prim": np.full((H, W), -1, np.int32),
+            "t": np.zeros((H, W), np.float32)}
+
+
+def _tri_ptrs(out):
+    return [C.c_void_p(out[k].ctypes.data) for k in ("rgba", "rgba8", "depth", "prim", "t")]
+
+
+def tri_render(cam, tri9: np.ndarray, colors: np.ndarray, W: int, H: int, max_depth=1.0, spp=1,
+               island_id=0, num_islands=1, crosshair=False, rows=None, threads=0):
+    """One rank's surface image by brute-force closest hit (deviceCode.cpp:
+    340-416 with ModelCH :176-188).  tri9: (n,9) float32 vertices in the
+    rank's list order (primitive id = row); colors: (n,) colour ids.
+    Returns rgba (H,W,4), rgba8, depth, prim (-1 = miss), t."""
+    tri9 = np.ascontiguousarray(tri9, dtype=np.float32).reshape(-1, 9)
+    colors = np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+    assert len(colors) == len(tri9)
+    p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair)
+    out = _tri_out(H, W)
+    row0, row1 = (0, H) if rows is None else rows
+    rc = lib().orc_tri_render(C.byref(p), C.c_void_p(tri9.ctypes.data),
+                              C.c_void_p(colors.ctypes.data), C.c_longlong(len(tri9)),
+                              row0, row1, *_tri_ptrs(out), int(threads))
+    if rc != 0:
+        raise ValueError(f"orc_tri_render failed ({rc})")
+    return out
+
+
+def tri_render_bvh(cam, nodes: np.ndarray, tris: np.ndarray, root: int, W: int, H: int,
+                   max_depth=1.0, spp=1, island_id=0, num_islands=1, crosshair=False, rows=None,
+                   threads=0):
+    """The same image through a BVH in the library's device layout (nodes
+    (n,16) 4-byte words, tris (m,12) float32), for meshes too large for brute
+    force.  prim holds primitive ids."""
+    nodes = np.ascontiguousarray(nodes).view(np.uint32).reshape(-1, 16)
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
+    p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair)
+    out = _tri_out(H, W)
+    row0, row1 = (0, H) if rows is None else rows
+    f = lib().orc_tri_render_bvh
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int, C.c_int] + \
+        [C.c_void_p] * 5 + [C.c_int]
+    rc = f(C.byref(p), C.c_void_p(nodes.ctypes.data if len(nodes) else None),
+           C.c_void_p(tris.ctypes.data), int(root), row0, row1, *_tri_ptrs(out), int(threads))
+    if rc != 0:
+        raise ValueError(f"orc_tri_render_bvh failed ({rc})")
+    return out
+
+
+def random_color(idx: int):
+    out = (C.c_float * 3)()
+    lib().orc_random_color(C.c_uint32(idx), out)
+    return tuple(out)
+
+
+def mesh_split(verts: np.ndarray, tris: np.ndarray, bounds, n: int):
+    """MeshSplitter, median strategy (chopSuey.cpp:60-198), restated in
+    numpy: returns (tris reordered cluster by cluster, [(first, last, lo, hi)]).
+
+    Per split: the cluster with most triangles (first of equals) is removed;
+    axis = longest side of its domain (y if > x and >= z, z if > x and >= y,
+    else x); primitive references (bbox centroids of the original triangles,
+    one array for the whole mesh) are sorted over the cluster's index range
+    by that coordinate; the plane is the centroid at position first +
+    size_t(N / float(n)); triangles with min vertex < plane go left (order
+    kept); L then R are appended when non-empty, until n clusters exist
+    (at least one split).  A cluster coming back whole with an unchanged
+    domain would repeat forever (the reference recurses without end): error.
+    Ties in the sort are broken by primitive id."""
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    t = np.array(tris, np.int32).reshape(-1, 3)
+    N = len(t)
+    tv = v[t]                                   # (N,3,3)
+    cen = ((tv.min(axis=1) + tv.max(axis=1)) * np.float32(0.5)).astype(np.float32)
+    ref_prim = np.arange(N)
+    ref_cen = cen.copy()
+    lo0 = np.array(bounds[0], np.float32)
+    hi0 = np.array(bounds[1], np.float32)
+    clusters = [(0, N, lo0.copy(), hi0.copy())]
+    num = int(np.float32(N) / np.float32(n))
+    while True:
+        sizes = [c[1] - c[0] for c in clusters]
+        k = int(np.argmax(sizes))
+        first, last, lo, hi = clusters.pop(k)
+        s = (hi - lo).astype(np.float32)
+        axis = 0
+        if s[1] > s[0] and s[1] >= s[2]:
+            axis = 1
+        elif s[2] > s[0] and s[2] >= s[1]:
+            axis = 2
+        o = np.lexsort((ref_prim[first:last], ref_cen[first:last, axis]))
+        ref_prim[first:last] = ref_prim[first:last][o]
+        ref_cen[first:last] = ref_cen[first:last][o]
+        plane = ref_cen[min(first + num, N - 1), axis]
+        mins = t[first:last]
+        mv = v[mins][:, :, axis].min(axis=1)
+        left = mv < plane
+        t[first:last] = np.concatenate([mins[left], mins[~left]])
+        split = first + int(left.sum())
+        if split == last:
+            split = first
+        L = (first, split, lo.copy(), hi.copy())
+        L[3][axis] = plane
+        R = (split, last, lo.copy(), hi.copy())
+        R[2][axis] = plane
+        before = len(clusters)
+        if L[1] - L[0] > 0:
+            clusters.append(L)
+        if R[1] - R[0] > 0:
+            clusters.append(R)
+        if len(clusters) < before + 2 and np.array_equal(clusters[-1][2], lo) \
+                and np.array_equal(clusters[-1][3], hi):
+            raise ValueError("cluster cannot be split")
+        if len(clusters) >= n:
+            break
+    return t, clusters

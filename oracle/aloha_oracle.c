@@ -1061,3 +1061,253 @@ int orc_diff_render64(const orc_params *p, const orc_brick *bricks, int nb, doub
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* surface path: closest hit + ModelCH shading                         */
+/* deviceCode.cpp:59-67 (randomColor), :176-188 (ModelCH),             */
+/* :340-416 (renderFrame with traceRay(world)), :94-107 (generateRay)  */
+/* ------------------------------------------------------------------ */
+/* Ray/triangle test: Woop, Benthin, Wald, "Watertight Ray/Triangle
+ * Intersection", JCGT 2(1) 2013 -- shear to ray space, edge functions
+ * recomputed in double when one is exactly zero.  Closest hit: smallest
+ * t > 0 below tmax = 1e10, ties to the smaller primitive id. */
+typedef struct {
+  float o[3], d[3];
+  int kx, ky, kz;
+  float sx, sy, sz;
+} tri_ray;
+
+static void tri_ray_setup(const float o[3], const float d[3], tri_ray *r) {
+  for (int k = 0; k < 3; ++k) { r->o[k] = o[k]; r->d[k] = d[k]; }
+  int kz = 0;
+  if (fabsf(d[1]) > fabsf(d[kz])) kz = 1;
+  if (fabsf(d[2]) > fabsf(d[kz])) kz = 2;
+  int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+  if (d[kz] < 0.f) { int t = kx; kx = ky; ky = t; }
+  r->kx = kx; r->ky = ky; r->kz = kz;
+  r->sx = d[kx] / d[kz];
+  r->sy = d[ky] / d[kz];
+  r->sz = 1.f / d[kz];
+}
+
+/* t of the hit, -1 on a miss; v: three vertices of 3 floats */
+static float tri_hit(const tri_ray *r, const float *v0, const float *v1, const float *v2) {
+  float A[3], B[3], Cv[3];
+  for (int k = 0; k < 3; ++k) {
+    A[k] = v0[k] - r->o[k];
+    B[k] = v1[k] - r->o[k];
+    Cv[k] = v2[k] - r->o[k];
+  }
+  const float ax = A[r->kx] - r->sx * A[r->kz], ay = A[r->ky] - r->sy * A[r->kz];
+  const float bx = B[r->kx] - r->sx * B[r->kz], by = B[r->ky] - r->sy * B[r->kz];
+  const float cx = Cv[r->kx] - r->sx * Cv[r->kz], cy = Cv[r->ky] - r->sy * Cv[r->kz];
+  float u = cx * by - cy * bx, v = ax * cy - ay * cx, w = bx * ay - by * ax;
+  if (u == 0.f || v == 0.f || w == 0.f) {
+    u = (float)((double)cx * (double)by - (double)cy * (double)bx);
+    v = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+    w = (float)((double)bx * (double)ay - (double)by * (double)ax);
+  }
+  if ((u < 0.f || v < 0.f || w < 0.f) && (u > 0.f || v > 0.f || w > 0.f)) return -1.f;
+  const float det = u + v + w;
+  if (det == 0.f) return -1.f;
+  const float az = r->sz * A[r->kz], bz = r->sz * B[r->kz], cz = r->sz * Cv[r->kz];
+  const float T = u * az + v * bz + w * cz;
+  return T / det;
+}
+
+static void pixel_dir(const orc_params *p, int px, int py, float d[3]) {
+  const float su = (float)px + .5f, sv = (float)py + .5f;
+  for (int a = 0; a < 3; ++a) d[a] = (p->dir00[a] + su * p->du[a]) + sv * p->dv[a];
+  const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (int a = 0; a < 3; ++a) d[a] = d[a] / len;
+  for (int a = 0; a < 3; ++a)
+    if (fabsf(d[a]) < 1e-5f) d[a] = 1e-5f;
+}
+
+/* randomColor(idx), deviceCode.cpp:59-67 */
+void orc_random_color(uint32_t idx, float out[3]) {
+  const uint32_t r = idx * 13u * 17u + 0x234235u;
+  const uint32_t g = idx * 7u * 3u * 5u + 0x773477u;
+  const uint32_t b = idx * 11u * 19u + 0x223766u;
+  out[0] = (float)(r & 255u) / 255.f;
+  out[1] = (float)(g & 255u) / 255.f;
+  out[2] = (float)(b & 255u) / 255.f;
+}
+
+/* ModelCH's geometric normal and the raygen's headlight colour */
+static void tri_shade(const float d[3], const float *v0, const float *v1, const float *v2,
+                      uint32_t color, float c[4]) {
+  const float e1x = v1[0] - v0[0], e1y = v1[1] - v0[1], e1z = v1[2] - v0[2];
+  const float e2x = v2[0] - v0[0], e2y = v2[1] - v0[1], e2z = v2[2] - v0[2];
+  const float nx = e1y * e2z - e1z * e2y;
+  const float ny = e1z * e2x - e1x * e2z;
+  const float nz = e1x * e2y - e1y * e2x;
+  const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
+  const float gx = nx * inv, gy = ny * inv, gz = nz * inv;
+  const float lam = fmaxf(0.f, (-d[0]) * gx + (-d[1]) * gy + (-d[2]) * gz);
+  float cd[3];
+  orc_random_color(color, cd);
+  for (int k = 0; k < 3; ++k) c[k] = .2f + cd[k] * lam;
+  c[3] = 1.f;
+}
+
+/* per-pixel image value of a rank (accum_id 0): depth, float RGBA, RGBA8 */
+static void tri_finish(const orc_params *p, int px, int py, const float c[4], float z,
+                       size_t pix, float *rgba, uint32_t *rgba8, float *depth) {
+  const float inv_spp = 1.f / (float)p->spp;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < p->spp; ++s)
+    for (int k = 0; k < 4; ++k) acc[k] = acc[k] + c[k];
+  if (depth) depth[pix] = z;
+  if (rgba)
+    for (int k = 0; k < 4; ++k) rgba[4 * pix + k] = (acc[k] * 1.f) * inv_spp;
+  if (rgba8) {
+    const int cross = p->crosshair && (px == p->W / 2 || py == p->H / 2);
+    float q[4];
+    for (int k = 0; k < 4; ++k) {
+      float a = acc[k] * 1.f;
+      if (cross) a = 1.f - a;
+      q[k] = a * inv_spp;
+    }
+    rgba8[pix] = orc_make_rgba(q);
+  }
+}
+
+static int island_pixel_of(const orc_params *p, int px, int py) {
+  const int gx = (px + p->W - 1) / p->W;
+  const int block = px / 16 + gx * (py / 16);
+  return block % p->num_islands == p->island_id;
+}
+
+/* One rank's surface image by brute force over its triangles (tri9:
+ * ntris x 9 floats in the rank's list order, primitive id = index;
+ * color: colour id per triangle).  Rows [row0,row1); every output may be
+ * NULL; hit_prim/hit_t: -1 / 0 for a miss.  accum_id 0, no heat map. */
+int orc_tri_render(const orc_params *p, const float *tri9, const uint32_t *color, long long ntris,
+                   int row0, int row1, float *rgba, uint32_t *rgba8, float *depth,
+                   int32_t *hit_prim, float *hit_t, int nthreads) {
+  if (!p || p->W <= 0 || p->H <= 0 || p->spp < 1 || p->num_islands < 1 || ntris < 0) return -1;
+  if (row0 < 0) row0 = 0;
+  if (row1 > p->H) row1 = p->H;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int py = row0; py < row1; ++py)
+    for (int px = 0; px < p->W; ++px) {
+      const size_t pix = (size_t)py * p->W + px;
+      float c[4] = {0.f, 0.f, 0.f, 0.f}, z = 1.f, best = 1e10f, d[3];
+      long long hit = -1;
+      if (island_pixel_of(p, px, py)) {
+        tri_ray r;
+        pixel_dir(p, px, py, d);
+        tri_ray_setup(p->org, d, &r);
+        for (long long i = 0; i < ntris; ++i) {
+          const float *v = tri9 + 9 * i;
+          const float t = tri_hit(&r, v, v + 3, v + 6);
+          if (t > 0.f && t < best) { best = t; hit = i; }
+        }
+        if (hit >= 0) {
+          const float *v = tri9 + 9 * hit;
+          tri_shade(d, v, v + 3, v + 6, color[hit], c);
+          z = best / p->max_depth;
+        }
+      }
+      tri_finish(p, px, py, c, z, pix, rgba, rgba8, depth);
+      if (hit_prim) hit_prim[pix] = (int32_t)hit;
+      if (hit_t) hit_t[pix] = hit >= 0 ? best : 0.f;
+    }
+  return 0;
+}
+
+/* The same image by walking a BVH in the library's device layout
+ * (nodes: 16 x 4-byte words each -- left box xy, right box xy, both z,
+ * child refs; tris: 12 floats each, vertex w = primitive id / colour id
+ * bits).  Used where brute force is too slow; the traversal is ordered
+ * and culled differently from the kernel's but returns the same closest
+ * hit (smallest t, then smallest primitive id). */
+int orc_tri_render_bvh(const orc_params *p, const void *nodes, const float *tris, int32_t root,
+                       int row0, int row1, float *rgba, uint32_t *rgba8, float *depth,
+                       int32_t *hit_prim, float *hit_t, int nthreads) {
+  if (!p || p->W <= 0 || p->H <= 0 || p->spp < 1 || p->num_islands < 1) return -1;
+  if (row0 < 0) row0 = 0;
+  if (row1 > p->H) row1 = p->H;
+  const float *nf = (const float *)nodes;
+  const int32_t *ni = (const int32_t *)nodes;
+  int overflow = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int py = row0; py < row1; ++py)
+    for (int px = 0; px < p->W; ++px) {
+      const size_t pix = (size_t)py * p->W + px;
+      float c[4] = {0.f, 0.f, 0.f, 0.f}, z = 1.f, best = 1e10f, d[3];
+      long long hit = -1;
+      int32_t hit_id = -1;
+      if (island_pixel_of(p, px, py) && root != INT_MIN) {
+        tri_ray r;
+        pixel_dir(p, px, py, d);
+        tri_ray_setup(p->org, d, &r);
+        /* double-precision slab tests widened by 1e-6 relative: culling
+         * only has to be conservative */
+        int32_t stack[64];
+        int sp = 0;
+        stack[sp++] = root;
+        while (sp > 0) {
+          const int32_t ref = stack[--sp];
+          if (ref >= 0) {
+            const float *n = nf + 16 * (size_t)ref;
+            for (int side = 1; side >= 0; --side) {
+              const float *xy = n + 4 * side;
+              const float lo[3] = {xy[0], xy[2], n[8 + 2 * side]};
+              const float hi[3] = {xy[1], xy[3], n[9 + 2 * side]};
+              double t0 = 0.0, t1 = (double)best * (1.0 + 1e-6);
+              int ok = 1;
+              for (int a = 0; a < 3 && ok; ++a) {
+                const double inv = 1.0 / (double)d[a];
+                double ta = ((double)lo[a] - p->org[a]) * inv, tb = ((double)hi[a] - p->org[a]) * inv;
+                if (ta > tb) { const double t = ta; ta = tb; tb = t; }
+                ta -= 1e-6 * fabs(ta) + 1e-12;
+                tb += 1e-6 * fabs(tb) + 1e-12;
+                if (ta > t0) t0 = ta;
+                if (tb < t1) t1 = tb;
+                ok = t0 <= t1;
+              }
+              if (ok) {
+                if (sp < 64)
+                  stack[sp++] = ni[16 * (size_t)ref + 12 + side];
+                else
+                  overflow = 1;
+              }
+            }
+          } else {
+            const uint32_t leaf = ~(uint32_t)ref;
+            const long long first = leaf >> 3, cnt = (leaf & 7u) + 1;
+            for (long long i = first; i < first + cnt; ++i) {
+              const float *t12 = tris + 12 * (size_t)i;
+              int32_t prim;
+              memcpy(&prim, t12 + 3, 4);
+              const float t = tri_hit(&r, t12, t12 + 4, t12 + 8);
+              if (t > 0.f && (t < best || (t == best && prim < hit_id))) {
+                best = t;
+                hit = i;
+                hit_id = prim;
+              }
+            }
+          }
+        }
+        if (hit >= 0) {
+          const float *t12 = tris + 12 * (size_t)hit;
+          uint32_t col;
+          memcpy(&col, t12 + 7, 4);
+          tri_shade(d, t12, t12 + 4, t12 + 8, col, c);
+          z = best / p->max_depth;
+        }
+      }
+      tri_finish(p, px, py, c, z, pix, rgba, rgba8, depth);
+      if (hit_prim) hit_prim[pix] = hit_id;
+      if (hit_t) hit_t[pix] = hit >= 0 ? best : 0.f;
+    }
+  return overflow ? -2 : 0;
+}

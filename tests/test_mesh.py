@@ -1,0 +1,274 @@
+"""Surface path, host side (no GPU): the sphere generator, MeshSplitter
+(chopSuey.cpp:48-229) against its numpy restatement in the oracle, the .tri
+byte layout (chopSuey.cpp:200-218, waikikiRenderer.cpp:394-451), the
+cluster colour ids (waikikiRenderer.cpp:554-557), and the BVH build -- its
+structure and, through the oracle's CPU walk, the closest hit of every pixel
+against brute force."""
+import struct
+
+import numpy as np
+import pytest
+
+import aloha_amd as A
+import oracle as O
+
+
+def _domain(lo=(0, 0, 0), hi=(1, 1, 1)):
+    return A.Box3f(tuple(map(float, lo)), tuple(map(float, hi)))
+
+
+@pytest.fixture(scope="module")
+def spheres():
+    return A.Mesh.spheres(64, 8, 12, _domain(), 0.02, 0.08, 0.1, seed=7)
+
+
+def test_sphere_counts_and_bounds(spheres):
+    info = spheres.info()
+    assert info["ntris"] == 64 * 2 * 12 * 7 and info["nverts"] == 64 * (2 + 7 * 12)
+    v, t = spheres.arrays()
+    assert t.min() == 0 and t.max() == len(v) - 1
+    # centres are at least one radius inside; displacement adds <= 10 %
+    assert v.min() >= -0.1 * 0.08 and v.max() <= 1 + 0.1 * 0.08
+    b = info["bounds"]
+    assert np.allclose(b.lower, v.min(0)) and np.allclose(b.upper, v.max(0))
+    again = A.Mesh.spheres(64, 8, 12, _domain(), 0.02, 0.08, 0.1, seed=7).arrays()
+    assert np.array_equal(again[0], v) and np.array_equal(again[1], t)
+
+
+def test_sphere_winding_is_outward():
+    """Without displacement every triangle's cross(v1-v0, v2-v0) points away
+    from its sphere's centre, so the headlight term max(0, -d.n) lights the
+    side facing the camera."""
+    m = A.Mesh.spheres(3, 6, 9, _domain(), 0.05, 0.1, 0.0, seed=3)
+    v, t = m.arrays()
+    per = 2 * 9 * 5
+    for s in range(3):
+        tri = v[t[s * per:(s + 1) * per]].astype(np.float64)
+        centre = v[t[s * per:(s + 1) * per]].reshape(-1, 3).mean(0)
+        n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+        out = tri.mean(1) - centre
+        assert (np.einsum("ij,ij->i", n, out) > 0).all()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 13])
+def test_mesh_split_matches_restatement(n):
+    m = A.Mesh.spheres(48, 6, 10, _domain((0, 0, 0), (2, 1, 1.5)), 0.02, 0.06, 0.1, seed=11)
+    v, t0 = m.arrays()
+    b = m.info()["bounds"]
+    ref_t, ref_cl = O.mesh_split(v, t0, (b.lower, b.upper), n)
+    got = m.split(n)
+    v1, t1 = m.arrays()
+    cl = m.clusters()
+    # n=1 reads primRefs[N] in the reference (past the end); both clamp to
+    # the last reference, so every triangle is left of the plane and the one
+    # cluster returns with its domain clipped
+    assert got == len(cl) == len(ref_cl) and got >= n
+    assert np.array_equal(t1, ref_t)
+    for (f, l, bx), (rf, rl, rlo, rhi) in zip(cl, ref_cl):
+        assert (f, l) == (rf, rl)
+        assert np.array_equal(np.array(bx.lower, np.float32), rlo)
+        assert np.array_equal(np.array(bx.upper, np.float32), rhi)
+    # clusters tile the triangle list; the multiset of triangles is unchanged
+    assert sorted((f, l) for f, l, _ in cl)[0][0] == 0
+    assert sum(l - f for f, l, _ in cl) == len(t0)
+    assert np.array_equal(np.sort(t1.view("i4,i4,i4"), axis=0), np.sort(t0.view("i4,i4,i4"), axis=0))
+
+
+def test_first_split_plane_closed_form():
+    """The first plane is the centroid at rank size_t(N/float(n)) along the
+    longest model axis (chopSuey.cpp:130-147), triangles with min vertex
+    below it on the left."""
+    m = A.Mesh.spheres(40, 6, 8, _domain((0, 0, 0), (1, 3, 1)), 0.02, 0.05, 0.1, seed=5)
+    v, t = m.arrays()
+    tv = v[t]
+    cen = (tv.min(1) + tv.max(1)) * np.float32(0.5)
+    n = 4
+    num = int(np.float32(len(t)) / np.float32(n))
+    plane = np.sort(cen[:, 1])[num]               # y is the longest axis
+    m.split(2)                                    # at least one split: exactly 2 clusters
+    (f0, l0, b0), (f1, l1, b1) = m.clusters()
+    # with n=2 the rank is N/2
+    plane2 = np.sort(cen[:, 1])[int(np.float32(len(t)) / np.float32(2))]
+    assert b0.upper[1] == plane2 and b1.lower[1] == plane2
+    assert l0 - f0 == int((tv[:, :, 1].min(1) < plane2).sum())
+    assert plane <= plane2
+
+
+def test_mesh_split_errors():
+    with pytest.raises(A.AlohaError):
+        A.Mesh.from_arrays(np.zeros((3, 3)), np.array([[0, 1, 3]]))
+    m = A.Mesh.from_arrays(np.eye(3), np.array([[0, 1, 2]]))
+    with pytest.raises(A.AlohaError):
+        m.split(0)
+    with pytest.raises(A.AlohaError):   # one triangle cannot become two clusters
+        m.split(2)
+    with pytest.raises(ValueError):
+        O.mesh_split(np.eye(3, dtype=np.float32), np.array([[0, 1, 2]]), ((0, 0, 0), (1, 1, 1)), 2)
+
+
+def test_tri_file_layout_and_roundtrip(tmp_path, spheres):
+    m = A.Mesh.spheres(20, 5, 7, _domain(), 0.02, 0.08, 0.1, seed=9)
+    m.split(5)
+    v, t = m.arrays()
+    cl = m.clusters()
+    b = m.info()["bounds"]
+    path = tmp_path / "m.tri"
+    m.write_tri(str(path))
+    raw = path.read_bytes()
+    # the reader's layout (waikikiRenderer.cpp:403-447)
+    nc, = struct.unpack_from("<Q", raw, 0)
+    assert nc == len(cl)
+    assert struct.unpack_from("<6f", raw, 8) == tuple(np.float32(x) for x in (*b.lower, *b.upper))
+    nv, = struct.unpack_from("<Q", raw, 32)
+    assert nv == len(v)
+    assert np.array_equal(np.frombuffer(raw, np.float32, 3 * nv, 40).reshape(-1, 3), v)
+    off = 40 + 12 * nv
+    for f, l, bx in cl:
+        ni, = struct.unpack_from("<Q", raw, off)
+        assert ni == l - f
+        assert struct.unpack_from("<6f", raw, off + 8) == tuple(np.float32(x) for x in (*bx.lower, *bx.upper))
+        assert np.array_equal(np.frombuffer(raw, np.int32, 3 * ni, off + 32).reshape(-1, 3), t[f:l])
+        off += 32 + 12 * ni
+    assert off == len(raw)
+    back = A.Mesh.read_tri(str(path))
+    v2, t2 = back.arrays()
+    assert np.array_equal(v2, v) and np.array_equal(t2, t)
+    assert [(f, l) for f, l, _ in back.clusters()] == [(f, l) for f, l, _ in cl]
+    (tmp_path / "bad.tri").write_bytes(raw[:50])
+    with pytest.raises(A.AlohaError):
+        A.Mesh.read_tri(str(tmp_path / "bad.tri"))
+
+
+def test_color_ids_and_round_robin():
+    # 8 clusters on 8 ranks: rank r <- cluster r, colour id r
+    rr = A.Partitioner.round_robin(8, 8)
+    ranks = [rr.rank_bricks(r) for r in range(8)]
+    assert ranks == [[r] for r in range(8)]
+    assert A.mesh_color_ids(ranks, 8) == [[r] for r in range(8)]
+    # 10 clusters on 4 ranks: blocks of ceil(10/4)=3, colour ids r*(10//4)+i
+    rr = A.Partitioner.round_robin(10, 4)
+    ranks = [rr.rank_bricks(r) for r in range(4)]
+    assert ranks == [[0, 1, 2], [3, 4, 5], [6, 7, 8], [9]]
+    assert A.mesh_color_ids(ranks, 10) == [[0, 1, 2], [2, 3, 4], [4, 5, 6], [6]]
+
+
+def test_random_color_closed_form():
+    for idx in (0, 1, 7, 255, 1 << 20, 0xFFFFFFFF):
+        r = (idx * 221 + 0x234235) & 0xFFFFFFFF
+        g = (idx * 105 + 0x773477) & 0xFFFFFFFF
+        b = (idx * 209 + 0x223766) & 0xFFFFFFFF
+        want = tuple(float(np.float32(x & 255) / np.float32(255)) for x in (r, g, b))
+        assert O.random_color(idx) == pytest.approx(want, abs=0)
+
+
+def _check_bvh(nodes, tris, root, tri9):
+    """Every triangle in exactly one leaf; every child box contains its
+    subtree's triangles; leaves hold 1..8 triangles."""
+    seen = np.zeros(len(tri9), np.int32)
+    nf = nodes.view(np.float32)
+    ni = nodes.view(np.int32)
+
+    def walk(ref, lo, hi, depth):
+        if ref >= 0:
+            for side in (0, 1):
+                xy = nf[ref, 4 * side:4 * side + 4]
+                z = nf[ref, 8 + 2 * side:10 + 2 * side]
+                clo = np.array([xy[0], xy[2], z[0]])
+                chi = np.array([xy[1], xy[3], z[1]])
+                if lo is not None:
+                    assert (clo >= lo - 1e-6).all() and (chi <= hi + 1e-6).all()
+                walk(int(ni[ref, 12 + side]), clo, chi, depth + 1)
+        else:
+            leaf = (~ref) & 0xFFFFFFFF
+            first, cnt = leaf >> 3, (leaf & 7) + 1
+            for s in range(first, first + cnt):
+                prim = int(tris[s, 3:4].view(np.int32)[0])
+                seen[prim] += 1
+                v = tris[s].reshape(3, 4)[:, :3]
+                assert np.array_equal(v.reshape(-1), tri9[prim])
+                if lo is not None:
+                    assert (v >= lo).all() and (v <= hi).all()
+            assert depth <= 32
+    walk(root, None, None, 0)
+    assert (seen == 1).all()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_bvh_walk_equals_brute_force(seed):
+    """The library's BVH (host build of the device layout) walked by the
+    oracle gives every pixel the same closest hit, t bit for bit and
+    primitive id, as brute force over the rank's list."""
+    m = A.Mesh.spheres(24, 6, 10, _domain(), 0.03, 0.1, 0.1, seed=seed)
+    m.split(4)
+    ids = [1, 3]
+    colors = [5, 9]
+    tri9 = m.rank_tri9(ids)
+    ncl = [l - f for f, l, _ in m.clusters()]
+    col = np.repeat(colors, [ncl[i] for i in ids]).astype(np.uint32)
+    nodes, tris, root, depth = m.build_bvh(ids, colors)
+    assert 0 < depth <= 32 and len(tris) == len(tri9)
+    _check_bvh(nodes, tris, root, tri9)
+    b = m.info()["bounds"]
+    W, H = 96, 80
+    cam = A.synth_camera(b, W, H, 40.0, 1.6, seed=seed)
+    md = A.max_depth(b, cam["org"])
+    bf = O.tri_render(cam, tri9, col, W, H, max_depth=md)
+    bv = O.tri_render_bvh(cam, nodes, tris, root, W, H, max_depth=md)
+    assert (bf["prim"] >= 0).sum() > 150
+    assert np.array_equal(bf["prim"], bv["prim"])
+    assert np.array_equal(bf["t"], bv["t"])
+    assert np.array_equal(bf["rgba8"], bv["rgba8"])
+    assert np.array_equal(bf["rgba"], bv["rgba"])
+
+
+def test_bvh_depth_bound_on_degenerate_input():
+    """Many triangles with one centroid (no SAH plane) still give leaves of
+    at most 8 and depth <= 32 (object-median fallback)."""
+    k = 3000
+    v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (k, 1))
+    t = np.arange(3 * k, dtype=np.int32).reshape(-1, 3)
+    m = A.Mesh.from_arrays(v, t)
+    nodes, tris, root, depth = m.build_bvh([0], [0])
+    assert depth <= 32 and len(tris) == k
+    _check_bvh(nodes, tris, root, m.rank_tri9([0]))
+
+
+def test_shading_closed_form():
+    """One triangle facing the camera: colour .2 + randomColor(id) * cos,
+    depth t / maxDepth, a miss is transparent black at depth 1."""
+    v = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    cam = {"org": np.array([0, 0, 4], np.float32),
+           "dir_00": np.array([-0.5, -0.5, -1], np.float32),
+           "dir_du": np.array([1 / 8, 0, 0], np.float32),
+           "dir_dv": np.array([0, 1 / 8, 0], np.float32)}
+    out = O.tri_render(cam, v.reshape(1, 9), np.array([3], np.uint32), 8, 8, max_depth=8.0)
+    # pixel (4,4): direction (1/16, 1/16, -1) normalised, hits z=0 at t = 4*len
+    d = np.array([1 / 16, 1 / 16, -1], np.float64)
+    t = 4 * np.linalg.norm(d)
+    cos = 1 / np.linalg.norm(d)                  # n = (0,0,1) for this winding
+    cd = np.array(O.random_color(3))
+    assert out["prim"][4, 4] == 0
+    assert out["t"][4, 4] == pytest.approx(t, rel=1e-6)
+    assert out["depth"][4, 4] == pytest.approx(t / 8, rel=1e-6)
+    assert np.allclose(out["rgba"][4, 4], [*(0.2 + cd * cos), 1.0], atol=1e-6)
+    assert out["prim"][0, 7] == -1 and out["depth"][0, 7] == 1.0
+    assert (out["rgba"][0, 7] == 0).all() and out["rgba8"][0, 7] == 0
+    # reversed winding: back face, headlight term 0 -> grey .2
+    back = O.tri_render(cam, v[[0, 2, 1]].reshape(1, 9), np.array([3], np.uint32), 8, 8,
+                        max_depth=8.0)
+    assert np.allclose(back["rgba"][4, 4], [0.2, 0.2, 0.2, 1.0])
+
+
+def test_watertight_shared_edges():
+    """Two triangles sharing a diagonal: every ray through the square hits
+    one of them (no cracks), also rays through the shared edge."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    W = H = 64
+    # the image spans x, y in [0.08, 0.92]; pixels (i, i) aim exactly at the diagonal
+    cam = {"org": np.array([0.5, 0.5, 3], np.float32),
+           "dir_00": np.array([-0.14, -0.14, -1], np.float32),
+           "dir_du": np.array([0.28 / W, 0, 0], np.float32),
+           "dir_dv": np.array([0, 0.28 / H, 0], np.float32)}
+    out = O.tri_render(cam, v.reshape(2, 9), np.zeros(2, np.uint32), W, H, max_depth=8.0)
+    assert (out["prim"] >= 0).all()
+    assert (out["prim"] == 0).sum() > 0 and (out["prim"] == 1).sum() > 0
