@@ -432,6 +432,12 @@ class Mesh:
         return cls(h)
 
     @classmethod
+    def read_obj(cls, path: str) -> "Mesh":
+        h = C.c_void_p()
+        check(lib().aloha_mesh_read_obj(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def read_tri(cls, path: str) -> "Mesh":
         h = C.c_void_p()
         check(lib().aloha_tri_read(str(path).encode(), C.byref(h)))

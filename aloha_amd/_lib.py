@@ -130,6 +130,7 @@ _SIGS = [
     ("aloha_mesh_synth_spheres", C.c_int, [_I, _I, _I, C.POINTER(Box3f), C.c_float, C.c_float, C.c_float,
                                            C.c_uint64, C.POINTER(_P)]),
     ("aloha_mesh_split", C.c_int, [_P, _I, _IP]),
+    ("aloha_mesh_read_obj", C.c_int, [C.c_char_p, C.POINTER(_P)]),
     ("aloha_tri_write", C.c_int, [C.c_char_p, _P]),
     ("aloha_tri_read", C.c_int, [C.c_char_p, C.POINTER(_P)]),
     ("aloha_tris_upload", C.c_int, [_P, _P, _I, _IP, _IP, _P, _IP]),

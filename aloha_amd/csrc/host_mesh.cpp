@@ -604,3 +604,105 @@ extern "C" int aloha_tris_build_host(const aloha_mesh *mesh, int32_t n, const in
   if (tris && nt) std::memcpy(tris, rb.dev_tris.data(), rb.dev_tris.size() * sizeof(float));
   return ALOHA_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Wavefront OBJ input (the mesh mode of chopSuey, chopSuey.cpp:400-421, and
+// Mesh::loadOBJ, mesh.cpp:39-85): positions and the triangulated faces of the
+// first shape, as MeshSplitter reads geoms[0] only.  Polygons are fanned
+// (v0, vi, vi+1); indices may be negative (relative to the vertices read so
+// far) or carry /vt/vn parts, which are ignored.
+// ---------------------------------------------------------------------------
+extern "C" int aloha_mesh_read_obj(const char *path, aloha_mesh **out) {
+  if (!path || !out) return ALOHA_FAIL_GLOBAL(ALOHA_ERR_ARG, "aloha_mesh_read_obj: NULL");
+  std::ifstream f(path);
+  if (!f) return ALOHA_FAIL_GLOBAL(ALOHA_ERR_IO, "aloha_mesh_read_obj: cannot open %s", path);
+  auto *m = new aloha_mesh;
+  std::string line;
+  bool shape_has_faces = false, done = false;
+  int64_t lineno = 0;
+  std::vector<int64_t> poly;
+  while (!done && std::getline(f, line)) {
+    ++lineno;
+    const char *s = line.c_str();
+    while (*s == ' ' || *s == '\t') ++s;
+    if ((s[0] == 'o' || s[0] == 'g') && (s[1] == ' ' || s[1] == '\t' || s[1] == 0)) {
+      if (shape_has_faces) done = true;  // a second shape starts: geoms[0] is complete
+      continue;
+    }
+    if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
+      float v[3];
+      char *end = nullptr;
+      const char *p = s + 2;
+      for (int k = 0; k < 3; ++k) {
+        v[k] = std::strtof(p, &end);
+        if (end == p) {
+          delete m;
+          return ALOHA_FAIL_GLOBAL(ALOHA_ERR_IO, "aloha_mesh_read_obj: bad vertex on line %lld",
+                                   (long long)lineno);
+        }
+        p = end;
+      }
+      m->verts.insert(m->verts.end(), v, v + 3);
+      continue;
+    }
+    if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
+      poly.clear();
+      const char *p = s + 2;
+      const int64_t nv = (int64_t)m->verts.size() / 3;
+      while (true) {
+        while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
+        if (!*p) break;
+        char *end = nullptr;
+        const long long idx = std::strtoll(p, &end, 10);
+        if (end == p || idx == 0) {
+          delete m;
+          return ALOHA_FAIL_GLOBAL(ALOHA_ERR_IO, "aloha_mesh_read_obj: bad face on line %lld",
+                                   (long long)lineno);
+        }
+        const int64_t v = idx > 0 ? idx - 1 : nv + idx;
+        if (v < 0 || v >= nv) {
+          delete m;
+          return ALOHA_FAIL_GLOBAL(ALOHA_ERR_IO, "aloha_mesh_read_obj: index out of range on line %lld",
+                                   (long long)lineno);
+        }
+        poly.push_back(v);
+        p = end;
+        while (*p && *p != ' ' && *p != '\t' && *p != '\r') ++p;  // skip /vt/vn
+      }
+      if (poly.size() < 3) {
+        delete m;
+        return ALOHA_FAIL_GLOBAL(ALOHA_ERR_IO, "aloha_mesh_read_obj: face with %d vertices on line %lld",
+                                 (int)poly.size(), (long long)lineno);
+      }
+      for (size_t k = 1; k + 1 < poly.size(); ++k) {
+        m->tris.push_back((int32_t)poly[0]);
+        m->tris.push_back((int32_t)poly[k]);
+        m->tris.push_back((int32_t)poly[k + 1]);
+      }
+      shape_has_faces = true;
+    }
+  }
+  // vertices listed after the first shape still belong to the shared array
+  while (std::getline(f, line)) {
+    const char *s = line.c_str();
+    while (*s == ' ' || *s == '\t') ++s;
+    if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
+      float v[3] = {0, 0, 0};
+      char *end = nullptr;
+      const char *p = s + 2;
+      for (int k = 0; k < 3; ++k) {
+        v[k] = std::strtof(p, &end);
+        p = end;
+      }
+      m->verts.insert(m->verts.end(), v, v + 3);
+    }
+  }
+  if (m->verts.size() / 3 > (size_t)INT32_MAX) {
+    delete m;
+    return ALOHA_FAIL_GLOBAL(ALOHA_ERR_LIMIT, "aloha_mesh_read_obj: too many vertices");
+  }
+  box_empty(m->bounds);
+  for (size_t i = 0; i < m->verts.size() / 3; ++i) box_extend(m->bounds, &m->verts[3 * i]);
+  *out = m;
+  return ALOHA_OK;
+}

@@ -103,6 +103,19 @@ void Renderer::check(int rc) {
 }
 
 Renderer::Renderer(int commRank, int commSize, int numIslands, const std::string &inFileName) {
+  // mode from the file extension (waikikiRenderer.cpp:390-394)
+  if (inFileName.size() >= 4 && inFileName.compare(inFileName.size() - 4, 4, ".tri") == 0) {
+    aloha_mesh *mesh = nullptr;
+    check(aloha_tri_read(inFileName.c_str(), &mesh));
+    try {
+      initMesh(commRank, commSize, numIslands, mesh);
+    } catch (...) {
+      aloha_mesh_destroy(mesh);
+      throw;
+    }
+    aloha_mesh_destroy(mesh);
+    return;
+  }
   aloha_box3i cr, vr;
   aloha_box3f sr;
   int32_t n = 0;
@@ -130,9 +143,11 @@ Renderer::Renderer(int commRank, int commSize, int numIslands, const box3f &boun
   init(commRank, commSize, numIslands, bricks, voxels, dtype);
 }
 
-void Renderer::init(int commRank, int commSize, int nIslands,
-                    const std::vector<aloha_brick_desc> &bricks,
-                    const std::vector<const void *> &voxels, int dtype) {
+Renderer::Renderer(int commRank, int commSize, int numIslands, const aloha_mesh *mesh) {
+  initMesh(commRank, commSize, numIslands, mesh);
+}
+
+void Renderer::initRanks(int commRank, int commSize, int nIslands) {
   if (nIslands < 1 || commSize < nIslands || commSize % nIslands != 0)
     throw std::invalid_argument("aloha: commSize must be a multiple of numIslands");
   commWorld.rank = commRank;
@@ -140,8 +155,49 @@ void Renderer::init(int commRank, int commSize, int nIslands,
   numIslands = nIslands;
   islandSize_ = commSize / nIslands;
   islandID = commRank / islandSize_;
-  mode = Mode::Volume;
   check(aloha_ctx_create(0, &ctx_));
+}
+
+// TriMesh set-up (waikikiRenderer.cpp:396-593): clusters in blocks per rank
+// (partitionRoundRobin), rank r's i-th cluster coloured by
+// randomColor(r * (numClusters / ranks) + i), one BVH per rank.
+void Renderer::initMesh(int commRank, int commSize, int nIslands, const aloha_mesh *mesh) {
+  if (!mesh) throw std::invalid_argument("aloha: mesh is NULL");
+  initRanks(commRank, commSize, nIslands);
+  mode = Mode::TriMesh;
+  aloha_box3f b;
+  int32_t nc = 0;
+  check(aloha_mesh_info(mesh, nullptr, nullptr, &b, &nc));
+  modelBounds = {{b.lower[0], b.lower[1], b.lower[2]}, {b.upper[0], b.upper[1], b.upper[2]}};
+  const int n = nc > 0 ? nc : 1;
+  check(aloha_partitioner_create_rr(n, islandSize_, &part_));
+  perRank.assign(islandSize_, {});
+  std::vector<int32_t> offs{0}, ids;
+  std::vector<uint32_t> colors;
+  const int per = n / islandSize_;
+  for (int r = 0; r < islandSize_; ++r) {
+    int32_t cnt = 0;
+    std::vector<int32_t> got(n + 1);
+    check(aloha_partitioner_rank_bricks(part_, r, got.data(), (int32_t)got.size(), &cnt));
+    perRank[r].assign(got.begin(), got.begin() + cnt);
+    for (int i = 0; i < cnt; ++i) {
+      ids.push_back(got[i]);
+      colors.push_back((uint32_t)(r * per + i));
+    }
+    offs.push_back((int32_t)ids.size());
+  }
+  geomIds_.resize(islandSize_);
+  check(aloha_tris_upload(ctx_, mesh, islandSize_, offs.data(), ids.data(), colors.data(),
+                          geomIds_.data()));
+  compositeOrder.resize(islandSize_);
+  for (int r = 0; r < islandSize_; ++r) compositeOrder[r] = r;
+}
+
+void Renderer::init(int commRank, int commSize, int nIslands,
+                    const std::vector<aloha_brick_desc> &bricks,
+                    const std::vector<const void *> &voxels, int dtype) {
+  initRanks(commRank, commSize, nIslands);
+  mode = Mode::Volume;
   std::vector<aloha_box3f> domains(bricks.size());
   for (size_t i = 0; i < bricks.size(); ++i) domains[i] = bricks[i].space_range;
   check(aloha_partitioner_create_kd(domains.data(), (int32_t)domains.size(), islandSize_, &part_));
@@ -167,9 +223,9 @@ Renderer::~Renderer() {
 
 void Renderer::freeImages() {
   for (void *p : {(void *)d_rank_rgba_, (void *)d_rank_rgba8_, (void *)d_rank_depth_,
-                  (void *)d_rank_accum_, (void *)d_comp_})
+                  (void *)d_rank_accum_, (void *)d_comp_, (void *)d_comp_depth_})
     if (p) (void)hipFree(p);
-  d_rank_rgba_ = d_rank_accum_ = d_comp_ = d_rank_depth_ = nullptr;
+  d_rank_rgba_ = d_rank_accum_ = d_comp_ = d_rank_depth_ = d_comp_depth_ = nullptr;
   d_rank_rgba8_ = nullptr;
 }
 
@@ -209,6 +265,7 @@ void Renderer::handleResizeEvent(const vec2i &newSize) {
   alloc((void **)&d_rank_depth_, P * npix * 4);
   alloc((void **)&d_rank_accum_, P * npix * 16);
   alloc((void **)&d_comp_, npix * 16);
+  alloc((void **)&d_comp_depth_, npix * 4);
   accumID = 0;
 }
 
@@ -305,23 +362,33 @@ void Renderer::render(uint32_t *fb) {
   const int W = fbSize.x, H = fbSize.y;
   const size_t npix = (size_t)W * H;
   std::vector<int32_t> offs{0}, ids;
-  for (int r = 0; r < islandSize_; ++r) {
-    for (int b : perRank[r]) ids.push_back(brickIds_[b]);
-    offs.push_back((int32_t)ids.size());
-  }
+  if (mode == Mode::Volume)
+    for (int r = 0; r < islandSize_; ++r) {
+      for (int b : perRank[r]) ids.push_back(brickIds_[b]);
+      offs.push_back((int32_t)ids.size());
+    }
   for (int i = 0; i < numIslands; ++i) {
     p.island_id = i;
     const size_t base = (size_t)i * islandSize_ * npix;
     aloha_images img{d_rank_rgba_ + 4 * base, d_rank_rgba8_ + base, d_rank_depth_ + base,
                      d_rank_accum_ + 4 * base, nullptr};
-    check(aloha_render_ranks(ctx_, &p, W, H, islandSize_, offs.data(), ids.data(), &img));
+    if (mode == Mode::TriMesh)
+      check(aloha_render_tris(ctx_, &p, W, H, islandSize_, geomIds_.data(), &img, nullptr));
+    else
+      check(aloha_render_ranks(ctx_, &p, W, H, islandSize_, offs.data(), ids.data(), &img));
   }
-  // composite order expanded over islands (:1043-1050)
-  std::vector<int32_t> order;
-  for (int i = 0; i < numIslands; ++i)
-    for (int r : compositeOrder) order.push_back(i * islandSize_ + r);
-  check(aloha_composite_over(ctx_, d_rank_rgba_, commWorld.size, order.data(), W, H, d_comp_,
-                             commWorld.rank == displayRank ? fb : nullptr));
+  if (mode == Mode::TriMesh) {
+    // IceT Z_BUFFER / REDUCE with an RGBA8 colour and float depth (:903-909)
+    check(aloha_composite_zmin(ctx_, d_rank_rgba8_, d_rank_depth_, commWorld.size, W, H,
+                               commWorld.rank == displayRank ? fb : nullptr, d_comp_depth_));
+  } else {
+    // composite order expanded over islands (:1043-1050)
+    std::vector<int32_t> order;
+    for (int i = 0; i < numIslands; ++i)
+      for (int r : compositeOrder) order.push_back(i * islandSize_ + r);
+    check(aloha_composite_over(ctx_, d_rank_rgba_, commWorld.size, order.data(), W, H, d_comp_,
+                               commWorld.rank == displayRank ? fb : nullptr));
+  }
   check(aloha_sync(ctx_));
   if (commWorld.rank != displayRank) {
     // IceT hands non-display ranks no image; their fb keeps their own frame

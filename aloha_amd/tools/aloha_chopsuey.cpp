@@ -1,8 +1,10 @@
-// aloha_chopsuey -- bricks a raw volume into a .vol file, the volume mode of
-// chopSuey (chopSuey.cpp:361-440, :232-359), over the aloha_amd C ABI.
+// aloha_chopsuey -- chopSuey (chopSuey.cpp:361-440) over the aloha_amd C ABI:
+// bricks a raw volume into a .vol file (VolumeSplitter, :232-359), or splits
+// an .obj mesh into clusters and writes a .tri file (MeshSplitter, :48-229).
 //
 //   aloha_chopsuey in.raw -dims X Y Z [-type uint8|uint16|float | -bpc N]
 //                  [-n numBricks] [-o out.vol]
+//   aloha_chopsuey in.obj [-n numClusters] [-o out.tri]
 #include <cfloat>
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +43,24 @@ int main(int argc, char **argv) {
     } else return usage("unknown cmdline arg '" + a + "'");
   }
   if (in.empty()) return usage("no filename specified");
+  if (in.size() >= 4 && in.compare(in.size() - 4, 4, ".obj") == 0) {
+    aloha_mesh *mesh = nullptr;
+    int32_t count = 0;
+    if (aloha_mesh_read_obj(in.c_str(), &mesh)) return usage(aloha_last_error(nullptr));
+    if (aloha_mesh_split(mesh, n, &count) || aloha_tri_write(out.c_str(), mesh)) {
+      aloha_mesh_destroy(mesh);
+      return usage(aloha_last_error(nullptr));
+    }
+    std::vector<int64_t> ranges(2 * (size_t)count);
+    std::vector<aloha_box3f> bounds(count);
+    aloha_mesh_clusters(mesh, ranges.data(), bounds.data(), count, &count);
+    for (int i = 0; i < count; ++i)
+      printf("%lld %lld [(%g,%g,%g),(%g,%g,%g)]\n", (long long)ranges[2 * i],
+             (long long)ranges[2 * i + 1], bounds[i].lower[0], bounds[i].lower[1],
+             bounds[i].lower[2], bounds[i].upper[0], bounds[i].upper[1], bounds[i].upper[2]);
+    aloha_mesh_destroy(mesh);
+    return 0;
+  }
   if (!dims[0] || !dims[1] || !dims[2]) return usage("no input dimensions specified");
   if (bpc != 1 && bpc != 2 && bpc != 4) return usage("no data type/bpc specified");
   const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];

@@ -329,6 +329,10 @@ int aloha_mesh_synth_spheres(int32_t num_spheres, int32_t rings, int32_t segment
  * domain's longest axis, triangles with min vertex < plane to the left.
  * Reorders the mesh's triangles cluster by cluster. */
 int aloha_mesh_split(aloha_mesh *mesh, int32_t num_clusters, int32_t *count);
+/* Wavefront OBJ (Mesh::loadOBJ mesh.cpp:39-85 as chopSuey uses it,
+ * chopSuey.cpp:400-421): all vertices, the triangulated faces of the first
+ * shape (MeshSplitter reads geoms[0] only); bounds over all vertices. */
+int aloha_mesh_read_obj(const char *path, aloha_mesh **out);
 /* .tri files (MeshSplitter::saveTris chopSuey.cpp:200-218; reader
  * waikikiRenderer.cpp:394-451).  An unsplit mesh is written as one cluster. */
 int aloha_tri_write(const char *path, const aloha_mesh *mesh);

@@ -81,18 +81,24 @@ struct DistributedRenderer {
 struct Renderer : mpi::DistributedRenderer {
   enum class Mode { TriMesh, Volume };
 
-  // Loads a .vol brick file, assigns bricks to commSize virtual ranks with
-  // partitionKD and uploads them (waikikiRenderer.cpp:344-846).
+  // Loads a .vol brick file (Mode::Volume: bricks to commSize virtual ranks
+  // with partitionKD) or a .tri cluster file (Mode::TriMesh: clusters in
+  // blocks per rank, partitionRoundRobin) and uploads it
+  // (waikikiRenderer.cpp:344-846).
   Renderer(int commRank, int commSize, int numIslands, const std::string &inFileName);
   // Same from bricks already in memory (x-fastest voxels, encoding dtype).
   Renderer(int commRank, int commSize, int numIslands, const box3f &modelBounds,
            const std::vector<aloha_brick_desc> &bricks, const std::vector<const void *> &voxels,
            int dtype);
+  // Mode::TriMesh from a mesh in memory (split into clusters or not); the
+  // mesh is not kept after the upload.
+  Renderer(int commRank, int commSize, int numIslands, const aloha_mesh *mesh);
   ~Renderer();
 
-  // One frame: every virtual rank marches its bricks, the rank images are
-  // 'over'-composited in visibility order and packed to RGBA8 into the
-  // device buffer fbPointer (waikikiRenderer.cpp:1015-1080).
+  // One frame: every virtual rank marches its bricks (or traces its
+  // clusters), the rank images are 'over'-composited in visibility order
+  // (depth-composited for meshes) and packed to RGBA8 into the device buffer
+  // fbPointer (waikikiRenderer.cpp:1015-1080).
   void render(uint32_t *fbPointer);
 
   void handleCameraEvent(const vec3f &org, const vec3f &dir_00, const vec3f &dir_du,
@@ -103,15 +109,18 @@ struct Renderer : mpi::DistributedRenderer {
 
   void resetAccum() { accumID = 0; }
   void setDt(float dt) { this->dt = dt; }
-  // the last composited frame as float RGBA (device, fbSize.x*fbSize.y*4)
+  // the last composited frame as float RGBA (device, fbSize.x*fbSize.y*4;
+  // Mode::Volume -- the depth composite of meshes is RGBA8 only)
   const float *compositeRGBA() const { return d_comp_; }
+  // the last depth-composited depth buffer (device, Mode::TriMesh)
+  const float *compositeDepth() const { return d_comp_depth_; }
   // status of the last failing call ("" if none)
   const std::string &lastError() const { return error_; }
 
   Mode mode = Mode::Volume;
   box3f modelBounds{};
   std::vector<int> compositeOrder;            // rank ids, front to back
-  std::vector<std::vector<int>> perRank;      // brick ids per rank
+  std::vector<std::vector<int>> perRank;      // brick (cluster) ids per rank
   int accumID = 0;
   float dt = 2.f;
   vec2i fbSize{1, 1};
@@ -129,17 +138,21 @@ struct Renderer : mpi::DistributedRenderer {
   void init(int commRank, int commSize, int numIslands,
             const std::vector<aloha_brick_desc> &bricks, const std::vector<const void *> &voxels,
             int dtype);
+  void initRanks(int commRank, int commSize, int numIslands);
+  void initMesh(int commRank, int commSize, int numIslands, const aloha_mesh *mesh);
   void check(int rc);
   void freeImages();
   aloha_ctx *ctx_ = nullptr;
   aloha_partitioner *part_ = nullptr;
   std::vector<int> brickIds_;
+  std::vector<int32_t> geomIds_;  // Mode::TriMesh: one geometry per rank
   int islandSize_ = 1;
   float *d_rank_rgba_ = nullptr;
   uint32_t *d_rank_rgba8_ = nullptr;
   float *d_rank_depth_ = nullptr;
   float *d_rank_accum_ = nullptr;
   float *d_comp_ = nullptr;
+  float *d_comp_depth_ = nullptr;
   std::string error_;
 };
 

@@ -198,3 +198,36 @@ def test_full_size_sampled_rows():
                                    rows=(int(y), int(y) + 1))
             assert np.array_equal(d[y], ref["depth"][y]), (r, y)
             assert np.array_equal(c[y], ref["rgba8"][y]), (r, y)
+
+
+def test_headless_tri_renderer_api(tmp_path):
+    """aloha_headless on a .tri file: maui::Renderer in TriMesh mode
+    (round-robin clusters, colour ids, per-rank images, depth composite)
+    against the oracle, 2 frames (accumulated), display rank and rank 1."""
+    import subprocess
+    from pathlib import Path
+    print("\n[case] aloha_headless .tri, 4 ranks", flush=True)
+    wl = WL.surface(num_spheres=30, rings=6, segments=10, num_clusters=6, num_ranks=4, W=96, H=72,
+                    seed=41, rmin=0.06, rmax=0.15, name="headless")
+    path = tmp_path / "s.tri"
+    wl.mesh.write_tri(str(path))
+    cam = [str(float(x)) for k in ("org", "dir_00", "dir_du", "dir_dv") for x in wl.cam[k]]
+    exe = Path(A.__file__).resolve().parent / "bin" / "aloha_headless"
+    refs = [_oracle_rank(wl, r) for r in range(4)]
+    o8, oz = O.composite_zmin(np.stack([x["rgba8"] for x in refs]), np.stack([x["depth"] for x in refs]))
+    for rank in (0, 1):
+        out = tmp_path / f"f{rank}"
+        r = subprocess.run([str(exe), str(path), "-ranks", "4", "-rank", str(rank), "-size",
+                            str(wl.W), str(wl.H), "-camera", *cam, "-frames", "2", "-o", str(out)],
+                           capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0
+        got8 = np.fromfile(str(out) + ".rgba8", np.uint32).reshape(wl.H, wl.W)
+        gotz = np.fromfile(str(out) + ".depth", np.float32).reshape(wl.H, wl.W)
+        assert np.array_equal(gotz, oz)
+        # frames 0 and 1 accumulate the same colour: (c + c) * (1/2) == c for the pack
+        if rank == 0:
+            assert np.array_equal(got8, o8)
+        else:
+            assert np.array_equal(got8, refs[1]["rgba8"])
+    assert (oz < 1).sum() > 1000

@@ -272,3 +272,58 @@ def test_watertight_shared_edges():
     out = O.tri_render(cam, v.reshape(2, 9), np.zeros(2, np.uint32), W, H, max_depth=8.0)
     assert (out["prim"] >= 0).all()
     assert (out["prim"] == 0).sum() > 0 and (out["prim"] == 1).sum() > 0
+
+
+OBJ = """# two shapes; MeshSplitter reads the first
+o first
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+f 1 2 3 4
+f 1/1/1 -2/2/2 -1/3/3
+g second
+v 0 0 1
+f 1 2 5
+v 2 2 2
+"""
+
+
+def test_read_obj_first_shape(tmp_path):
+    p = tmp_path / "m.obj"
+    p.write_text(OBJ)
+    m = A.Mesh.read_obj(str(p))
+    v, t = m.arrays()
+    # all 6 vertices; the quad fans into (0,1,2), (0,2,3); "-2/..-1" -> (0,2,3)
+    assert len(v) == 6 and v[5].tolist() == [2, 2, 2]
+    assert t.tolist() == [[0, 1, 2], [0, 2, 3], [0, 2, 3]]
+    b = m.info()["bounds"]
+    assert tuple(b.lower) == (0, 0, 0) and tuple(b.upper) == (2, 2, 2)
+    (tmp_path / "bad.obj").write_text("v 0 0 0\nf 1 2 9\n")
+    with pytest.raises(A.AlohaError):
+        A.Mesh.read_obj(str(tmp_path / "bad.obj"))
+
+
+def test_chopsuey_obj_to_tri(tmp_path):
+    """aloha_chopsuey in.obj -n 4 -o out.tri == Mesh.read_obj + split + write_tri."""
+    import subprocess
+    from pathlib import Path
+    src = A.Mesh.spheres(30, 5, 8, _domain(), 0.03, 0.1, 0.1, seed=17)
+    v, t = src.arrays()
+    obj = tmp_path / "s.obj"
+    with open(obj, "w") as f:
+        f.write("o spheres\n")
+        f.writelines(f"v {x:.9g} {y:.9g} {z:.9g}\n" for x, y, z in v.astype(float))
+        f.writelines(f"f {a + 1} {b + 1} {c + 1}\n" for a, b, c in t)
+    exe = Path(A.__file__).resolve().parent / "bin" / "aloha_chopsuey"
+    r = subprocess.run([str(exe), str(obj), "-n", "4", "-o", str(tmp_path / "s.tri")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert len(r.stdout.strip().splitlines()) == 4
+    got = A.Mesh.read_tri(str(tmp_path / "s.tri"))
+    ref = A.Mesh.read_obj(str(obj))
+    ref.split(4)
+    gv, gt = got.arrays()
+    rv, rt = ref.arrays()
+    assert np.array_equal(gv, v) and np.array_equal(gv, rv) and np.array_equal(gt, rt)
+    assert [(f, l) for f, l, _ in got.clusters()] == [(f, l) for f, l, _ in ref.clusters()]
