@@ -320,6 +320,32 @@ class Context:
         self._check(lib().aloha_render_composite(self._h, C.byref(params), W, H, len(ranks), offs,
                                                  ids, o, p(out_rgba), p(out_rgba8), p(nsamples)))
 
+    def set_march_schedule(self, ranks_lanes: int, fused_lanes: int) -> None:
+        self._check(lib().aloha_set_march_schedule(self._h, ranks_lanes, fused_lanes))
+
+    def march_schedule(self) -> tuple[int, int]:
+        r, f = C.c_int32(), C.c_int32()
+        self._check(lib().aloha_get_march_schedule(self._h, C.byref(r), C.byref(f)))
+        return r.value, f.value
+
+    def autotune_march(self, params: RenderParams, W: int, H: int, ranks: Sequence[Sequence[int]],
+                       out: RankImages, order: Sequence[int] | None = None, out_rgba=None,
+                       out_rgba8=None, reps: int = 5):
+        """Pick the fastest lanes-per-ray schedule for these launches; returns
+        the chosen (ranks_lanes, fused_lanes) and the timing table."""
+        offs, ids = _rank_table(ranks)
+        o = (C.c_int32 * len(order))(*order) if order is not None else None
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        imgs = out.as_c()
+        table = (C.c_float * 12)()
+        rows = C.c_int32()
+        self._check(lib().aloha_autotune_march(self._h, C.byref(params), W, H, len(ranks), offs, ids,
+                                               o, C.byref(imgs), p(out_rgba), p(out_rgba8), reps,
+                                               table, C.byref(rows)))
+        t = [(int(table[3 * i]), float(table[3 * i + 1]), float(table[3 * i + 2]))
+             for i in range(rows.value)]
+        return self.march_schedule(), t
+
     def footprint(self, params: RenderParams, W: int, H: int,
                   ranks: Sequence[Sequence[int]]) -> int:
         offs, ids = _rank_table(ranks)

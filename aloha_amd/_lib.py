@@ -117,6 +117,10 @@ _SIGS = [
     ("aloha_batch_autotune", C.c_int, [_P, _I, _FP, _FP, _I, _IP]),
     ("aloha_batch_capture", C.c_int, [_P]),
     ("aloha_batch_launch", C.c_int, [_P]),
+    ("aloha_set_march_schedule", C.c_int, [_P, _I, _I]),
+    ("aloha_get_march_schedule", C.c_int, [_P, _IP, _IP]),
+    ("aloha_autotune_march", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP,
+                                       C.POINTER(Images), _P, _P, _I, _FP, _IP]),
     ("aloha_render_footprint", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP,
                                          C.POINTER(C.c_uint64)]),
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),

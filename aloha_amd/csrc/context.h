@@ -37,6 +37,8 @@ struct aloha_ctx {
   float cam[12] = {};
   bool has_cam = false;
   std::vector<TriGeom> geoms;
+  // lanes per ray of the per-rank and fused march launches (1, 2, 4, 8)
+  int march_lanes = 1, fused_lanes = 1;
   std::string err;
 };
 

@@ -136,6 +136,33 @@ void rank_rect(const float cam[12], const std::vector<DevBrick> &bricks, const i
   out[3] = (int32_t)std::min<double>(H, std::max(0.0, y1));
 }
 
+// The march advances t by repeated addition t += dt, so every t it reaches
+// must satisfy t + dt > t, and one brick visit must stay a bounded loop:
+// dt >= 2^-20 x the farthest brick corner's distance from org (t + dt then
+// moves by many ulps), and a brick diagonal of at most 2^22 steps.
+int check_steps(aloha_ctx *ctx, const float cam[12], const std::vector<DevBrick> &bricks,
+                const int32_t *ids, int n, float dt) {
+  for (int k = 0; k < n; ++k) {
+    const DevBrick &b = bricks[ids[k]];
+    double far2 = 0.0, diag2 = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      double d2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        const double v = ((c >> a) & 1) ? b.space_hi[a] : b.space_lo[a];
+        d2 += (v - cam[a]) * (v - cam[a]);
+      }
+      far2 = std::max(far2, d2);
+    }
+    for (int a = 0; a < 3; ++a)
+      diag2 += ((double)b.space_hi[a] - b.space_lo[a]) * ((double)b.space_hi[a] - b.space_lo[a]);
+    if (std::sqrt(far2) * 0x1p-20 > (double)dt || std::sqrt(diag2) > 0x1p22 * (double)dt)
+      return fail(ctx, ALOHA_ERR_LIMIT,
+                  "render: dt %g too small for brick %d (farthest corner at %g, diagonal %g)",
+                  dt, ids[k], std::sqrt(far2), std::sqrt(diag2));
+  }
+  return ALOHA_OK;
+}
+
 // Fill MarchArgs from the context and a validated rank table.
 int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t H, int32_t P,
                 const int32_t *rank_offsets, const int32_t *brick_ids, MarchArgs &a) {
@@ -159,6 +186,7 @@ int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t
       return fail(ctx, ALOHA_ERR_ARG, "render: brick id %d not uploaded", brick_ids[k]);
   if (!ctx->d_tf || ctx->tf_n < 1) return fail(ctx, ALOHA_ERR_STATE, "render: no transfer function set");
   if (!ctx->has_cam) return fail(ctx, ALOHA_ERR_STATE, "render: no camera set");
+  if (int rc = check_steps(ctx, ctx->cam, ctx->bricks, brick_ids, nb, p->dt)) return rc;
   if (int rc = sync_bricks(ctx)) return rc;
   std::memset(&a, 0, sizeof a);
   std::memcpy(a.org, ctx->cam, sizeof a.org);
@@ -385,7 +413,7 @@ extern "C" int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *par
   a.depth = out->depth;
   a.accum = (float4 *)out->accum;
   a.nsamples = out->nsamples;
-  HIP_CHECK(ctx, launch_march(a, false, ctx->stream));
+  HIP_CHECK(ctx, launch_march(a, false, ctx->march_lanes, ctx->stream));
   return ALOHA_OK;
 }
 
@@ -412,7 +440,101 @@ extern "C" int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params 
   a.rgba = (float4 *)d_out_rgba;
   a.rgba8 = d_out_rgba8;
   a.nsamples = d_nsamples;
-  HIP_CHECK(ctx, launch_march_composite(a, ctx->stream));
+  HIP_CHECK(ctx, launch_march_composite(a, ctx->fused_lanes, ctx->stream));
+  return ALOHA_OK;
+}
+
+namespace {
+bool lanes_ok(int n) { return n == 1 || n == 2 || n == 4 || n == 8; }
+}  // namespace
+
+extern "C" int aloha_set_march_schedule(aloha_ctx *ctx, int32_t ranks_lanes, int32_t fused_lanes) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_set_march_schedule: ctx is NULL");
+  if (!lanes_ok(ranks_lanes) || !lanes_ok(fused_lanes))
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_set_march_schedule: lanes per ray must be 1, 2, 4 or 8");
+  ctx->march_lanes = ranks_lanes;
+  ctx->fused_lanes = fused_lanes;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_get_march_schedule(const aloha_ctx *ctx, int32_t *ranks_lanes,
+                                        int32_t *fused_lanes) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_get_march_schedule: ctx is NULL");
+  if (ranks_lanes) *ranks_lanes = ctx->march_lanes;
+  if (fused_lanes) *fused_lanes = ctx->fused_lanes;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_autotune_march(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                    int32_t H, int32_t P, const int32_t *rank_offsets,
+                                    const int32_t *brick_ids, const int32_t *order,
+                                    const aloha_images *images, float *d_out_rgba,
+                                    uint32_t *d_out_rgba8, int32_t reps, float *table,
+                                    int32_t *rows) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_autotune_march: ctx is NULL");
+  if (!images || reps < 1) return fail(ctx, ALOHA_ERR_ARG, "aloha_autotune_march: bad argument");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  if (params->accum_id > 0 && !images->accum)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_autotune_march: accum_id > 0 needs the accum buffer");
+  const bool fused = order && (d_out_rgba || d_out_rgba8);
+  MarchArgs f = a;
+  if (fused) {
+    if (params->accum_id != 0 || (params->flags & (ALOHA_FLAG_HEATMAP | ALOHA_FLAG_CROSSHAIR)))
+      return fail(ctx, ALOHA_ERR_ARG, "aloha_autotune_march: the fused form needs accum_id 0, no flags");
+    std::vector<char> seen(P, 0);
+    for (int k = 0; k < P; ++k) {
+      if (order[k] < 0 || order[k] >= P || seen[order[k]])
+        return fail(ctx, ALOHA_ERR_ARG, "aloha_autotune_march: order is not a permutation");
+      seen[order[k]] = 1;
+      f.order[k] = order[k];
+    }
+    f.rgba = (float4 *)d_out_rgba;
+    f.rgba8 = d_out_rgba8;
+  }
+  a.rgba = (float4 *)images->rgba;
+  a.rgba8 = images->rgba8;
+  a.depth = images->depth;
+  a.accum = (float4 *)images->accum;
+  a.nsamples = images->nsamples;
+  hipEvent_t e0, e1;
+  HIP_CHECK(ctx, hipEventCreate(&e0));
+  HIP_CHECK(ctx, hipEventCreate(&e1));
+  auto time = [&](auto &&launch, float &ms) {
+    hipError_t e = launch();  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+    for (int r = 0; e == hipSuccess && r < reps; ++r) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    return e;
+  };
+  const int opts[4] = {1, 2, 4, 8};
+  float best_r = 1e30f, best_f = 1e30f;
+  int lr = ctx->march_lanes, lf = ctx->fused_lanes;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) {
+    const int g = opts[i];
+    float mr = 0.f, mf = 0.f;
+    e = time([&] { return launch_march(a, false, g, ctx->stream); }, mr);
+    if (e == hipSuccess && fused) e = time([&] { return launch_march_composite(f, g, ctx->stream); }, mf);
+    if (e != hipSuccess) break;
+    if (table) {
+      table[3 * i] = (float)g;
+      table[3 * i + 1] = mr;
+      table[3 * i + 2] = fused ? mf : 0.f;
+    }
+    if (mr < best_r) best_r = mr, lr = g;
+    if (fused && mf < best_f) best_f = mf, lf = g;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(ctx, ALOHA_ERR_HIP, "aloha_autotune_march: %s", hipGetErrorString(e));
+  ctx->march_lanes = lr;
+  ctx->fused_lanes = lf;
+  if (rows) *rows = 4;
   return ALOHA_OK;
 }
 
@@ -441,7 +563,7 @@ extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params 
   hipError_t e = hipMemsetAsync(bits, 0, words * 4, ctx->stream);
   a.touched = bits;
   a.bricks = ctx->d_bricks;
-  if (e == hipSuccess) e = launch_march(a, true, ctx->stream);
+  if (e == hipSuccess) e = launch_march(a, true, 1, ctx->stream);
   std::vector<uint32_t> host(words);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipMemcpy(host.data(), bits, words * 4, hipMemcpyDeviceToHost);
@@ -569,6 +691,8 @@ int build_diff(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t 
       return fail(ctx, ALOHA_ERR_ARG, "diff: brick id %d not uploaded", brick_ids[k]);
   for (int i = 0; i < 12 * V; ++i)
     if (!std::isfinite(cams[i])) return fail(ctx, ALOHA_ERR_ARG, "diff: non-finite camera");
+  for (int v = 0; v < V; ++v)
+    if (int rc = check_steps(ctx, cams + 12 * v, ctx->bricks, brick_ids, nb, p->dt)) return rc;
   if (!ctx->d_tf || ctx->tf_n < 1) return fail(ctx, ALOHA_ERR_STATE, "diff: no transfer function set");
   if (int rc = sync_bricks(ctx)) return rc;
   std::memset(&a, 0, sizeof a);
@@ -696,6 +820,8 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
         return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_create: order of view %d is not a permutation", v);
       seen[o] = 1;
     }
+    if (int rc2 = check_steps(ctx, cams + 12 * v, ctx->bricks, brick_ids, rank_offsets[P], params->dt))
+      return rc2;
     for (int s = 0; s < P; ++s)
       rank_rect(cams + 12 * v, ctx->bricks, brick_ids + rank_offsets[s],
                 rank_offsets[s + 1] - rank_offsets[s], W, H, &rects[((size_t)v * P + s) * 4]);

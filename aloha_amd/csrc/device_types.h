@@ -114,8 +114,9 @@ struct TrisArgs {
 };
 
 // launchers (march.hip, composite.hip)
-hipError_t launch_march(const MarchArgs &a, bool footprint, hipStream_t s);
-hipError_t launch_march_composite(const MarchArgs &a, hipStream_t s);
+// group: lanes per ray (1, 2, 4 or 8) of the per-rank and fused kernels
+hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream_t s);
+hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s);
 hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_blocks, hipStream_t s);
 hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
                             int32_t px, int32_t py, int32_t pz, hipStream_t s);

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "device_types.h"
 #include "march_device.h"
 
@@ -63,6 +65,141 @@ __device__ __forceinline__ void integrate(const MarchArgs &a, const float4 *s_tf
     c.z = c.z + (w * tf.z) * shade;
     c.w = c.w + w;
   }
+}
+
+// integrateBrick with the samples of one ray spread over a group of G lanes:
+// per step, lane gl evaluates sample k0 + gl (position, 7 taps, normal, TF,
+// shade -- the costly, independent part), then every lane of the group
+// applies the G samples in order with the sequential 'over' recurrence and
+// the loop test (t < t1 && alpha < .99) before each.  The positions come from
+// the same chain of additions t += dt, so the result is integrate()'s bit for
+// bit; samples past the termination are computed and dropped.
+//
+// Control flow is wave-uniform wherever lanes exchange values: the step loop
+// runs while any lane of the wave has work (a ballot), the shuffles execute
+// on every lane, and each group's work is predicated on `run`.  All lanes of
+// a group hold the same ray state, so the predicates are group-uniform.
+template <int G>
+__device__ __forceinline__ void group_integrate(const MarchArgs &a, const float4 *s_tf, bool run,
+                                                int bid, const float o[3], const float d[3],
+                                                float t0, float t1, float jitter, float4 &c,
+                                                uint32_t &ns, int gl, int gbase) {
+  const float dt = a.dt;
+  const float segs = floorf((t0 - dt * jitter) / dt);
+  float t = dt * (jitter + segs);
+  if (t < t0) t += dt;
+  // the brick record is read only by lanes that march it
+  const DevBrick *dbp = run ? a.bricks + bid : nullptr;
+  bool go = run && t < t1 && c.w < .99f;
+  while (__any(go)) {
+    float tk = t;
+    for (int j = 0; j < gl; ++j) tk += dt;
+    float tn = tk;  // base of the next step: G additions in all
+    for (int j = gl; j < G; ++j) tn += dt;
+    const bool valid = go && tk < t1;
+    float4 tf = make_float4(0.f, 0.f, 0.f, 0.f);
+    float shade = 0.f;
+    if (valid) {
+      const DevBrick &db = *dbp;
+      const BrickView b{db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1], db.n[2]};
+      const float x = (o[0] + tk * d[0]) - db.vox_lo[0];
+      const float y = (o[1] + tk * d[1]) - db.vox_lo[1];
+      const float z = (o[2] + tk * d[2]) - db.vox_lo[2];
+      Sample s;
+      if (!sample_fast(b, x, y, z, s)) s = sample_exact<false>(b, nullptr, db, x, y, z);
+      const float ex = s.gx + 1e-4f, ey = s.gy + 1e-4f, ez = s.gz + 1e-4f;
+      const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+      const float nx = s.gx / len, ny = s.gy / len, nz = s.gz / len;
+      tf = tf_lookup(s_tf, a.tf_n, s.val);
+      shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
+    }
+    const int vbits = valid ? 1 : 0;
+    bool live = go;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      int vj = vbits;
+      float tx = tf.x, ty = tf.y, tz = tf.z, tw = tf.w, sh = shade;
+      if constexpr (G > 1) {
+        const int src = gbase + j;
+        vj = __shfl(vbits, src);
+        tx = __shfl(tf.x, src), ty = __shfl(tf.y, src), tz = __shfl(tf.z, src);
+        tw = __shfl(tf.w, src), sh = __shfl(shade, src);
+      }
+      live = live && vj && c.w < .99f;
+      if (live) {
+        ++ns;
+        const float w = (1.f - c.w) * tw;
+        c.x = c.x + (w * tx) * sh;
+        c.y = c.y + (w * ty) * sh;
+        c.z = c.z + (w * tz) * sh;
+        c.w = c.w + w;
+      }
+    }
+    if (go) t = tn;
+    go = go && t < t1 && c.w < .99f;
+  }
+}
+
+// rank_pixel for a group of G lanes sharing pixel `pixel` (live: the group
+// has a pixel to march).  Same brick loop, in wave-uniform control flow.
+template <int G>
+__device__ __forceinline__ float4 group_rank_pixel(const MarchArgs &a, const float4 *s_tf,
+                                                   bool live, int slot, const float org[3],
+                                                   const float d[3], uint32_t pixel, float &z,
+                                                   uint32_t &ns, int gl, int gbase) {
+  const int b0 = live ? a.rank_off[slot] : 0, b1 = live ? a.rank_off[slot + 1] : 0;
+  uint32_t rng = rng_seed(pixel, (uint32_t)a.accum_id);
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int sid = 0; sid < a.spp; ++sid) {
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tmin = 0.f;
+    bool go = live;
+    while (__any(go)) {
+      int hit = -1;
+      float best = 0.f, bnear = 0.f, bt0 = 0.f, bfar = 0.f;
+      if (go) {
+        for (int k = b0; k < b1; ++k) {
+          const int bid = a.rank_bricks[k];
+          const DevBrick &db = a.bricks[bid];
+          float t0, t1, rep, tn;
+          if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
+          if (t0 > tmin) {
+            rep = t0;
+            tn = t0;
+          } else if (t1 > tmin) {
+            rep = t1;
+            tn = 0.f;
+          } else {
+            continue;
+          }
+          if (hit < 0 || rep < best) {
+            hit = bid;
+            best = rep;
+            bnear = tn;
+            bt0 = t0;
+            bfar = t1;
+          }
+        }
+      }
+      const bool run = go && hit >= 0;
+      float j = 0.f;
+      if (run) {
+        bnear = bnear + tmin;
+        j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
+      }
+      group_integrate<G>(a, s_tf, run, hit, org, d, bt0, bfar, j, c, ns, gl, gbase);
+      if (run) {
+        z = fminf(z, bnear / a.max_depth);
+        tmin = bfar + 1e-20f;
+      }
+      go = run && c.w < .99f;
+    }
+    acc.x = acc.x + c.x;
+    acc.y = acc.y + c.y;
+    acc.z = acc.z + c.z;
+    acc.w = acc.w + c.w;
+  }
+  return acc;
 }
 
 // One rank's spp-summed colour of one pixel: renderFrame's sample loop and
@@ -211,51 +348,248 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   }
 }
 
-// Fused raygen + sort-last composite: each pixel marches every virtual rank
-// in visibility order (a.order) from transparent, exactly as that rank's
-// image would hold it, and blends it 'over' the running colour with the
-// composite kernel's arithmetic.  The result equals march_kernel +
-// over_kernel bit for bit, without the ranks' images in HBM.  Ranks whose
-// rectangle misses the pixel contribute exact zeros and are skipped.
-__global__ __launch_bounds__(kBlock) void march_composite_kernel(const MarchArgs a) {
-  extern __shared__ float4 s_tf[];
-  const int bx = blockIdx.x * kTileW, by = blockIdx.y * kTileH;
-  bool active = false;
-  for (int s = 0; s < a.P; ++s) active |= block_meets(a, s, bx, by);  // block-uniform
-  if (active) {
-    load_tf(a, s_tf);
-    __syncthreads();
+// ---- group kernels: 32x8-pixel workgroup tiles, G lanes per ray ---------
+//
+// A workgroup owns one 32x8 tile of one rank image (per-rank form) or of the
+// final image (fused form).  Its waves pull batches of 64/G consecutive
+// pixels from an LDS counter, one pixel per group of G lanes, march each
+// pixel with its group (group_integrate),
+// and stage the per-pixel results in LDS; then the 256 threads write the
+// tile row by row (128-byte rows for the 4-byte outputs).  The per-rank grid
+// lists the tiles that meet a rank's projected rectangle first, so the long
+// marches start at once, and the tiles that miss every pixel of their rank
+// after them: those only write the miss result.
+constexpr int kGW = 32, kGH = 8;
+static_assert(kGW * kGH == kBlock, "one pixel per thread in the epilogue");
+
+struct Staged {
+  float4 acc[kBlock];
+  float z[kBlock];
+  uint32_t ns[kBlock];
+  float heat[kBlock];
+  int next;
+};
+
+// tile range of rank slot s's rectangle: [tx0, tx0+ntx) x [ty0, ty0+nty)
+__device__ __forceinline__ void rect_tiles(const MarchArgs &a, int s, int &tx0, int &ty0, int &ntx,
+                                           int &nty) {
+  const int32_t *rc = a.rect[s];
+  if (rc[0] >= rc[2] || rc[1] >= rc[3]) {
+    tx0 = ty0 = ntx = nty = 0;
+    return;
   }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int px = bx + (wave & 1) * 8 + (lane & 7);
-  const int py = by + (wave >> 1) * 8 + (lane >> 3);
+  tx0 = rc[0] / kGW;
+  ty0 = rc[1] / kGH;
+  ntx = (rc[2] + kGW - 1) / kGW - tx0;
+  nty = (rc[3] + kGH - 1) / kGH - ty0;
+}
+
+// Per-pixel epilogue of a rank image (renderFrame :395-415), as march_kernel.
+__device__ __forceinline__ void write_rank_pixel(const MarchArgs &a, long long out_idx, int px,
+                                                 int py, float4 acc, float z, uint32_t ns,
+                                                 float heat_t) {
+  const int W = a.W, H = a.H;
+  if (a.depth) a.depth[out_idx] = z;
+  if (a.nsamples) a.nsamples[out_idx] = ns;
+  if (a.flags & ALOHA_FLAG_HEATMAP) {
+    const float3 h = heat_map(heat_t);
+    acc = make_float4(h.x + .5f * acc.x, h.y + .5f * acc.y, h.z + .5f * acc.z, .5f + .5f * acc.w);
+  }
+  if (a.accum) {
+    if (a.accum_id > 0) {
+      const float4 prev = a.accum[out_idx];
+      acc.x = acc.x + prev.x;
+      acc.y = acc.y + prev.y;
+      acc.z = acc.z + prev.z;
+      acc.w = acc.w + prev.w;
+    }
+    a.accum[out_idx] = acc;
+  }
+  const float inv_frames = 1.f / (float)(a.accum_id + 1);
+  const float inv_spp = 1.f / (float)a.spp;
+  const float4 fr = make_float4(acc.x * inv_frames, acc.y * inv_frames, acc.z * inv_frames,
+                                acc.w * inv_frames);
+  if (a.rgba)
+    a.rgba[out_idx] = make_float4(fr.x * inv_spp, fr.y * inv_spp, fr.z * inv_spp, fr.w * inv_spp);
+  if (a.rgba8) {
+    float4 q = fr;
+    if ((a.flags & ALOHA_FLAG_CROSSHAIR) && (px == W / 2 || py == H / 2))
+      q = make_float4(1.f - q.x, 1.f - q.y, 1.f - q.z, 1.f - q.w);
+    a.rgba8[out_idx] = pack8(q.x * inv_spp) | (pack8(q.y * inv_spp) << 8) |
+                       (pack8(q.z * inv_spp) << 16) | (pack8(q.w * inv_spp) << 24);
+  }
+}
+
+// Pixel order inside a 32x8 tile: four 8x8 sub-tiles, row-major inside
+// each, so a batch of 64 (32, 16, 8) consecutive indices is an 8x8 (8x4, 8x2,
+// 8x1) pixel block -- compact footprints for the lanes of a wave.
+__device__ __forceinline__ void tile_pixel(int idx, int &lx, int &ly) {
+  lx = (idx >> 6) * 8 + (idx & 7);
+  ly = (idx >> 3) & 7;
+}
+__device__ __forceinline__ int tile_index(int lx, int ly) { return (lx >> 3) * 64 + ly * 8 + (lx & 7); }
+
+// Miss result of every pixel of one 32x8 tile of rank slot `slot`.
+__device__ __forceinline__ void fill_tile(const MarchArgs &a, int slot, int bx, int by) {
+  const int t = threadIdx.x;
+  const int px = bx + t % kGW, py = by + t / kGW;
   if (px >= a.W || py >= a.H) return;
-  const int pixel = px + a.W * py;
-  float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t ns = 0;
-  if (active && island_pixel(a, px, py)) {
-    float d[3];
-    pixel_ray(a.dir00, a.du, a.dv, px, py, d);
-    const float inv_spp = 1.f / (float)a.spp;
-    for (int k = 0; k < a.P; ++k) {
-      const int slot = a.order[k];  // wave-uniform
-      if (!in_rect(a, slot, px, py)) continue;
-      float z = 1.f;
-      const float4 acc = rank_pixel<false>(a, s_tf, slot, a.org, d, (uint32_t)pixel, z, ns);
-      // rank image value (accum_id 0): (acc * 1) * (1/spp)
-      const float4 c = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
-                                   (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
-      const float w = 1.f - C.w;
-      C.x = C.x + w * c.x;
-      C.y = C.y + w * c.y;
-      C.z = C.z + w * c.z;
-      C.w = C.w + w * c.w;
+  long long clock_begin = 0;
+  if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
+  const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
+                           ? (float)(clock64() - clock_begin) * (a.heat_scale / (float)a.spp)
+                           : 0.f;
+  write_rank_pixel(a, (long long)slot * a.W * a.H + px + (long long)a.W * py, px, py,
+                   make_float4(0.f, 0.f, 0.f, 0.f), 1.f, 0u, heat_t);
+}
+
+// Per-rank images.  Blocks [0, active) march the tiles that meet their rank's
+// rectangle (rank by rank); the `fill_blocks` blocks after them stride over
+// all (rank, tile) pairs and write the miss result of the tiles outside the
+// rectangles -- few blocks, many tiles each, so the cheap part costs no
+// per-block launch latency.
+template <int G>
+__global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, int tiles_x,
+                                                             int tiles_y, int fill_blocks) {
+  extern __shared__ float4 s_tf[];
+  __shared__ Staged st;
+  int b = (int)blockIdx.x, slot = -1, tx = 0, ty = 0;
+  for (int s = 0; s < a.P; ++s) {
+    int tx0, ty0, ntx, nty;
+    rect_tiles(a, s, tx0, ty0, ntx, nty);
+    if (b < ntx * nty) {
+      slot = s;
+      tx = tx0 + b % ntx;
+      ty = ty0 + b / ntx;
+      break;
+    }
+    b -= ntx * nty;
+  }
+  if (slot < 0) {  // fill part (block-uniform)
+    const int per = tiles_x * tiles_y;
+    for (int f = b; f < a.P * per; f += fill_blocks) {
+      const int s = f / per, t = f - s * per;
+      const int fx = t % tiles_x, fy = t / tiles_x;
+      int tx0, ty0, ntx, nty;
+      rect_tiles(a, s, tx0, ty0, ntx, nty);
+      if (fx >= tx0 && fx < tx0 + ntx && fy >= ty0 && fy < ty0 + nty) continue;  // marched
+      fill_tile(a, s, fx * kGW, fy * kGH);
+    }
+    return;
+  }
+  const int bx = tx * kGW, by = ty * kGH;
+  load_tf(a, s_tf);
+  if (threadIdx.x == 0) st.next = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
+  // one counter update per wave: its 64/G groups take consecutive pixels;
+  // the batch base is wave-uniform, so the loop exit is a scalar branch.
+  // Each pass takes a fresh batch, so kBlock passes always suffice.
+  for (int pass = 0; pass < kBlock; ++pass) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&st.next, 64 / G);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+    if (base >= kBlock) break;
+    // every lane stays in the loop body (predicated), so the group march
+    // below runs in wave-uniform control flow
+    const int idx = base + lane / G;
+    int lx, ly;
+    tile_pixel(idx, lx, ly);
+    const int px = bx + lx, py = by + ly;
+    long long clock_begin = 0;
+    if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
+    float z = 1.f;
+    uint32_t ns = 0;
+    const bool live = idx < kBlock && px < a.W && py < a.H && in_rect(a, slot, px, py) &&
+                      island_pixel(a, px, py);
+    float d[3] = {1.f, 0.f, 0.f};
+    if (live) pixel_ray(a.dir00, a.du, a.dv, px, py, d);
+    const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, a.org, d,
+                                           (uint32_t)(px + a.W * py), z, ns, gl, gbase);
+    if (gl == 0 && idx < kBlock) {
+      st.acc[idx] = acc;
+      st.z[idx] = z;
+      st.ns[idx] = ns;
+      if (a.flags & ALOHA_FLAG_HEATMAP)
+        st.heat[idx] = (float)(clock64() - clock_begin) * (a.heat_scale / (float)a.spp);
     }
   }
+  __syncthreads();
+  // epilogue: row-major over the tile, 128-byte rows for 4-byte outputs
+  const int t = threadIdx.x;
+  const int px = bx + t % kGW, py = by + t / kGW;
+  if (px >= a.W || py >= a.H) return;
+  const int k = tile_index(t % kGW, t / kGW);
+  write_rank_pixel(a, (long long)slot * a.W * a.H + px + (long long)a.W * py, px, py, st.acc[k],
+                   st.z[k], st.ns[k], st.heat[k]);
+}
+
+// Fused raygen + composite with lane groups: each pixel's ranks in
+// visibility order, each rank's march shared by the group's G lanes, blended
+// 'over' exactly as march_composite_kernel does.
+template <int G>
+__global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a) {
+  extern __shared__ float4 s_tf[];
+  __shared__ Staged st;
+  const int bx = blockIdx.x * kGW, by = blockIdx.y * kGH;
+  bool active = false;
+  for (int s = 0; s < a.P; ++s) {
+    const int32_t *rc = a.rect[s];
+    active |= bx < rc[2] && bx + kGW > rc[0] && by < rc[3] && by + kGH > rc[1];
+  }
+  if (active) {
+    load_tf(a, s_tf);
+    if (threadIdx.x == 0) st.next = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
+    const float inv_spp = 1.f / (float)a.spp;
+    for (int pass = 0; pass < kBlock; ++pass) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&st.next, 64 / G);
+      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+      if (base >= kBlock) break;
+      const int idx = base + lane / G;
+      int lx, ly;
+      tile_pixel(idx, lx, ly);
+      const int px = bx + lx, py = by + ly;
+      float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+      uint32_t ns = 0;
+      const bool pix = idx < kBlock && px < a.W && py < a.H && island_pixel(a, px, py);
+      float d[3] = {1.f, 0.f, 0.f};
+      if (pix) pixel_ray(a.dir00, a.du, a.dv, px, py, d);
+      for (int k = 0; k < a.P; ++k) {  // uniform: every lane walks every slot
+        const int slot = a.order[k];
+        const bool live = pix && in_rect(a, slot, px, py);
+        float z = 1.f;
+        const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, a.org, d,
+                                               (uint32_t)(px + a.W * py), z, ns, gl, gbase);
+        if (live) {
+          const float4 c = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
+                                       (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
+          const float w = 1.f - C.w;
+          C.x = C.x + w * c.x;
+          C.y = C.y + w * c.y;
+          C.z = C.z + w * c.z;
+          C.w = C.w + w * c.w;
+        }
+      }
+      if (gl == 0 && idx < kBlock) {
+        st.acc[idx] = C;
+        st.ns[idx] = ns;
+      }
+    }
+    __syncthreads();
+  }
+  const int t = threadIdx.x;
+  const int px = bx + t % kGW, py = by + t / kGW;
+  if (px >= a.W || py >= a.H) return;
+  const int k = tile_index(t % kGW, t / kGW);
+  const float4 C = active ? st.acc[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int pixel = px + a.W * py;
   if (a.rgba) a.rgba[pixel] = C;
   if (a.rgba8)
     a.rgba8[pixel] = pack8(C.x) | (pack8(C.y) << 8) | (pack8(C.z) << 16) | (pack8(C.w) << 24);
-  if (a.nsamples) a.nsamples[pixel] = ns;
+  if (a.nsamples) a.nsamples[pixel] = active ? st.ns[k] : 0u;
 }
 
 // Batched form: V views of the same ranks in one launch.  A work item is
@@ -353,9 +687,16 @@ __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dty
 
 }  // namespace
 
-hipError_t launch_march_composite(const MarchArgs &a, hipStream_t s) {
-  const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, 1);
-  march_composite_kernel<<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
+hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) {
+  const dim3 grid((a.W + kGW - 1) / kGW, (a.H + kGH - 1) / kGH, 1);
+  const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
+  switch (group) {
+    case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a); break;
+    case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a); break;
+    case 4: march_composite_group_kernel<4><<<grid, kBlock, lds, s>>>(a); break;
+    case 8: march_composite_group_kernel<8><<<grid, kBlock, lds, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -381,13 +722,37 @@ hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_block
   return hipGetLastError();
 }
 
-hipError_t launch_march(const MarchArgs &a, bool footprint, hipStream_t s) {
-  const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, a.P);
+// fill blocks of the per-rank launch: a few per CU, each striding over tiles
+constexpr int kFillBlocks = 1024;
+
+hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream_t s) {
   const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
-  if (footprint)
+  if (footprint) {
+    const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, a.P);
     march_kernel<true><<<grid, kBlock, lds, s>>>(a);
-  else
-    march_kernel<false><<<grid, kBlock, lds, s>>>(a);
+    return hipGetLastError();
+  }
+  // active tiles of every rank, then the fill blocks
+  const int tiles_x = (a.W + kGW - 1) / kGW, tiles_y = (a.H + kGH - 1) / kGH;
+  const long long fill_tiles = (long long)a.P * tiles_x * tiles_y;
+  if (fill_tiles > INT32_MAX) return hipErrorInvalidValue;
+  const int fill_blocks = (int)std::min<long long>(fill_tiles, kFillBlocks);
+  long long blocks = fill_blocks;
+  for (int r = 0; r < a.P; ++r) {
+    const int32_t *rc = a.rect[r];
+    if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
+    const int tx0 = rc[0] / kGW, ty0 = rc[1] / kGH;
+    blocks += (long long)((rc[2] + kGW - 1) / kGW - tx0) * ((rc[3] + kGH - 1) / kGH - ty0);
+  }
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)blocks);
+  switch (group) {
+    case 1: march_group_kernel<1><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
+    case 2: march_group_kernel<2><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
+    case 4: march_group_kernel<4><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
+    case 8: march_group_kernel<8><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -105,6 +105,15 @@ def main():
         ctx.render_ranks(params, W, H, ranks, imgs)
         ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8)
 
+    # lanes-per-ray schedule of the per-rank and fused launches, chosen by
+    # timing every choice on these exact launches (untimed; results are
+    # identical for every schedule)
+    fout = torch.empty_like(out)
+    fout8 = torch.empty_like(out8)
+    schedule, sched_table = ctx.autotune_march(params, W, H, ranks, imgs, wl.order, fout, fout8,
+                                               reps=10)
+    log(f"[bench] march schedule (lanes per ray: ranks, fused) = {schedule}")
+
     def timed(fn, n):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -133,8 +142,6 @@ def main():
     ms_comp = timed(lambda: ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8), args.steps)
     log(f"[bench] render {ms_render:.4f} ms, composite {ms_comp:.4f} ms")
     # fused form: the same frame (bit-identical result) in one launch
-    fout = torch.empty_like(out)
-    fout8 = torch.empty_like(out8)
     fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
     for _ in range(args.warmup):
         fused()
@@ -165,19 +172,20 @@ def main():
                        if not k.startswith("_")}
     dominant = "render" if ms_render >= ms_comp else "composite"
     roof = {
-        "kernel": "march_kernel" if dominant == "render" else "over_kernel",
+        "kernel": f"march_group_kernel<{schedule[0]}>" if dominant == "render" else "over_kernel",
         "bound": "hbm",
         "achieved": round(render_gbs if dominant == "render" else comp_gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round((render_gbs if dominant == "render" else comp_gbs) / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get("march_kernel" if dominant == "render" else "over_kernel"),
+        "traffic": traffic.get(f"march_group_kernel<{schedule[0]}>" if dominant == "render"
+                               else "over_kernel"),
         "traffic_source": "profiles/traffic.json (rocprofv3 --pmc, calibrated)" if traffic else None,
         "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
         "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
     }
     kernels = {
-        "march_kernel": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
+        f"march_group_kernel<{schedule[0]}>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
                          "GB/s": round(render_gbs, 1), "frac_hbm": round(render_gbs / HBM_PEAK_GBS, 4),
                          "Gsamples/s": round(samples / (ms_render * 1e-3) / 1e9, 3)},
         "over_kernel": {"ms": round(ms_comp, 5), "bytes": int(comp_bytes),
@@ -211,6 +219,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded)",
+        "schedule": {"lanes_per_ray_ranks": schedule[0], "lanes_per_ray_fused": schedule[1],
+                     "autotune": [{"lanes": l, "ranks_ms": round(r, 4), "fused_ms": round(f, 4)}
+                                  for l, r, f in sched_table]},
         "config": {"workload": desc, "virtual_ranks": P, "bricks": len(ids),
                    "samples_per_frame": samples, "touched_voxels": touched,
                    "brick_bytes_padded": ctx.brick_bytes(), "upload_s_incl_pcie": round(upload_s, 3)},

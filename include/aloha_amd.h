@@ -218,6 +218,21 @@ int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params *params, in
                            const int32_t *brick_ids, const int32_t *order, float *d_out_rgba,
                            uint32_t *d_out_rgba8, uint32_t *d_nsamples);
 
+/* Schedule of the per-rank (aloha_render_ranks) and fused
+ * (aloha_render_composite) march launches: lanes per ray, 1, 2, 4 or 8.  With
+ * n > 1 the n lanes of a group evaluate n consecutive samples of one ray at
+ * once and composite them in order; every schedule gives the same images bit
+ * for bit.  Default 1. */
+int aloha_set_march_schedule(aloha_ctx *ctx, int32_t ranks_lanes, int32_t fused_lanes);
+int aloha_get_march_schedule(const aloha_ctx *ctx, int32_t *ranks_lanes, int32_t *fused_lanes);
+/* Time both launches for every lanes-per-ray choice with these arguments
+ * (reps launches each, on the context stream, synchronous) and keep the
+ * fastest.  The fused launch is timed when order and an output are given.
+ * table (optional): 4 rows {lanes, ranks ms, fused ms}; *rows = 4. */
+int aloha_autotune_march(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                         int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
+                         const int32_t *order, const aloha_images *images, float *d_out_rgba,
+                         uint32_t *d_out_rgba8, int32_t reps, float *table, int32_t *rows);
 /* Count distinct voxels read by the march of the same launch (measurement:
  * compulsory bytes).  Synchronous. */
 int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,

@@ -9,15 +9,17 @@ import oracle as O
 from aloha_amd import RankImages
 
 
-def gpu_render(wl, ranks=None, P=None, *, accum=None, **params):
+def gpu_render(wl, ranks=None, P=None, *, accum=None, lanes=1, tf=None, **params):
     """Render every rank of `ranks` (brick-id lists; default: the workload's
-    kd assignment) on the GPU.  Returns numpy arrays (P,H,W[,4])."""
+    kd assignment) on the GPU with `lanes` lanes per ray.  Returns numpy
+    arrays (P,H,W[,4])."""
     import torch
     ranks = wl.rank_bricks if ranks is None else ranks
     P = len(ranks)
     with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ctx.set_march_schedule(lanes, lanes)
         ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
-        ctx.set_tf(wl.tf)
+        ctx.set_tf(wl.tf if tf is None else tf)
         ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
         out = RankImages.alloc(P, wl.W, wl.H, accum=True, nsamples=True)
         if accum is not None:

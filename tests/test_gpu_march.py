@@ -114,3 +114,59 @@ def test_fused_render_composite_bit_exact(which, c0, c1):
         assert torch.equal(out, ref)
         assert torch.equal(out8, ref8)
         assert int(ns.sum()) == int(imgs.nsamples.sum()) > 0
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_lane_schedules_identical(c0, lanes):
+    """Every lanes-per-ray schedule gives the 1-lane images bit for bit, with
+    the epilogue exercised (spp, accumulation, islands, crosshair) and with a
+    low-opacity TF whose rays run hundreds of samples (many group steps)."""
+    _case(f"blobs128 P=2, {lanes} lanes per ray vs 1")
+    acc = np.random.default_rng(4).random((2, c0.H, c0.W, 4), dtype=np.float32)
+    kw = dict(spp=2, accum_id=1, island_id=1, num_islands=2, flags=A.FLAG_CROSSHAIR)
+    one = gpu_render(c0, accum=acc, lanes=1, **kw)
+    many = gpu_render(c0, accum=acc, lanes=lanes, **kw)
+    for k in one:
+        assert np.array_equal(one[k], many[k]), k
+    thin = c0.tf.copy()
+    thin[:, 3] *= 0.03
+    one = gpu_render(c0, lanes=1, tf=thin)
+    many = gpu_render(c0, lanes=lanes, tf=thin)
+    for k in one:
+        assert np.array_equal(one[k], many[k]), k
+    assert one["nsamples"].max() > 100
+
+
+def test_lane_schedules_brick_loop_and_fused(c1):
+    """fbm512, one rank holding all 8 bricks (the brick loop) and the fused
+    launch, for every schedule, against the oracle-checked 1-lane result."""
+    import torch
+    _case("fbm512 P=1 all bricks + fused, lanes 1/2/4/8")
+    ranks = [list(range(len(c1.bricks)))]
+    ref = gpu_render(c1, ranks, lanes=1)
+    for lanes in (2, 4, 8):
+        g = gpu_render(c1, ranks, lanes=lanes)
+        for k in ref:
+            assert np.array_equal(ref[k], g[k]), (lanes, k)
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ids = [ctx.upload_brick(d, v) for d, v in zip(c1.bricks, c1.voxels)]
+        ctx.set_tf(c1.tf)
+        ctx.set_camera(c1.cam["org"], c1.cam["dir_00"], c1.cam["dir_du"], c1.cam["dir_dv"])
+        rk = [[ids[i] for i in r] for r in c1.rank_bricks]
+        outs = []
+        for lanes in (1, 2, 4, 8):
+            ctx.set_march_schedule(1, lanes)
+            o = torch.zeros((c1.H, c1.W, 4), device="cuda")
+            o8 = torch.zeros((c1.H, c1.W), dtype=torch.int32, device="cuda")
+            ns = torch.zeros((c1.H, c1.W), dtype=torch.int32, device="cuda")
+            ctx.render_composite(c1.params(), c1.W, c1.H, rk, c1.order, o, o8, ns)
+            outs.append((o, o8, ns))
+        torch.cuda.synchronize()
+        for o, o8, ns in outs[1:]:
+            assert torch.equal(o, outs[0][0]) and torch.equal(o8, outs[0][1])
+            assert torch.equal(ns, outs[0][2])
+        imgs = A.RankImages.alloc(len(rk), c1.W, c1.H)
+        (r_l, f_l), table = ctx.autotune_march(c1.params(), c1.W, c1.H, rk, imgs, c1.order,
+                                               outs[0][0], outs[0][1], reps=2)
+        print("autotune", r_l, f_l, table)
+        assert r_l in (1, 2, 4, 8) and f_l in (1, 2, 4, 8) and len(table) == 4
