@@ -19,7 +19,8 @@
 // dTF is accumulated in LDS per workgroup and flushed with one global atomic
 // per touched entry; dDensity with global float atomics on the brick's
 // unpadded voxel grid (clamped corners receive the weight of the clamped-away
-// neighbour, exactly as the forward reads them).
+// neighbour, exactly as the forward reads them), summed per voxel of the
+// shared 20-voxel footprint before the atomics where it fits.
 #include <hip/hip_runtime.h>
 
 #include "device_types.h"
@@ -222,6 +223,130 @@ __device__ __forceinline__ void scatter_tap(float *grad, const DevBrick &db, flo
   }
 }
 
+// x index ax + scx + ii of a footprint row, without dynamic register indexing
+template <int II>
+__device__ __forceinline__ void add_x(float row[3], int scx, float v) {
+  if (II == 0) {
+    row[0] += scx ? 0.f : v;
+    row[1] += scx ? v : 0.f;
+  } else {
+    row[1] += scx ? 0.f : v;
+    row[2] += scx ? v : 0.f;
+  }
+}
+
+__device__ __forceinline__ void atomic_voxel(float *grad, const DevBrick &db, int x, int y, int z,
+                                             float v) {
+  if (v == 0.f) return;
+  x = clampi(x, 0, db.n[0] - 1);
+  y = clampi(y, 0, db.n[1] - 1);
+  z = clampi(z, 0, db.n[2] - 1);
+  atomicAdd(&grad[((size_t)z * db.n[1] + y) * db.n[0] + x], v);
+}
+
+// Scatter of all seven taps of one sample through the shared 20-voxel
+// footprint (sample_fast's frame: R = 3 x-columns of the value tap's y/z
+// pair, one extra y row, one extra z plane).  Each tap's eight corner
+// weights are summed per footprint voxel in registers, so the sample issues
+// at most 20 atomics instead of 56; the sum equals the tap-by-tap scatter up
+// to float reassociation.  gV: d/d(value tap); gd: d/d(gradient component),
+// i.e. +gd for the +0.5 tap and -gd for the -0.5 tap of each axis.  Returns
+// false, scattering nothing, when the footprint does not fit.
+__device__ __forceinline__ bool scatter_fast(float *grad, const DevBrick &db, float x, float y,
+                                             float z, float gV, const float gd[3]) {
+  const float xm = x - .5f, xp = x + .5f;
+  const float ym = y - .5f, yp = y + .5f;
+  const float zm = z - .5f, zp = z + .5f;
+  const float fxm = floorf(xm), fxc = floorf(x), fxp = floorf(xp);
+  const float fym = floorf(ym), fyc = floorf(y), fyp = floorf(yp);
+  const float fzm = floorf(zm), fzc = floorf(z), fzp = floorf(zp);
+  const int ax = (int)fxm, cx = (int)fxc, bx = (int)fxp;
+  const int ay = (int)fym, cy = (int)fyc, by = (int)fyp;
+  const int az = (int)fzm, cz = (int)fzc, bz = (int)fzp;
+  const int scx = cx - ax, scy = cy - ay, scz = cz - az;
+  const bool ok = ((unsigned)scx <= 1u) & ((unsigned)scy <= 1u) & ((unsigned)scz <= 1u) &
+                  (bx == ax + 1) & (by == ay + 1) & (bz == az + 1) & (ax >= -1) &
+                  (ax <= db.n[0]) & (ay >= -1) & (ay <= db.n[1]) & (az >= -1) & (az <= db.n[2]);
+  if (!ok) return false;
+  const float wxm = xm - fxm, wxc = x - fxc, wxp = xp - fxp;
+  const float wym = ym - fym, wyc = y - fyc, wyp = yp - fyp;
+  const float wzm = zm - fzm, wzc = z - fzc, wzp = zp - fzp;
+  const float gXp = gd[0], gXm = -gd[0], gYp = gd[1], gYm = -gd[1], gZp = gd[2], gZm = -gd[2];
+  float A[2][2][3];  // A[kk][jj][i]: voxel (ax + i, cy + jj, cz + kk)
+  float YN[2][2];    // YN[kk][ii]: voxel (cx + ii, yn, cz + kk)
+  float ZN[2][2];    // ZN[jj][ii]: voxel (cx + ii, cy + jj, zn)
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      // value tap and the two x taps share the y rows cy, cy+1 and z planes cz, cz+1
+      const float wyz = (jj ? wyc : 1.f - wyc) * (kk ? wzc : 1.f - wzc);
+      const float v0 = gV * wyz * (1.f - wxc), v1 = gV * wyz * wxc;
+      A[kk][jj][0] = (scx ? 0.f : v0) + gXm * wyz * (1.f - wxm);
+      A[kk][jj][1] = (scx ? v0 : v1) + gXm * wyz * wxm + gXp * wyz * (1.f - wxp);
+      A[kk][jj][2] = (scx ? v1 : 0.f) + gXp * wyz * wxp;
+      YN[kk][jj] = 0.f;
+      ZN[kk][jj] = 0.f;
+    }
+  // y taps: x pair (cx, cx+1), z planes (cz, cz+1); rows ay, ay+1, ay+2
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const float wz = kk ? wzc : 1.f - wzc;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+      const float wxz = (ii ? wxc : 1.f - wxc) * wz;
+      const float r0 = gYm * wxz * (1.f - wym);
+      const float r1 = gYm * wxz * wym + gYp * wxz * (1.f - wyp);
+      const float r2 = gYp * wxz * wyp;
+      YN[kk][ii] += scy ? r0 : r2;  // the row outside the value tap's pair
+      if (ii == 0) {
+        add_x<0>(A[kk][0], scx, scy ? r1 : r0);
+        add_x<0>(A[kk][1], scx, scy ? r2 : r1);
+      } else {
+        add_x<1>(A[kk][0], scx, scy ? r1 : r0);
+        add_x<1>(A[kk][1], scx, scy ? r2 : r1);
+      }
+    }
+  }
+  // z taps: x pair (cx, cx+1), y rows (cy, cy+1); planes az, az+1, az+2
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj) {
+    const float wy = jj ? wyc : 1.f - wyc;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+      const float wxy = (ii ? wxc : 1.f - wxc) * wy;
+      const float p0 = gZm * wxy * (1.f - wzm);
+      const float p1 = gZm * wxy * wzm + gZp * wxy * (1.f - wzp);
+      const float p2 = gZp * wxy * wzp;
+      ZN[jj][ii] += scz ? p0 : p2;
+      if (ii == 0) {
+        add_x<0>(A[0][jj], scx, scz ? p1 : p0);
+        add_x<0>(A[1][jj], scx, scz ? p2 : p1);
+      } else {
+        add_x<1>(A[0][jj], scx, scz ? p1 : p0);
+        add_x<1>(A[1][jj], scx, scz ? p2 : p1);
+      }
+    }
+  }
+  const int yn = scy ? ay : ay + 2;
+  const int zn = scz ? az : az + 2;
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) atomic_voxel(grad, db, ax + i, cy + jj, cz + kk, A[kk][jj][i]);
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) atomic_voxel(grad, db, cx + ii, yn, cz + kk, YN[kk][ii]);
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) atomic_voxel(grad, db, cx + ii, cy + jj, zn, ZN[jj][ii]);
+  return true;
+}
+
 __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a) {
   extern __shared__ float4 s_mem[];
   float4 *s_tf = s_mem;
@@ -329,6 +454,8 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a)
           const float x = (org[0] + t * d[0]) - db.vox_lo[0];
           const float y = (org[1] + t * d[1]) - db.vox_lo[1];
           const float z = (org[2] + t * d[2]) - db.vox_lo[2];
+          if (dv == 0.f && dg[0] == 0.f && dg[1] == 0.f && dg[2] == 0.f) continue;
+          if (scatter_fast(grad, db, x, y, z, dv, dg)) continue;
           scatter_tap(grad, db, x, y, z, dv);
           scatter_tap(grad, db, x + .5f, y, z, dg[0]);
           scatter_tap(grad, db, x - .5f, y, z, -dg[0]);
