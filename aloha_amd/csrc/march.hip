@@ -524,72 +524,136 @@ __global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, 
                    st.z[k], st.ns[k], st.heat[k]);
 }
 
-// Fused raygen + composite with lane groups: each pixel's ranks in
-// visibility order, each rank's march shared by the group's G lanes, blended
-// 'over' exactly as march_composite_kernel does.
+// Fused raygen + composite.  The work items of a tile are (rank, pixel)
+// pairs of the ranks whose rectangle meets it, taken in visibility order in
+// chunks of up to kFusedChunk ranks: lane groups march items in parallel and
+// store each rank's pixel colour (its rank image value, accum_id 0) in LDS;
+// then each thread blends its pixel's chunk colours 'over' its running
+// colour in order -- march_composite's arithmetic, so the frame equals the
+// rank images composited by over_kernel bit for bit.  A rank that misses a
+// pixel contributes exact zeros, which 'over' leaves unchanged.
+// The fused tile is 16x4 pixels (two 8x4 halves, row-major inside each), a
+// quarter of the per-rank tile: its items are (rank, pixel) pairs, so the
+// 256 threads still have up to kFusedChunk x 64 items per chunk while the
+// grid has enough workgroups to spread the few tiles the volume covers.
+constexpr int kFusedChunk = 8;
+constexpr int kFW = 16, kFH = 4, kFPix = kFW * kFH;
+static_assert(kBlock % kFPix == 0 && kFPix % 64 == 0, "items of one batch share a rank");
+
+__device__ __forceinline__ void fused_pixel(int idx, int &lx, int &ly) {
+  lx = (idx >> 5) * 8 + (idx & 7);
+  ly = (idx >> 3) & 3;
+}
+__device__ __forceinline__ int fused_index(int lx, int ly) { return (lx >> 3) * 32 + ly * 8 + (lx & 7); }
+
+struct FusedStage {
+  float4 col[kFusedChunk][kFPix];
+  uint32_t ns[kFPix];
+  int ranks[ALOHA_MAX_RANKS];
+  int nranks, next;
+};
+
+// Transparent-black frame pixels of one fused tile outside every rectangle.
+__device__ __forceinline__ void fused_fill(const MarchArgs &a, int bx, int by) {
+  const int t = threadIdx.x;
+  if (t >= kFPix) return;
+  const int px = bx + t % kFW, py = by + t / kFW;
+  if (px >= a.W || py >= a.H) return;
+  const int pixel = px + a.W * py;
+  if (a.rgba) a.rgba[pixel] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (a.rgba8) a.rgba8[pixel] = 0u;
+  if (a.nsamples) a.nsamples[pixel] = 0u;
+}
+
+// Grid: the tiles of the rectangles' bounding box (u[0..3] = tile range
+// x0, y0, nx, ny) first, then `fill_blocks` blocks striding over the other
+// tiles of the frame.
 template <int G>
-__global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a) {
+__global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a,
+                                                                       int4 u, int tiles_x,
+                                                                       int tiles_y,
+                                                                       int fill_blocks) {
   extern __shared__ float4 s_tf[];
-  __shared__ Staged st;
-  const int bx = blockIdx.x * kGW, by = blockIdx.y * kGH;
-  bool active = false;
-  for (int s = 0; s < a.P; ++s) {
-    const int32_t *rc = a.rect[s];
-    active |= bx < rc[2] && bx + kGW > rc[0] && by < rc[3] && by + kGH > rc[1];
+  __shared__ FusedStage fs;
+  const int nact = u.z * u.w;
+  if ((int)blockIdx.x >= nact) {  // fill part (block-uniform)
+    for (int f = (int)blockIdx.x - nact; f < tiles_x * tiles_y; f += fill_blocks) {
+      const int fx = f % tiles_x, fy = f / tiles_x;
+      if (fx >= u.x && fx < u.x + u.z && fy >= u.y && fy < u.y + u.w) continue;
+      fused_fill(a, fx * kFW, fy * kFH);
+    }
+    return;
   }
-  if (active) {
-    load_tf(a, s_tf);
-    if (threadIdx.x == 0) st.next = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
-    const float inv_spp = 1.f / (float)a.spp;
-    for (int pass = 0; pass < kBlock; ++pass) {
+  const int bx = (u.x + (int)blockIdx.x % u.z) * kFW, by = (u.y + (int)blockIdx.x / u.z) * kFH;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    int n = 0;
+    for (int k = 0; k < a.P; ++k) {
+      const int s = a.order[k];
+      const int32_t *rc = a.rect[s];
+      if (bx < rc[2] && bx + kFW > rc[0] && by < rc[3] && by + kFH > rc[1]) fs.ranks[n++] = s;
+    }
+    fs.nranks = n;
+  }
+  if (t < kFPix) fs.ns[t] = 0u;
+  __syncthreads();
+  const int nranks = fs.nranks;  // block-uniform
+  float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (nranks > 0) load_tf(a, s_tf);
+  const int lane = t & 63, gl = lane % G, gbase = lane - gl;
+  const float inv_spp = 1.f / (float)a.spp;
+  const int own = fused_index(t % kFW, (t / kFW) % kFH);  // pixel of thread t < kFPix
+  for (int c0 = 0; c0 < nranks; c0 += kFusedChunk) {
+    const int nc = min(kFusedChunk, nranks - c0);
+    if (t == 0) fs.next = 0;
+    __syncthreads();  // TF staged, counter reset, previous chunk blended
+    const int items = nc * kFPix;
+    for (int pass = 0; pass < kFusedChunk * kFPix; ++pass) {
       int base = 0;
-      if (lane == 0) base = atomicAdd(&st.next, 64 / G);
+      if (lane == 0) base = atomicAdd(&fs.next, 64 / G);
       base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-      if (base >= kBlock) break;
-      const int idx = base + lane / G;
+      if (base >= items) break;
+      // 64/G consecutive items share one rank (kFPix is a multiple of 64)
+      const int item = base + lane / G;
+      const int r = item / kFPix, idx = item - r * kFPix;
+      const int slot = fs.ranks[c0 + min(r, nc - 1)];
       int lx, ly;
-      tile_pixel(idx, lx, ly);
+      fused_pixel(idx, lx, ly);
       const int px = bx + lx, py = by + ly;
-      float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-      uint32_t ns = 0;
-      const bool pix = idx < kBlock && px < a.W && py < a.H && island_pixel(a, px, py);
+      const bool live = item < items && px < a.W && py < a.H && in_rect(a, slot, px, py) &&
+                        island_pixel(a, px, py);
       float d[3] = {1.f, 0.f, 0.f};
-      if (pix) pixel_ray(a.dir00, a.du, a.dv, px, py, d);
-      for (int k = 0; k < a.P; ++k) {  // uniform: every lane walks every slot
-        const int slot = a.order[k];
-        const bool live = pix && in_rect(a, slot, px, py);
-        float z = 1.f;
-        const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, a.org, d,
-                                               (uint32_t)(px + a.W * py), z, ns, gl, gbase);
-        if (live) {
-          const float4 c = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
-                                       (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
-          const float w = 1.f - C.w;
-          C.x = C.x + w * c.x;
-          C.y = C.y + w * c.y;
-          C.z = C.z + w * c.z;
-          C.w = C.w + w * c.w;
-        }
-      }
-      if (gl == 0 && idx < kBlock) {
-        st.acc[idx] = C;
-        st.ns[idx] = ns;
+      if (live) pixel_ray(a.dir00, a.du, a.dv, px, py, d);
+      float z = 1.f;
+      uint32_t ns = 0;
+      const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, a.org, d,
+                                             (uint32_t)(px + a.W * py), z, ns, gl, gbase);
+      if (gl == 0 && item < items) {
+        fs.col[r][idx] = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
+                                     (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
+        if (ns) atomicAdd(&fs.ns[idx], ns);
       }
     }
     __syncthreads();
+    if (t < kFPix)
+      for (int r = 0; r < nc; ++r) {
+        const float4 c = fs.col[r][own];
+        const float w = 1.f - C.w;
+        C.x = C.x + w * c.x;
+        C.y = C.y + w * c.y;
+        C.z = C.z + w * c.z;
+        C.w = C.w + w * c.w;
+      }
   }
-  const int t = threadIdx.x;
-  const int px = bx + t % kGW, py = by + t / kGW;
+  __syncthreads();
+  if (t >= kFPix) return;
+  const int px = bx + t % kFW, py = by + t / kFW;
   if (px >= a.W || py >= a.H) return;
-  const int k = tile_index(t % kGW, t / kGW);
-  const float4 C = active ? st.acc[k] : make_float4(0.f, 0.f, 0.f, 0.f);
   const int pixel = px + a.W * py;
   if (a.rgba) a.rgba[pixel] = C;
   if (a.rgba8)
     a.rgba8[pixel] = pack8(C.x) | (pack8(C.y) << 8) | (pack8(C.z) << 16) | (pack8(C.w) << 24);
-  if (a.nsamples) a.nsamples[pixel] = active ? st.ns[k] : 0u;
+  if (a.nsamples) a.nsamples[pixel] = fs.ns[own];
 }
 
 // Batched form: V views of the same ranks in one launch.  A work item is
@@ -687,14 +751,35 @@ __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dty
 
 }  // namespace
 
+// fill blocks of the per-rank and fused launches: a few per CU, each
+// striding over tiles
+constexpr int kFillBlocks = 1024;
+
 hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) {
-  const dim3 grid((a.W + kGW - 1) / kGW, (a.H + kGH - 1) / kGH, 1);
+  const int tiles_x = (a.W + kFW - 1) / kFW, tiles_y = (a.H + kFH - 1) / kFH;
+  // bounding box of the ranks' rectangles, in fused tiles
+  int x0 = a.W, y0 = a.H, x1 = 0, y1 = 0;
+  for (int r = 0; r < a.P; ++r) {
+    const int32_t *rc = a.rect[r];
+    if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
+    x0 = std::min(x0, (int)rc[0]), y0 = std::min(y0, (int)rc[1]);
+    x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
+  }
+  int4 u = make_int4(0, 0, 0, 0);
+  if (x0 < x1 && y0 < y1) {
+    u.x = x0 / kFW, u.y = y0 / kFH;
+    u.z = (x1 + kFW - 1) / kFW - u.x, u.w = (y1 + kFH - 1) / kFH - u.y;
+  }
+  const long long all = (long long)tiles_x * tiles_y;
+  if (all > INT32_MAX) return hipErrorInvalidValue;
+  const int fill_blocks = (int)std::min<long long>(all, kFillBlocks);
+  const dim3 grid((unsigned)(u.z * u.w + fill_blocks));
   const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
   switch (group) {
-    case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a); break;
-    case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a); break;
-    case 4: march_composite_group_kernel<4><<<grid, kBlock, lds, s>>>(a); break;
-    case 8: march_composite_group_kernel<8><<<grid, kBlock, lds, s>>>(a); break;
+    case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
+    case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
+    case 4: march_composite_group_kernel<4><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
+    case 8: march_composite_group_kernel<8><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -721,9 +806,6 @@ hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_block
   }
   return hipGetLastError();
 }
-
-// fill blocks of the per-rank launch: a few per CU, each striding over tiles
-constexpr int kFillBlocks = 1024;
 
 hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream_t s) {
   const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
