@@ -569,13 +569,13 @@ class Batch:
 
     __del__ = close
 
-    def set_schedule(self, tile_shape: int, persistent_blocks: int) -> None:
-        self.ctx._check(lib().aloha_batch_set_schedule(self._h, tile_shape, persistent_blocks))
+    def set_schedule(self, lanes_per_ray: int, persistent_blocks: int) -> None:
+        self.ctx._check(lib().aloha_batch_set_schedule(self._h, lanes_per_ray, persistent_blocks))
 
     def schedule(self):
         t, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         self.ctx._check(lib().aloha_batch_get_schedule(self._h, C.byref(t), C.byref(b), C.byref(c)))
-        return {"tile_shape": t.value, "persistent_blocks": b.value, "captured": bool(c.value)}
+        return {"lanes_per_ray": t.value, "persistent_blocks": b.value, "captured": bool(c.value)}
 
     def autotune(self, reps: int = 3):
         best = C.c_float()

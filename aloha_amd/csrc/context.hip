@@ -757,9 +757,9 @@ struct aloha_batch {
   aloha_ctx *ctx = nullptr;
   BatchArgs b;
   float *d_cams = nullptr;
-  int32_t *d_orders = nullptr, *d_rects = nullptr;
+  int32_t *d_orders = nullptr, *d_rects = nullptr, *d_boxes = nullptr, *d_box_off = nullptr;
   uint32_t *d_counter = nullptr;
-  int tile_shape = 0;
+  int lanes = 2;
   int persistent_blocks = 0;
   int cus = 0;
   hipStream_t stream = nullptr;  // capture / replay stream (not the null stream)
@@ -776,12 +776,7 @@ void batch_drop_graph(aloha_batch *bt) {
   bt->graph = nullptr;
 }
 
-void batch_tiles(aloha_batch *bt) {
-  const int bw = 2 * kBatchTileW[bt->tile_shape], bh = 2 * kBatchTileH[bt->tile_shape];
-  bt->b.tiles_x = (bt->b.m.W + bw - 1) / bw;
-  bt->b.tiles_y = (bt->b.m.H + bh - 1) / bh;
-  bt->b.total_items = bt->b.tiles_x * bt->b.tiles_y * bt->b.V;
-}
+bool lanes_valid(int n) { return n == 1 || n == 2 || n == 4 || n == 8; }
 }  // namespace
 
 extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
@@ -826,6 +821,30 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
       rank_rect(cams + 12 * v, ctx->bricks, brick_ids + rank_offsets[s],
                 rank_offsets[s + 1] - rank_offsets[s], W, H, &rects[((size_t)v * P + s) * 4]);
   }
+  // per view: bounding box of the rectangles in fused tiles, and the prefix
+  // of the active work items
+  std::vector<int32_t> boxes((size_t)V * 4, 0), box_off((size_t)V + 1, 0);
+  for (int v = 0; v < V; ++v) {
+    int x0 = W, y0 = H, x1 = 0, y1 = 0;
+    for (int s = 0; s < P; ++s) {
+      const int32_t *rc = &rects[((size_t)v * P + s) * 4];
+      if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
+      x0 = std::min(x0, (int)rc[0]), y0 = std::min(y0, (int)rc[1]);
+      x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
+    }
+    int32_t *bx = &boxes[(size_t)v * 4];
+    if (x0 < x1 && y0 < y1) {
+      bx[0] = x0 / kFusedTileW, bx[1] = y0 / kFusedTileH;
+      bx[2] = (x1 + kFusedTileW - 1) / kFusedTileW - bx[0];
+      bx[3] = (y1 + kFusedTileH - 1) / kFusedTileH - bx[1];
+    }
+    const int64_t next = (int64_t)box_off[v] + (int64_t)bx[2] * bx[3];
+    if (next > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many tiles");
+    box_off[v + 1] = (int32_t)next;
+  }
+  const int tiles_x = (W + kFusedTileW - 1) / kFusedTileW, tiles_y = (H + kFusedTileH - 1) / kFusedTileH;
+  if ((int64_t)V * tiles_x * tiles_y > INT32_MAX)
+    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many tiles");
   auto *bt = new aloha_batch;
   bt->ctx = ctx;
   std::memset(&bt->b, 0, sizeof bt->b);
@@ -840,10 +859,14 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   if (e == hipSuccess) e = hipMalloc(&bt->d_cams, sizeof(float) * 12 * (size_t)V);
   if (e == hipSuccess) e = hipMalloc(&bt->d_orders, sizeof(int32_t) * (size_t)V * P);
   if (e == hipSuccess) e = hipMalloc(&bt->d_rects, sizeof(int32_t) * rects.size());
+  if (e == hipSuccess) e = hipMalloc(&bt->d_boxes, sizeof(int32_t) * boxes.size());
+  if (e == hipSuccess) e = hipMalloc(&bt->d_box_off, sizeof(int32_t) * box_off.size());
   if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(uint32_t));
   if (e == hipSuccess) e = hipMemcpy(bt->d_cams, cams, sizeof(float) * 12 * (size_t)V, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_orders, orders, sizeof(int32_t) * (size_t)V * P, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_rects, rects.data(), sizeof(int32_t) * rects.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(bt->d_boxes, boxes.data(), sizeof(int32_t) * boxes.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(bt->d_box_off, box_off.data(), sizeof(int32_t) * box_off.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming);
@@ -854,8 +877,14 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   bt->b.cams = bt->d_cams;
   bt->b.orders = bt->d_orders;
   bt->b.rects = bt->d_rects;
+  bt->b.boxes = bt->d_boxes;
+  bt->b.box_off = bt->d_box_off;
   bt->b.counter = bt->d_counter;
-  batch_tiles(bt);
+  bt->b.tiles_x = tiles_x;
+  bt->b.tiles_y = tiles_y;
+  bt->b.total_active = box_off[V];
+  bt->b.fill_blocks = (int)std::min<int64_t>((int64_t)V * tiles_x * tiles_y, 1024);
+  bt->lanes = ctx->fused_lanes;
   *out = bt;
   return ALOHA_OK;
 }
@@ -865,7 +894,8 @@ extern "C" int aloha_batch_destroy(aloha_batch *bt) {
   (void)hipSetDevice(bt->ctx->device);
   if (bt->stream) (void)hipStreamSynchronize(bt->stream);
   batch_drop_graph(bt);
-  for (void *p : {(void *)bt->d_cams, (void *)bt->d_orders, (void *)bt->d_rects, (void *)bt->d_counter})
+  for (void *p : {(void *)bt->d_cams, (void *)bt->d_orders, (void *)bt->d_rects, (void *)bt->d_boxes,
+                  (void *)bt->d_box_off, (void *)bt->d_counter})
     if (p) (void)hipFree(p);
   if (bt->ev_in) (void)hipEventDestroy(bt->ev_in);
   if (bt->ev_out) (void)hipEventDestroy(bt->ev_out);
@@ -874,22 +904,22 @@ extern "C" int aloha_batch_destroy(aloha_batch *bt) {
   return ALOHA_OK;
 }
 
-extern "C" int aloha_batch_set_schedule(aloha_batch *bt, int32_t tile_shape,
+extern "C" int aloha_batch_set_schedule(aloha_batch *bt, int32_t lanes_per_ray,
                                         int32_t persistent_blocks) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_set_schedule: NULL");
-  if (tile_shape < 0 || tile_shape > 2 || persistent_blocks < 0 || persistent_blocks > (1 << 20))
-    return fail(bt->ctx, ALOHA_ERR_ARG, "aloha_batch_set_schedule: tile_shape 0..2, blocks >= 0");
-  if (tile_shape != bt->tile_shape || persistent_blocks != bt->persistent_blocks) batch_drop_graph(bt);
-  bt->tile_shape = tile_shape;
+  if (!lanes_valid(lanes_per_ray) || persistent_blocks < 0 || persistent_blocks > (1 << 20))
+    return fail(bt->ctx, ALOHA_ERR_ARG,
+                "aloha_batch_set_schedule: lanes per ray 1, 2, 4 or 8; blocks >= 0");
+  if (lanes_per_ray != bt->lanes || persistent_blocks != bt->persistent_blocks) batch_drop_graph(bt);
+  bt->lanes = lanes_per_ray;
   bt->persistent_blocks = persistent_blocks;
-  batch_tiles(bt);
   return ALOHA_OK;
 }
 
-extern "C" int aloha_batch_get_schedule(const aloha_batch *bt, int32_t *tile_shape,
+extern "C" int aloha_batch_get_schedule(const aloha_batch *bt, int32_t *lanes_per_ray,
                                         int32_t *persistent_blocks, int32_t *captured) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_get_schedule: NULL");
-  if (tile_shape) *tile_shape = bt->tile_shape;
+  if (lanes_per_ray) *lanes_per_ray = bt->lanes;
   if (persistent_blocks) *persistent_blocks = bt->persistent_blocks;
   if (captured) *captured = bt->exec != nullptr;
   return ALOHA_OK;
@@ -906,7 +936,7 @@ extern "C" int aloha_batch_launch(aloha_batch *bt) {
   if (bt->exec)
     HIP_CHECK(ctx, hipGraphLaunch(bt->exec, bt->stream));
   else
-    HIP_CHECK(ctx, launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream));
+    HIP_CHECK(ctx, launch_batch(bt->b, bt->lanes, bt->persistent_blocks, bt->stream));
   HIP_CHECK(ctx, hipEventRecord(bt->ev_out, bt->stream));
   HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, bt->ev_out, 0));
   return ALOHA_OK;
@@ -919,7 +949,7 @@ extern "C" int aloha_batch_capture(aloha_batch *bt) {
   batch_drop_graph(bt);
   HIP_CHECK(ctx, hipStreamSynchronize(bt->stream));
   HIP_CHECK(ctx, hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream);
+  hipError_t e = launch_batch(bt->b, bt->lanes, bt->persistent_blocks, bt->stream);
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(bt->stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -936,8 +966,9 @@ extern "C" int aloha_batch_capture(aloha_batch *bt) {
   return ALOHA_OK;
 }
 
-// Times every schedule (3 tile shapes x {one block per item, persistent with
-// 2/4/8 blocks per CU}) over `reps` launches each and keeps the fastest.  The
+// Times every schedule (1/2/4/8 lanes per ray x {one block per item,
+// persistent with 2/4/8 blocks per CU}) over `reps` launches each and keeps
+// the fastest.  The
 // schedule never changes the samples or their arithmetic, so the images do
 // not depend on the choice.  Synchronous.
 extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_ms,
@@ -951,17 +982,17 @@ extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_m
   HIP_CHECK(ctx, hipEventCreate(&e0));
   HIP_CHECK(ctx, hipEventCreate(&e1));
   float best = 1e30f;
-  int best_shape = 0, best_blocks = 0, n = 0;
+  int best_lanes = bt->lanes, best_blocks = 0, n = 0;
   const int blocks_opts[4] = {0, 2, 4, 8};
-  for (int shape = 0; shape < 3; ++shape)
+  const int lane_opts[4] = {1, 2, 4, 8};
+  for (int lanes : lane_opts)
     for (int bo : blocks_opts) {
-      bt->tile_shape = shape;
+      bt->lanes = lanes;
       bt->persistent_blocks = bo ? bo * bt->cus : 0;
-      batch_tiles(bt);
-      hipError_t e = launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream);  // warm
+      hipError_t e = launch_batch(bt->b, bt->lanes, bt->persistent_blocks, bt->stream);  // warm
       if (e == hipSuccess) e = hipEventRecord(e0, bt->stream);
       for (int r = 0; e == hipSuccess && r < reps; ++r)
-        e = launch_batch(bt->b, bt->tile_shape, bt->persistent_blocks, bt->stream);
+        e = launch_batch(bt->b, bt->lanes, bt->persistent_blocks, bt->stream);
       if (e == hipSuccess) e = hipEventRecord(e1, bt->stream);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.f;
@@ -973,22 +1004,21 @@ extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_m
       }
       ms /= (float)reps;
       if (table && n < table_cap) {
-        table[3 * n] = (float)shape;
+        table[3 * n] = (float)lanes;
         table[3 * n + 1] = (float)bt->persistent_blocks;
         table[3 * n + 2] = ms;
       }
       ++n;
       if (ms < best) {
         best = ms;
-        best_shape = shape;
+        best_lanes = lanes;
         best_blocks = bt->persistent_blocks;
       }
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  bt->tile_shape = best_shape;
+  bt->lanes = best_lanes;
   bt->persistent_blocks = best_blocks;
-  batch_tiles(bt);
   if (best_ms) *best_ms = best;
   if (table_n) *table_n = n;
   return ALOHA_OK;

@@ -52,18 +52,20 @@ struct MarchArgs {
   int32_t order[ALOHA_MAX_RANKS];  // fused composite: rank slots front to back
 };
 
+// Fused frame tiles (single fused frame and every batched view): 16x4 pixels.
+constexpr int kFusedTileW = 16, kFusedTileH = 4;
+
 // Batched form: V views of one rank table in one launch.
 struct BatchArgs {
   MarchArgs m;              // ranks, bricks, TF, params; m.rgba/rgba8/nsamples are V x H x W
   const float *cams;        // V x 12 (org, dir_00, dir_du, dir_dv)
   const int32_t *orders;    // V x P rank slots, front to back
   const int32_t *rects;     // V x P x 4 pixel rectangles
+  const int32_t *boxes;     // V x 4: fused-tile bounding box of the view's rectangles (x0, y0, nx, ny)
+  const int32_t *box_off;   // V + 1: prefix sums of nx * ny (active work items)
   uint32_t *counter;        // persistent schedule: next work item
-  int32_t V, tiles_x, tiles_y, total_items;
+  int32_t V, tiles_x, tiles_y, total_active, fill_blocks;
 };
-// tile shapes: 0 = 8x8 pixels per wave, 1 = 16x4, 2 = 4x16 (blocks are 2x2 waves)
-constexpr int kBatchTileW[3] = {8, 16, 4};
-constexpr int kBatchTileH[3] = {8, 4, 16};
 
 // Differentiable form: one rank marching its bricks, seen from V views.
 constexpr int kMaxViews = 16;
@@ -117,7 +119,8 @@ struct TrisArgs {
 // group: lanes per ray (1, 2, 4 or 8) of the per-rank and fused kernels
 hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream_t s);
 hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s);
-hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_blocks, hipStream_t s);
+// lanes: lanes per ray (1, 2, 4, 8); persistent_blocks 0: one block per active tile
+hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s);
 hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
                             int32_t px, int32_t py, int32_t pz, hipStream_t s);
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);

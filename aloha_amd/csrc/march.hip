@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, 
 // 256 threads still have up to kFusedChunk x 64 items per chunk while the
 // grid has enough workgroups to spread the few tiles the volume covers.
 constexpr int kFusedChunk = 8;
-constexpr int kFW = 16, kFH = 4, kFPix = kFW * kFH;
+constexpr int kFW = kFusedTileW, kFH = kFusedTileH, kFPix = kFW * kFH;
 static_assert(kBlock % kFPix == 0 && kFPix % 64 == 0, "items of one batch share a rank");
 
 __device__ __forceinline__ void fused_pixel(int idx, int &lx, int &ly) {
@@ -553,44 +553,62 @@ struct FusedStage {
   int nranks, next;
 };
 
+// Where a fused tile's frame comes from: the single frame of a launch
+// (MarchArgs camera, order, rectangles) or one view of a batch (device
+// tables).  Pointers into the kernel arguments are read-only constant memory.
+struct FrameSrc {
+  const MarchArgs &a;
+  __device__ const float *org() const { return a.org; }
+  __device__ const float *dir00() const { return a.dir00; }
+  __device__ const float *du() const { return a.du; }
+  __device__ const float *dv() const { return a.dv; }
+  __device__ int order(int k) const { return a.order[k]; }
+  __device__ const int32_t *rect(int s) const { return a.rect[s]; }
+  __device__ long long out_base() const { return 0; }
+};
+
+struct ViewSrc {
+  const BatchArgs &b;
+  int view;
+  __device__ const float *org() const { return b.cams + 12 * view; }
+  __device__ const float *dir00() const { return b.cams + 12 * view + 3; }
+  __device__ const float *du() const { return b.cams + 12 * view + 6; }
+  __device__ const float *dv() const { return b.cams + 12 * view + 9; }
+  __device__ int order(int k) const { return b.orders[b.m.P * view + k]; }
+  __device__ const int32_t *rect(int s) const { return b.rects + 4 * (b.m.P * view + s); }
+  __device__ long long out_base() const { return (long long)view * b.m.W * b.m.H; }
+};
+
+__device__ __forceinline__ bool in_rc(const int32_t *rc, int px, int py) {
+  return px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
+}
+
 // Transparent-black frame pixels of one fused tile outside every rectangle.
-__device__ __forceinline__ void fused_fill(const MarchArgs &a, int bx, int by) {
+template <class Src>
+__device__ __forceinline__ void fused_fill(const MarchArgs &a, const Src &src, int bx, int by) {
   const int t = threadIdx.x;
   if (t >= kFPix) return;
   const int px = bx + t % kFW, py = by + t / kFW;
   if (px >= a.W || py >= a.H) return;
-  const int pixel = px + a.W * py;
+  const long long pixel = src.out_base() + px + (long long)a.W * py;
   if (a.rgba) a.rgba[pixel] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (a.rgba8) a.rgba8[pixel] = 0u;
   if (a.nsamples) a.nsamples[pixel] = 0u;
 }
 
-// Grid: the tiles of the rectangles' bounding box (u[0..3] = tile range
-// x0, y0, nx, ny) first, then `fill_blocks` blocks striding over the other
-// tiles of the frame.
-template <int G>
-__global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a,
-                                                                       int4 u, int tiles_x,
-                                                                       int tiles_y,
-                                                                       int fill_blocks) {
-  extern __shared__ float4 s_tf[];
-  __shared__ FusedStage fs;
-  const int nact = u.z * u.w;
-  if ((int)blockIdx.x >= nact) {  // fill part (block-uniform)
-    for (int f = (int)blockIdx.x - nact; f < tiles_x * tiles_y; f += fill_blocks) {
-      const int fx = f % tiles_x, fy = f / tiles_x;
-      if (fx >= u.x && fx < u.x + u.z && fy >= u.y && fy < u.y + u.w) continue;
-      fused_fill(a, fx * kFW, fy * kFH);
-    }
-    return;
-  }
-  const int bx = (u.x + (int)blockIdx.x % u.z) * kFW, by = (u.y + (int)blockIdx.x / u.z) * kFH;
+// One fused 16x4 tile at (bx, by): rank list, chunks of (rank, pixel) items
+// marched by lane groups, in-order blend, output.  The TF must be staged in
+// s_tf before the call; every thread of the block calls it.  Safe to call
+// repeatedly on the same `fs` when a barrier separates the calls.
+template <int G, class Src>
+__device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, FusedStage &fs,
+                                           const float4 *s_tf, int bx, int by) {
   const int t = threadIdx.x;
   if (t == 0) {
     int n = 0;
     for (int k = 0; k < a.P; ++k) {
-      const int s = a.order[k];
-      const int32_t *rc = a.rect[s];
+      const int s = src.order(k);
+      const int32_t *rc = src.rect(s);
       if (bx < rc[2] && bx + kFW > rc[0] && by < rc[3] && by + kFH > rc[1]) fs.ranks[n++] = s;
     }
     fs.nranks = n;
@@ -599,14 +617,13 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
   __syncthreads();
   const int nranks = fs.nranks;  // block-uniform
   float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (nranks > 0) load_tf(a, s_tf);
   const int lane = t & 63, gl = lane % G, gbase = lane - gl;
   const float inv_spp = 1.f / (float)a.spp;
   const int own = fused_index(t % kFW, (t / kFW) % kFH);  // pixel of thread t < kFPix
   for (int c0 = 0; c0 < nranks; c0 += kFusedChunk) {
     const int nc = min(kFusedChunk, nranks - c0);
     if (t == 0) fs.next = 0;
-    __syncthreads();  // TF staged, counter reset, previous chunk blended
+    __syncthreads();  // counter reset, previous chunk blended
     const int items = nc * kFPix;
     for (int pass = 0; pass < kFusedChunk * kFPix; ++pass) {
       int base = 0;
@@ -620,13 +637,13 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
       int lx, ly;
       fused_pixel(idx, lx, ly);
       const int px = bx + lx, py = by + ly;
-      const bool live = item < items && px < a.W && py < a.H && in_rect(a, slot, px, py) &&
+      const bool live = item < items && px < a.W && py < a.H && in_rc(src.rect(slot), px, py) &&
                         island_pixel(a, px, py);
       float d[3] = {1.f, 0.f, 0.f};
-      if (live) pixel_ray(a.dir00, a.du, a.dv, px, py, d);
+      if (live) pixel_ray(src.dir00(), src.du(), src.dv(), px, py, d);
       float z = 1.f;
       uint32_t ns = 0;
-      const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, a.org, d,
+      const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, src.org(), d,
                                              (uint32_t)(px + a.W * py), z, ns, gl, gbase);
       if (gl == 0 && item < items) {
         fs.col[r][idx] = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
@@ -649,81 +666,99 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
   if (t >= kFPix) return;
   const int px = bx + t % kFW, py = by + t / kFW;
   if (px >= a.W || py >= a.H) return;
-  const int pixel = px + a.W * py;
+  const long long pixel = src.out_base() + px + (long long)a.W * py;
   if (a.rgba) a.rgba[pixel] = C;
   if (a.rgba8)
     a.rgba8[pixel] = pack8(C.x) | (pack8(C.y) << 8) | (pack8(C.z) << 16) | (pack8(C.w) << 24);
   if (a.nsamples) a.nsamples[pixel] = fs.ns[own];
 }
 
-// Batched form: V views of the same ranks in one launch.  A work item is
-// (view, block tile); each pixel marches the ranks in that view's visibility
-// order and blends them as march_composite_kernel does, so every view's
-// image equals its own fused (and plain) frame bit for bit.  The wave tile
-// shape (TW x TH pixels, TW*TH = 64) and persistence (a work counter that
-// blocks pull from until it runs out) are schedule-only choices.
-template <int TW, int TH>
-__device__ __forceinline__ void batch_item(const BatchArgs &b, const float4 *s_tf, int item) {
-  const MarchArgs &a = b.m;
-  constexpr int BW = 2 * TW, BH = 2 * TH;
-  const int per_view = b.tiles_x * b.tiles_y;
-  const int view = item / per_view;
-  const int tile = item - view * per_view;
-  const int bx = (tile % b.tiles_x) * BW, by = (tile / b.tiles_x) * BH;
-  const float *cam = b.cams + 12 * view;
-  const int32_t *order = b.orders + a.P * view;
-  const int32_t *rects = b.rects + 4 * a.P * view;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int px = bx + (wave & 1) * TW + (lane % TW);
-  const int py = by + (wave >> 1) * TH + (lane / TW);
-  if (px >= a.W || py >= a.H) return;
-  const int pixel = px + a.W * py;
-  float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t ns = 0;
-  if (island_pixel(a, px, py)) {
-    float d[3];
-    pixel_ray(cam + 3, cam + 6, cam + 9, px, py, d);
-    const float inv_spp = 1.f / (float)a.spp;
-    for (int k = 0; k < a.P; ++k) {
-      const int slot = order[k];  // wave-uniform
-      const int32_t *rc = rects + 4 * slot;
-      if (!(px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3])) continue;
-      float z = 1.f;
-      const float4 acc = rank_pixel<false>(a, s_tf, slot, cam, d, (uint32_t)pixel, z, ns);
-      const float4 c = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
-                                   (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
-      const float w = 1.f - C.w;
-      C.x = C.x + w * c.x;
-      C.y = C.y + w * c.y;
-      C.z = C.z + w * c.z;
-      C.w = C.w + w * c.w;
-    }
-  }
-  const size_t o = (size_t)view * a.W * a.H + pixel;
-  if (a.rgba) a.rgba[o] = C;
-  if (a.rgba8)
-    a.rgba8[o] = pack8(C.x) | (pack8(C.y) << 8) | (pack8(C.z) << 16) | (pack8(C.w) << 24);
-  if (a.nsamples) a.nsamples[o] = ns;
-}
-
-template <int TW, int TH, bool kPersistent>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
+// Single fused frame.  Grid: the tiles of the rectangles' bounding box
+// (u = tile range x0, y0, nx, ny) first, then `fill_blocks` blocks striding
+// over the other tiles of the frame.
+template <int G>
+__global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a,
+                                                                       int4 u, int tiles_x,
+                                                                       int tiles_y,
+                                                                       int fill_blocks) {
   extern __shared__ float4 s_tf[];
-  load_tf(b.m, s_tf);
-  __syncthreads();
-  if (!kPersistent) {
-    batch_item<TW, TH>(b, s_tf, blockIdx.x);
+  __shared__ FusedStage fs;
+  const FrameSrc src{a};
+  const int nact = u.z * u.w;
+  if ((int)blockIdx.x >= nact) {  // fill part (block-uniform)
+    for (int f = (int)blockIdx.x - nact; f < tiles_x * tiles_y; f += fill_blocks) {
+      const int fx = f % tiles_x, fy = f / tiles_x;
+      if (fx >= u.x && fx < u.x + u.z && fy >= u.y && fy < u.y + u.w) continue;
+      fused_fill(a, src, fx * kFW, fy * kFH);
+    }
     return;
   }
+  load_tf(a, s_tf);
+  fused_tile<G>(a, src, fs, s_tf, (u.x + (int)blockIdx.x % u.z) * kFW,
+                (u.y + (int)blockIdx.x / u.z) * kFH);
+}
+
+// Batched form: V views of the same ranks in one launch.  The active work
+// items are the fused tiles inside each view's bounding box (b.boxes, prefix
+// b.box_off); each is one fused_tile of that view, so every view's image
+// equals its single fused frame bit for bit.  Then the blocks stride over the
+// remaining tiles of every view writing transparent black.  kPersistent:
+// gridDim.x blocks pull active items from a counter until it runs out;
+// otherwise one block per active item plus b.fill_blocks fill blocks.
+__device__ __forceinline__ void batch_active(const BatchArgs &b, int item, int &view, int &bx,
+                                             int &by) {
+  int lo = 0, hi = b.V;  // largest view with box_off[view] <= item
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (b.box_off[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  view = lo;
+  const int k = item - b.box_off[view];
+  const int32_t *bx4 = b.boxes + 4 * view;
+  bx = (bx4[0] + k % bx4[2]) * kFW;
+  by = (bx4[1] + k / bx4[2]) * kFH;
+}
+
+__device__ __forceinline__ void batch_fill(const BatchArgs &b, int first, int stride) {
+  const int per = b.tiles_x * b.tiles_y;
+  for (int f = first; f < b.V * per; f += stride) {
+    const int view = f / per, t = f - view * per;
+    const int fx = t % b.tiles_x, fy = t / b.tiles_x;
+    const int32_t *bx4 = b.boxes + 4 * view;
+    if (fx >= bx4[0] && fx < bx4[0] + bx4[2] && fy >= bx4[1] && fy < bx4[1] + bx4[3]) continue;
+    fused_fill(b.m, ViewSrc{b, view}, fx * kFW, fy * kFH);
+  }
+}
+
+template <int G, bool kPersistent>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
+  extern __shared__ float4 s_tf[];
+  __shared__ FusedStage fs;
   __shared__ int s_item;
-  for (;;) {
+  if (!kPersistent) {
+    if ((int)blockIdx.x >= b.total_active) {
+      batch_fill(b, (int)blockIdx.x - b.total_active, b.fill_blocks);
+      return;
+    }
+    load_tf(b.m, s_tf);
+    int view, bx, by;
+    batch_active(b, (int)blockIdx.x, view, bx, by);
+    fused_tile<G>(b.m, ViewSrc{b, view}, fs, s_tf, bx, by);
+    return;
+  }
+  load_tf(b.m, s_tf);
+  for (int it = 0; it <= b.total_active; ++it) {  // bounded: each pass takes a fresh item
     if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter, 1u);
     __syncthreads();
     const int item = s_item;
-    __syncthreads();  // every wave has read s_item before it is replaced
-    if (item >= b.total_items) break;
-    batch_item<TW, TH>(b, s_tf, item);
+    __syncthreads();  // every wave has read s_item; the previous tile's epilogue is done
+    if (item >= b.total_active) break;
+    int view, bx, by;
+    batch_active(b, item, view, bx, by);
+    fused_tile<G>(b.m, ViewSrc{b, view}, fs, s_tf, bx, by);
   }
+  batch_fill(b, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Work-counter reset as a kernel node (graph-capturable without a memset node).
@@ -785,23 +820,26 @@ hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t launch_batch(const BatchArgs &b, int tile_shape, int persistent_blocks,
-                        hipStream_t s) {
+hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s) {
   const size_t lds = sizeof(float4) * (size_t)(b.m.tf_n + 4);
   if (persistent_blocks > 0) {
     reset_counter_kernel<<<1, 1, 0, s>>>(b.counter);
     const dim3 grid(persistent_blocks);
-    switch (tile_shape) {
-      case 1: batch_kernel<16, 4, true><<<grid, kBlock, lds, s>>>(b); break;
-      case 2: batch_kernel<4, 16, true><<<grid, kBlock, lds, s>>>(b); break;
-      default: batch_kernel<8, 8, true><<<grid, kBlock, lds, s>>>(b); break;
+    switch (lanes) {
+      case 1: batch_kernel<1, true><<<grid, kBlock, lds, s>>>(b); break;
+      case 2: batch_kernel<2, true><<<grid, kBlock, lds, s>>>(b); break;
+      case 4: batch_kernel<4, true><<<grid, kBlock, lds, s>>>(b); break;
+      case 8: batch_kernel<8, true><<<grid, kBlock, lds, s>>>(b); break;
+      default: return hipErrorInvalidValue;
     }
   } else {
-    const dim3 grid(b.total_items);
-    switch (tile_shape) {
-      case 1: batch_kernel<16, 4, false><<<grid, kBlock, lds, s>>>(b); break;
-      case 2: batch_kernel<4, 16, false><<<grid, kBlock, lds, s>>>(b); break;
-      default: batch_kernel<8, 8, false><<<grid, kBlock, lds, s>>>(b); break;
+    const dim3 grid((unsigned)(b.total_active + b.fill_blocks));
+    switch (lanes) {
+      case 1: batch_kernel<1, false><<<grid, kBlock, lds, s>>>(b); break;
+      case 2: batch_kernel<2, false><<<grid, kBlock, lds, s>>>(b); break;
+      case 4: batch_kernel<4, false><<<grid, kBlock, lds, s>>>(b); break;
+      case 8: batch_kernel<8, false><<<grid, kBlock, lds, s>>>(b); break;
+      default: return hipErrorInvalidValue;
     }
   }
   return hipGetLastError();

@@ -370,7 +370,7 @@ def batch_bench(args, timed):
             "speedup_vs_per_view": round(ms_views / ms_batch, 2),
             "speedup_vs_per_brick": round(ms_bricks / ms_batch, 2),
             "autotune": {"chosen": sched, "best_ms": round(best, 4),
-                         "table": [{"tile_shape": t, "persistent_blocks": pb, "ms": round(m, 4)}
+                         "table": [{"lanes_per_ray": t, "persistent_blocks": pb, "ms": round(m, 4)}
                                    for t, pb, m in table]},
             "last_view_equal_to_per_view_sequence": equal}
 

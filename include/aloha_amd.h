@@ -269,16 +269,18 @@ int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *params, int32_
                        const int32_t *brick_ids, const int32_t *orders, float *d_out_rgba,
                        uint32_t *d_out_rgba8, uint32_t *d_nsamples, aloha_batch **out);
 int aloha_batch_destroy(aloha_batch *batch);
-/* Schedule: tile_shape 0 = 8x8 pixels per wave, 1 = 16x4, 2 = 4x16;
- * persistent_blocks 0 = one workgroup per work item, else that many
- * workgroups pulling (view, tile) items from a counter.  Drops a captured graph. */
-int aloha_batch_set_schedule(aloha_batch *batch, int32_t tile_shape, int32_t persistent_blocks);
-int aloha_batch_get_schedule(const aloha_batch *batch, int32_t *tile_shape,
+/* Schedule: lanes_per_ray 1, 2, 4 or 8 (the fused frame's lane groups,
+ * aloha_set_march_schedule); persistent_blocks 0 = one workgroup per active
+ * 16x4 tile, else that many workgroups pulling (view, tile) items from a
+ * counter.  Images do not depend on it.  Drops a captured graph. */
+int aloha_batch_set_schedule(aloha_batch *batch, int32_t lanes_per_ray, int32_t persistent_blocks);
+int aloha_batch_get_schedule(const aloha_batch *batch, int32_t *lanes_per_ray,
                              int32_t *persistent_blocks, int32_t *captured);
-/* Time every schedule over reps launches, keep the fastest; table (optional)
- * receives {tile_shape, persistent_blocks, ms} per candidate.  Synchronous. */
+/* Time every schedule (4 x 4) over reps launches, keep the fastest; table
+ * (optional) receives {lanes_per_ray, persistent_blocks, ms} per candidate.
+ * Synchronous. */
 int aloha_batch_autotune(aloha_batch *batch, int32_t reps, float *best_ms, float *table,
-                         int32_t table_cap, int32_t *table_n);
+                         int32_t cap, int32_t *n);
 /* Capture one launch (counter reset + batch kernel) into a hipGraph. */
 int aloha_batch_capture(aloha_batch *batch);
 /* Run the batch (the graph if captured), ordered on the context stream. */

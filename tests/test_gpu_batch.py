@@ -44,15 +44,15 @@ def test_every_schedule_and_graph_replay_bit_exact(wl):
     V = len(wl.cams)
     out = torch.empty((V, wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
     b = A.Batch(ctx, wl.params(), wl.W, wl.H, wl.cams, ranks, wl.orders, out_rgba=out)
-    for shape in (0, 1, 2):
+    for lanes in (1, 2, 4, 8):
         for blocks in (0, 8, 64):
             out.fill_(-1.0)
-            b.set_schedule(shape, blocks)
+            b.set_schedule(lanes, blocks)
             b.launch()
             torch.cuda.synchronize()
-            assert torch.equal(out, ref), (shape, blocks)
+            assert torch.equal(out, ref), (lanes, blocks)
     best, table = b.autotune(reps=2)
-    assert len(table) == 12 and best > 0
+    assert len(table) == 16 and best > 0
     b.capture()
     assert b.schedule()["captured"]
     for rep in range(3):
