@@ -102,9 +102,7 @@ __device__ __forceinline__ void ray_dir(const DiffArgs &a, int view, int px, int
   }
 }
 
-__device__ __forceinline__ BrickView view_of(const DevBrick &db) {
-  return BrickView{db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1], db.n[2]};
-}
+__device__ __forceinline__ BrickView view_of(const DevBrick &db) { return brick_view(db); }
 
 struct Shaded {
   Sample s;

@@ -842,9 +842,7 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
     if (next > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many tiles");
     box_off[v + 1] = (int32_t)next;
   }
-  const int tiles_x = (W + kFusedTileW - 1) / kFusedTileW, tiles_y = (H + kFusedTileH - 1) / kFusedTileH;
-  if ((int64_t)V * tiles_x * tiles_y > INT32_MAX)
-    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many tiles");
+  if ((int64_t)V * H > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many rows");
   auto *bt = new aloha_batch;
   bt->ctx = ctx;
   std::memset(&bt->b, 0, sizeof bt->b);
@@ -880,10 +878,8 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   bt->b.boxes = bt->d_boxes;
   bt->b.box_off = bt->d_box_off;
   bt->b.counter = bt->d_counter;
-  bt->b.tiles_x = tiles_x;
-  bt->b.tiles_y = tiles_y;
   bt->b.total_active = box_off[V];
-  bt->b.fill_blocks = (int)std::min<int64_t>((int64_t)V * tiles_x * tiles_y, 1024);
+  bt->b.fill_blocks = (int)std::min<int64_t>((int64_t)V * H, 1024);
   bt->lanes = ctx->fused_lanes;
   *out = bt;
   return ALOHA_OK;

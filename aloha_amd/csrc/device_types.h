@@ -64,7 +64,8 @@ struct BatchArgs {
   const int32_t *boxes;     // V x 4: fused-tile bounding box of the view's rectangles (x0, y0, nx, ny)
   const int32_t *box_off;   // V + 1: prefix sums of nx * ny (active work items)
   uint32_t *counter;        // persistent schedule: next work item
-  int32_t V, tiles_x, tiles_y, total_active, fill_blocks;
+  int32_t V, total_active;  // views, active work items (box_off[V])
+  int32_t fill_blocks;      // non-persistent schedule: blocks that write the rows outside the boxes
 };
 
 // Differentiable form: one rank marching its bricks, seen from V views.

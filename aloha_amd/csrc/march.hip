@@ -46,7 +46,7 @@ __device__ __forceinline__ void integrate(const MarchArgs &a, const float4 *s_tf
   const float segs = floorf((t0 - dt * jitter) / dt);
   float t = dt * (jitter + segs);
   if (t < t0) t += dt;
-  const BrickView b{db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1], db.n[2]};
+  const BrickView b = brick_view(db);
   for (; t < t1 && c.w < .99f; t += dt) {
     ++ns;
     const float x = (o[0] + t * d[0]) - db.vox_lo[0];
@@ -88,8 +88,17 @@ __device__ __forceinline__ void group_integrate(const MarchArgs &a, const float4
   const float segs = floorf((t0 - dt * jitter) / dt);
   float t = dt * (jitter + segs);
   if (t < t0) t += dt;
-  // the brick record is read only by lanes that march it
-  const DevBrick *dbp = run ? a.bricks + bid : nullptr;
+  // the brick record is read once, by the lanes that march it (inside the
+  // step loop a conditional load could not be hoisted)
+  BrickView b{nullptr, 0u, 0u, 0, 0, 0};
+  float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
+  if (run) {
+    const DevBrick &db = a.bricks[bid];
+    b = brick_view(db);
+    lo0 = db.vox_lo[0];
+    lo1 = db.vox_lo[1];
+    lo2 = db.vox_lo[2];
+  }
   bool go = run && t < t1 && c.w < .99f;
   while (__any(go)) {
     float tk = t;
@@ -100,13 +109,11 @@ __device__ __forceinline__ void group_integrate(const MarchArgs &a, const float4
     float4 tf = make_float4(0.f, 0.f, 0.f, 0.f);
     float shade = 0.f;
     if (valid) {
-      const DevBrick &db = *dbp;
-      const BrickView b{db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1], db.n[2]};
-      const float x = (o[0] + tk * d[0]) - db.vox_lo[0];
-      const float y = (o[1] + tk * d[1]) - db.vox_lo[1];
-      const float z = (o[2] + tk * d[2]) - db.vox_lo[2];
+      const float x = (o[0] + tk * d[0]) - lo0;
+      const float y = (o[1] + tk * d[1]) - lo1;
+      const float z = (o[2] + tk * d[2]) - lo2;
       Sample s;
-      if (!sample_fast(b, x, y, z, s)) s = sample_exact<false>(b, nullptr, db, x, y, z);
+      if (!sample_fast(b, x, y, z, s)) s = sample_exact<false>(b, nullptr, a.bricks[bid], x, y, z);
       const float ex = s.gx + 1e-4f, ey = s.gy + 1e-4f, ez = s.gz + 1e-4f;
       const float len = sqrtf(ex * ex + ey * ey + ez * ez);
       const float nx = s.gx / len, ny = s.gy / len, nz = s.gz / len;
@@ -429,18 +436,32 @@ __device__ __forceinline__ void tile_pixel(int idx, int &lx, int &ly) {
 }
 __device__ __forceinline__ int tile_index(int lx, int ly) { return (lx >> 3) * 64 + ly * 8 + (lx & 7); }
 
-// Miss result of every pixel of one 32x8 tile of rank slot `slot`.
-__device__ __forceinline__ void fill_tile(const MarchArgs &a, int slot, int bx, int by) {
-  const int t = threadIdx.x;
-  const int px = bx + t % kGW, py = by + t / kGW;
-  if (px >= a.W || py >= a.H) return;
-  long long clock_begin = 0;
-  if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
-  const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
-                           ? (float)(clock64() - clock_begin) * (a.heat_scale / (float)a.spp)
-                           : 0.f;
-  write_rank_pixel(a, (long long)slot * a.W * a.H + px + (long long)a.W * py, px, py,
-                   make_float4(0.f, 0.f, 0.f, 0.f), 1.f, 0u, heat_t);
+// Fill part shared by the launches: `nimg` images of H rows of W pixels.
+// Fill block fb of nfb takes a contiguous run of rows and calls put(img, x,
+// y) for every pixel outside its image's marched box [x0, x1) x [y0, y1)
+// (box(img, x0, y0, x1, y1), whole tiles, written by the march blocks);
+// the 256 threads take 256 consecutive pixels per pass, so each store
+// instruction covers whole 64-byte lines.  The row bookkeeping is
+// block-uniform: one division per block, no per-tile index arithmetic.
+template <class Box, class Put>
+__device__ __forceinline__ void fill_rows(int W, int H, int nimg, int fb, int nfb, const Box &box,
+                                          const Put &put) {
+  const int rows = nimg * H;  // < 2^31, checked by the launchers
+  const int per = (rows + nfb - 1) / nfb;
+  const int r0 = per * fb, r1 = min(r0 + per, rows);
+  if (r0 >= r1) return;
+  int img = r0 / H, y = r0 - img * H;
+  int x0, y0, x1, y1;
+  box(img, x0, y0, x1, y1);
+  for (int r = r0; r < r1; ++r) {
+    const bool band = y >= y0 && y < y1;
+    for (int x = (int)threadIdx.x; x < W; x += kBlock)
+      if (!band || x < x0 || x >= x1) put(img, x, y);
+    if (++y == H && r + 1 < r1) {
+      y = 0;
+      box(++img, x0, y0, x1, y1);
+    }
+  }
 }
 
 // Per-rank images.  Blocks [0, active) march the tiles that meet their rank's
@@ -449,8 +470,7 @@ __device__ __forceinline__ void fill_tile(const MarchArgs &a, int slot, int bx, 
 // rectangles -- few blocks, many tiles each, so the cheap part costs no
 // per-block launch latency.
 template <int G>
-__global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, int tiles_x,
-                                                             int tiles_y, int fill_blocks) {
+__global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, int fill_blocks) {
   extern __shared__ float4 s_tf[];
   __shared__ Staged st;
   int b = (int)blockIdx.x, slot = -1, tx = 0, ty = 0;
@@ -465,16 +485,24 @@ __global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, 
     }
     b -= ntx * nty;
   }
-  if (slot < 0) {  // fill part (block-uniform)
-    const int per = tiles_x * tiles_y;
-    for (int f = b; f < a.P * per; f += fill_blocks) {
-      const int s = f / per, t = f - s * per;
-      const int fx = t % tiles_x, fy = t / tiles_x;
-      int tx0, ty0, ntx, nty;
-      rect_tiles(a, s, tx0, ty0, ntx, nty);
-      if (fx >= tx0 && fx < tx0 + ntx && fy >= ty0 && fy < ty0 + nty) continue;  // marched
-      fill_tile(a, s, fx * kGW, fy * kGH);
-    }
+  if (slot < 0) {  // fill part (block-uniform): the pixels outside the marched tiles
+    const bool heat = a.flags & ALOHA_FLAG_HEATMAP;
+    const float heat_scale = a.heat_scale / (float)a.spp;
+    fill_rows(
+        a.W, a.H, a.P, b, fill_blocks,
+        [&](int s, int &x0, int &y0, int &x1, int &y1) {
+          int tx0, ty0, ntx, nty;
+          rect_tiles(a, s, tx0, ty0, ntx, nty);
+          x0 = tx0 * kGW, x1 = (tx0 + ntx) * kGW, y0 = ty0 * kGH, y1 = (ty0 + nty) * kGH;
+        },
+        [&](int s, int px, int py) {
+          // a miss costs no march time: the heat map shows the clock read itself
+          long long clock_begin = 0;
+          if (heat) clock_begin = clock64();
+          const float heat_t = heat ? (float)(clock64() - clock_begin) * heat_scale : 0.f;
+          write_rank_pixel(a, (long long)s * a.W * a.H + px + (long long)a.W * py, px, py,
+                           make_float4(0.f, 0.f, 0.f, 0.f), 1.f, 0u, heat_t);
+        });
     return;
   }
   const int bx = tx * kGW, by = ty * kGH;
@@ -583,14 +611,8 @@ __device__ __forceinline__ bool in_rc(const int32_t *rc, int px, int py) {
   return px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
 }
 
-// Transparent-black frame pixels of one fused tile outside every rectangle.
-template <class Src>
-__device__ __forceinline__ void fused_fill(const MarchArgs &a, const Src &src, int bx, int by) {
-  const int t = threadIdx.x;
-  if (t >= kFPix) return;
-  const int px = bx + t % kFW, py = by + t / kFW;
-  if (px >= a.W || py >= a.H) return;
-  const long long pixel = src.out_base() + px + (long long)a.W * py;
+// Transparent black (no rank reaches the pixel) at frame pixel `pixel`.
+__device__ __forceinline__ void fused_clear(const MarchArgs &a, long long pixel) {
   if (a.rgba) a.rgba[pixel] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (a.rgba8) a.rgba8[pixel] = 0u;
   if (a.nsamples) a.nsamples[pixel] = 0u;
@@ -678,19 +700,19 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
 // over the other tiles of the frame.
 template <int G>
 __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a,
-                                                                       int4 u, int tiles_x,
-                                                                       int tiles_y,
+                                                                       int4 u,
                                                                        int fill_blocks) {
   extern __shared__ float4 s_tf[];
   __shared__ FusedStage fs;
   const FrameSrc src{a};
   const int nact = u.z * u.w;
   if ((int)blockIdx.x >= nact) {  // fill part (block-uniform)
-    for (int f = (int)blockIdx.x - nact; f < tiles_x * tiles_y; f += fill_blocks) {
-      const int fx = f % tiles_x, fy = f / tiles_x;
-      if (fx >= u.x && fx < u.x + u.z && fy >= u.y && fy < u.y + u.w) continue;
-      fused_fill(a, src, fx * kFW, fy * kFH);
-    }
+    fill_rows(
+        a.W, a.H, 1, (int)blockIdx.x - nact, fill_blocks,
+        [&](int, int &x0, int &y0, int &x1, int &y1) {
+          x0 = u.x * kFW, y0 = u.y * kFH, x1 = (u.x + u.z) * kFW, y1 = (u.y + u.w) * kFH;
+        },
+        [&](int, int px, int py) { fused_clear(a, px + (long long)a.W * py); });
     return;
   }
   load_tf(a, s_tf);
@@ -720,15 +742,17 @@ __device__ __forceinline__ void batch_active(const BatchArgs &b, int item, int &
   by = (bx4[1] + k / bx4[2]) * kFH;
 }
 
-__device__ __forceinline__ void batch_fill(const BatchArgs &b, int first, int stride) {
-  const int per = b.tiles_x * b.tiles_y;
-  for (int f = first; f < b.V * per; f += stride) {
-    const int view = f / per, t = f - view * per;
-    const int fx = t % b.tiles_x, fy = t / b.tiles_x;
-    const int32_t *bx4 = b.boxes + 4 * view;
-    if (fx >= bx4[0] && fx < bx4[0] + bx4[2] && fy >= bx4[1] && fy < bx4[1] + bx4[3]) continue;
-    fused_fill(b.m, ViewSrc{b, view}, fx * kFW, fy * kFH);
-  }
+__device__ __forceinline__ void batch_fill(const BatchArgs &b, int fb, int nfb) {
+  fill_rows(
+      b.m.W, b.m.H, b.V, fb, nfb,
+      [&](int view, int &x0, int &y0, int &x1, int &y1) {
+        const int32_t *bx4 = b.boxes + 4 * view;
+        x0 = bx4[0] * kFW, y0 = bx4[1] * kFH;
+        x1 = (bx4[0] + bx4[2]) * kFW, y1 = (bx4[1] + bx4[3]) * kFH;
+      },
+      [&](int view, int px, int py) {
+        fused_clear(b.m, (long long)view * b.m.W * b.m.H + px + (long long)b.m.W * py);
+      });
 }
 
 template <int G, bool kPersistent>
@@ -791,7 +815,6 @@ __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dty
 constexpr int kFillBlocks = 1024;
 
 hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) {
-  const int tiles_x = (a.W + kFW - 1) / kFW, tiles_y = (a.H + kFH - 1) / kFH;
   // bounding box of the ranks' rectangles, in fused tiles
   int x0 = a.W, y0 = a.H, x1 = 0, y1 = 0;
   for (int r = 0; r < a.P; ++r) {
@@ -805,16 +828,14 @@ hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) 
     u.x = x0 / kFW, u.y = y0 / kFH;
     u.z = (x1 + kFW - 1) / kFW - u.x, u.w = (y1 + kFH - 1) / kFH - u.y;
   }
-  const long long all = (long long)tiles_x * tiles_y;
-  if (all > INT32_MAX) return hipErrorInvalidValue;
-  const int fill_blocks = (int)std::min<long long>(all, kFillBlocks);
+  const int fill_blocks = std::min<int>(a.H, kFillBlocks);
   const dim3 grid((unsigned)(u.z * u.w + fill_blocks));
   const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
   switch (group) {
-    case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
-    case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
-    case 4: march_composite_group_kernel<4><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
-    case 8: march_composite_group_kernel<8><<<grid, kBlock, lds, s>>>(a, u, tiles_x, tiles_y, fill_blocks); break;
+    case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
+    case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
+    case 4: march_composite_group_kernel<4><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
+    case 8: march_composite_group_kernel<8><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -852,11 +873,10 @@ hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream
     march_kernel<true><<<grid, kBlock, lds, s>>>(a);
     return hipGetLastError();
   }
-  // active tiles of every rank, then the fill blocks
-  const int tiles_x = (a.W + kGW - 1) / kGW, tiles_y = (a.H + kGH - 1) / kGH;
-  const long long fill_tiles = (long long)a.P * tiles_x * tiles_y;
-  if (fill_tiles > INT32_MAX) return hipErrorInvalidValue;
-  const int fill_blocks = (int)std::min<long long>(fill_tiles, kFillBlocks);
+  // active tiles of every rank, then the fill blocks (runs of image rows)
+  const long long rows = (long long)a.P * a.H;
+  if (rows > INT32_MAX) return hipErrorInvalidValue;
+  const int fill_blocks = (int)std::min<long long>(rows, kFillBlocks);
   long long blocks = fill_blocks;
   for (int r = 0; r < a.P; ++r) {
     const int32_t *rc = a.rect[r];
@@ -867,10 +887,10 @@ hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream
   if (blocks > INT32_MAX) return hipErrorInvalidValue;
   const dim3 grid((unsigned)blocks);
   switch (group) {
-    case 1: march_group_kernel<1><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
-    case 2: march_group_kernel<2><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
-    case 4: march_group_kernel<4><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
-    case 8: march_group_kernel<8><<<grid, kBlock, lds, s>>>(a, tiles_x, tiles_y, fill_blocks); break;
+    case 1: march_group_kernel<1><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
+    case 2: march_group_kernel<2><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
+    case 4: march_group_kernel<4><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
+    case 8: march_group_kernel<8><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -35,13 +35,24 @@ __device__ __forceinline__ float rng_next(uint32_t &s) {
   return (float)(s >> 8) * (1.0f / 16777216.0f);
 }
 
+// Voxel reads go through global-address-space pointers, so the compiler
+// emits global loads (vector-memory counter only) rather than flat loads,
+// which also count against the LDS counter and so would serialize with the
+// TF lookups and lane shuffles.
+typedef const float __attribute__((address_space(1))) gfloat;
+
 // o points at padded voxel (-1,-1,-1), so every index the taps form is
 // non-negative after +1 and all offsets are 24-bit products.
 struct BrickView {
-  const float *o;
+  const gfloat *o;
   uint32_t px, pxy;
   int n0, n1, n2;
 };
+
+__device__ __forceinline__ BrickView brick_view(const DevBrick &db) {
+  return BrickView{(const gfloat *)db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1],
+                   db.n[2]};
+}
 
 __device__ __forceinline__ uint32_t voff(const BrickView &b, int i, int j, int k) {
   return (uint32_t)(i + 1) + __umul24((uint32_t)(j + 1), b.px) + __umul24((uint32_t)(k + 1), b.pxy);
@@ -73,7 +84,7 @@ __device__ __forceinline__ float tap_exact(const BrickView &b, uint32_t *touched
       for (int j = 0; j < 2; ++j)
         for (int i = 0; i < 2; ++i) mark(touched, db, ix + i, iy + j, iz + k);
   }
-  const float *p = b.o + voff(b, ix, iy, iz);
+  const gfloat *p = b.o + voff(b, ix, iy, iz);
   const float c00 = lerp_(p[0], p[1], wx);
   const float c10 = lerp_(p[b.px], p[b.px + 1], wx);
   const float c01 = lerp_(p[b.pxy], p[b.pxy + 1], wx);
@@ -121,31 +132,33 @@ __device__ __forceinline__ bool sample_fast(const BrickView &b, float x, float y
   const float wzm = zm - fzm, wzc = z - fzc, wzp = zp - fzp;
   const uint32_t px = b.px, pxy = b.pxy;
 
-  // R[jj][kk][i] = V(ax+i, cy+jj, cz+kk)
-  const float *r = b.o + voff(b, ax, cy, cz);
+  // R[jj][kk][i] = V(ax+i, cy+jj, cz+kk): four x-rows of 3 voxels, one row
+  // pointer each so the x offsets are immediates
+  const gfloat *r00 = b.o + voff(b, ax, cy, cz);
+  const gfloat *r01 = r00 + pxy;
+  const gfloat *row[2][2] = {{r00, r01}, {r00 + px, r01 + px}};
   float R[2][2][3];
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) R[jj][kk][i] = r[i + jj * px + kk * pxy];
+      for (int i = 0; i < 3; ++i) R[jj][kk][i] = row[jj][kk][i];
   // the y row outside the value tap's pair, and the z row likewise
   const int yn = scy ? ay : ay + 2;
   const int zn = scz ? az : az + 2;
-  const float *ry = b.o + voff(b, cx, yn, cz);
-  const float *rz = b.o + voff(b, cx, cy, zn);
+  const gfloat *ry = b.o + voff(b, cx, yn, cz);
+  const gfloat *rz = b.o + voff(b, cx, cy, zn);
+  const gfloat *ry1 = ry + pxy, *rz1 = rz + px;
   float YN[2][2], ZN[2][2];
-#pragma unroll
-  for (int kk = 0; kk < 2; ++kk) {
-    YN[kk][0] = ry[kk * pxy];
-    YN[kk][1] = ry[kk * pxy + 1];
-  }
-#pragma unroll
-  for (int jj = 0; jj < 2; ++jj) {
-    ZN[jj][0] = rz[jj * px];
-    ZN[jj][1] = rz[jj * px + 1];
-  }
+  YN[0][0] = ry[0];
+  YN[0][1] = ry[1];
+  YN[1][0] = ry1[0];
+  YN[1][1] = ry1[1];
+  ZN[0][0] = rz[0];
+  ZN[0][1] = rz[1];
+  ZN[1][0] = rz1[0];
+  ZN[1][1] = rz1[1];
 
   // value tap: x-lerps on the (cx, cx+1) pair
   float cX[2][2];
