@@ -39,10 +39,18 @@ struct aloha_ctx {
   std::vector<TriGeom> geoms;
   // lanes per ray of the per-rank and fused march launches (1, 2, 4, 8)
   int march_lanes = 1, fused_lanes = 1;
+  // persistent per-rank launch: waves per lanes-per-ray option (and the
+  // dynamic LDS size they were computed for)
+  int q_waves[4] = {0, 0, 0, 0};
+  size_t q_waves_lds[4] = {0, 0, 0, 0};
   std::string err;
 };
 
 namespace aloha {
+
+// Launch arguments of the persistent per-rank kernel for march arguments
+// `a`; sets waves to its grid for `lanes` lanes per ray.
+int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, int &waves);
 
 // Record a message on ctx (and as the thread's global error); returns code.
 int fail(aloha_ctx *ctx, int code, const char *fmt, ...);

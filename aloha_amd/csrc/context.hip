@@ -398,6 +398,34 @@ extern "C" int aloha_camera_set(aloha_ctx *ctx, const float org[3], const float 
   return ALOHA_OK;
 }
 
+namespace aloha {
+
+int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, int &waves) {
+  int64_t total = 0;
+  q.rect_off[0] = 0;
+  for (int s = 0; s < a.P; ++s) {
+    const int32_t *rc = a.rect[s];
+    if (rc[0] < rc[2] && rc[1] < rc[3]) total += (int64_t)(rc[2] - rc[0]) * (rc[3] - rc[1]);
+    if (total > INT32_MAX / 2)
+      return fail(ctx, ALOHA_ERR_LIMIT, "render: rank rectangles too large");
+    q.rect_off[s + 1] = (int32_t)total;
+  }
+  // persistent grid: as many one-wave blocks as the GPU holds at once
+  const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
+  const size_t lds = march_tables_bytes(a);
+  if (ctx->q_waves[li] == 0 || ctx->q_waves_lds[li] != lds) {
+    int per_cu = 0, cus = 0;
+    HIP_CHECK(ctx, ranks_occupancy(lanes, lds, &per_cu));
+    HIP_CHECK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    ctx->q_waves[li] = std::max(1, per_cu) * std::max(1, cus);
+    ctx->q_waves_lds[li] = lds;
+  }
+  waves = ctx->q_waves[li];
+  return ALOHA_OK;
+}
+
+}  // namespace aloha
+
 extern "C" int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
                                   int32_t H, int32_t P, const int32_t *rank_offsets,
                                   const int32_t *brick_ids, const aloha_images *out) {
@@ -413,7 +441,10 @@ extern "C" int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *par
   a.depth = out->depth;
   a.accum = (float4 *)out->accum;
   a.nsamples = out->nsamples;
-  HIP_CHECK(ctx, launch_march(a, false, ctx->march_lanes, ctx->stream));
+  RanksArgs q;
+  int waves = 0;
+  if (int rc = prepare_ranks(ctx, a, ctx->march_lanes, q, waves)) return rc;
+  HIP_CHECK(ctx, launch_march_ranks(a, q, ctx->march_lanes, waves, ctx->stream));
   return ALOHA_OK;
 }
 
@@ -518,7 +549,10 @@ extern "C" int aloha_autotune_march(aloha_ctx *ctx, const aloha_render_params *p
   for (int i = 0; i < 4 && e == hipSuccess; ++i) {
     const int g = opts[i];
     float mr = 0.f, mf = 0.f;
-    e = time([&] { return launch_march(a, false, g, ctx->stream); }, mr);
+    RanksArgs q;
+    int waves = 0;
+    if (int rc = prepare_ranks(ctx, a, g, q, waves)) return rc;
+    e = time([&] { return launch_march_ranks(a, q, g, waves, ctx->stream); }, mr);
     if (e == hipSuccess && fused) e = time([&] { return launch_march_composite(f, g, ctx->stream); }, mf);
     if (e != hipSuccess) break;
     if (table) {
@@ -563,7 +597,7 @@ extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params 
   hipError_t e = hipMemsetAsync(bits, 0, words * 4, ctx->stream);
   a.touched = bits;
   a.bricks = ctx->d_bricks;
-  if (e == hipSuccess) e = launch_march(a, true, 1, ctx->stream);
+  if (e == hipSuccess) e = launch_march_footprint(a, ctx->stream);
   std::vector<uint32_t> host(words);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipMemcpy(host.data(), bits, words * 4, hipMemcpyDeviceToHost);

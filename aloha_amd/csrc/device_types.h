@@ -52,6 +52,12 @@ struct MarchArgs {
   int32_t order[ALOHA_MAX_RANKS];  // fused composite: rank slots front to back
 };
 
+// Persistent per-rank launch: the pixels of the ranks' rectangles, numbered
+// rank after rank, row-major; rect_off[s] is the first of rank slot s.
+struct RanksArgs {
+  int32_t rect_off[ALOHA_MAX_RANKS + 1];
+};
+
 // Fused frame tiles (single fused frame and every batched view): 16x4 pixels.
 constexpr int kFusedTileW = 16, kFusedTileH = 4;
 
@@ -117,8 +123,22 @@ struct TrisArgs {
 };
 
 // launchers (march.hip, composite.hip)
-// group: lanes per ray (1, 2, 4 or 8) of the per-rank and fused kernels
-hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream_t s);
+// group / lanes: lanes per ray (1, 2, 4 or 8) of the march kernels
+// footprint pass (voxel bitset of one frame, march_kernel<true>)
+hipError_t launch_march_footprint(const MarchArgs &a, hipStream_t s);
+// persistent per-rank launch: `waves` one-wave blocks
+hipError_t launch_march_ranks(const MarchArgs &a, const RanksArgs &q, int group, int waves,
+                              hipStream_t s);
+// Dynamic LDS of the march launches: the TF with its zero borders
+// (tf_n + 4 float4), a 64-byte brick record per rank_bricks entry,
+// rank_off[0..P] and the P pixel rectangles.
+constexpr size_t kLdsBrickBytes = 64;
+inline size_t march_tables_bytes(const MarchArgs &a) {
+  return sizeof(float4) * (size_t)(a.tf_n + 4) + kLdsBrickBytes * (size_t)a.rank_off[a.P] +
+         sizeof(int32_t) * (size_t)(a.P + 1) + sizeof(int32_t) * 4 * (size_t)a.P;
+}
+// resident one-wave blocks per CU of the persistent per-rank kernel
+hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu);
 hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s);
 // lanes: lanes per ray (1, 2, 4, 8); persistent_blocks 0: one block per active tile
 hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s);

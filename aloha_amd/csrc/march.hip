@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device_types.h"
 #include "march_device.h"
@@ -67,72 +68,268 @@ __device__ __forceinline__ void integrate(const MarchArgs &a, const float4 *s_tf
   }
 }
 
-// integrateBrick with the samples of one ray spread over a group of G lanes:
-// per step, lane gl evaluates sample k0 + gl (position, 7 taps, normal, TF,
-// shade -- the costly, independent part), then every lane of the group
-// applies the G samples in order with the sequential 'over' recurrence and
-// the loop test (t < t1 && alpha < .99) before each.  The positions come from
-// the same chain of additions t += dt, so the result is integrate()'s bit for
-// bit; samples past the termination are computed and dropped.
-//
-// Control flow is wave-uniform wherever lanes exchange values: the step loop
-// runs while any lane of the wave has work (a ballot), the shuffles execute
-// on every lane, and each group's work is predicated on `run`.  All lanes of
-// a group hold the same ray state, so the predicates are group-uniform.
-template <int G>
-__device__ __forceinline__ void group_integrate(const MarchArgs &a, const float4 *s_tf, bool run,
-                                                int bid, const float o[3], const float d[3],
-                                                float t0, float t1, float jitter, float4 &c,
-                                                uint32_t &ns, int gl, int gbase) {
-  const float dt = a.dt;
-  const float segs = floorf((t0 - dt * jitter) / dt);
-  float t = dt * (jitter + segs);
-  if (t < t0) t += dt;
-  // the brick record is read once, by the lanes that march it (inside the
-  // step loop a conditional load could not be hoisted)
-  BrickView b{nullptr, 0u, 0u, 0, 0, 0};
-  float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
-  if (run) {
-    const DevBrick &db = a.bricks[bid];
-    b = brick_view(db);
-    lo0 = db.vox_lo[0];
-    lo1 = db.vox_lo[1];
-    lo2 = db.vox_lo[2];
+__device__ __forceinline__ void load_tf(const MarchArgs &a, float4 *s_tf) {
+  for (int i = threadIdx.x; i < a.tf_n + 4; i += blockDim.x)
+    s_tf[i] = (i >= 2 && i < a.tf_n + 2) ? a.tf[i - 2] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// What the march reads of one brick, cached in LDS per block (64 B): the
+// brick selection and segment set-up then cost an LDS read instead of a
+// dependent global load on every ray transition.
+struct LdsBrick {
+  float lo[3], hi[3];  // spaceRange
+  float vlo[3];        // voxelRange.lower
+  uint32_t px, pxy;
+  int32_t n[3];
+  const float *origin;
+};
+static_assert(sizeof(LdsBrick) == kLdsBrickBytes, "LDS brick record");
+
+// The launch's dynamic LDS: the TF with its zero borders (tf_n + 4 float4),
+// then one LdsBrick per rank_bricks entry, rank_off[0..P] and the ranks'
+// pixel rectangles -- every table a lane indexes with a per-lane value, so
+// that such reads are LDS reads, not vector loads from the kernel
+// arguments.
+struct MarchTables {
+  const float4 *tf;
+  const LdsBrick *brick;  // indexed like rank_bricks
+  const int32_t *roff;    // rank_off[0..P]
+  const int32_t *rect;    // rect[0..P) x 4
+};
+
+// Stage the TF, the rank table and the ranks' brick records in the dynamic
+// LDS `smem`; the caller synchronizes before use.
+__device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *smem) {
+  load_tf(a, smem);
+  LdsBrick *sb = reinterpret_cast<LdsBrick *>(smem + a.tf_n + 4);
+  const int nrb = a.rank_off[a.P];
+  int32_t *roff = reinterpret_cast<int32_t *>(sb + nrb);
+  for (int i = threadIdx.x; i < nrb; i += blockDim.x) {
+    const DevBrick &db = a.bricks[a.rank_bricks[i]];
+    LdsBrick r;
+    for (int k = 0; k < 3; ++k) {
+      r.lo[k] = db.space_lo[k];
+      r.hi[k] = db.space_hi[k];
+      r.vlo[k] = db.vox_lo[k];
+      r.n[k] = db.n[k];
+    }
+    r.px = (uint32_t)db.px;
+    r.pxy = (uint32_t)db.pxy;
+    r.origin = db.origin;
+    sb[i] = r;
   }
-  bool go = run && t < t1 && c.w < .99f;
-  while (__any(go)) {
+  int32_t *rect = roff + a.P + 1;
+  for (int i = threadIdx.x; i <= a.P; i += blockDim.x) roff[i] = a.rank_off[i];
+  for (int i = threadIdx.x; i < 4 * a.P; i += blockDim.x) rect[i] = a.rect[i / 4][i % 4];
+  return MarchTables{smem, sb, roff, rect};
+}
+
+// Compile-time loop: f(std::integral_constant<int, J>) for J in [J0, N).
+template <int J0, int N, class F>
+__device__ __forceinline__ void static_for(const F &f) {
+  if constexpr (J0 < N) {
+    f(std::integral_constant<int, J0>{});
+    static_for<J0 + 1, N>(f);
+  }
+}
+
+// Value of lane gbase + J of the caller's lane group (groups of G aligned
+// lanes), on every lane of the group.  Pairs and quads use a DPP quad
+// permutation (one VALU move, no LDS round trip); groups of 8 use
+// ds_bpermute.  Every lane of the wave must be active.
+template <int G, int J>
+__device__ __forceinline__ float group_bcast(float v, int gbase) {
+  if constexpr (G == 1) {
+    return v;
+  } else if constexpr (G == 2 || G == 4) {
+    constexpr int ctrl = G == 4 ? J * 0x55 : (J | (J << 2) | ((2 + J) << 4) | ((2 + J) << 6));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+  } else {
+    return __shfl(v, gbase + J);
+  }
+}
+
+template <int G, int J>
+__device__ __forceinline__ int group_bcast_i(int v, int gbase) {
+  return __float_as_int(group_bcast<G, J>(__int_as_float(v), gbase));
+}
+
+// One ray of a work list: rank slot (-1: the ray can reach no brick, its
+// result is the miss result), pixel, and the caller's tag for it.
+struct ListRay {
+  int slot, px, py, tag;
+};
+
+// renderFrame's per-pixel work (sample loop, brick do/while, integrateBrick;
+// deviceCode.cpp:362-393 and :256-298) for the rays of a work list [0,
+// count), marched by lane groups of G lanes with persistent-lane scheduling:
+// a group takes the next list entry (an LDS counter) as soon as its ray is
+// done, so groups stay busy while entries remain instead of waiting for the
+// longest ray of a fixed batch.
+//
+// Each group runs a small state machine, one transition or one march step
+// per loop trip: kFetch (take an entry, set up the ray), kSelect (closest
+// brick hit from tmin: set up the segment, or end the spp sample when there
+// is none), kStep (G samples of the segment).  In a step lane gl evaluates
+// sample k0 + gl (position, 7 taps, normal, TF, shade -- the costly,
+// independent part); then every lane of the group applies the G samples in
+// order with the sequential 'over' recurrence and the loop test (t < t1 &&
+// alpha < .99) before each.  Positions come from the same chain of additions
+// t += dt, the segment start from firstSampleT, the jitter from the same RNG
+// draws in the same order, so every ray's result equals rank_pixel()'s bit
+// for bit; samples past the termination are computed and dropped.
+//
+// Control flow is wave-uniform wherever lanes exchange values: the loop runs
+// while any group of the wave has a ray (a ballot), the broadcasts execute on
+// every lane, and each group's work is predicated on its state, which is the
+// same on all of its lanes.  fetch(want) is called by every lane of the
+// wave and returns the next list index on the lanes with want set (the
+// first lane of each group without a ray); entry(k) gives list entry k;
+// store(ray, acc, z, ns, clock_begin) is called on the group's first lane
+// when its ray is done.
+template <int G, class Fetch, class Entry, class Store>
+__device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables &tb,
+                                           const float org[3], const float dir00[3],
+                                           const float du[3], const float dv[3],
+                                           const Fetch &fetch, int count, const Entry &entry,
+                                           const Store &store) {
+  enum : int { kFetch = 0, kSelect = 1, kStep = 2, kDone = 3 };
+  const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
+  const float dt = a.dt;
+  int state = kFetch, b0 = 0, b1 = 0, sid = 0;
+  ListRay ray{-1, 0, 0, 0};
+  uint32_t rng = 0, ns = 0;
+  float d[3] = {1.f, 0.f, 0.f};
+  float4 c = make_float4(0.f, 0.f, 0.f, 0.f), acc = c;
+  float z = 1.f, tmin = 0.f, t = 0.f, t1 = 0.f;
+  BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
+  float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
+  int bid = 0;
+  long long clock_begin = 0;
+  // end of one spp sample: sum it, start the next or hand the ray back
+  auto end_sample = [&]() {
+    acc.x = acc.x + c.x;
+    acc.y = acc.y + c.y;
+    acc.z = acc.z + c.z;
+    acc.w = acc.w + c.w;
+    if (++sid < a.spp) {
+      c = make_float4(0.f, 0.f, 0.f, 0.f);
+      tmin = 0.f;
+      state = kSelect;
+    } else {
+      if (gl == 0) store(ray, acc, z, ns, clock_begin);
+      state = kFetch;
+    }
+  };
+  // kSelect: closest reported brick hit from tmin (VolBrickIsect, ties to
+  // the lower index); sets up the segment or ends the spp sample
+  auto select = [&]() {
+    int hit = -1;
+    float best = 0.f, bnear = 0.f, bt0 = 0.f, bfar = 0.f;
+    for (int kb = b0; kb < b1; ++kb) {
+      const LdsBrick &db = tb.brick[kb];
+      float s0, s1, rep, tn;
+      if (!slab(org, d, tmin, 1e10f, db.lo, db.hi, s0, s1)) continue;
+      if (s0 > tmin) {
+        rep = s0;
+        tn = s0;
+      } else if (s1 > tmin) {
+        rep = s1;
+        tn = 0.f;
+      } else {
+        continue;
+      }
+      if (hit < 0 || rep < best) {  // ties keep the earlier entry
+        hit = kb;
+        best = rep;
+        bnear = tn;
+        bt0 = s0;
+        bfar = s1;
+      }
+    }
+    if (hit < 0) {
+      end_sample();
+      return;
+    }
+    bnear = bnear + tmin;
+    const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
+    // firstSampleT
+    const float segs = floorf((bt0 - dt * j) / dt);
+    t = dt * (j + segs);
+    if (t < bt0) t += dt;
+    t1 = bfar;
+    bid = a.rank_bricks[hit];
+    const LdsBrick &db = tb.brick[hit];
+    bv = BrickView{(const gfloat *)db.origin, db.px, db.pxy, db.n[0], db.n[1], db.n[2]};
+    lo0 = db.vlo[0];
+    lo1 = db.vlo[1];
+    lo2 = db.vlo[2];
+    z = fminf(z, bnear / a.max_depth);
+    tmin = bfar + 1e-20f;
+    if (t < t1 && c.w < .99f) state = kStep;
+    else if (!(c.w < .99f)) end_sample();  // else: select the next brick next trip
+  };
+  // One trip: a group whose segment ended selects its next brick (or ends
+  // its ray), a group without a ray takes the next entry and selects its
+  // first brick, then every group with a segment steps -- so a ray's
+  // transitions cost no extra trips.
+  while (__any(state != kDone)) {
+    if (state == kSelect) select();
+    // kFetch: the group's first lane takes an entry; all lanes broadcast
+    int k = fetch(state == kFetch && gl == 0);
+    k = group_bcast_i<G, 0>(k, gbase);
+    if (state == kFetch) {
+      if (k >= count) {
+        state = kDone;
+      } else {
+        const ListRay r = entry(k);
+        if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
+        ray = r;
+        // slot -1: a pixel no brick can reach (no brick to select: the miss result)
+        b0 = r.slot < 0 ? 0 : tb.roff[r.slot];
+        b1 = r.slot < 0 ? 0 : tb.roff[r.slot + 1];
+        pixel_ray(dir00, du, dv, r.px, r.py, d);
+        rng = rng_seed((uint32_t)(r.px + a.W * r.py), (uint32_t)a.accum_id);
+        sid = 0;
+        ns = 0;
+        z = 1.f;
+        acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        c = acc;
+        tmin = 0.f;
+        state = kSelect;
+        select();
+      }
+    }
+    // kStep: G samples of the group's segment
+    const bool go = state == kStep;
     float tk = t;
     for (int j = 0; j < gl; ++j) tk += dt;
     float tn = tk;  // base of the next step: G additions in all
     for (int j = gl; j < G; ++j) tn += dt;
     const bool valid = go && tk < t1;
     float4 tf = make_float4(0.f, 0.f, 0.f, 0.f);
-    float shade = 0.f;
+    // an evaluated sample has shade >= 0.1 or NaN, so shade < 0 marks "no
+    // sample" (one value less to broadcast)
+    float shade = -1.f;
     if (valid) {
-      const float x = (o[0] + tk * d[0]) - lo0;
-      const float y = (o[1] + tk * d[1]) - lo1;
-      const float z = (o[2] + tk * d[2]) - lo2;
+      const float x = (org[0] + tk * d[0]) - lo0;
+      const float y = (org[1] + tk * d[1]) - lo1;
+      const float zz = (org[2] + tk * d[2]) - lo2;
       Sample s;
-      if (!sample_fast(b, x, y, z, s)) s = sample_exact<false>(b, nullptr, a.bricks[bid], x, y, z);
+      if (!sample_fast(bv, x, y, zz, s)) s = sample_exact<false>(bv, nullptr, a.bricks[bid], x, y, zz);
       const float ex = s.gx + 1e-4f, ey = s.gy + 1e-4f, ez = s.gz + 1e-4f;
       const float len = sqrtf(ex * ex + ey * ey + ez * ez);
       const float nx = s.gx / len, ny = s.gy / len, nz = s.gz / len;
-      tf = tf_lookup(s_tf, a.tf_n, s.val);
+      tf = tf_lookup(tb.tf, a.tf_n, s.val);
       shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
     }
-    const int vbits = valid ? 1 : 0;
     bool live = go;
-#pragma unroll
-    for (int j = 0; j < G; ++j) {
-      int vj = vbits;
-      float tx = tf.x, ty = tf.y, tz = tf.z, tw = tf.w, sh = shade;
-      if constexpr (G > 1) {
-        const int src = gbase + j;
-        vj = __shfl(vbits, src);
-        tx = __shfl(tf.x, src), ty = __shfl(tf.y, src), tz = __shfl(tf.z, src);
-        tw = __shfl(tf.w, src), sh = __shfl(shade, src);
-      }
-      live = live && vj && c.w < .99f;
+    static_for<0, G>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const float tx = group_bcast<G, j>(tf.x, gbase), ty = group_bcast<G, j>(tf.y, gbase);
+      const float tz = group_bcast<G, j>(tf.z, gbase), tw = group_bcast<G, j>(tf.w, gbase);
+      const float sh = group_bcast<G, j>(shade, gbase);
+      live = live && !(sh < 0.f) && c.w < .99f;
       if (live) {
         ++ns;
         const float w = (1.f - c.w) * tw;
@@ -141,72 +338,15 @@ __device__ __forceinline__ void group_integrate(const MarchArgs &a, const float4
         c.z = c.z + (w * tz) * sh;
         c.w = c.w + w;
       }
+    });
+    if (go) {
+      t = tn;
+      if (!(t < t1 && c.w < .99f)) {  // segment over
+        if (c.w < .99f) state = kSelect;
+        else end_sample();
+      }
     }
-    if (go) t = tn;
-    go = go && t < t1 && c.w < .99f;
   }
-}
-
-// rank_pixel for a group of G lanes sharing pixel `pixel` (live: the group
-// has a pixel to march).  Same brick loop, in wave-uniform control flow.
-template <int G>
-__device__ __forceinline__ float4 group_rank_pixel(const MarchArgs &a, const float4 *s_tf,
-                                                   bool live, int slot, const float org[3],
-                                                   const float d[3], uint32_t pixel, float &z,
-                                                   uint32_t &ns, int gl, int gbase) {
-  const int b0 = live ? a.rank_off[slot] : 0, b1 = live ? a.rank_off[slot + 1] : 0;
-  uint32_t rng = rng_seed(pixel, (uint32_t)a.accum_id);
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int sid = 0; sid < a.spp; ++sid) {
-    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-    float tmin = 0.f;
-    bool go = live;
-    while (__any(go)) {
-      int hit = -1;
-      float best = 0.f, bnear = 0.f, bt0 = 0.f, bfar = 0.f;
-      if (go) {
-        for (int k = b0; k < b1; ++k) {
-          const int bid = a.rank_bricks[k];
-          const DevBrick &db = a.bricks[bid];
-          float t0, t1, rep, tn;
-          if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
-          if (t0 > tmin) {
-            rep = t0;
-            tn = t0;
-          } else if (t1 > tmin) {
-            rep = t1;
-            tn = 0.f;
-          } else {
-            continue;
-          }
-          if (hit < 0 || rep < best) {
-            hit = bid;
-            best = rep;
-            bnear = tn;
-            bt0 = t0;
-            bfar = t1;
-          }
-        }
-      }
-      const bool run = go && hit >= 0;
-      float j = 0.f;
-      if (run) {
-        bnear = bnear + tmin;
-        j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
-      }
-      group_integrate<G>(a, s_tf, run, hit, org, d, bt0, bfar, j, c, ns, gl, gbase);
-      if (run) {
-        z = fminf(z, bnear / a.max_depth);
-        tmin = bfar + 1e-20f;
-      }
-      go = run && c.w < .99f;
-    }
-    acc.x = acc.x + c.x;
-    acc.y = acc.y + c.y;
-    acc.z = acc.z + c.z;
-    acc.w = acc.w + c.w;
-  }
-  return acc;
 }
 
 // One rank's spp-summed colour of one pixel: renderFrame's sample loop and
@@ -275,11 +415,6 @@ __device__ __forceinline__ bool block_meets(const MarchArgs &a, int slot, int bx
 __device__ __forceinline__ bool in_rect(const MarchArgs &a, int slot, int px, int py) {
   const int32_t *rc = a.rect[slot];
   return px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
-}
-
-__device__ __forceinline__ void load_tf(const MarchArgs &a, float4 *s_tf) {
-  for (int i = threadIdx.x; i < a.tf_n + 4; i += kBlock)
-    s_tf[i] = (i >= 2 && i < a.tf_n + 2) ? a.tf[i - 2] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // island mask with the reference's gridSize formula (deviceCode.cpp:346-361):
@@ -355,40 +490,44 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   }
 }
 
-// ---- group kernels: 32x8-pixel workgroup tiles, G lanes per ray ---------
-//
-// A workgroup owns one 32x8 tile of one rank image (per-rank form) or of the
-// final image (fused form).  Its waves pull batches of 64/G consecutive
-// pixels from an LDS counter, one pixel per group of G lanes, march each
-// pixel with its group (group_integrate),
-// and stage the per-pixel results in LDS; then the 256 threads write the
-// tile row by row (128-byte rows for the 4-byte outputs).  The per-rank grid
-// lists the tiles that meet a rank's projected rectangle first, so the long
-// marches start at once, and the tiles that miss every pixel of their rank
-// after them: those only write the miss result.
-constexpr int kGW = 32, kGH = 8;
-static_assert(kGW * kGH == kBlock, "one pixel per thread in the epilogue");
-
-struct Staged {
-  float4 acc[kBlock];
-  float z[kBlock];
-  uint32_t ns[kBlock];
-  float heat[kBlock];
-  int next;
-};
-
-// tile range of rank slot s's rectangle: [tx0, tx0+ntx) x [ty0, ty0+nty)
-__device__ __forceinline__ void rect_tiles(const MarchArgs &a, int s, int &tx0, int &ty0, int &ntx,
-                                           int &nty) {
-  const int32_t *rc = a.rect[s];
-  if (rc[0] >= rc[2] || rc[1] >= rc[3]) {
-    tx0 = ty0 = ntx = nty = 0;
-    return;
+// Whether the primary ray (org, d) reaches a brick of rank `slot`: the first
+// brick selection of march_list (tmin = 0) finds a hit.  A ray that
+// reaches none keeps the miss result (no sample, colour 0, depth 1).
+__device__ __forceinline__ bool ray_meets_rank(const MarchTables &tb, int slot, const float org[3],
+                                               const float d[3]) {
+  for (int k = tb.roff[slot]; k < tb.roff[slot + 1]; ++k) {
+    const LdsBrick &db = tb.brick[k];
+    float t0, t1;
+    if (slab(org, d, 0.f, 1e10f, db.lo, db.hi, t0, t1) && (t0 > 0.f || t1 > 0.f)) return true;
   }
-  tx0 = rc[0] / kGW;
-  ty0 = rc[1] / kGH;
-  ntx = (rc[2] + kGW - 1) / kGW - tx0;
-  nty = (rc[3] + kGH - 1) / kGH - ty0;
+  return false;
+}
+
+// Stream compaction of the items i in [0, n) with live(i) into list, in
+// index order (a ballot and popcount per wave, wave totals through wcnt);
+// returns the count, block-uniform.  Every thread of the block calls it; it
+// ends with a barrier, so LDS written before the call is visible after it.
+template <class Live>
+__device__ __forceinline__ int compact_items(int n, uint16_t *list, int *wcnt, const Live &live) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int total = 0;
+  for (int i0 = 0; i0 < n; i0 += kBlock) {
+    const int i = i0 + t;
+    const bool on = i < n && live(i);
+    const uint64_t m = __ballot(on);
+    if (lane == 0) wcnt[w] = __popcll(m);
+    __syncthreads();
+    int base = total, sum = total;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+      base += k < w ? wcnt[k] : 0;
+      sum += wcnt[k];
+    }
+    if (on) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)i;
+    __syncthreads();  // wcnt is rewritten by the next round; the list is complete
+    total = sum;
+  }
+  return total;
 }
 
 // Per-pixel epilogue of a rank image (renderFrame :395-415), as march_kernel.
@@ -427,15 +566,6 @@ __device__ __forceinline__ void write_rank_pixel(const MarchArgs &a, long long o
   }
 }
 
-// Pixel order inside a 32x8 tile: four 8x8 sub-tiles, row-major inside
-// each, so a batch of 64 (32, 16, 8) consecutive indices is an 8x8 (8x4, 8x2,
-// 8x1) pixel block -- compact footprints for the lanes of a wave.
-__device__ __forceinline__ void tile_pixel(int idx, int &lx, int &ly) {
-  lx = (idx >> 6) * 8 + (idx & 7);
-  ly = (idx >> 3) & 7;
-}
-__device__ __forceinline__ int tile_index(int lx, int ly) { return (lx >> 3) * 64 + ly * 8 + (lx & 7); }
-
 // Fill part shared by the launches: `nimg` images of H rows of W pixels.
 // Fill block fb of nfb takes a contiguous run of rows and calls put(img, x,
 // y) for every pixel outside its image's marched box [x0, x1) x [y0, y1)
@@ -464,92 +594,101 @@ __device__ __forceinline__ void fill_rows(int W, int H, int nimg, int fb, int nf
   }
 }
 
-// Per-rank images.  Blocks [0, active) march the tiles that meet their rank's
-// rectangle (rank by rank); the `fill_blocks` blocks after them stride over
-// all (rank, tile) pairs and write the miss result of the tiles outside the
-// rectangles -- few blocks, many tiles each, so the cheap part costs no
-// per-block launch latency.
+// ---------------------------------------------------------------------------
+// Persistent per-rank launch.
+//
+// The work of a frame is every pixel of every rank's rectangle (the pixels a
+// rank's bricks can cover) -- q.rect_off[s] numbers them rank after rank,
+// row-major -- plus the miss result everywhere else.  The grid is one wave
+// per block, as many as the GPU holds at once.  Wave w takes the rectangle
+// pixels in runs of kRun consecutive pixels (a row segment, so the rays of a
+// run are coherent): runs w, w + nw, w + 2 nw, ... -- a static round-robin
+// that spreads every region's cost over all waves without a single atomic.
+// Lane groups march the rays with march_list (a pixel whose ray reaches no
+// brick of its rank just stores the miss result); a wave that is done then
+// writes the miss result of its share of the image rows outside the
+// rectangles.  Static shares need no counters, so there is nothing to reset
+// between launches.
+constexpr int kRun = 16;
+
 template <int G>
-__global__ __launch_bounds__(kBlock) void march_group_kernel(const MarchArgs a, int fill_blocks) {
-  extern __shared__ float4 s_tf[];
-  __shared__ Staged st;
-  int b = (int)blockIdx.x, slot = -1, tx = 0, ty = 0;
-  for (int s = 0; s < a.P; ++s) {
-    int tx0, ty0, ntx, nty;
-    rect_tiles(a, s, tx0, ty0, ntx, nty);
-    if (b < ntx * nty) {
-      slot = s;
-      tx = tx0 + b % ntx;
-      ty = ty0 + b / ntx;
-      break;
-    }
-    b -= ntx * nty;
-  }
-  if (slot < 0) {  // fill part (block-uniform): the pixels outside the marched tiles
-    const bool heat = a.flags & ALOHA_FLAG_HEATMAP;
-    const float heat_scale = a.heat_scale / (float)a.spp;
-    fill_rows(
-        a.W, a.H, a.P, b, fill_blocks,
-        [&](int s, int &x0, int &y0, int &x1, int &y1) {
-          int tx0, ty0, ntx, nty;
-          rect_tiles(a, s, tx0, ty0, ntx, nty);
-          x0 = tx0 * kGW, x1 = (tx0 + ntx) * kGW, y0 = ty0 * kGH, y1 = (ty0 + nty) * kGH;
-        },
-        [&](int s, int px, int py) {
-          // a miss costs no march time: the heat map shows the clock read itself
-          long long clock_begin = 0;
-          if (heat) clock_begin = clock64();
-          const float heat_t = heat ? (float)(clock64() - clock_begin) * heat_scale : 0.f;
-          write_rank_pixel(a, (long long)s * a.W * a.H + px + (long long)a.W * py, px, py,
-                           make_float4(0.f, 0.f, 0.f, 0.f), 1.f, 0u, heat_t);
-        });
-    return;
-  }
-  const int bx = tx * kGW, by = ty * kGH;
-  load_tf(a, s_tf);
-  if (threadIdx.x == 0) st.next = 0;
+__global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, const RanksArgs q) {
+  extern __shared__ float4 smem[];
+  __shared__ int32_t s_off[ALOHA_MAX_RANKS + 1];
+  const MarchTables tb = load_tables(a, smem);
+  for (int i = threadIdx.x; i <= a.P; i += blockDim.x) s_off[i] = q.rect_off[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
-  // one counter update per wave: its 64/G groups take consecutive pixels;
-  // the batch base is wave-uniform, so the loop exit is a scalar branch.
-  // Each pass takes a fresh batch, so kBlock passes always suffice.
-  for (int pass = 0; pass < kBlock; ++pass) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&st.next, 64 / G);
-    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-    if (base >= kBlock) break;
-    // every lane stays in the loop body (predicated), so the group march
-    // below runs in wave-uniform control flow
-    const int idx = base + lane / G;
-    int lx, ly;
-    tile_pixel(idx, lx, ly);
-    const int px = bx + lx, py = by + ly;
-    long long clock_begin = 0;
-    if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
-    float z = 1.f;
-    uint32_t ns = 0;
-    const bool live = idx < kBlock && px < a.W && py < a.H && in_rect(a, slot, px, py) &&
-                      island_pixel(a, px, py);
-    float d[3] = {1.f, 0.f, 0.f};
-    if (live) pixel_ray(a.dir00, a.du, a.dv, px, py, d);
-    const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, a.org, d,
-                                           (uint32_t)(px + a.W * py), z, ns, gl, gbase);
-    if (gl == 0 && idx < kBlock) {
-      st.acc[idx] = acc;
-      st.z[idx] = z;
-      st.ns[idx] = ns;
-      if (a.flags & ALOHA_FLAG_HEATMAP)
-        st.heat[idx] = (float)(clock64() - clock_begin) * (a.heat_scale / (float)a.spp);
+  const int lane = (int)threadIdx.x, w = (int)blockIdx.x, nw = (int)gridDim.x;
+  const int total = q.rect_off[a.P];
+  const float heat_scale = a.heat_scale / (float)a.spp;
+  // this wave's position in its stream of pixels (wave-uniform)
+  int pos = 0;
+  const int count = total > 0 ? ((total + kRun - 1) / kRun + nw - 1 - w) / nw * kRun : 0;
+  auto fetch = [&](bool want) {
+    const uint64_t m = __ballot(want);
+    const int k = pos + __popcll(m & ((1ull << lane) - 1ull));
+    pos += __popcll(m);
+    return k;
+  };
+  // stream index -> rectangle pixel index (or -1 past the end)
+  auto pixel_of = [&](int k) {
+    const int v = (w + (k / kRun) * nw) * kRun + k % kRun;
+    return v < total ? v : -1;
+  };
+  march_list<G>(
+      a, tb, a.org, a.dir00, a.du, a.dv, fetch, count,
+      [&](int k) {
+        const int v = pixel_of(k);
+        if (v < 0) return ListRay{-1, 0, 0, -1};
+        int lo = 0, hi = a.P;  // the slot s with s_off[s] <= v < s_off[s + 1]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= v) lo = mid;
+          else hi = mid;
+        }
+        const int32_t *rc = tb.rect + 4 * lo;
+        const int rw = rc[2] - rc[0], l = v - s_off[lo], row = l / rw;
+        const int px = rc[0] + (l - row * rw), py = rc[1] + row;
+        if (px >= a.W || py >= a.H) return ListRay{-1, 0, 0, -1};  // nothing to write
+        // tag: the rank slot (the slot is -1 when no brick can be reached)
+        return ListRay{island_pixel(a, px, py) ? lo : -1, px, py, lo};
+      },
+      [&](const ListRay &r, float4 acc, float z, uint32_t ns, long long clock_begin) {
+        if (r.tag < 0) return;
+        const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
+                                 ? (float)(clock64() - clock_begin) * heat_scale
+                                 : 0.f;
+        write_rank_pixel(a, (long long)r.tag * a.W * a.H + r.px + (long long)a.W * r.py, r.px,
+                         r.py, acc, z, ns, heat_t);
+      });
+
+  // fill: the miss result of every pixel outside the rectangles, rows
+  // w, w + nw, ... of the P x H rank image rows
+  const bool heat = a.flags & ALOHA_FLAG_HEATMAP;
+  const bool plain = !(a.flags & (ALOHA_FLAG_HEATMAP | ALOHA_FLAG_CROSSHAIR)) &&
+                     !(a.accum && a.accum_id > 0);
+  for (int r = w; r < a.P * a.H; r += nw) {
+    const int s = r / a.H, y = r - s * a.H;
+    const int32_t *rc = tb.rect + 4 * s;
+    const bool band = rc[0] < rc[2] && y >= rc[1] && y < rc[3];
+    for (int x = lane; x < a.W; x += 64) {
+      if (band && x >= rc[0] && x < rc[2]) continue;  // a rectangle pixel: marched
+      const long long o = (long long)s * a.W * a.H + x + (long long)a.W * y;
+      if (plain) {
+        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (a.depth) a.depth[o] = 1.f;
+        if (a.nsamples) a.nsamples[o] = 0u;
+        if (a.accum) a.accum[o] = zero;
+        if (a.rgba) a.rgba[o] = zero;
+        if (a.rgba8) a.rgba8[o] = 0u;
+      } else {
+        long long clock_begin = 0;
+        if (heat) clock_begin = clock64();
+        const float heat_t = heat ? (float)(clock64() - clock_begin) * heat_scale : 0.f;
+        write_rank_pixel(a, o, x, y, make_float4(0.f, 0.f, 0.f, 0.f), 1.f, 0u, heat_t);
+      }
     }
   }
-  __syncthreads();
-  // epilogue: row-major over the tile, 128-byte rows for 4-byte outputs
-  const int t = threadIdx.x;
-  const int px = bx + t % kGW, py = by + t / kGW;
-  if (px >= a.W || py >= a.H) return;
-  const int k = tile_index(t % kGW, t / kGW);
-  write_rank_pixel(a, (long long)slot * a.W * a.H + px + (long long)a.W * py, px, py, st.acc[k],
-                   st.z[k], st.ns[k], st.heat[k]);
 }
 
 // Fused raygen + composite.  The work items of a tile are (rank, pixel)
@@ -578,6 +717,9 @@ struct FusedStage {
   float4 col[kFusedChunk][kFPix];
   uint32_t ns[kFPix];
   int ranks[ALOHA_MAX_RANKS];
+  int32_t rc[ALOHA_MAX_RANKS][4];  // their rectangles
+  uint16_t list[kFusedChunk * kFPix];  // live (rank, pixel) items of the chunk
+  int wcnt[kBlock / 64];
   int nranks, next;
 };
 
@@ -624,14 +766,17 @@ __device__ __forceinline__ void fused_clear(const MarchArgs &a, long long pixel)
 // repeatedly on the same `fs` when a barrier separates the calls.
 template <int G, class Src>
 __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, FusedStage &fs,
-                                           const float4 *s_tf, int bx, int by) {
+                                           const MarchTables &tb, int bx, int by) {
   const int t = threadIdx.x;
   if (t == 0) {
     int n = 0;
     for (int k = 0; k < a.P; ++k) {
       const int s = src.order(k);
       const int32_t *rc = src.rect(s);
-      if (bx < rc[2] && bx + kFW > rc[0] && by < rc[3] && by + kFH > rc[1]) fs.ranks[n++] = s;
+      if (bx < rc[2] && bx + kFW > rc[0] && by < rc[3] && by + kFH > rc[1]) {
+        for (int i = 0; i < 4; ++i) fs.rc[n][i] = rc[i];
+        fs.ranks[n++] = s;
+      }
     }
     fs.nranks = n;
   }
@@ -639,40 +784,44 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
   __syncthreads();
   const int nranks = fs.nranks;  // block-uniform
   float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int lane = t & 63, gl = lane % G, gbase = lane - gl;
   const float inv_spp = 1.f / (float)a.spp;
   const int own = fused_index(t % kFW, (t / kFW) % kFH);  // pixel of thread t < kFPix
   for (int c0 = 0; c0 < nranks; c0 += kFusedChunk) {
     const int nc = min(kFusedChunk, nranks - c0);
-    if (t == 0) fs.next = 0;
-    __syncthreads();  // counter reset, previous chunk blended
+    __syncthreads();  // previous chunk blended
     const int items = nc * kFPix;
-    for (int pass = 0; pass < kFusedChunk * kFPix; ++pass) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&fs.next, 64 / G);
-      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-      if (base >= items) break;
-      // 64/G consecutive items share one rank (kFPix is a multiple of 64)
-      const int item = base + lane / G;
+    // a rank whose bricks the pixel's ray misses contributes exact zeros
+    for (int i = t; i < items; i += kBlock) fs.col[i / kFPix][i % kFPix] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t == 0) fs.next = 0;
+    // the items whose ray reaches one of the rank's bricks, in item order
+    const int count = compact_items(items, fs.list, fs.wcnt, [&](int item) {
       const int r = item / kFPix, idx = item - r * kFPix;
-      const int slot = fs.ranks[c0 + min(r, nc - 1)];
+      const int slot = fs.ranks[c0 + r];
       int lx, ly;
       fused_pixel(idx, lx, ly);
       const int px = bx + lx, py = by + ly;
-      const bool live = item < items && px < a.W && py < a.H && in_rc(src.rect(slot), px, py) &&
-                        island_pixel(a, px, py);
-      float d[3] = {1.f, 0.f, 0.f};
-      if (live) pixel_ray(src.dir00(), src.du(), src.dv(), px, py, d);
-      float z = 1.f;
-      uint32_t ns = 0;
-      const float4 acc = group_rank_pixel<G>(a, s_tf, live, slot, src.org(), d,
-                                             (uint32_t)(px + a.W * py), z, ns, gl, gbase);
-      if (gl == 0 && item < items) {
-        fs.col[r][idx] = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
-                                     (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
-        if (ns) atomicAdd(&fs.ns[idx], ns);
-      }
-    }
+      if (px >= a.W || py >= a.H || !in_rc(fs.rc[c0 + r], px, py) || !island_pixel(a, px, py))
+        return false;
+      float d[3];
+      pixel_ray(src.dir00(), src.du(), src.dv(), px, py, d);
+      return ray_meets_rank(tb, slot, src.org(), d);
+    });
+    march_list<G>(
+        a, tb, src.org(), src.dir00(), src.du(), src.dv(),
+        [&](bool want) { return want ? atomicAdd(&fs.next, 1) : 0; }, count,
+        [&](int k) {
+          const int item = fs.list[k];
+          const int r = item / kFPix, idx = item - r * kFPix;
+          int lx, ly;
+          fused_pixel(idx, lx, ly);
+          return ListRay{fs.ranks[c0 + r], bx + lx, by + ly, item};
+        },
+        [&](const ListRay &ray, float4 acc, float, uint32_t ns, long long) {
+          const int item = ray.tag, r = item / kFPix, idx = item - r * kFPix;
+          fs.col[r][idx] = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
+                                       (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
+          if (ns) atomicAdd(&fs.ns[idx], ns);
+        });
     __syncthreads();
     if (t < kFPix)
       for (int r = 0; r < nc; ++r) {
@@ -715,9 +864,9 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
         [&](int, int px, int py) { fused_clear(a, px + (long long)a.W * py); });
     return;
   }
-  load_tf(a, s_tf);
-  fused_tile<G>(a, src, fs, s_tf, (u.x + (int)blockIdx.x % u.z) * kFW,
-                (u.y + (int)blockIdx.x / u.z) * kFH);
+  const int item = (int)blockIdx.x;  // row-major tiles of the box
+  const MarchTables tb = load_tables(a, s_tf);
+  fused_tile<G>(a, src, fs, tb, (u.x + item % u.z) * kFW, (u.y + item / u.z) * kFH);
 }
 
 // Batched form: V views of the same ranks in one launch.  The active work
@@ -765,13 +914,14 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
       batch_fill(b, (int)blockIdx.x - b.total_active, b.fill_blocks);
       return;
     }
-    load_tf(b.m, s_tf);
+    const int item = (int)blockIdx.x;
+    const MarchTables tb = load_tables(b.m, s_tf);
     int view, bx, by;
-    batch_active(b, (int)blockIdx.x, view, bx, by);
-    fused_tile<G>(b.m, ViewSrc{b, view}, fs, s_tf, bx, by);
+    batch_active(b, item, view, bx, by);
+    fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
     return;
   }
-  load_tf(b.m, s_tf);
+  const MarchTables tb = load_tables(b.m, s_tf);
   for (int it = 0; it <= b.total_active; ++it) {  // bounded: each pass takes a fresh item
     if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter, 1u);
     __syncthreads();
@@ -780,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
     if (item >= b.total_active) break;
     int view, bx, by;
     batch_active(b, item, view, bx, by);
-    fused_tile<G>(b.m, ViewSrc{b, view}, fs, s_tf, bx, by);
+    fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
   }
   batch_fill(b, (int)blockIdx.x, (int)gridDim.x);
 }
@@ -830,7 +980,7 @@ hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) 
   }
   const int fill_blocks = std::min<int>(a.H, kFillBlocks);
   const dim3 grid((unsigned)(u.z * u.w + fill_blocks));
-  const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
+  const size_t lds = march_tables_bytes(a);
   switch (group) {
     case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
     case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
@@ -842,7 +992,7 @@ hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) 
 }
 
 hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s) {
-  const size_t lds = sizeof(float4) * (size_t)(b.m.tf_n + 4);
+  const size_t lds = march_tables_bytes(b.m);
   if (persistent_blocks > 0) {
     reset_counter_kernel<<<1, 1, 0, s>>>(b.counter);
     const dim3 grid(persistent_blocks);
@@ -866,31 +1016,30 @@ hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hi
   return hipGetLastError();
 }
 
-hipError_t launch_march(const MarchArgs &a, bool footprint, int group, hipStream_t s) {
-  const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4);
-  if (footprint) {
-    const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, a.P);
-    march_kernel<true><<<grid, kBlock, lds, s>>>(a);
-    return hipGetLastError();
-  }
-  // active tiles of every rank, then the fill blocks (runs of image rows)
-  const long long rows = (long long)a.P * a.H;
-  if (rows > INT32_MAX) return hipErrorInvalidValue;
-  const int fill_blocks = (int)std::min<long long>(rows, kFillBlocks);
-  long long blocks = fill_blocks;
-  for (int r = 0; r < a.P; ++r) {
-    const int32_t *rc = a.rect[r];
-    if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
-    const int tx0 = rc[0] / kGW, ty0 = rc[1] / kGH;
-    blocks += (long long)((rc[2] + kGW - 1) / kGW - tx0) * ((rc[3] + kGH - 1) / kGH - ty0);
-  }
-  if (blocks > INT32_MAX) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)blocks);
+hipError_t launch_march_footprint(const MarchArgs &a, hipStream_t s) {
+  const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, a.P);
+  march_kernel<true><<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu) {
   switch (group) {
-    case 1: march_group_kernel<1><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
-    case 2: march_group_kernel<2><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
-    case 4: march_group_kernel<4><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
-    case 8: march_group_kernel<8><<<grid, kBlock, lds, s>>>(a, fill_blocks); break;
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<1>, 64, lds);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<2>, 64, lds);
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<4>, 64, lds);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<8>, 64, lds);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_march_ranks(const MarchArgs &a, const RanksArgs &q, int group, int waves,
+                              hipStream_t s) {
+  const size_t lds = march_tables_bytes(a);
+  switch (group) {
+    case 1: march_persist_kernel<1><<<waves, 64, lds, s>>>(a, q); break;
+    case 2: march_persist_kernel<2><<<waves, 64, lds, s>>>(a, q); break;
+    case 4: march_persist_kernel<4><<<waves, 64, lds, s>>>(a, q); break;
+    case 8: march_persist_kernel<8><<<waves, 64, lds, s>>>(a, q); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

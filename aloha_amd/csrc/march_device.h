@@ -84,11 +84,14 @@ __device__ __forceinline__ float tap_exact(const BrickView &b, uint32_t *touched
       for (int j = 0; j < 2; ++j)
         for (int i = 0; i < 2; ++i) mark(touched, db, ix + i, iy + j, iz + k);
   }
-  const gfloat *p = b.o + voff(b, ix, iy, iz);
-  const float c00 = lerp_(p[0], p[1], wx);
-  const float c10 = lerp_(p[b.px], p[b.px + 1], wx);
-  const float c01 = lerp_(p[b.pxy], p[b.pxy + 1], wx);
-  const float c11 = lerp_(p[b.px + b.pxy], p[b.px + b.pxy + 1], wx);
+  // 32-bit element indices per corner row: no 64-bit address offsets stay
+  // live across a caller's loop for this (rarely taken) path
+  const uint32_t i00 = voff(b, ix, iy, iz), i10 = i00 + b.px, i01 = i00 + b.pxy,
+                 i11 = i10 + b.pxy;
+  const float c00 = lerp_(b.o[i00], b.o[i00 + 1u], wx);
+  const float c10 = lerp_(b.o[i10], b.o[i10 + 1u], wx);
+  const float c01 = lerp_(b.o[i01], b.o[i01 + 1u], wx);
+  const float c11 = lerp_(b.o[i11], b.o[i11 + 1u], wx);
   const float c0 = lerp_(c00, c10, wy);
   const float c1 = lerp_(c01, c11, wy);
   return lerp_(c0, c1, wz);
@@ -101,11 +104,26 @@ struct Sample {
 template <bool kFoot>
 __device__ __forceinline__ Sample sample_exact(const BrickView &b, uint32_t *touched,
                                                const DevBrick &db, float x, float y, float z) {
+  // one tap at a time (scheduling barriers between them): this path is
+  // rare, and keeping its 56 loads in flight at once would raise the
+  // register count of every kernel that inlines it
   Sample s;
-  s.gx = tap_exact<kFoot>(b, touched, db, x + .5f, y, z) - tap_exact<kFoot>(b, touched, db, x - .5f, y, z);
-  s.gy = tap_exact<kFoot>(b, touched, db, x, y + .5f, z) - tap_exact<kFoot>(b, touched, db, x, y - .5f, z);
-  s.gz = tap_exact<kFoot>(b, touched, db, x, y, z + .5f) - tap_exact<kFoot>(b, touched, db, x, y, z - .5f);
+  const float xp = tap_exact<kFoot>(b, touched, db, x + .5f, y, z);
+  __builtin_amdgcn_sched_barrier(0);
+  const float xm = tap_exact<kFoot>(b, touched, db, x - .5f, y, z);
+  __builtin_amdgcn_sched_barrier(0);
+  const float yp = tap_exact<kFoot>(b, touched, db, x, y + .5f, z);
+  __builtin_amdgcn_sched_barrier(0);
+  const float ym = tap_exact<kFoot>(b, touched, db, x, y - .5f, z);
+  __builtin_amdgcn_sched_barrier(0);
+  const float zp = tap_exact<kFoot>(b, touched, db, x, y, z + .5f);
+  __builtin_amdgcn_sched_barrier(0);
+  const float zm = tap_exact<kFoot>(b, touched, db, x, y, z - .5f);
+  __builtin_amdgcn_sched_barrier(0);
   s.val = tap_exact<kFoot>(b, touched, db, x, y, z);
+  s.gx = xp - xm;
+  s.gy = yp - ym;
+  s.gz = zp - zm;
   return s;
 }
 
