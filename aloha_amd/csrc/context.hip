@@ -201,6 +201,8 @@ int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t
   a.H = H;
   a.spp = p->spp;
   a.accum_id = p->accum_id;
+  a.inv_frames = 1.f / (float)(p->accum_id + 1);
+  a.inv_spp = 1.f / (float)p->spp;
   a.island_id = p->island_id;
   a.num_islands = p->num_islands;
   a.jitter_mode = p->jitter_mode;
@@ -401,14 +403,16 @@ extern "C" int aloha_camera_set(aloha_ctx *ctx, const float org[3], const float 
 namespace aloha {
 
 int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, int &waves) {
+  const int run = rank_run(lanes);
   int64_t total = 0;
-  q.rect_off[0] = 0;
+  q.run_off[0] = 0;
   for (int s = 0; s < a.P; ++s) {
     const int32_t *rc = a.rect[s];
-    if (rc[0] < rc[2] && rc[1] < rc[3]) total += (int64_t)(rc[2] - rc[0]) * (rc[3] - rc[1]);
-    if (total > INT32_MAX / 2)
+    if (rc[0] < rc[2] && rc[1] < rc[3])
+      total += (int64_t)(rc[3] - rc[1]) * ((rc[2] - rc[0] + run - 1) / run);
+    if (total > INT32_MAX / 128)
       return fail(ctx, ALOHA_ERR_LIMIT, "render: rank rectangles too large");
-    q.rect_off[s + 1] = (int32_t)total;
+    q.run_off[s + 1] = (int32_t)total;
   }
   // persistent grid: as many one-wave blocks as the GPU holds at once
   const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;

@@ -32,6 +32,7 @@ struct DevBrick {
 struct MarchArgs {
   float org[3], dir00[3], du[3], dv[3];
   float dt, max_depth, jitter_value, heat_scale;
+  float inv_frames, inv_spp;  // 1 / (accum_id + 1), 1 / spp (host IEEE divisions)
   int32_t W, H, spp, accum_id;
   int32_t island_id, num_islands, jitter_mode, flags;
   int32_t P, tf_n;
@@ -52,10 +53,13 @@ struct MarchArgs {
   int32_t order[ALOHA_MAX_RANKS];  // fused composite: rank slots front to back
 };
 
-// Persistent per-rank launch: the pixels of the ranks' rectangles, numbered
-// rank after rank, row-major; rect_off[s] is the first of rank slot s.
+// Persistent per-rank launch: the pixels of the ranks' rectangles in runs of
+// up to rank_run(G) = 64 / G consecutive pixels of one rectangle row (G lanes
+// per ray), numbered rank after rank, row-major; run_off[s] is the first run
+// of rank slot s.
+constexpr int rank_run(int lanes) { return 64 / lanes; }
 struct RanksArgs {
-  int32_t rect_off[ALOHA_MAX_RANKS + 1];
+  int32_t run_off[ALOHA_MAX_RANKS + 1];
 };
 
 // Fused frame tiles (single fused frame and every batched view): 16x4 pixels.

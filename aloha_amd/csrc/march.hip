@@ -289,7 +289,8 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
         b0 = r.slot < 0 ? 0 : tb.roff[r.slot];
         b1 = r.slot < 0 ? 0 : tb.roff[r.slot + 1];
         pixel_ray(dir00, du, dv, r.px, r.py, d);
-        rng = rng_seed((uint32_t)(r.px + a.W * r.py), (uint32_t)a.accum_id);
+        if (a.jitter_mode != ALOHA_JITTER_FIXED)
+          rng = rng_seed((uint32_t)(r.px + a.W * r.py), (uint32_t)a.accum_id);
         sid = 0;
         ns = 0;
         z = 1.f;
@@ -302,10 +303,14 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
     }
     // kStep: G samples of the group's segment
     const bool go = state == kStep;
-    float tk = t;
-    for (int j = 0; j < gl; ++j) tk += dt;
-    float tn = tk;  // base of the next step: G additions in all
-    for (int j = gl; j < G; ++j) tn += dt;
+    // the chain t, t + dt, (t + dt) + dt, ...: this lane's sample is element
+    // gl, the next step's base element G
+    float tk = t, tn = t;
+#pragma unroll
+    for (int j = 1; j <= G; ++j) {
+      tn += dt;
+      if (j == gl) tk = tn;
+    }
     const bool valid = go && tk < t1;
     float4 tf = make_float4(0.f, 0.f, 0.f, 0.f);
     // an evaluated sample has shade >= 0.1 or NaN, so shade < 0 marks "no
@@ -420,16 +425,18 @@ __device__ __forceinline__ bool in_rect(const MarchArgs &a, int slot, int px, in
 // island mask with the reference's gridSize formula (deviceCode.cpp:346-361):
 // gridSize.x = iDivUp(pixel.x, W) is 1 for every column but the first
 __device__ __forceinline__ bool island_pixel(const MarchArgs &a, int px, int py) {
-  const int gx = (px + a.W - 1) / a.W;
+  if (a.num_islands == 1) return a.island_id == 0;  // block_id % 1 == 0
+  const int gx = px > 0 ? 1 : 0;  // (px + W - 1) / W for 0 <= px < W
   const int block_id = px / 16 + gx * (py / 16);
   return block_id % a.num_islands == a.island_id;
 }
 
-// Per-rank raygen: grid.z = rank slot, every pixel of every rank image is
-// written.  Blocks that miss the slot's projected rectangle write the miss
-// result without staging the TF or generating rays.
+// Footprint pass: the per-lane march of every rank pixel (grid.z = rank
+// slot) with each corner read marking its voxel in the `touched` bitset;
+// blocks that miss the slot's projected rectangle do nothing.
 template <bool kFoot>
 __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
+  static_assert(kFoot, "the footprint pass is the only per-lane march launch");
   extern __shared__ float4 s_tf[];
   const int slot = blockIdx.z;
   const int bx = blockIdx.x * kTileW, by = blockIdx.y * kTileH;
@@ -442,51 +449,12 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   const int px = bx + (wave & 1) * 8 + (lane & 7);
   const int py = by + (wave >> 1) * 8 + (lane >> 3);
   if (px >= a.W || py >= a.H) return;
-  const int W = a.W, H = a.H;
-  const int pixel = px + W * py;
-  const long long out_idx = (long long)slot * W * H + pixel;
-
-  long long clock_begin = 0;
-  if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   float z = 1.f;
   uint32_t ns = 0;
   if (active && in_rect(a, slot, px, py) && island_pixel(a, px, py)) {
     float d[3];
     pixel_ray(a.dir00, a.du, a.dv, px, py, d);
-    acc = rank_pixel<kFoot>(a, s_tf, slot, a.org, d, (uint32_t)pixel, z, ns);
-  }
-  if (kFoot) return;
-  if (a.depth) a.depth[out_idx] = z;
-  if (a.nsamples) a.nsamples[out_idx] = ns;
-  if (a.flags & ALOHA_FLAG_HEATMAP) {
-    const long long clock_end = clock64();
-    const float t = (float)(clock_end - clock_begin) * (a.heat_scale / (float)a.spp);
-    const float3 h = heat_map(t);
-    acc = make_float4(h.x + .5f * acc.x, h.y + .5f * acc.y, h.z + .5f * acc.z, .5f + .5f * acc.w);
-  }
-  if (a.accum) {
-    if (a.accum_id > 0) {
-      const float4 prev = a.accum[out_idx];
-      acc.x = acc.x + prev.x;
-      acc.y = acc.y + prev.y;
-      acc.z = acc.z + prev.z;
-      acc.w = acc.w + prev.w;
-    }
-    a.accum[out_idx] = acc;
-  }
-  const float inv_frames = 1.f / (float)(a.accum_id + 1);
-  const float inv_spp = 1.f / (float)a.spp;
-  const float4 fr = make_float4(acc.x * inv_frames, acc.y * inv_frames, acc.z * inv_frames,
-                                acc.w * inv_frames);
-  if (a.rgba)
-    a.rgba[out_idx] = make_float4(fr.x * inv_spp, fr.y * inv_spp, fr.z * inv_spp, fr.w * inv_spp);
-  if (a.rgba8) {
-    float4 q = fr;
-    if ((a.flags & ALOHA_FLAG_CROSSHAIR) && (px == W / 2 || py == H / 2))
-      q = make_float4(1.f - q.x, 1.f - q.y, 1.f - q.z, 1.f - q.w);
-    a.rgba8[out_idx] = pack8(q.x * inv_spp) | (pack8(q.y * inv_spp) << 8) |
-                       (pack8(q.z * inv_spp) << 16) | (pack8(q.w * inv_spp) << 24);
+    (void)rank_pixel<kFoot>(a, s_tf, slot, a.org, d, (uint32_t)(px + a.W * py), z, ns);
   }
 }
 
@@ -551,8 +519,7 @@ __device__ __forceinline__ void write_rank_pixel(const MarchArgs &a, long long o
     }
     a.accum[out_idx] = acc;
   }
-  const float inv_frames = 1.f / (float)(a.accum_id + 1);
-  const float inv_spp = 1.f / (float)a.spp;
+  const float inv_frames = a.inv_frames, inv_spp = a.inv_spp;
   const float4 fr = make_float4(acc.x * inv_frames, acc.y * inv_frames, acc.z * inv_frames,
                                 acc.w * inv_frames);
   if (a.rgba)
@@ -598,60 +565,71 @@ __device__ __forceinline__ void fill_rows(int W, int H, int nimg, int fb, int nf
 // Persistent per-rank launch.
 //
 // The work of a frame is every pixel of every rank's rectangle (the pixels a
-// rank's bricks can cover) -- q.rect_off[s] numbers them rank after rank,
-// row-major -- plus the miss result everywhere else.  The grid is one wave
-// per block, as many as the GPU holds at once.  Wave w takes the rectangle
-// pixels in runs of kRun consecutive pixels (a row segment, so the rays of a
-// run are coherent): runs w, w + nw, w + 2 nw, ... -- a static round-robin
-// that spreads every region's cost over all waves without a single atomic.
-// Lane groups march the rays with march_list (a pixel whose ray reaches no
-// brick of its rank just stores the miss result); a wave that is done then
-// writes the miss result of its share of the image rows outside the
-// rectangles.  Static shares need no counters, so there is nothing to reset
-// between launches.
-constexpr int kRun = 16;
+// rank's bricks can cover), cut into runs of up to 64 / G consecutive pixels
+// of one rectangle row (one run per group of the wave, so the entries of one
+// fetch fall in at most two runs) -- q.run_off[s] numbers the runs rank after
+// rank, row-major -- plus the miss result everywhere else.  The grid is one wave
+// per block, as many as the GPU holds at once.  Wave w takes runs w, w + nw,
+// w + 2 nw, ...: a static round-robin that spreads every region's cost over
+// all waves without a single atomic, while the rays of one run (a row
+// segment) stay coherent.  A run is decoded once per fetch with wave-uniform
+// (scalar) arithmetic.  Lane groups march the rays with march_list (a pixel
+// whose ray reaches no brick of its rank just stores the miss result); a
+// wave that is done then writes the miss result of its share of the image
+// rows outside the rectangles.  Static shares need no counters, so there is
+// nothing to reset between launches.
+// Pixels [x0, x1) of row py of rank slot's rectangle (slot -1: past the end).
+struct RunSpan {
+  int slot, x0, x1, py;
+};
+
+template <int kRun>
+__device__ __forceinline__ RunSpan run_span(const MarchArgs &a, const RanksArgs &q, int run) {
+  if (run >= q.run_off[a.P]) return RunSpan{-1, 0, 0, 0};
+  int s = 0;
+  while (q.run_off[s + 1] <= run) ++s;  // ends: run < run_off[P]
+  const int32_t *rc = a.rect[s];
+  const int cpr = (rc[2] - rc[0] + kRun - 1) / kRun;  // runs per row
+  const int l = run - q.run_off[s], row = l / cpr;
+  const int x0 = rc[0] + (l - row * cpr) * kRun;
+  return RunSpan{s, x0, min(x0 + kRun, (int)rc[2]), rc[1] + row};
+}
 
 template <int G>
 __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, const RanksArgs q) {
+  constexpr int kRun = rank_run(G);
   extern __shared__ float4 smem[];
-  __shared__ int32_t s_off[ALOHA_MAX_RANKS + 1];
   const MarchTables tb = load_tables(a, smem);
-  for (int i = threadIdx.x; i <= a.P; i += blockDim.x) s_off[i] = q.rect_off[i];
   __syncthreads();
   const int lane = (int)threadIdx.x, w = (int)blockIdx.x, nw = (int)gridDim.x;
-  const int total = q.rect_off[a.P];
+  const int runs = q.run_off[a.P];
   const float heat_scale = a.heat_scale / (float)a.spp;
-  // this wave's position in its stream of pixels (wave-uniform)
-  int pos = 0;
-  const int count = total > 0 ? ((total + kRun - 1) / kRun + nw - 1 - w) / nw * kRun : 0;
+  // stream index k of this wave: run w + (k / kRun) * nw, pixel k % kRun of it
+  const int count = runs > w ? (runs - w + nw - 1) / nw * kRun : 0;
+  int pos = 0;         // next stream index (wave-uniform)
+  int j0 = 0;          // run index (in the stream) of the last fetch's first entry
+  RunSpan span0{-1, 0, 0, 0}, span1{-1, 0, 0, 0};  // its run and the next
   auto fetch = [&](bool want) {
     const uint64_t m = __ballot(want);
+    const int n = __popcll(m);
     const int k = pos + __popcll(m & ((1ull << lane) - 1ull));
-    pos += __popcll(m);
+    if (n > 0) {  // wave-uniform: decode the (at most two) runs the entries fall in
+      j0 = pos / kRun;
+      span0 = run_span<kRun>(a, q, w + j0 * nw);
+      if ((pos + n - 1) / kRun > j0) span1 = run_span<kRun>(a, q, w + (j0 + 1) * nw);
+      pos += n;
+    }
     return k;
-  };
-  // stream index -> rectangle pixel index (or -1 past the end)
-  auto pixel_of = [&](int k) {
-    const int v = (w + (k / kRun) * nw) * kRun + k % kRun;
-    return v < total ? v : -1;
   };
   march_list<G>(
       a, tb, a.org, a.dir00, a.du, a.dv, fetch, count,
       [&](int k) {
-        const int v = pixel_of(k);
-        if (v < 0) return ListRay{-1, 0, 0, -1};
-        int lo = 0, hi = a.P;  // the slot s with s_off[s] <= v < s_off[s + 1]
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (s_off[mid] <= v) lo = mid;
-          else hi = mid;
-        }
-        const int32_t *rc = tb.rect + 4 * lo;
-        const int rw = rc[2] - rc[0], l = v - s_off[lo], row = l / rw;
-        const int px = rc[0] + (l - row * rw), py = rc[1] + row;
-        if (px >= a.W || py >= a.H) return ListRay{-1, 0, 0, -1};  // nothing to write
+        const RunSpan sp = k / kRun == j0 ? span0 : span1;
+        const int px = sp.x0 + k % kRun, py = sp.py;
+        if (sp.slot < 0 || px >= sp.x1 || px >= a.W || py >= a.H)
+          return ListRay{-1, 0, 0, -1};  // nothing to write
         // tag: the rank slot (the slot is -1 when no brick can be reached)
-        return ListRay{island_pixel(a, px, py) ? lo : -1, px, py, lo};
+        return ListRay{island_pixel(a, px, py) ? sp.slot : -1, px, py, sp.slot};
       },
       [&](const ListRay &r, float4 acc, float z, uint32_t ns, long long clock_begin) {
         if (r.tag < 0) return;
@@ -784,7 +762,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
   __syncthreads();
   const int nranks = fs.nranks;  // block-uniform
   float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-  const float inv_spp = 1.f / (float)a.spp;
+  const float inv_spp = a.inv_spp;
   const int own = fused_index(t % kFW, (t / kFW) % kFH);  // pixel of thread t < kFPix
   for (int c0 = 0; c0 < nranks; c0 += kFusedChunk) {
     const int nc = min(kFusedChunk, nranks - c0);
