@@ -36,6 +36,7 @@ constexpr int kTile = 16;
 
 struct Visit {
   int bid;        // brick id
+  int slot;       // its index in the brick list (a.brick_ids, a.dvox)
   float *grad;    // its density-gradient grid (may be null)
   float t0, t1;   // clipped interval
   float j;        // lattice jitter
@@ -51,7 +52,7 @@ __device__ int visits(const DiffArgs &a, const float org[3], const float d[3], u
   float tmin = 0.f;
   int nv = 0;
   while (nv < max_visits) {
-    int hit = -1;
+    int hit = -1, hslot = 0;
     float *hgrad = nullptr;
     float best = 0.f, bt0 = 0.f, bt1 = 0.f;
     for (int k = 0; k < a.nb; ++k) {
@@ -64,6 +65,7 @@ __device__ int visits(const DiffArgs &a, const float org[3], const float d[3], u
       else continue;
       if (hit < 0 || rep < best) {
         hit = bid;
+        hslot = k;
         hgrad = a.dvox[k];
         best = rep;
         bt0 = t0;
@@ -76,7 +78,7 @@ __device__ int visits(const DiffArgs &a, const float org[3], const float d[3], u
       const float dt = a.dt;
       const float segs = floorf((bt0 - dt * j) / dt);
       const float tf = dt * (j + segs);
-      Visit v{hit, hgrad, bt0, bt1, j, tf < bt0 ? segs + 1.f : segs};
+      Visit v{hit, hslot, hgrad, bt0, bt1, j, tf < bt0 ? segs + 1.f : segs};
       if (vis) vis[want < 0 ? nv : 0] = v;
     }
     ++nv;
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void diff_forward_kernel(const DiffArgs a) 
     const float dt = a.dt;
     const float segs = floorf((bt0 - dt * j) / dt);
     const float tfirst = dt * (j + segs);
-    const Visit v{hit, nullptr, bt0, bt1, j, tfirst < bt0 ? segs + 1.f : segs};
+    const Visit v{hit, 0, nullptr, bt0, bt1, j, tfirst < bt0 ? segs + 1.f : segs};
     const DevBrick db = a.bricks[hit];
     for (int k = 0;; ++k) {
       const float t = lattice_t(a, v, k);
@@ -195,10 +197,52 @@ __global__ __launch_bounds__(kBlock) void diff_forward_kernel(const DiffArgs a) 
   a.nsamples[out] = ns;
 }
 
+// Where a brick visit's density gradient goes.  Global float atomics run at
+// about 20 G/s on MI355X whatever the address pattern, and neighbouring rays
+// and samples hit the same voxels many times, so each workgroup first sums
+// into an LDS hash table of (brick slot, voxel) -> partial gradient and, at
+// its end, issues one global atomic per distinct voxel.  A brick whose voxel
+// index does not fit the 24-bit key, and any update whose short probe run is
+// full, goes straight to a global atomic.  Sums are reassociated either way
+// (float atomics are unordered).
+constexpr int kCacheBits = 12;
+constexpr int kCacheSlots = 1 << kCacheBits;
+constexpr int kCacheProbes = 8;
+constexpr uint32_t kCacheEmpty = 0xFFFFFFFFu;
+
+struct GradSink {
+  float *grad;     // the brick's unpadded gradient grid
+  uint32_t *keys;  // LDS table keys (null: no caching for this brick)
+  float *vals;     // LDS table partial sums
+  uint32_t tag;    // brick slot << 24
+};
+
+// LDS and global views of the table and the grid, so each add is an LDS or a
+// global atomic (a generic pointer would make both flat atomics)
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(1))) float glb_float;
+
+__device__ __forceinline__ void sink_add(const GradSink &s, uint32_t idx, float v) {
+  if (s.keys) {
+    const uint32_t key = s.tag | idx;
+    uint32_t h = (key * 2654435761u) >> (32 - kCacheBits);
+    for (int p = 0; p < kCacheProbes; ++p) {
+      const uint32_t k = atomicCAS(&s.keys[h], kCacheEmpty, key);
+      if (k == kCacheEmpty || k == key) {
+        __hip_atomic_fetch_add((lds_float *)s.vals + h, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+      }
+      h = (h + 1u) & (uint32_t)(kCacheSlots - 1);
+    }
+  }
+  __hip_atomic_fetch_add((glb_float *)s.grad + idx, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Scatter d(loss)/d(tap value) onto the 8 corners of one trilinear tap.
-__device__ __forceinline__ void scatter_tap(float *grad, const DevBrick &db, float x, float y,
-                                            float z, float g) {
-  if (g == 0.f || grad == nullptr) return;
+__device__ __forceinline__ void scatter_tap(const GradSink &sink, const DevBrick &db, float x,
+                                            float y, float z, float g) {
+  if (g == 0.f) return;
   const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
   const float wx = x - fx, wy = y - fy, wz = z - fz;
   const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
@@ -215,7 +259,7 @@ __device__ __forceinline__ void scatter_tap(float *grad, const DevBrick &db, flo
       for (int i = 0; i < 2; ++i) {
         const int xx = clampi(ix + i, 0, nx - 1);
         const float w = (i ? wx : 1.f - wx) * w_yz;
-        atomicAdd(&grad[((size_t)zz * ny + yy) * nx + xx], g * w);
+        sink_add(sink, (uint32_t)(((size_t)zz * ny + yy) * nx + xx), g * w);
       }
     }
   }
@@ -233,13 +277,13 @@ __device__ __forceinline__ void add_x(float row[3], int scx, float v) {
   }
 }
 
-__device__ __forceinline__ void atomic_voxel(float *grad, const DevBrick &db, int x, int y, int z,
-                                             float v) {
+__device__ __forceinline__ void atomic_voxel(const GradSink &sink, const DevBrick &db, int x, int y,
+                                             int z, float v) {
   if (v == 0.f) return;
   x = clampi(x, 0, db.n[0] - 1);
   y = clampi(y, 0, db.n[1] - 1);
   z = clampi(z, 0, db.n[2] - 1);
-  atomicAdd(&grad[((size_t)z * db.n[1] + y) * db.n[0] + x], v);
+  sink_add(sink, (uint32_t)(((size_t)z * db.n[1] + y) * db.n[0] + x), v);
 }
 
 // Scatter of all seven taps of one sample through the shared 20-voxel
@@ -250,8 +294,8 @@ __device__ __forceinline__ void atomic_voxel(float *grad, const DevBrick &db, in
 // to float reassociation.  gV: d/d(value tap); gd: d/d(gradient component),
 // i.e. +gd for the +0.5 tap and -gd for the -0.5 tap of each axis.  Returns
 // false, scattering nothing, when the footprint does not fit.
-__device__ __forceinline__ bool scatter_fast(float *grad, const DevBrick &db, float x, float y,
-                                             float z, float gV, const float gd[3]) {
+__device__ __forceinline__ bool scatter_fast(const GradSink &sink, const DevBrick &db, float x,
+                                             float y, float z, float gV, const float gd[3]) {
   const float xm = x - .5f, xp = x + .5f;
   const float ym = y - .5f, yp = y + .5f;
   const float zm = z - .5f, zp = z + .5f;
@@ -333,15 +377,15 @@ __device__ __forceinline__ bool scatter_fast(float *grad, const DevBrick &db, fl
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) atomic_voxel(grad, db, ax + i, cy + jj, cz + kk, A[kk][jj][i]);
+      for (int i = 0; i < 3; ++i) atomic_voxel(sink, db, ax + i, cy + jj, cz + kk, A[kk][jj][i]);
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii) atomic_voxel(grad, db, cx + ii, yn, cz + kk, YN[kk][ii]);
+    for (int ii = 0; ii < 2; ++ii) atomic_voxel(sink, db, cx + ii, yn, cz + kk, YN[kk][ii]);
 #pragma unroll
   for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii) atomic_voxel(grad, db, cx + ii, cy + jj, zn, ZN[jj][ii]);
+    for (int ii = 0; ii < 2; ++ii) atomic_voxel(sink, db, cx + ii, cy + jj, zn, ZN[jj][ii]);
   return true;
 }
 
@@ -349,8 +393,14 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a)
   extern __shared__ float4 s_mem[];
   float4 *s_tf = s_mem;
   float *s_dtf = (float *)(s_mem + a.tf_n + 4);  // tf_n x 4
+  uint32_t *s_keys = (uint32_t *)(s_dtf + 4 * a.tf_n);
+  float *s_vals = (float *)(s_keys + kCacheSlots);
   load_tf(a, s_tf);
   for (int i = threadIdx.x; i < 4 * a.tf_n; i += kBlock) s_dtf[i] = 0.f;
+  for (int i = threadIdx.x; i < kCacheSlots; i += kBlock) {
+    s_keys[i] = kCacheEmpty;
+    s_vals[i] = 0.f;
+  }
   __syncthreads();
 
   const int view = blockIdx.z;
@@ -401,6 +451,8 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a)
       }
       const DevBrick db = a.bricks[v.bid];
       float *grad = v.grad;
+      const bool keyed = (int64_t)db.n[0] * db.n[1] * db.n[2] <= (1 << 24);
+      const GradSink sink{grad, keyed ? s_keys : nullptr, s_vals, (uint32_t)v.slot << 24};
       for (int k = (int)cnt - 1; k >= 0; --k) {
         const float t = lattice_t(a, v, k);
         const Shaded r = shade_sample(a, s_tf, db, org, d, t);
@@ -453,14 +505,14 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a)
           const float y = (org[1] + t * d[1]) - db.vox_lo[1];
           const float z = (org[2] + t * d[2]) - db.vox_lo[2];
           if (dv == 0.f && dg[0] == 0.f && dg[1] == 0.f && dg[2] == 0.f) continue;
-          if (scatter_fast(grad, db, x, y, z, dv, dg)) continue;
-          scatter_tap(grad, db, x, y, z, dv);
-          scatter_tap(grad, db, x + .5f, y, z, dg[0]);
-          scatter_tap(grad, db, x - .5f, y, z, -dg[0]);
-          scatter_tap(grad, db, x, y + .5f, z, dg[1]);
-          scatter_tap(grad, db, x, y - .5f, z, -dg[1]);
-          scatter_tap(grad, db, x, y, z + .5f, dg[2]);
-          scatter_tap(grad, db, x, y, z - .5f, -dg[2]);
+          if (scatter_fast(sink, db, x, y, z, dv, dg)) continue;
+          scatter_tap(sink, db, x, y, z, dv);
+          scatter_tap(sink, db, x + .5f, y, z, dg[0]);
+          scatter_tap(sink, db, x - .5f, y, z, -dg[0]);
+          scatter_tap(sink, db, x, y + .5f, z, dg[1]);
+          scatter_tap(sink, db, x, y - .5f, z, -dg[1]);
+          scatter_tap(sink, db, x, y, z + .5f, dg[2]);
+          scatter_tap(sink, db, x, y, z - .5f, -dg[2]);
         }
       }
     }
@@ -469,6 +521,12 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a)
   if (a.dtf)
     for (int i = threadIdx.x; i < 4 * a.tf_n; i += kBlock)
       if (s_dtf[i] != 0.f) atomicAdd(&a.dtf[i], s_dtf[i]);
+  // one global atomic per distinct cached voxel
+  for (int i = threadIdx.x; i < kCacheSlots; i += kBlock) {
+    const uint32_t key = s_keys[i];
+    const float v = s_vals[i];
+    if (key != kCacheEmpty && v != 0.f) atomicAdd(&a.dvox[key >> 24][key & 0xFFFFFFu], v);
+  }
 }
 
 }  // namespace
@@ -477,7 +535,10 @@ hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s) {
   const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
   const size_t tf_lds = sizeof(float4) * (size_t)(a.tf_n + 4);
   if (backward)
-    diff_backward_kernel<<<grid, kBlock, tf_lds + sizeof(float) * 4 * (size_t)a.tf_n, s>>>(a);
+    diff_backward_kernel<<<grid, kBlock,
+                           tf_lds + sizeof(float) * 4 * (size_t)a.tf_n +
+                               (sizeof(uint32_t) + sizeof(float)) * (size_t)kCacheSlots,
+                           s>>>(a);
   else
     diff_forward_kernel<<<grid, kBlock, tf_lds, s>>>(a);
   return hipGetLastError();
