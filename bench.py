@@ -327,8 +327,8 @@ def batch_bench(args, timed):
     torch.cuda.synchronize()
     samples = int(ns.to(torch.int64).sum())
     best, table = b.autotune(reps=3)
-    sched = b.schedule()
     b.capture()
+    sched = b.schedule()  # the timed schedule, after the graph capture
     n = max(3, min(args.steps, 10))
     ms_batch = timed(b.launch, n)
     # plain per-view sequence
