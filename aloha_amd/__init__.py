@@ -355,12 +355,33 @@ class Context:
         return v.value
 
     def composite_over(self, rank_rgba, order: Sequence[int], W: int, H: int, out_rgba=None,
-                       out_rgba8=None) -> None:
+                       out_rgba8=None, rects=None) -> None:
+        """Ordered 'over' of the rank images.  rects (P x 4 x0, y0, x1, y1, e.g.
+        from rank_rects): the sparse form, reading each rank only inside its
+        rectangle (the images must be transparent outside it)."""
         o = (C.c_int32 * len(order))(*order)
-        self._check(lib().aloha_composite_over(
-            self._h, C.c_void_p(rank_rgba.data_ptr()), len(order), o, W, H,
-            C.c_void_p(out_rgba.data_ptr()) if out_rgba is not None else None,
-            C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None))
+        outs = (C.c_void_p(out_rgba.data_ptr()) if out_rgba is not None else None,
+                C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None)
+        if rects is None:
+            self._check(lib().aloha_composite_over(
+                self._h, C.c_void_p(rank_rgba.data_ptr()), len(order), o, W, H, *outs))
+        else:
+            flat = [int(v) for r in rects for v in r]
+            if len(flat) != 4 * len(order):
+                raise ValueError("rects: need 4 integers per rank")
+            rc = (C.c_int32 * len(flat))(*flat)
+            self._check(lib().aloha_composite_over_sparse(
+                self._h, C.c_void_p(rank_rgba.data_ptr()), len(order), o, rc, W, H, *outs))
+
+    def rank_rects(self, params: RenderParams, W: int, H: int,
+                   ranks: Sequence[Sequence[int]]) -> list[tuple[int, int, int, int]]:
+        """The pixel rectangles outside of which render_ranks with these
+        arguments writes transparent black (x0, y0, x1, y1 per rank)."""
+        offs, ids = _rank_table(ranks)
+        out = (C.c_int32 * (4 * len(ranks)))()
+        self._check(lib().aloha_rank_rects(self._h, C.byref(params), W, H, len(ranks), offs, ids,
+                                           out))
+        return [tuple(out[4 * s:4 * s + 4]) for s in range(len(ranks))]
 
     def composite_over_rgba8(self, rank_rgba8, order: Sequence[int], W: int, H: int,
                              out_rgba8) -> None:

@@ -1,7 +1,8 @@
 // composite.hip -- sort-last compositing of the virtual ranks' images on the
 // device, replacing IceT (waikikiRenderer.cpp:894-917, :1043-1079):
 //   over        ordered BLEND of premultiplied float RGBA, front to back in
-//               the kd visibility order, onto transparent black;
+//               the kd visibility order, onto transparent black (dense, or
+//               sparse: reading each rank only inside its pixel rectangle);
 //   over_rgba8  the same over the ranks' 8-bit images (ubyte colour format);
 //   zmin        Z_BUFFER: the smallest depth wins, ties to the lower rank.
 // Each is one streaming pass: every rank image is read once with 16-byte
@@ -37,6 +38,35 @@ __global__ __launch_bounds__(kBlock) void over_kernel(const float4 *__restrict__
     C.y = C.y + w * s.y;
     C.z = C.z + w * s.z;
     C.w = C.w + w * s.w;
+  }
+  if (out) out[i] = C;
+  if (out8) out8[i] = pack_rgba(C);
+}
+
+// Sparse 'over' (IceT composites sparse images: runs of empty pixels are
+// skipped; here the empty region of rank s is everything outside its pixel
+// rectangle c.rect[s]).  Blending a transparent-black pixel leaves C exactly
+// unchanged (C + w * 0 = C, w >= 0), so skipping those reads gives the dense
+// result bit for bit while reading only the pixels inside the rectangles.
+// Grid: x covers a row in kBlock-pixel runs, y = row.
+__global__ __launch_bounds__(kBlock) void over_sparse_kernel(const float4 *__restrict__ imgs,
+                                                             const CompositeArgs c, int W,
+                                                             float4 *__restrict__ out,
+                                                             uint32_t *__restrict__ out8) {
+  const int px = (int)(blockIdx.x * kBlock + threadIdx.x), py = (int)blockIdx.y;
+  if (px >= W) return;
+  const int64_t i = px + (int64_t)W * py;
+  float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int k = 0; k < c.P; ++k) {
+    const int s = c.order[k];  // k is uniform: scalar loads
+    const int32_t *rc = c.rect[s];
+    if (px < rc[0] || px >= rc[2] || py < rc[1] || py >= rc[3]) continue;
+    const float4 v = imgs[(int64_t)s * c.npix + i];
+    const float w = 1.f - C.w;
+    C.x = C.x + w * v.x;
+    C.y = C.y + w * v.y;
+    C.z = C.z + w * v.z;
+    C.w = C.w + w * v.w;
   }
   if (out) out[i] = C;
   if (out8) out8[i] = pack_rgba(C);
@@ -87,6 +117,13 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBl
 hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, float4 *out,
                                  uint32_t *out8, hipStream_t s) {
   over_kernel<<<blocks_for(c.npix), kBlock, 0, s>>>(imgs, c, out, out8);
+  return hipGetLastError();
+}
+
+hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs &c, int32_t W,
+                                        int32_t H, float4 *out, uint32_t *out8, hipStream_t s) {
+  const dim3 grid((unsigned)((W + kBlock - 1) / kBlock), (unsigned)H);
+  over_sparse_kernel<<<grid, kBlock, 0, s>>>(imgs, c, W, out, out8);
   return hipGetLastError();
 }
 

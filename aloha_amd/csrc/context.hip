@@ -644,6 +644,40 @@ extern "C" int aloha_composite_over(aloha_ctx *ctx, const float *d_rank_rgba, in
   return ALOHA_OK;
 }
 
+extern "C" int aloha_composite_over_sparse(aloha_ctx *ctx, const float *d_rank_rgba, int32_t P,
+                                           const int32_t *order, const int32_t *rects, int32_t W,
+                                           int32_t H, float *d_out_rgba, uint32_t *d_out_rgba8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_composite_over_sparse: ctx is NULL");
+  if (!d_rank_rgba || !rects || W < 1 || H < 1)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_composite_over_sparse: bad argument");
+  CompositeArgs c;
+  if (int rc = check_order(ctx, order, P, c)) return rc;
+  c.npix = (int64_t)W * H;
+  for (int s = 0; s < P; ++s)
+    for (int k = 0; k < 4; ++k) c.rect[s][k] = rects[4 * s + k];
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  HIP_CHECK(ctx, launch_composite_over_sparse((const float4 *)d_rank_rgba, c, W, H,
+                                              (float4 *)d_out_rgba, d_out_rgba8, ctx->stream));
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_rank_rects(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                int32_t H, int32_t P, const int32_t *rank_offsets,
+                                const int32_t *brick_ids, int32_t *rects) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_rank_rects: ctx is NULL");
+  if (!rects) return fail(ctx, ALOHA_ERR_ARG, "aloha_rank_rects: rects is NULL");
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  // a heat map colours every pixel, and accumulation adds earlier frames'
+  // pixels: then no pixel is known to be transparent
+  const bool whole = (params->flags & ALOHA_FLAG_HEATMAP) || params->accum_id > 0;
+  for (int s = 0; s < P; ++s) {
+    const int32_t full[4] = {0, 0, W, H};
+    for (int k = 0; k < 4; ++k) rects[4 * s + k] = whole ? full[k] : a.rect[s][k];
+  }
+  return ALOHA_OK;
+}
+
 extern "C" int aloha_composite_over_rgba8(aloha_ctx *ctx, const uint32_t *d_rank_rgba8,
                                           int32_t P, const int32_t *order, int32_t W, int32_t H,
                                           uint32_t *d_out_rgba8) {

@@ -101,6 +101,8 @@ struct CompositeArgs {
   int32_t P;
   int64_t npix;
   int32_t order[ALOHA_MAX_RANKS];
+  // sparse form: rank s's image is transparent outside [x0,x1) x [y0,y1)
+  int32_t rect[ALOHA_MAX_RANKS][4];
 };
 
 // Surface path: one rank's triangle geometry (BVH nodes of 4 float4, see
@@ -151,6 +153,9 @@ hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], floa
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
 hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, float4 *out,
                                  uint32_t *out8, hipStream_t s);
+// sparse form: reads rank pixels only inside c.rect (W x H images)
+hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs &c, int32_t W,
+                                        int32_t H, float4 *out, uint32_t *out8, hipStream_t s);
 hipError_t launch_composite_over_rgba8(const uint32_t *imgs, const CompositeArgs &c,
                                        uint32_t *out8, hipStream_t s);
 hipError_t launch_composite_zmin(const uint32_t *imgs, const float *depth, int32_t P,

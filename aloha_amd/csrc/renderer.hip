@@ -2,7 +2,7 @@
 // the C ABI (waikikiRenderer.cpp, DistributedRenderer.cpp).  All virtual
 // ranks live in this process and on this GPU: the MPI event broadcast
 // becomes a local queue, the per-rank OWL launches become one
-// aloha_render_ranks launch per island, and IceT becomes aloha_composite_over.
+// aloha_render_ranks launch per island, and IceT becomes aloha_composite_over_sparse.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -382,12 +382,19 @@ void Renderer::render(uint32_t *fb) {
     check(aloha_composite_zmin(ctx_, d_rank_rgba8_, d_rank_depth_, commWorld.size, W, H,
                                commWorld.rank == displayRank ? fb : nullptr, d_comp_depth_));
   } else {
-    // composite order expanded over islands (:1043-1050)
+    // composite order expanded over islands (:1043-1050); each rank image is
+    // read only inside the rectangle its bricks project to (transparent
+    // outside it; the whole image while frames accumulate)
     std::vector<int32_t> order;
     for (int i = 0; i < numIslands; ++i)
       for (int r : compositeOrder) order.push_back(i * islandSize_ + r);
-    check(aloha_composite_over(ctx_, d_rank_rgba_, commWorld.size, order.data(), W, H, d_comp_,
-                               commWorld.rank == displayRank ? fb : nullptr));
+    std::vector<int32_t> rects(4 * (size_t)islandSize_);
+    check(aloha_rank_rects(ctx_, &p, W, H, islandSize_, offs.data(), ids.data(), rects.data()));
+    std::vector<int32_t> all;
+    for (int i = 0; i < numIslands; ++i) all.insert(all.end(), rects.begin(), rects.end());
+    check(aloha_composite_over_sparse(ctx_, d_rank_rgba_, commWorld.size, order.data(),
+                                      all.data(), W, H, d_comp_,
+                                      commWorld.rank == displayRank ? fb : nullptr));
   }
   check(aloha_sync(ctx_));
   if (commWorld.rank != displayRank) {

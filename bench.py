@@ -3,8 +3,11 @@
 
 One step = one frame: every virtual rank ray-marches its bricks into its
 float RGBA + RGBA8 + depth image (one launch for all ranks), then the ranks'
-images are 'over'-composited in kd visibility order (one launch).  Inputs
-(bricks, TF, camera) are resident in HBM before timing starts.
+images are 'over'-composited in kd visibility order (one launch, reading each
+rank image only inside the pixel rectangle its bricks project to -- outside
+it the image is transparent black; the dense composite is timed beside it
+and gives the same frame).  Inputs (bricks, TF, camera) are resident in HBM
+before timing starts.
 
 Default workload: 512^3 float32 fBm volume, chopSuey-split into 8 bricks
 (2x2x2, ghost voxels), one brick per virtual rank, 1024x1024, dt = 0.5.
@@ -101,9 +104,14 @@ def main():
     log(f"[bench] {len(ids)} bricks uploaded ({ctx.brick_bytes() / 1e6:.0f} MB padded) "
         f"in {upload_s:.2f}s incl. PCIe")
 
+    # the pixel rectangles outside of which each rank image is transparent:
+    # the composite reads the rank images only inside them (IceT's sparse
+    # images); the dense composite is timed beside it and gives the same frame
+    rects = ctx.rank_rects(params, W, H, ranks)
+
     def frame():
         ctx.render_ranks(params, W, H, ranks, imgs)
-        ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8)
+        ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8, rects=rects)
 
     # lanes-per-ray schedule of the per-rank and fused launches, chosen by
     # timing every choice on these exact launches (untimed; results are
@@ -139,8 +147,16 @@ def main():
 
     # per-kernel averages over the same number of launches
     ms_render = timed(lambda: ctx.render_ranks(params, W, H, ranks, imgs), args.steps)
-    ms_comp = timed(lambda: ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8), args.steps)
-    log(f"[bench] render {ms_render:.4f} ms, composite {ms_comp:.4f} ms")
+    ms_comp = timed(lambda: ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8, rects=rects),
+                    args.steps)
+    dense = torch.empty_like(out)
+    dense8 = torch.empty_like(out8)
+    ms_dense = timed(lambda: ctx.composite_over(imgs.rgba, wl.order, W, H, dense, dense8),
+                     args.steps)
+    torch.cuda.synchronize()
+    sparse_equal = bool(torch.equal(dense, out) and torch.equal(dense8, out8))
+    log(f"[bench] render {ms_render:.4f} ms, composite {ms_comp:.4f} ms (sparse; dense "
+        f"{ms_dense:.4f} ms, equal {sparse_equal})")
     # fused form: the same frame (bit-identical result) in one launch
     fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
     for _ in range(args.warmup):
@@ -161,7 +177,10 @@ def main():
     # algorithmic bytes per launch
     npix = W * H
     render_bytes = touched * 4 + wl.tf.nbytes + P * npix * (16 + 4 + 4)
-    comp_bytes = P * npix * 16 + npix * (16 + 4)
+    rect_pix = sum(max(0, x1 - x0) * max(0, y1 - y0) for x0, y0, x1, y1 in rects)
+    comp_bytes = rect_pix * 16 + npix * (16 + 4)  # sparse: rank pixels inside the rectangles
+    dense_bytes = P * npix * 16 + npix * (16 + 4)
+    dense_gbs = dense_bytes / (ms_dense * 1e-3) / 1e9
     render_gbs = render_bytes / (ms_render * 1e-3) / 1e9
     comp_gbs = comp_bytes / (ms_comp * 1e-3) / 1e9
     traffic = {}
@@ -172,14 +191,14 @@ def main():
                        if not k.startswith("_")}
     dominant = "render" if ms_render >= ms_comp else "composite"
     roof = {
-        "kernel": f"march_persist_kernel<{schedule[0]}>" if dominant == "render" else "over_kernel",
+        "kernel": f"march_persist_kernel<{schedule[0]}>" if dominant == "render" else "over_sparse_kernel",
         "bound": "hbm",
         "achieved": round(render_gbs if dominant == "render" else comp_gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round((render_gbs if dominant == "render" else comp_gbs) / HBM_PEAK_GBS, 4),
         "traffic": traffic.get(f"march_persist_kernel<{schedule[0]}>" if dominant == "render"
-                               else "over_kernel"),
+                               else "over_sparse_kernel"),
         "traffic_source": "profiles/traffic.json (rocprofv3 --pmc, calibrated)" if traffic else None,
         "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
         "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
@@ -188,8 +207,12 @@ def main():
         f"march_persist_kernel<{schedule[0]}>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
                          "GB/s": round(render_gbs, 1), "frac_hbm": round(render_gbs / HBM_PEAK_GBS, 4),
                          "Gsamples/s": round(samples / (ms_render * 1e-3) / 1e9, 3)},
-        "over_kernel": {"ms": round(ms_comp, 5), "bytes": int(comp_bytes),
-                        "GB/s": round(comp_gbs, 1), "frac_hbm": round(comp_gbs / HBM_PEAK_GBS, 4)},
+        "over_sparse_kernel": {"ms": round(ms_comp, 5), "bytes": int(comp_bytes),
+                               "GB/s": round(comp_gbs, 1),
+                               "frac_hbm": round(comp_gbs / HBM_PEAK_GBS, 4),
+                               "equal_to_dense": sparse_equal},
+        "over_kernel": {"ms": round(ms_dense, 5), "bytes": int(dense_bytes),
+                        "GB/s": round(dense_gbs, 1), "frac_hbm": round(dense_gbs / HBM_PEAK_GBS, 4)},
     }
 
     cpu = None

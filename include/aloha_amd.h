@@ -245,6 +245,24 @@ int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params, in
 int aloha_composite_over(aloha_ctx *ctx, const float *d_rank_rgba, int32_t P,
                          const int32_t *order, int32_t W, int32_t H, float *d_out_rgba,
                          uint32_t *d_out_rgba8);
+/* Sparse form of aloha_composite_over.  IceT composites sparse images (it
+ * skips runs of empty pixels); here rects[4*s .. 4*s+3] = x0, y0, x1, y1
+ * bounds the non-transparent pixels of rank s's image, and only those are
+ * read.  Bit-identical to the dense form whenever every rank pixel outside
+ * its rectangle is transparent black -- which aloha_render_ranks guarantees
+ * for the rectangles of aloha_rank_rects. */
+int aloha_composite_over_sparse(aloha_ctx *ctx, const float *d_rank_rgba, int32_t P,
+                                const int32_t *order, const int32_t *rects, int32_t W,
+                                int32_t H, float *d_out_rgba, uint32_t *d_out_rgba8);
+/* Pixel rectangles (P x 4: x0, y0, x1, y1) outside of which
+ * aloha_render_ranks with these arguments writes transparent black float
+ * RGBA: each rank's bricks projected with the current camera (the whole
+ * image for a rank whose bricks reach the camera plane, and for every rank
+ * when the heat map is on or accum_id > 0).  No counterpart in the
+ * reference (its ranks' images go to IceT as they are). */
+int aloha_rank_rects(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                     int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
+                     int32_t *rects);
 /* Same over P RGBA8 rank images (the reference's ubyte path, :908, :915). */
 int aloha_composite_over_rgba8(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, int32_t P,
                                const int32_t *order, int32_t W, int32_t H,

@@ -149,6 +149,47 @@ def test_fused_render_composite_bit_exact(which, c0, c1):
         assert int(ns.sum()) == int(imgs.nsamples.sum()) > 0
 
 
+@pytest.mark.parametrize("which", ["c0", "c1", "odd", "heat"])
+def test_sparse_composite_equals_dense(which, c0, c1):
+    """The sparse composite over aloha_rank_rects' rectangles gives the dense
+    composite of render_ranks' images bit for bit (float and RGBA8); with the
+    heat map on, the rectangles are the whole image."""
+    import torch
+    if which == "odd":
+        raw = WL.synth_blobs_u8((72, 60, 54), 12, 0x5EED7, 3.0, 10.0, 0.3, 1.0)
+        wl = WL.make_volume_workload("blobs_odd", raw, 27, 13, 203, 117, cam_seed=0x5EED8)
+    else:
+        wl = c1 if which == "c1" else c0
+    _case(f"{wl.name} sparse vs dense composite ({which})")
+    params = wl.params(flags=A.FLAG_HEATMAP) if which == "heat" else wl.params()
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
+        ctx.set_tf(wl.tf)
+        ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+        ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
+        imgs = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H)
+        ctx.render_ranks(params, wl.W, wl.H, ranks, imgs)
+        rects = ctx.rank_rects(params, wl.W, wl.H, ranks)
+        dense = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
+        dense8 = torch.empty((wl.H, wl.W), dtype=torch.int32, device="cuda")
+        sparse, sparse8 = torch.empty_like(dense), torch.empty_like(dense8)
+        ctx.composite_over(imgs.rgba, wl.order, wl.W, wl.H, dense, dense8)
+        ctx.composite_over(imgs.rgba, wl.order, wl.W, wl.H, sparse, sparse8, rects=rects)
+        torch.cuda.synchronize()
+        assert torch.equal(sparse, dense) and torch.equal(sparse8, dense8)
+        area = sum(max(0, x1 - x0) * max(0, y1 - y0) for x0, y0, x1, y1 in rects)
+        if which == "heat":
+            assert all(r == (0, 0, wl.W, wl.H) for r in rects)
+        else:
+            assert area < wl.num_ranks * wl.W * wl.H  # something was skipped
+            # outside its rectangle every rank pixel is transparent black
+            rgba = imgs.rgba.cpu().numpy()
+            for s, (x0, y0, x1, y1) in enumerate(rects):
+                m = np.ones((wl.H, wl.W), bool)
+                m[max(y0, 0):y1, max(x0, 0):x1] = False
+                assert not rgba[s][m].any()
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_lane_schedules_identical(c0, lanes):
     """Every lanes-per-ray schedule gives the 1-lane images bit for bit, with
