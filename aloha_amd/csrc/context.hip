@@ -403,7 +403,7 @@ extern "C" int aloha_camera_set(aloha_ctx *ctx, const float org[3], const float 
 namespace aloha {
 
 int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, int &waves) {
-  const int run = rank_run(lanes);
+  const int run = kRankRun;
   int64_t total = 0;
   q.run_off[0] = 0;
   for (int s = 0; s < a.P; ++s) {
@@ -416,7 +416,7 @@ int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, i
   }
   // persistent grid: as many one-wave blocks as the GPU holds at once
   const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
-  const size_t lds = march_tables_bytes(a);
+  const size_t lds = ranks_lds_bytes(a);
   if (ctx->q_waves[li] == 0 || ctx->q_waves_lds[li] != lds) {
     int per_cu = 0, cus = 0;
     HIP_CHECK(ctx, ranks_occupancy(lanes, lds, &per_cu));

@@ -54,10 +54,9 @@ struct MarchArgs {
 };
 
 // Persistent per-rank launch: the pixels of the ranks' rectangles in runs of
-// up to rank_run(G) = 64 / G consecutive pixels of one rectangle row (G lanes
-// per ray), numbered rank after rank, row-major; run_off[s] is the first run
-// of rank slot s.
-constexpr int rank_run(int lanes) { return 64 / lanes; }
+// up to kRankRun consecutive pixels of one rectangle row, numbered rank after
+// rank, row-major; run_off[s] is the first run of rank slot s.
+constexpr int kRankRun = 16;
 struct RanksArgs {
   int32_t run_off[ALOHA_MAX_RANKS + 1];
 };
@@ -139,9 +138,15 @@ hipError_t launch_march_ranks(const MarchArgs &a, const RanksArgs &q, int group,
 // (tf_n + 4 float4), a 64-byte brick record per rank_bricks entry,
 // rank_off[0..P] and the P pixel rectangles.
 constexpr size_t kLdsBrickBytes = 64;
-inline size_t march_tables_bytes(const MarchArgs &a) {
+__host__ __device__ inline size_t march_tables_bytes(const MarchArgs &a) {
   return sizeof(float4) * (size_t)(a.tf_n + 4) + kLdsBrickBytes * (size_t)a.rank_off[a.P] +
          sizeof(int32_t) * (size_t)(a.P + 1) + sizeof(int32_t) * 4 * (size_t)a.P;
+}
+// The persistent per-rank launch adds a list of 64 ray records (64 B each)
+// after the tables, at the next 16-byte boundary past them.
+constexpr size_t kRayRecBytes = 64;
+__host__ __device__ inline size_t ranks_lds_bytes(const MarchArgs &a) {
+  return (march_tables_bytes(a) / 16 + 1) * 16 + 64 * kRayRecBytes;
 }
 // resident one-wave blocks per CU of the persistent per-rank kernel
 hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu);

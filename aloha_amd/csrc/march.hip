@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <climits>
 #include <type_traits>
 
 #include "device_types.h"
@@ -160,172 +161,258 @@ struct ListRay {
   int slot, px, py, tag;
 };
 
+enum : int { kFetch = 0, kSelect = 1, kStep = 2, kDone = 3 };
+
+// The march state of one ray: kFetch (no ray), kSelect (closest brick hit
+// from tmin next), kStep (marching segment [t, t1) of brick kb), kDone.
+// In a lane group every lane holds the same values.
+struct RayState {
+  int state, b0, b1, sid, kb, bid;  // kb: rank_bricks index, bid: brick id
+  ListRay ray;
+  uint32_t rng, ns;
+  float d[3];
+  float4 c, acc;  // this spp sample's colour, the spp sum
+  float z, tmin, t, t1;
+  long long clock_begin;
+};
+
+__device__ __forceinline__ void ray_state_init(RayState &s) {
+  s.state = kFetch;
+  s.b0 = s.b1 = s.sid = s.kb = s.bid = 0;
+  s.ray = ListRay{-1, 0, 0, 0};
+  s.rng = s.ns = 0;
+  s.d[0] = 1.f, s.d[1] = 0.f, s.d[2] = 0.f;
+  s.c = s.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  s.z = 1.f, s.tmin = 0.f, s.t = 0.f, s.t1 = 0.f;
+  s.clock_begin = 0;
+}
+
+// End of one spp sample: sum it, start the next or hand the ray back
+// (store(ray, acc, z, ns, clock_begin), called where `owner`).
+template <class Store>
+__device__ __forceinline__ void end_sample(const MarchArgs &a, RayState &s, bool owner,
+                                           const Store &store) {
+  s.acc.x = s.acc.x + s.c.x;
+  s.acc.y = s.acc.y + s.c.y;
+  s.acc.z = s.acc.z + s.c.z;
+  s.acc.w = s.acc.w + s.c.w;
+  if (++s.sid < a.spp) {
+    s.c = make_float4(0.f, 0.f, 0.f, 0.f);
+    s.tmin = 0.f;
+    s.state = kSelect;
+  } else {
+    if (owner) store(s.ray, s.acc, s.z, s.ns, s.clock_begin);
+    s.state = kFetch;
+  }
+}
+
+// kSelect: the closest reported brick hit from tmin (VolBrickIsect, ties to
+// the lower index; deviceCode.cpp:366-389): sets up the segment (firstSampleT,
+// depth, next tmin) or ends the spp sample.  Returns whether a brick was hit.
+template <class Store>
+__device__ __forceinline__ bool select_brick(const MarchArgs &a, const MarchTables &tb,
+                                             const float org[3], RayState &s, bool owner,
+                                             const Store &store) {
+  const float dt = a.dt;
+  int hit = -1;
+  float best = 0.f, bnear = 0.f, bt0 = 0.f, bfar = 0.f;
+  for (int kb = s.b0; kb < s.b1; ++kb) {
+    const LdsBrick &db = tb.brick[kb];
+    float s0, s1, rep, tn;
+    if (!slab(org, s.d, s.tmin, 1e10f, db.lo, db.hi, s0, s1)) continue;
+    if (s0 > s.tmin) {
+      rep = s0;
+      tn = s0;
+    } else if (s1 > s.tmin) {
+      rep = s1;
+      tn = 0.f;
+    } else {
+      continue;
+    }
+    if (hit < 0 || rep < best) {  // ties keep the earlier entry
+      hit = kb;
+      best = rep;
+      bnear = tn;
+      bt0 = s0;
+      bfar = s1;
+    }
+  }
+  if (hit < 0) {
+    end_sample(a, s, owner, store);
+    return false;
+  }
+  bnear = bnear + s.tmin;
+  const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(s.rng);
+  // firstSampleT
+  const float segs = floorf((bt0 - dt * j) / dt);
+  float t = dt * (j + segs);
+  if (t < bt0) t += dt;
+  s.t = t;
+  s.t1 = bfar;
+  s.kb = hit;
+  s.bid = a.rank_bricks[hit];
+  s.z = fminf(s.z, bnear / a.max_depth);
+  s.tmin = bfar + 1e-20f;
+  if (t < s.t1 && s.c.w < .99f) s.state = kStep;
+  else if (!(s.c.w < .99f)) end_sample(a, s, owner, store);  // else: the next brick next
+  return true;
+}
+
+// Take ray r: generateRay at the pixel centre, the RNG seeded per (pixel,
+// accumID), then the first brick selection.
+template <class Store>
+__device__ __forceinline__ bool begin_ray(const MarchArgs &a, const MarchTables &tb,
+                                          const float org[3], const float dir00[3],
+                                          const float du[3], const float dv[3], RayState &s,
+                                          const ListRay &r, bool owner, const Store &store) {
+  s.ray = r;
+  // slot -1: a pixel no brick can reach (no brick to select: the miss result)
+  s.b0 = r.slot < 0 ? 0 : tb.roff[r.slot];
+  s.b1 = r.slot < 0 ? 0 : tb.roff[r.slot + 1];
+  pixel_ray(dir00, du, dv, r.px, r.py, s.d);
+  if (a.jitter_mode != ALOHA_JITTER_FIXED)
+    s.rng = rng_seed((uint32_t)(r.px + a.W * r.py), (uint32_t)a.accum_id);
+  s.sid = 0;
+  s.ns = 0;
+  s.z = 1.f;
+  s.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  s.c = s.acc;
+  s.tmin = 0.f;
+  s.state = kSelect;
+  return select_brick(a, tb, org, s, owner, store);
+}
+
+// A ray after begin_ray, as one lane set it up for a lane group to march
+// (64 B, four 16-byte LDS words).  ns, c and acc are zero at that point:
+// the first selection takes no sample, and a sample that ends there sums
+// zeros.
+struct RayRec {
+  int4 w0;    // slot, px, py, tag
+  float4 w1;  // d[0..2], rng (bits)
+  float4 w2;  // z, tmin, t, t1
+  int4 w3;    // kb, bid, state | sid << 2, b0 | b1 << 16
+};
+static_assert(sizeof(RayRec) == 64, "LDS ray record");
+
+__device__ __forceinline__ void rec_pack(RayRec &r, const RayState &s) {
+  r.w0 = make_int4(s.ray.slot, s.ray.px, s.ray.py, s.ray.tag);
+  r.w1 = make_float4(s.d[0], s.d[1], s.d[2], __uint_as_float(s.rng));
+  r.w2 = make_float4(s.z, s.tmin, s.t, s.t1);
+  r.w3 = make_int4(s.kb, s.bid, s.state | (s.sid << 2), s.b0 | (s.b1 << 16));
+}
+
+__device__ __forceinline__ void rec_adopt(RayState &s, const RayRec &r) {
+  s.ray = ListRay{r.w0.x, r.w0.y, r.w0.z, r.w0.w};
+  s.d[0] = r.w1.x, s.d[1] = r.w1.y, s.d[2] = r.w1.z;
+  s.rng = __float_as_uint(r.w1.w);
+  s.z = r.w2.x, s.tmin = r.w2.y, s.t = r.w2.z, s.t1 = r.w2.w;
+  s.kb = r.w3.x, s.bid = r.w3.y;
+  s.state = r.w3.z & 3, s.sid = r.w3.z >> 2;
+  s.b0 = r.w3.w & 0xFFFF, s.b1 = r.w3.w >> 16;
+  s.ns = 0;
+  s.c = s.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // renderFrame's per-pixel work (sample loop, brick do/while, integrateBrick;
 // deviceCode.cpp:362-393 and :256-298) for the rays of a work list [0,
 // count), marched by lane groups of G lanes with persistent-lane scheduling:
-// a group takes the next list entry (an LDS counter) as soon as its ray is
-// done, so groups stay busy while entries remain instead of waiting for the
-// longest ray of a fixed batch.
+// a group takes the next list entry as soon as its ray is done, so groups
+// stay busy while entries remain instead of waiting for the longest ray of a
+// fixed batch.
 //
-// Each group runs a small state machine, one transition or one march step
-// per loop trip: kFetch (take an entry, set up the ray), kSelect (closest
-// brick hit from tmin: set up the segment, or end the spp sample when there
-// is none), kStep (G samples of the segment).  In a step lane gl evaluates
-// sample k0 + gl (position, 7 taps, normal, TF, shade -- the costly,
-// independent part); then every lane of the group applies the G samples in
-// order with the sequential 'over' recurrence and the loop test (t < t1 &&
-// alpha < .99) before each.  Positions come from the same chain of additions
-// t += dt, the segment start from firstSampleT, the jitter from the same RNG
-// draws in the same order, so every ray's result equals rank_pixel()'s bit
-// for bit; samples past the termination are computed and dropped.
+// Each group runs the RayState machine, one transition or one march step per
+// loop trip.  In a step lane gl evaluates sample k0 + gl (position, 7 taps,
+// normal, TF, shade -- the costly, independent part); then every lane of the
+// group applies the G samples in order with the sequential 'over' recurrence
+// and the loop test (t < t1 && alpha < .99) before each.  Positions come from
+// the same chain of additions t += dt, the segment start from firstSampleT,
+// the jitter from the same RNG draws in the same order, so every ray's result
+// equals rank_pixel()'s bit for bit; samples past the termination are
+// computed and dropped.  A segment that ends in a rank of one brick ends the
+// spp sample at once: the next selection could only test that brick again,
+// and from tmin = its exit (+1e-20) its slab interval [max(tmin, near), exit)
+// is empty, so that selection finds no hit.
 //
 // Control flow is wave-uniform wherever lanes exchange values: the loop runs
 // while any group of the wave has a ray (a ballot), the broadcasts execute on
 // every lane, and each group's work is predicated on its state, which is the
 // same on all of its lanes.  fetch(want) is called by every lane of the
-// wave and returns the next list index on the lanes with want set (the
-// first lane of each group without a ray); entry(k) gives list entry k;
-// store(ray, acc, z, ns, clock_begin) is called on the group's first lane
-// when its ray is done.
-template <int G, class Fetch, class Entry, class Store>
+// wave and returns, on the lanes with want set (the first lane of each group
+// without a ray), the next list index, a negative value for "none yet, ask
+// again next trip" or count and above when the list is exhausted.  entry(k)
+// gives list entry k: a ListRay (kRec false: the group sets the ray up) or a
+// RayRec (kRec true: set up already).  store(ray, acc, z, ns, clock_begin) is
+// called on the group's first lane when its ray is done.
+template <int G, bool kRec, class Fetch, class Entry, class Store>
 __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables &tb,
                                            const float org[3], const float dir00[3],
                                            const float du[3], const float dv[3],
                                            const Fetch &fetch, int count, const Entry &entry,
                                            const Store &store) {
-  enum : int { kFetch = 0, kSelect = 1, kStep = 2, kDone = 3 };
   const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
+  const bool owner = gl == 0;
   const float dt = a.dt;
-  int state = kFetch, b0 = 0, b1 = 0, sid = 0;
-  ListRay ray{-1, 0, 0, 0};
-  uint32_t rng = 0, ns = 0;
-  float d[3] = {1.f, 0.f, 0.f};
-  float4 c = make_float4(0.f, 0.f, 0.f, 0.f), acc = c;
-  float z = 1.f, tmin = 0.f, t = 0.f, t1 = 0.f;
+  RayState s;
+  ray_state_init(s);
   BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
   float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
-  int bid = 0;
-  long long clock_begin = 0;
-  // end of one spp sample: sum it, start the next or hand the ray back
-  auto end_sample = [&]() {
-    acc.x = acc.x + c.x;
-    acc.y = acc.y + c.y;
-    acc.z = acc.z + c.z;
-    acc.w = acc.w + c.w;
-    if (++sid < a.spp) {
-      c = make_float4(0.f, 0.f, 0.f, 0.f);
-      tmin = 0.f;
-      state = kSelect;
-    } else {
-      if (gl == 0) store(ray, acc, z, ns, clock_begin);
-      state = kFetch;
-    }
-  };
-  // kSelect: closest reported brick hit from tmin (VolBrickIsect, ties to
-  // the lower index); sets up the segment or ends the spp sample
-  auto select = [&]() {
-    int hit = -1;
-    float best = 0.f, bnear = 0.f, bt0 = 0.f, bfar = 0.f;
-    for (int kb = b0; kb < b1; ++kb) {
-      const LdsBrick &db = tb.brick[kb];
-      float s0, s1, rep, tn;
-      if (!slab(org, d, tmin, 1e10f, db.lo, db.hi, s0, s1)) continue;
-      if (s0 > tmin) {
-        rep = s0;
-        tn = s0;
-      } else if (s1 > tmin) {
-        rep = s1;
-        tn = 0.f;
-      } else {
-        continue;
-      }
-      if (hit < 0 || rep < best) {  // ties keep the earlier entry
-        hit = kb;
-        best = rep;
-        bnear = tn;
-        bt0 = s0;
-        bfar = s1;
-      }
-    }
-    if (hit < 0) {
-      end_sample();
-      return;
-    }
-    bnear = bnear + tmin;
-    const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
-    // firstSampleT
-    const float segs = floorf((bt0 - dt * j) / dt);
-    t = dt * (j + segs);
-    if (t < bt0) t += dt;
-    t1 = bfar;
-    bid = a.rank_bricks[hit];
-    const LdsBrick &db = tb.brick[hit];
-    bv = BrickView{(const gfloat *)db.origin, db.px, db.pxy, db.n[0], db.n[1], db.n[2]};
-    lo0 = db.vlo[0];
-    lo1 = db.vlo[1];
-    lo2 = db.vlo[2];
-    z = fminf(z, bnear / a.max_depth);
-    tmin = bfar + 1e-20f;
-    if (t < t1 && c.w < .99f) state = kStep;
-    else if (!(c.w < .99f)) end_sample();  // else: select the next brick next trip
-  };
-  // One trip: a group whose segment ended selects its next brick (or ends
-  // its ray), a group without a ray takes the next entry and selects its
-  // first brick, then every group with a segment steps -- so a ray's
-  // transitions cost no extra trips.
-  while (__any(state != kDone)) {
-    if (state == kSelect) select();
-    // kFetch: the group's first lane takes an entry; all lanes broadcast
-    int k = fetch(state == kFetch && gl == 0);
+  bool fresh = false;  // a new segment: its brick view is still to be read
+  // One trip: a group whose segment ended in a rank of several bricks
+  // selects its next brick (or ends its ray), a group without a ray takes
+  // the next entry (and selects its first brick), then every group with a
+  // segment steps -- so a ray's transitions cost no extra trips.
+  while (__any(s.state != kDone)) {
+    if (s.state == kSelect) fresh |= select_brick(a, tb, org, s, owner, store);
+    int k = fetch(s.state == kFetch && owner);
     k = group_bcast_i<G, 0>(k, gbase);
-    if (state == kFetch) {
+    if (s.state == kFetch) {
       if (k >= count) {
-        state = kDone;
-      } else {
-        const ListRay r = entry(k);
-        if (a.flags & ALOHA_FLAG_HEATMAP) clock_begin = clock64();
-        ray = r;
-        // slot -1: a pixel no brick can reach (no brick to select: the miss result)
-        b0 = r.slot < 0 ? 0 : tb.roff[r.slot];
-        b1 = r.slot < 0 ? 0 : tb.roff[r.slot + 1];
-        pixel_ray(dir00, du, dv, r.px, r.py, d);
-        if (a.jitter_mode != ALOHA_JITTER_FIXED)
-          rng = rng_seed((uint32_t)(r.px + a.W * r.py), (uint32_t)a.accum_id);
-        sid = 0;
-        ns = 0;
-        z = 1.f;
-        acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        c = acc;
-        tmin = 0.f;
-        state = kSelect;
-        select();
+        s.state = kDone;
+      } else if (k >= 0) {
+        if constexpr (kRec) {
+          rec_adopt(s, entry(k));
+          fresh = true;
+        } else {
+          fresh |= begin_ray(a, tb, org, dir00, du, dv, s, entry(k), owner, store);
+        }
+        if (a.flags & ALOHA_FLAG_HEATMAP) s.clock_begin = clock64();
       }
     }
     // kStep: G samples of the group's segment
-    const bool go = state == kStep;
+    const bool go = s.state == kStep;
+    if (go && fresh) {
+      const LdsBrick &db = tb.brick[s.kb];
+      bv = BrickView{(const gfloat *)db.origin, db.px, db.pxy, db.n[0], db.n[1], db.n[2]};
+      lo0 = db.vlo[0];
+      lo1 = db.vlo[1];
+      lo2 = db.vlo[2];
+      fresh = false;
+    }
     // the chain t, t + dt, (t + dt) + dt, ...: this lane's sample is element
     // gl, the next step's base element G
-    float tk = t, tn = t;
+    float tk = s.t, tn = s.t;
 #pragma unroll
     for (int j = 1; j <= G; ++j) {
       tn += dt;
       if (j == gl) tk = tn;
     }
-    const bool valid = go && tk < t1;
+    const bool valid = go && tk < s.t1;
     float4 tf = make_float4(0.f, 0.f, 0.f, 0.f);
     // an evaluated sample has shade >= 0.1 or NaN, so shade < 0 marks "no
     // sample" (one value less to broadcast)
     float shade = -1.f;
     if (valid) {
+      const float *d = s.d;
       const float x = (org[0] + tk * d[0]) - lo0;
       const float y = (org[1] + tk * d[1]) - lo1;
       const float zz = (org[2] + tk * d[2]) - lo2;
-      Sample s;
-      if (!sample_fast(bv, x, y, zz, s)) s = sample_exact<false>(bv, nullptr, a.bricks[bid], x, y, zz);
-      const float ex = s.gx + 1e-4f, ey = s.gy + 1e-4f, ez = s.gz + 1e-4f;
+      Sample sm;
+      if (!sample_fast(bv, x, y, zz, sm))
+        sm = sample_exact<false>(bv, nullptr, a.bricks[s.bid], x, y, zz);
+      const float ex = sm.gx + 1e-4f, ey = sm.gy + 1e-4f, ez = sm.gz + 1e-4f;
       const float len = sqrtf(ex * ex + ey * ey + ez * ez);
-      const float nx = s.gx / len, ny = s.gy / len, nz = s.gz / len;
-      tf = tf_lookup(tb.tf, a.tf_n, s.val);
+      const float nx = sm.gx / len, ny = sm.gy / len, nz = sm.gz / len;
+      tf = tf_lookup(tb.tf, a.tf_n, sm.val);
       shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
     }
     bool live = go;
@@ -334,21 +421,21 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
       const float tx = group_bcast<G, j>(tf.x, gbase), ty = group_bcast<G, j>(tf.y, gbase);
       const float tz = group_bcast<G, j>(tf.z, gbase), tw = group_bcast<G, j>(tf.w, gbase);
       const float sh = group_bcast<G, j>(shade, gbase);
-      live = live && !(sh < 0.f) && c.w < .99f;
+      live = live && !(sh < 0.f) && s.c.w < .99f;
       if (live) {
-        ++ns;
-        const float w = (1.f - c.w) * tw;
-        c.x = c.x + (w * tx) * sh;
-        c.y = c.y + (w * ty) * sh;
-        c.z = c.z + (w * tz) * sh;
-        c.w = c.w + w;
+        ++s.ns;
+        const float w = (1.f - s.c.w) * tw;
+        s.c.x = s.c.x + (w * tx) * sh;
+        s.c.y = s.c.y + (w * ty) * sh;
+        s.c.z = s.c.z + (w * tz) * sh;
+        s.c.w = s.c.w + w;
       }
     });
     if (go) {
-      t = tn;
-      if (!(t < t1 && c.w < .99f)) {  // segment over
-        if (c.w < .99f) state = kSelect;
-        else end_sample();
+      s.t = tn;
+      if (!(s.t < s.t1 && s.c.w < .99f)) {  // segment over
+        if (s.c.w < .99f && s.b1 - s.b0 > 1) s.state = kSelect;
+        else end_sample(a, s, owner, store);
       }
     }
   }
@@ -565,80 +652,105 @@ __device__ __forceinline__ void fill_rows(int W, int H, int nimg, int fb, int nf
 // Persistent per-rank launch.
 //
 // The work of a frame is every pixel of every rank's rectangle (the pixels a
-// rank's bricks can cover), cut into runs of up to 64 / G consecutive pixels
-// of one rectangle row (one run per group of the wave, so the entries of one
-// fetch fall in at most two runs) -- q.run_off[s] numbers the runs rank after
-// rank, row-major -- plus the miss result everywhere else.  The grid is one wave
+// rank's bricks can cover), cut into runs of up to kRankRun consecutive pixels
+// of one rectangle row -- q.run_off[s] numbers the runs rank after rank,
+// row-major -- plus the miss result everywhere else.  The grid is one wave
 // per block, as many as the GPU holds at once.  Wave w takes runs w, w + nw,
 // w + 2 nw, ...: a static round-robin that spreads every region's cost over
 // all waves without a single atomic, while the rays of one run (a row
-// segment) stay coherent.  A run is decoded once per fetch with wave-uniform
-// (scalar) arithmetic.  Lane groups march the rays with march_list (a pixel
-// whose ray reaches no brick of its rank just stores the miss result); a
-// wave that is done then writes the miss result of its share of the image
-// rows outside the rectangles.  Static shares need no counters, so there is
-// nothing to reset between launches.
+// segment) stay coherent.
+//
+// The wave takes its runs 64 pixels (64 / kRankRun runs) at a time.  Every
+// lane sets up the ray of one of those pixels (generateRay, RNG seed, first
+// brick selection: the per-ray work, here at one ray per lane instead of one
+// per lane group); a pixel whose ray reaches no brick, or fails the island
+// test, stores its result right there, and the other rays go to a 64-entry
+// LDS list as RayRec records, in pixel order.  Lane groups then march the
+// list with march_list; when it is used up the wave sets up its next 64
+// pixels.  A wave that is done writes the miss result of its share of the
+// image rows outside the rectangles.  Static shares need no counters, so
+// there is nothing to reset between launches.
 // Pixels [x0, x1) of row py of rank slot's rectangle (slot -1: past the end).
 struct RunSpan {
   int slot, x0, x1, py;
 };
 
-template <int kRun>
 __device__ __forceinline__ RunSpan run_span(const MarchArgs &a, const RanksArgs &q, int run) {
   if (run >= q.run_off[a.P]) return RunSpan{-1, 0, 0, 0};
   int s = 0;
   while (q.run_off[s + 1] <= run) ++s;  // ends: run < run_off[P]
   const int32_t *rc = a.rect[s];
-  const int cpr = (rc[2] - rc[0] + kRun - 1) / kRun;  // runs per row
+  const int cpr = (rc[2] - rc[0] + kRankRun - 1) / kRankRun;  // runs per row
   const int l = run - q.run_off[s], row = l / cpr;
-  const int x0 = rc[0] + (l - row * cpr) * kRun;
-  return RunSpan{s, x0, min(x0 + kRun, (int)rc[2]), rc[1] + row};
+  const int x0 = rc[0] + (l - row * cpr) * kRankRun;
+  return RunSpan{s, x0, min(x0 + kRankRun, (int)rc[2]), rc[1] + row};
+}
+
+__device__ __forceinline__ long long rank_index(const MarchArgs &a, int slot, int px, int py) {
+  return (long long)slot * a.W * a.H + px + (long long)a.W * py;
 }
 
 template <int G>
 __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, const RanksArgs q) {
-  constexpr int kRun = rank_run(G);
+  constexpr int kRuns = 64 / kRankRun;  // runs per set-up pass
+  static_assert(64 % kRankRun == 0, "a set-up pass covers whole runs");
   extern __shared__ float4 smem[];
   const MarchTables tb = load_tables(a, smem);
+  RayRec *recs = reinterpret_cast<RayRec *>(smem + march_tables_bytes(a) / sizeof(float4) + 1);
   __syncthreads();
   const int lane = (int)threadIdx.x, w = (int)blockIdx.x, nw = (int)gridDim.x;
   const int runs = q.run_off[a.P];
   const float heat_scale = a.heat_scale / (float)a.spp;
-  // stream index k of this wave: run w + (k / kRun) * nw, pixel k % kRun of it
-  const int count = runs > w ? (runs - w + nw - 1) / nw * kRun : 0;
-  int pos = 0;         // next stream index (wave-uniform)
-  int j0 = 0;          // run index (in the stream) of the last fetch's first entry
-  RunSpan span0{-1, 0, 0, 0}, span1{-1, 0, 0, 0};  // its run and the next
-  auto fetch = [&](bool want) {
-    const uint64_t m = __ballot(want);
-    const int n = __popcll(m);
-    const int k = pos + __popcll(m & ((1ull << lane) - 1ull));
-    if (n > 0) {  // wave-uniform: decode the (at most two) runs the entries fall in
-      j0 = pos / kRun;
-      span0 = run_span<kRun>(a, q, w + j0 * nw);
-      if ((pos + n - 1) / kRun > j0) span1 = run_span<kRun>(a, q, w + (j0 + 1) * nw);
-      pos += n;
-    }
-    return k;
+  const int my_runs = runs > w ? (runs - w + nw - 1) / nw : 0;
+  const int passes = (my_runs + kRuns - 1) / kRuns;
+  auto store = [&](const ListRay &r, float4 acc, float z, uint32_t ns, long long clock_begin) {
+    if (r.tag < 0) return;
+    const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
+                             ? (float)(clock64() - clock_begin) * heat_scale
+                             : 0.f;
+    write_rank_pixel(a, rank_index(a, r.tag, r.px, r.py), r.px, r.py, acc, z, ns, heat_t);
   };
-  march_list<G>(
-      a, tb, a.org, a.dir00, a.du, a.dv, fetch, count,
-      [&](int k) {
-        const RunSpan sp = k / kRun == j0 ? span0 : span1;
-        const int px = sp.x0 + k % kRun, py = sp.py;
-        if (sp.slot < 0 || px >= sp.x1 || px >= a.W || py >= a.H)
-          return ListRay{-1, 0, 0, -1};  // nothing to write
-        // tag: the rank slot (the slot is -1 when no brick can be reached)
-        return ListRay{island_pixel(a, px, py) ? sp.slot : -1, px, py, sp.slot};
-      },
-      [&](const ListRay &r, float4 acc, float z, uint32_t ns, long long clock_begin) {
-        if (r.tag < 0) return;
-        const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
-                                 ? (float)(clock64() - clock_begin) * heat_scale
-                                 : 0.f;
-        write_rank_pixel(a, (long long)r.tag * a.W * a.H + r.px + (long long)a.W * r.py, r.px,
-                         r.py, acc, z, ns, heat_t);
-      });
+  // set-up pass p: lane l takes pixel l % kRankRun of the wave's run
+  // kRuns p + l / kRankRun; returns the number of records (wave-uniform)
+  auto setup = [&](int p) {
+    RunSpan sp{-1, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < kRuns; ++i) {  // wave-uniform decode of the pass's runs
+      const int j = kRuns * p + i;
+      const RunSpan si = j < my_runs ? run_span(a, q, w + j * nw) : RunSpan{-1, 0, 0, 0};
+      if (lane / kRankRun == i) sp = si;
+    }
+    const int px = sp.x0 + lane % kRankRun, py = sp.py;
+    RayState s;
+    ray_state_init(s);
+    if (sp.slot >= 0 && px < sp.x1 && px < a.W && py < a.H) {
+      if (a.flags & ALOHA_FLAG_HEATMAP) s.clock_begin = clock64();
+      // tag: the rank slot (the slot is -1 when no brick can be reached)
+      const ListRay r{island_pixel(a, px, py) ? sp.slot : -1, px, py, sp.slot};
+      begin_ray(a, tb, a.org, a.dir00, a.du, a.dv, s, r, true, store);
+    }
+    const bool live = s.state != kFetch;  // kFetch: no pixel, or its result is written
+    const uint64_t m = __ballot(live);
+    if (live) rec_pack(recs[__popcll(m & ((1ull << lane) - 1ull))], s);
+    return (int)__popcll(m);
+  };
+  int pass = 0, nrec = 0, next = 0;  // wave-uniform
+  auto fetch = [&](bool want) {
+    if (__any(want) && next >= nrec && pass < passes) {
+      __syncthreads();  // every record of the list has been read
+      do {
+        nrec = setup(pass++);
+      } while (nrec == 0 && pass < passes);
+      next = 0;
+      __syncthreads();  // the new list is written
+    }
+    const uint64_t m = __ballot(want);
+    const int k = next + __popcll(m & ((1ull << lane) - 1ull));
+    next += __popcll(m);
+    return k < nrec ? k : pass < passes ? -1 : INT_MAX;
+  };
+  march_list<G, true>(a, tb, a.org, a.dir00, a.du, a.dv, fetch, INT_MAX,
+                      [&](int k) { return recs[k]; }, store);
 
   // fill: the miss result of every pixel outside the rectangles, rows
   // w, w + nw, ... of the P x H rank image rows
@@ -651,7 +763,7 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
     const bool band = rc[0] < rc[2] && y >= rc[1] && y < rc[3];
     for (int x = lane; x < a.W; x += 64) {
       if (band && x >= rc[0] && x < rc[2]) continue;  // a rectangle pixel: marched
-      const long long o = (long long)s * a.W * a.H + x + (long long)a.W * y;
+      const long long o = rank_index(a, s, x, y);
       if (plain) {
         const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
         if (a.depth) a.depth[o] = 1.f;
@@ -784,7 +896,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
       pixel_ray(src.dir00(), src.du(), src.dv(), px, py, d);
       return ray_meets_rank(tb, slot, src.org(), d);
     });
-    march_list<G>(
+    march_list<G, false>(
         a, tb, src.org(), src.dir00(), src.du(), src.dv(),
         [&](bool want) { return want ? atomicAdd(&fs.next, 1) : 0; }, count,
         [&](int k) {
@@ -1012,7 +1124,7 @@ hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu) {
 
 hipError_t launch_march_ranks(const MarchArgs &a, const RanksArgs &q, int group, int waves,
                               hipStream_t s) {
-  const size_t lds = march_tables_bytes(a);
+  const size_t lds = ranks_lds_bytes(a);
   switch (group) {
     case 1: march_persist_kernel<1><<<waves, 64, lds, s>>>(a, q); break;
     case 2: march_persist_kernel<2><<<waves, 64, lds, s>>>(a, q); break;
