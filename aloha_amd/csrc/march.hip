@@ -764,13 +764,13 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
     for (int x = lane; x < a.W; x += 64) {
       if (band && x >= rc[0] && x < rc[2]) continue;  // a rectangle pixel: marched
       const long long o = rank_index(a, s, x, y);
-      if (plain) {
+      if (plain) {  // streaming stores: they overlap the march of the other waves
         const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (a.depth) a.depth[o] = 1.f;
-        if (a.nsamples) a.nsamples[o] = 0u;
-        if (a.accum) a.accum[o] = zero;
-        if (a.rgba) a.rgba[o] = zero;
-        if (a.rgba8) a.rgba8[o] = 0u;
+        if (a.depth) store_nt(a.depth + o, 1.f);
+        if (a.nsamples) store_nt(a.nsamples + o, 0u);
+        if (a.accum) store_nt(a.accum + o, zero);
+        if (a.rgba) store_nt(a.rgba + o, zero);
+        if (a.rgba8) store_nt(a.rgba8 + o, 0u);
       } else {
         long long clock_begin = 0;
         if (heat) clock_begin = clock64();
@@ -843,11 +843,12 @@ __device__ __forceinline__ bool in_rc(const int32_t *rc, int px, int py) {
   return px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
 }
 
-// Transparent black (no rank reaches the pixel) at frame pixel `pixel`.
+// Transparent black (no rank reaches the pixel) at frame pixel `pixel`
+// (streaming stores, as the per-rank fill).
 __device__ __forceinline__ void fused_clear(const MarchArgs &a, long long pixel) {
-  if (a.rgba) a.rgba[pixel] = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (a.rgba8) a.rgba8[pixel] = 0u;
-  if (a.nsamples) a.nsamples[pixel] = 0u;
+  if (a.rgba) store_nt(a.rgba + pixel, make_float4(0.f, 0.f, 0.f, 0.f));
+  if (a.rgba8) store_nt(a.rgba8 + pixel, 0u);
+  if (a.nsamples) store_nt(a.nsamples + pixel, 0u);
 }
 
 // One fused 16x4 tile at (bx, by): rank list, chunks of (rank, pixel) items

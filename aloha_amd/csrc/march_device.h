@@ -257,6 +257,19 @@ __device__ __forceinline__ bool slab(const float o[3], const float d[3], float t
   return t0 < t1;
 }
 
+// Streaming (nontemporal) vector stores: for output that nothing in the
+// launch reads back, so the writes do not displace the voxels the march
+// re-reads from L2.
+__device__ __forceinline__ void store_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(uint32_t *p, uint32_t v) {
+  __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ void store_nt(float4 *p, float4 v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 u = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(u, reinterpret_cast<f4 *>(p));
+}
+
 // ---- per-pixel helpers shared by the volume and surface raygens ----
 
 __device__ __forceinline__ float sat(float f) { return fminf(fmaxf(f, 0.f), 1.f); }

@@ -9,6 +9,21 @@ import oracle as O
 from aloha_amd import RankImages
 
 
+def poison(imgs):
+    """Fill freshly allocated output images with values no render writes
+    (NaN colours and depths, all-ones words), so a pixel the kernels skip
+    fails the comparison instead of passing on stale memory."""
+    for name in ("rgba", "depth"):
+        t = getattr(imgs, name, None)
+        if t is not None:
+            t.fill_(float("nan"))
+    for name in ("rgba8", "nsamples"):
+        t = getattr(imgs, name, None)
+        if t is not None:
+            t.fill_(-1)
+    return imgs
+
+
 def gpu_render(wl, ranks=None, P=None, *, accum=None, lanes=1, tf=None, **params):
     """Render every rank of `ranks` (brick-id lists; default: the workload's
     kd assignment) on the GPU with `lanes` lanes per ray.  Returns numpy
@@ -21,7 +36,7 @@ def gpu_render(wl, ranks=None, P=None, *, accum=None, lanes=1, tf=None, **params
         ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
         ctx.set_tf(wl.tf if tf is None else tf)
         ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
-        out = RankImages.alloc(P, wl.W, wl.H, accum=True, nsamples=True)
+        out = poison(RankImages.alloc(P, wl.W, wl.H, accum=True, nsamples=True))
         if accum is not None:
             out.accum.copy_(torch.from_numpy(accum))
         ctx.render_ranks(wl.params(**params), wl.W, wl.H, [[ids[i] for i in r] for r in ranks], out)

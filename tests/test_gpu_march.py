@@ -11,7 +11,7 @@ import pytest
 import aloha_amd as A
 from aloha_amd import workloads as WL
 
-from gpu_util import compare, gpu_render, oracle_render
+from gpu_util import compare, gpu_render, oracle_render, poison
 
 pytestmark = pytest.mark.gpu
 
@@ -114,7 +114,7 @@ def test_many_ranks_odd_image(lanes):
         ctx.set_tf(wl.tf)
         ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
         ranks = [[ids[i] for i in rb] for rb in wl.rank_bricks]
-        imgs = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H)
+        imgs = poison(A.RankImages.alloc(wl.num_ranks, wl.W, wl.H))
         ref = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
         out = torch.empty_like(ref)
         ctx.render_ranks(wl.params(), wl.W, wl.H, ranks, imgs)
@@ -139,7 +139,7 @@ def test_fused_render_composite_bit_exact(which, c0, c1):
         ctx.set_tf(wl.tf)
         ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
         ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
-        imgs = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, nsamples=True)
+        imgs = poison(A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, nsamples=True))
         ctx.render_ranks(wl.params(), wl.W, wl.H, ranks, imgs)
         ctx.composite_over(imgs.rgba, wl.order, wl.W, wl.H, ref, ref8)
         ctx.render_composite(wl.params(), wl.W, wl.H, ranks, wl.order, out, out8, ns)
@@ -167,7 +167,7 @@ def test_sparse_composite_equals_dense(which, c0, c1):
         ctx.set_tf(wl.tf)
         ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
         ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
-        imgs = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H)
+        imgs = poison(A.RankImages.alloc(wl.num_ranks, wl.W, wl.H))
         ctx.render_ranks(params, wl.W, wl.H, ranks, imgs)
         rects = ctx.rank_rects(params, wl.W, wl.H, ranks)
         dense = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
@@ -239,7 +239,7 @@ def test_lane_schedules_brick_loop_and_fused(c1):
         for o, o8, ns in outs[1:]:
             assert torch.equal(o, outs[0][0]) and torch.equal(o8, outs[0][1])
             assert torch.equal(ns, outs[0][2])
-        imgs = A.RankImages.alloc(len(rk), c1.W, c1.H)
+        imgs = poison(A.RankImages.alloc(len(rk), c1.W, c1.H))
         (r_l, f_l), table = ctx.autotune_march(c1.params(), c1.W, c1.H, rk, imgs, c1.order,
                                                outs[0][0], outs[0][1], reps=2)
         print("autotune", r_l, f_l, table)
