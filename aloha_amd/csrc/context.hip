@@ -403,13 +403,12 @@ extern "C" int aloha_camera_set(aloha_ctx *ctx, const float org[3], const float 
 namespace aloha {
 
 int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, int &waves) {
-  const int run = kRankRun;
   int64_t total = 0;
   q.run_off[0] = 0;
   for (int s = 0; s < a.P; ++s) {
     const int32_t *rc = a.rect[s];
     if (rc[0] < rc[2] && rc[1] < rc[3])
-      total += (int64_t)(rc[3] - rc[1]) * ((rc[2] - rc[0] + run - 1) / run);
+      total += (int64_t)(rc[3] - rc[1]) * ((rc[2] - rc[0] + kRankRun - 1) / kRankRun);
     if (total > INT32_MAX / 128)
       return fail(ctx, ALOHA_ERR_LIMIT, "render: rank rectangles too large");
     q.run_off[s + 1] = (int32_t)total;

@@ -55,7 +55,10 @@ struct MarchArgs {
 
 // Persistent per-rank launch: the pixels of the ranks' rectangles in runs of
 // up to kRankRun consecutive pixels of one rectangle row, numbered rank after
-// rank, row-major; run_off[s] is the first run of rank slot s.
+// rank, row-major (run_off[s] is the first run of rank slot s), the run list
+// cut into kRankParts contiguous parts.  Part x belongs to the waves
+// w = x (mod kRankParts), which take its runs round-robin.
+constexpr int kRankParts = 16;
 constexpr int kRankRun = 16;
 struct RanksArgs {
   int32_t run_off[ALOHA_MAX_RANKS + 1];
@@ -143,10 +146,12 @@ __host__ __device__ inline size_t march_tables_bytes(const MarchArgs &a) {
          sizeof(int32_t) * (size_t)(a.P + 1) + sizeof(int32_t) * 4 * (size_t)a.P;
 }
 // The persistent per-rank launch adds a list of 64 ray records (64 B each)
-// after the tables, at the next 16-byte boundary past them.
+// after the tables, at the next 16-byte boundary past them, then the run
+// offsets run_off[0..P].
 constexpr size_t kRayRecBytes = 64;
 __host__ __device__ inline size_t ranks_lds_bytes(const MarchArgs &a) {
-  return (march_tables_bytes(a) / 16 + 1) * 16 + 64 * kRayRecBytes;
+  return (march_tables_bytes(a) / 16 + 1) * 16 + 64 * kRayRecBytes +
+         sizeof(int32_t) * (size_t)(a.P + 1);
 }
 // resident one-wave blocks per CU of the persistent per-rank kernel
 hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu);

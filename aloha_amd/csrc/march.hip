@@ -652,57 +652,52 @@ __device__ __forceinline__ void fill_rows(int W, int H, int nimg, int fb, int nf
 // Persistent per-rank launch.
 //
 // The work of a frame is every pixel of every rank's rectangle (the pixels a
-// rank's bricks can cover), cut into runs of up to kRankRun consecutive pixels
-// of one rectangle row -- q.run_off[s] numbers the runs rank after rank,
-// row-major -- plus the miss result everywhere else.  The grid is one wave
-// per block, as many as the GPU holds at once.  Wave w takes runs w, w + nw,
-// w + 2 nw, ...: a static round-robin that spreads every region's cost over
-// all waves without a single atomic, while the rays of one run (a row
-// segment) stay coherent.
+// rank's bricks can cover) plus the miss result everywhere else.  The grid
+// is one wave per block, as many as the GPU holds at once.  The rectangle
+// pixels are cut into runs of up to kRankRun consecutive pixels of one row,
+// numbered rank after rank and row-major (q.run_off); the run list is cut
+// into kRankParts contiguous parts, and the waves w = x (mod kRankParts) --
+// all on one XCD, as workgroups are dealt to the XCDs in turn -- take part
+// x's runs round-robin.  So every wave gets the same number of runs (within
+// one), its runs are spread over the part (no wave holds a whole expensive
+// region), the rays of a run are neighbours (their voxel reads share cache
+// lines), and the voxels one XCD reads come from one part of the frame.
+// Static shares need no atomics and no counters to reset between launches.
 //
-// The wave takes its runs 64 pixels (64 / kRankRun runs) at a time.  Every
-// lane sets up the ray of one of those pixels (generateRay, RNG seed, first
-// brick selection: the per-ray work, here at one ray per lane instead of one
-// per lane group); a pixel whose ray reaches no brick, or fails the island
-// test, stores its result right there, and the other rays go to a 64-entry
-// LDS list as RayRec records, in pixel order.  Lane groups then march the
-// list with march_list; when it is used up the wave sets up its next 64
-// pixels.  A wave that is done writes the miss result of its share of the
-// image rows outside the rectangles.  Static shares need no counters, so
-// there is nothing to reset between launches.
-// Pixels [x0, x1) of row py of rank slot's rectangle (slot -1: past the end).
-struct RunSpan {
-  int slot, x0, x1, py;
-};
-
-__device__ __forceinline__ RunSpan run_span(const MarchArgs &a, const RanksArgs &q, int run) {
-  if (run >= q.run_off[a.P]) return RunSpan{-1, 0, 0, 0};
-  int s = 0;
-  while (q.run_off[s + 1] <= run) ++s;  // ends: run < run_off[P]
-  const int32_t *rc = a.rect[s];
-  const int cpr = (rc[2] - rc[0] + kRankRun - 1) / kRankRun;  // runs per row
-  const int l = run - q.run_off[s], row = l / cpr;
-  const int x0 = rc[0] + (l - row * cpr) * kRankRun;
-  return RunSpan{s, x0, min(x0 + kRankRun, (int)rc[2]), rc[1] + row};
-}
-
+// The wave takes its runs 64 / kRankRun at a time.  Every lane sets up the
+// ray of one pixel (generateRay, RNG seed, first brick selection: the
+// per-ray work, here at one ray per lane instead of one per lane group); a
+// pixel whose ray reaches no brick, or fails the island test, stores its
+// result right there, and the other rays go to a 64-entry LDS list as RayRec
+// records, in pixel order.  Lane groups then march the list with
+// march_list; when it is used up the wave sets up its next runs.  A wave
+// that is done writes the miss result of its share of the image rows outside
+// the rectangles.
 __device__ __forceinline__ long long rank_index(const MarchArgs &a, int slot, int px, int py) {
   return (long long)slot * a.W * a.H + px + (long long)a.W * py;
 }
 
 template <int G>
 __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, const RanksArgs q) {
-  constexpr int kRuns = 64 / kRankRun;  // runs per set-up pass
   static_assert(64 % kRankRun == 0, "a set-up pass covers whole runs");
+  constexpr int kRunsPerPass = 64 / kRankRun;
   extern __shared__ float4 smem[];
   const MarchTables tb = load_tables(a, smem);
   RayRec *recs = reinterpret_cast<RayRec *>(smem + march_tables_bytes(a) / sizeof(float4) + 1);
-  __syncthreads();
+  int32_t *roff = reinterpret_cast<int32_t *>(recs + 64);
   const int lane = (int)threadIdx.x, w = (int)blockIdx.x, nw = (int)gridDim.x;
+  for (int i = lane; i <= a.P; i += 64) roff[i] = q.run_off[i];
+  __syncthreads();
   const int runs = q.run_off[a.P];
+  // this wave's part [r0, r1) of the run list and its place among the
+  // part's nwp waves
+  const int parts = nw < kRankParts ? 1 : kRankParts;
+  const int part = w % parts, wi = w / parts, nwp = (nw - part + parts - 1) / parts;
+  const int r0 = (int)((long long)runs * part / parts);
+  const int r1 = (int)((long long)runs * (part + 1) / parts);
+  const int my_runs = r1 - r0 > wi ? (r1 - r0 - wi + nwp - 1) / nwp : 0;
+  const int passes = (my_runs + kRunsPerPass - 1) / kRunsPerPass;
   const float heat_scale = a.heat_scale / (float)a.spp;
-  const int my_runs = runs > w ? (runs - w + nw - 1) / nw : 0;
-  const int passes = (my_runs + kRuns - 1) / kRuns;
   auto store = [&](const ListRay &r, float4 acc, float z, uint32_t ns, long long clock_begin) {
     if (r.tag < 0) return;
     const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
@@ -711,28 +706,30 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
     write_rank_pixel(a, rank_index(a, r.tag, r.px, r.py), r.px, r.py, acc, z, ns, heat_t);
   };
   // set-up pass p: lane l takes pixel l % kRankRun of the wave's run
-  // kRuns p + l / kRankRun; returns the number of records (wave-uniform)
+  // kRunsPerPass p + l / kRankRun; returns the number of records (wave-uniform)
   auto setup = [&](int p) {
-    RunSpan sp{-1, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < kRuns; ++i) {  // wave-uniform decode of the pass's runs
-      const int j = kRuns * p + i;
-      const RunSpan si = j < my_runs ? run_span(a, q, w + j * nw) : RunSpan{-1, 0, 0, 0};
-      if (lane / kRankRun == i) sp = si;
-    }
-    const int px = sp.x0 + lane % kRankRun, py = sp.py;
+    const int m = kRunsPerPass * p + lane / kRankRun;  // the wave's m-th run
+    const int run = r0 + wi + m * nwp;
     RayState s;
     ray_state_init(s);
-    if (sp.slot >= 0 && px < sp.x1 && px < a.W && py < a.H) {
-      if (a.flags & ALOHA_FLAG_HEATMAP) s.clock_begin = clock64();
-      // tag: the rank slot (the slot is -1 when no brick can be reached)
-      const ListRay r{island_pixel(a, px, py) ? sp.slot : -1, px, py, sp.slot};
-      begin_ray(a, tb, a.org, a.dir00, a.du, a.dv, s, r, true, store);
+    if (m < my_runs) {
+      int slot = 0;  // the rank whose runs hold this one: run_off[slot] <= run < run_off[slot + 1]
+      while (roff[slot + 1] <= run) ++slot;
+      const int32_t *rc = tb.rect + 4 * slot;
+      const int l = run - roff[slot], cpr = (rc[2] - rc[0] + kRankRun - 1) / kRankRun;
+      const int row = l / cpr;
+      const int px = rc[0] + (l - row * cpr) * kRankRun + lane % kRankRun, py = rc[1] + row;
+      if (px < rc[2]) {
+        if (a.flags & ALOHA_FLAG_HEATMAP) s.clock_begin = clock64();
+        // tag: the rank slot (the slot is -1 when no brick can be reached)
+        const ListRay r{island_pixel(a, px, py) ? slot : -1, px, py, slot};
+        begin_ray(a, tb, a.org, a.dir00, a.du, a.dv, s, r, true, store);
+      }
     }
     const bool live = s.state != kFetch;  // kFetch: no pixel, or its result is written
-    const uint64_t m = __ballot(live);
-    if (live) rec_pack(recs[__popcll(m & ((1ull << lane) - 1ull))], s);
-    return (int)__popcll(m);
+    const uint64_t mk = __ballot(live);
+    if (live) rec_pack(recs[__popcll(mk & ((1ull << lane) - 1ull))], s);
+    return (int)__popcll(mk);
   };
   int pass = 0, nrec = 0, next = 0;  // wave-uniform
   auto fetch = [&](bool want) {
