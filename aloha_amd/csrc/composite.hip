@@ -48,28 +48,41 @@ __global__ __launch_bounds__(kBlock) void over_kernel(const float4 *__restrict__
 // rectangle c.rect[s]).  Blending a transparent-black pixel leaves C exactly
 // unchanged (C + w * 0 = C, w >= 0), so skipping those reads gives the dense
 // result bit for bit while reading only the pixels inside the rectangles.
-// Grid: x covers a row in kBlock-pixel runs, y = row.
+// Grid: x covers a row in runs of kSparsePix pixels (each thread two pixels
+// kBlock apart), y = row.  The output goes out with streaming stores.
+constexpr int kSparsePix = 2 * kBlock;
+
+__device__ __forceinline__ void store_nt(float4 *p, float4 v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 u = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(u, reinterpret_cast<f4 *>(p));
+}
+
 __global__ __launch_bounds__(kBlock) void over_sparse_kernel(const float4 *__restrict__ imgs,
                                                              const CompositeArgs c, int W,
                                                              float4 *__restrict__ out,
                                                              uint32_t *__restrict__ out8) {
-  const int px = (int)(blockIdx.x * kBlock + threadIdx.x), py = (int)blockIdx.y;
-  if (px >= W) return;
-  const int64_t i = px + (int64_t)W * py;
-  float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int k = 0; k < c.P; ++k) {
-    const int s = c.order[k];  // k is uniform: scalar loads
-    const int32_t *rc = c.rect[s];
-    if (px < rc[0] || px >= rc[2] || py < rc[1] || py >= rc[3]) continue;
-    const float4 v = imgs[(int64_t)s * c.npix + i];
-    const float w = 1.f - C.w;
-    C.x = C.x + w * v.x;
-    C.y = C.y + w * v.y;
-    C.z = C.z + w * v.z;
-    C.w = C.w + w * v.w;
+  const int py = (int)blockIdx.y;
+#pragma unroll
+  for (int h = 0; h < kSparsePix / kBlock; ++h) {
+    const int px = (int)(blockIdx.x * kSparsePix + h * kBlock + threadIdx.x);
+    if (px >= W) return;
+    const int64_t i = px + (int64_t)W * py;
+    float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < c.P; ++k) {
+      const int s = c.order[k];  // k is uniform: scalar loads
+      const int32_t *rc = c.rect[s];
+      if (px < rc[0] || px >= rc[2] || py < rc[1] || py >= rc[3]) continue;
+      const float4 v = imgs[(int64_t)s * c.npix + i];
+      const float w = 1.f - C.w;
+      C.x = C.x + w * v.x;
+      C.y = C.y + w * v.y;
+      C.z = C.z + w * v.z;
+      C.w = C.w + w * v.w;
+    }
+    if (out) store_nt(out + i, C);
+    if (out8) __builtin_nontemporal_store(pack_rgba(C), out8 + i);
   }
-  if (out) out[i] = C;
-  if (out8) out8[i] = pack_rgba(C);
 }
 
 __global__ __launch_bounds__(kBlock) void over_rgba8_kernel(const uint32_t *__restrict__ imgs,
@@ -122,7 +135,7 @@ hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, flo
 
 hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs &c, int32_t W,
                                         int32_t H, float4 *out, uint32_t *out8, hipStream_t s) {
-  const dim3 grid((unsigned)((W + kBlock - 1) / kBlock), (unsigned)H);
+  const dim3 grid((unsigned)((W + kSparsePix - 1) / kSparsePix), (unsigned)H);
   over_sparse_kernel<<<grid, kBlock, 0, s>>>(imgs, c, W, out, out8);
   return hipGetLastError();
 }

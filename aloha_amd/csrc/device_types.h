@@ -145,10 +145,10 @@ __host__ __device__ inline size_t march_tables_bytes(const MarchArgs &a) {
   return sizeof(float4) * (size_t)(a.tf_n + 4) + kLdsBrickBytes * (size_t)a.rank_off[a.P] +
          sizeof(int32_t) * (size_t)(a.P + 1) + sizeof(int32_t) * 4 * (size_t)a.P;
 }
-// The persistent per-rank launch adds a list of 64 ray records (64 B each)
+// The persistent per-rank launch adds a list of 64 ray records (48 B each)
 // after the tables, at the next 16-byte boundary past them, then the run
 // offsets run_off[0..P].
-constexpr size_t kRayRecBytes = 64;
+constexpr size_t kRayRecBytes = 48;
 __host__ __device__ inline size_t ranks_lds_bytes(const MarchArgs &a) {
   return (march_tables_bytes(a) / 16 + 1) * 16 + 64 * kRayRecBytes +
          sizeof(int32_t) * (size_t)(a.P + 1);

@@ -167,7 +167,7 @@ enum : int { kFetch = 0, kSelect = 1, kStep = 2, kDone = 3 };
 // from tmin next), kStep (marching segment [t, t1) of brick kb), kDone.
 // In a lane group every lane holds the same values.
 struct RayState {
-  int state, b0, b1, sid, kb, bid;  // kb: rank_bricks index, bid: brick id
+  int state, b0, b1, sid, kb;  // kb: the segment's rank_bricks index
   ListRay ray;
   uint32_t rng, ns;
   float d[3];
@@ -178,7 +178,7 @@ struct RayState {
 
 __device__ __forceinline__ void ray_state_init(RayState &s) {
   s.state = kFetch;
-  s.b0 = s.b1 = s.sid = s.kb = s.bid = 0;
+  s.b0 = s.b1 = s.sid = s.kb = 0;
   s.ray = ListRay{-1, 0, 0, 0};
   s.rng = s.ns = 0;
   s.d[0] = 1.f, s.d[1] = 0.f, s.d[2] = 0.f;
@@ -250,7 +250,6 @@ __device__ __forceinline__ bool select_brick(const MarchArgs &a, const MarchTabl
   s.t = t;
   s.t1 = bfar;
   s.kb = hit;
-  s.bid = a.rank_bricks[hit];
   s.z = fminf(s.z, bnear / a.max_depth);
   s.tmin = bfar + 1e-20f;
   if (t < s.t1 && s.c.w < .99f) s.state = kStep;
@@ -283,32 +282,34 @@ __device__ __forceinline__ bool begin_ray(const MarchArgs &a, const MarchTables 
 }
 
 // A ray after begin_ray, as one lane set it up for a lane group to march
-// (64 B, four 16-byte LDS words).  ns, c and acc are zero at that point:
+// (48 B, three 16-byte LDS words).  ns, c and acc are zero at that point:
 // the first selection takes no sample, and a sample that ends there sums
-// zeros.
+// zeros.  tmin is not stored: after a hit it is t1 + 1e-20 (t1 = the brick
+// exit), after a miss (which ends the first spp sample, so sid > 0) it is 0.
+// Brick indices fit 9 bits (ALOHA_MAX_RANK_BRICKS <= 256).
 struct RayRec {
-  int4 w0;    // slot, px, py, tag
+  int4 w0;    // px, py, tag, kb | b0 << 9 | b1 << 18
   float4 w1;  // d[0..2], rng (bits)
-  float4 w2;  // z, tmin, t, t1
-  int4 w3;    // kb, bid, state | sid << 2, b0 | b1 << 16
+  float4 w2;  // z, t, t1, state | sid << 2 (bits)
 };
-static_assert(sizeof(RayRec) == 64, "LDS ray record");
+static_assert(sizeof(RayRec) == kRayRecBytes, "LDS ray record");
+static_assert(ALOHA_MAX_RANK_BRICKS <= 256, "brick indices of a ray record");
 
 __device__ __forceinline__ void rec_pack(RayRec &r, const RayState &s) {
-  r.w0 = make_int4(s.ray.slot, s.ray.px, s.ray.py, s.ray.tag);
+  r.w0 = make_int4(s.ray.px, s.ray.py, s.ray.tag, s.kb | (s.b0 << 9) | (s.b1 << 18));
   r.w1 = make_float4(s.d[0], s.d[1], s.d[2], __uint_as_float(s.rng));
-  r.w2 = make_float4(s.z, s.tmin, s.t, s.t1);
-  r.w3 = make_int4(s.kb, s.bid, s.state | (s.sid << 2), s.b0 | (s.b1 << 16));
+  r.w2 = make_float4(s.z, s.t, s.t1, __int_as_float(s.state | (s.sid << 2)));
 }
 
 __device__ __forceinline__ void rec_adopt(RayState &s, const RayRec &r) {
-  s.ray = ListRay{r.w0.x, r.w0.y, r.w0.z, r.w0.w};
+  s.ray = ListRay{0, r.w0.x, r.w0.y, r.w0.z};  // slot: only begin_ray reads it
+  s.kb = r.w0.w & 0x1FF, s.b0 = (r.w0.w >> 9) & 0x1FF, s.b1 = r.w0.w >> 18;
   s.d[0] = r.w1.x, s.d[1] = r.w1.y, s.d[2] = r.w1.z;
   s.rng = __float_as_uint(r.w1.w);
-  s.z = r.w2.x, s.tmin = r.w2.y, s.t = r.w2.z, s.t1 = r.w2.w;
-  s.kb = r.w3.x, s.bid = r.w3.y;
-  s.state = r.w3.z & 3, s.sid = r.w3.z >> 2;
-  s.b0 = r.w3.w & 0xFFFF, s.b1 = r.w3.w >> 16;
+  s.z = r.w2.x, s.t = r.w2.y, s.t1 = r.w2.z;
+  const int ss = __float_as_int(r.w2.w);
+  s.state = ss & 3, s.sid = ss >> 2;
+  s.tmin = s.sid > 0 ? 0.f : s.t1 + 1e-20f;
   s.ns = 0;
   s.c = s.acc = make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -408,7 +409,7 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
       const float zz = (org[2] + tk * d[2]) - lo2;
       Sample sm;
       if (!sample_fast(bv, x, y, zz, sm))
-        sm = sample_exact<false>(bv, nullptr, a.bricks[s.bid], x, y, zz);
+        sm = sample_exact<false>(bv, nullptr, a.bricks[0], x, y, zz);  // no footprint: db unread
       const float ex = sm.gx + 1e-4f, ey = sm.gy + 1e-4f, ez = sm.gz + 1e-4f;
       const float len = sqrtf(ex * ex + ey * ey + ez * ez);
       const float nx = sm.gx / len, ny = sm.gy / len, nz = sm.gz / len;
@@ -545,30 +546,22 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   }
 }
 
-// Whether the primary ray (org, d) reaches a brick of rank `slot`: the first
-// brick selection of march_list (tmin = 0) finds a hit.  A ray that
-// reaches none keeps the miss result (no sample, colour 0, depth 1).
-__device__ __forceinline__ bool ray_meets_rank(const MarchTables &tb, int slot, const float org[3],
-                                               const float d[3]) {
-  for (int k = tb.roff[slot]; k < tb.roff[slot + 1]; ++k) {
-    const LdsBrick &db = tb.brick[k];
-    float t0, t1;
-    if (slab(org, d, 0.f, 1e10f, db.lo, db.hi, t0, t1) && (t0 > 0.f || t1 > 0.f)) return true;
-  }
-  return false;
-}
-
-// Stream compaction of the items i in [0, n) with live(i) into list, in
-// index order (a ballot and popcount per wave, wave totals through wcnt);
-// returns the count, block-uniform.  Every thread of the block calls it; it
-// ends with a barrier, so LDS written before the call is visible after it.
-template <class Live>
-__device__ __forceinline__ int compact_items(int n, uint16_t *list, int *wcnt, const Live &live) {
+// Lane-parallel ray set-up with stream compaction: every thread of the block
+// runs setup(i, state) for items i in [0, n) (a begin_ray, or nothing for an
+// item without a ray); the rays still marching (state other than kFetch) go
+// to recs as RayRec records, in item order.  Returns the count,
+// block-uniform.  Every thread of the block calls it; it ends with a
+// barrier, so LDS written before the call is visible after it.
+template <class Setup>
+__device__ __forceinline__ int compact_records(int n, RayRec *recs, int *wcnt, const Setup &setup) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int total = 0;
   for (int i0 = 0; i0 < n; i0 += kBlock) {
     const int i = i0 + t;
-    const bool on = i < n && live(i);
+    RayState st;
+    ray_state_init(st);
+    if (i < n) setup(i, st);
+    const bool on = st.state != kFetch;
     const uint64_t m = __ballot(on);
     if (lane == 0) wcnt[w] = __popcll(m);
     __syncthreads();
@@ -578,8 +571,8 @@ __device__ __forceinline__ int compact_items(int n, uint16_t *list, int *wcnt, c
       base += k < w ? wcnt[k] : 0;
       sum += wcnt[k];
     }
-    if (on) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)i;
-    __syncthreads();  // wcnt is rewritten by the next round; the list is complete
+    if (on) rec_pack(recs[base + __popcll(m & ((1ull << lane) - 1ull))], st);
+    __syncthreads();  // wcnt is rewritten by the next round; the records are complete
     total = sum;
   }
   return total;
@@ -780,8 +773,9 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
 
 // Fused raygen + composite.  The work items of a tile are (rank, pixel)
 // pairs of the ranks whose rectangle meets it, taken in visibility order in
-// chunks of up to kFusedChunk ranks: lane groups march items in parallel and
-// store each rank's pixel colour (its rank image value, accum_id 0) in LDS;
+// chunks of up to kFusedChunk ranks: every thread sets up the rays of its
+// items (compact_records), lane groups march the list in parallel and store
+// each rank's pixel colour (its rank image value, accum_id 0) in LDS;
 // then each thread blends its pixel's chunk colours 'over' its running
 // colour in order -- march_composite's arithmetic, so the frame equals the
 // rank images composited by over_kernel bit for bit.  A rank that misses a
@@ -789,8 +783,11 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
 // The fused tile is 16x4 pixels (two 8x4 halves, row-major inside each), a
 // quarter of the per-rank tile: its items are (rank, pixel) pairs, so the
 // 256 threads still have up to kFusedChunk x 64 items per chunk while the
-// grid has enough workgroups to spread the few tiles the volume covers.
-constexpr int kFusedChunk = 8;
+// grid has enough workgroups to spread the few tiles the volume covers.  Six
+// ranks per chunk keep the workgroup's LDS (the chunk's ray records and
+// colours, ~26 KB, plus the brick tables) small enough for four workgroups
+// per CU with 64 bricks.
+constexpr int kFusedChunk = 6;
 constexpr int kFW = kFusedTileW, kFH = kFusedTileH, kFPix = kFW * kFH;
 static_assert(kBlock % kFPix == 0 && kFPix % 64 == 0, "items of one batch share a rank");
 
@@ -805,7 +802,7 @@ struct FusedStage {
   uint32_t ns[kFPix];
   int ranks[ALOHA_MAX_RANKS];
   int32_t rc[ALOHA_MAX_RANKS][4];  // their rectangles
-  uint16_t list[kFusedChunk * kFPix];  // live (rank, pixel) items of the chunk
+  RayRec recs[kFusedChunk * kFPix];  // the chunk's (rank, pixel) rays, set up
   int wcnt[kBlock / 64];
   int nranks, next;
 };
@@ -881,35 +878,29 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
     // a rank whose bricks the pixel's ray misses contributes exact zeros
     for (int i = t; i < items; i += kBlock) fs.col[i / kFPix][i % kFPix] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (t == 0) fs.next = 0;
-    // the items whose ray reaches one of the rank's bricks, in item order
-    const int count = compact_items(items, fs.list, fs.wcnt, [&](int item) {
+    auto store = [&](const ListRay &ray, float4 acc, float, uint32_t ns, long long) {
+      const int item = ray.tag, r = item / kFPix, idx = item - r * kFPix;
+      fs.col[r][idx] = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
+                                   (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
+      if (ns) atomicAdd(&fs.ns[idx], ns);
+    };
+    // every item's ray set up by its own thread; the rays that reach a brick
+    // of their rank go to the list (the others contribute the zeros above)
+    const int count = compact_records(items, fs.recs, fs.wcnt, [&](int item, RayState &st) {
       const int r = item / kFPix, idx = item - r * kFPix;
       const int slot = fs.ranks[c0 + r];
       int lx, ly;
       fused_pixel(idx, lx, ly);
       const int px = bx + lx, py = by + ly;
       if (px >= a.W || py >= a.H || !in_rc(fs.rc[c0 + r], px, py) || !island_pixel(a, px, py))
-        return false;
-      float d[3];
-      pixel_ray(src.dir00(), src.du(), src.dv(), px, py, d);
-      return ray_meets_rank(tb, slot, src.org(), d);
+        return;
+      begin_ray(a, tb, src.org(), src.dir00(), src.du(), src.dv(), st, ListRay{slot, px, py, item},
+                true, store);
     });
-    march_list<G, false>(
+    march_list<G, true>(
         a, tb, src.org(), src.dir00(), src.du(), src.dv(),
         [&](bool want) { return want ? atomicAdd(&fs.next, 1) : 0; }, count,
-        [&](int k) {
-          const int item = fs.list[k];
-          const int r = item / kFPix, idx = item - r * kFPix;
-          int lx, ly;
-          fused_pixel(idx, lx, ly);
-          return ListRay{fs.ranks[c0 + r], bx + lx, by + ly, item};
-        },
-        [&](const ListRay &ray, float4 acc, float, uint32_t ns, long long) {
-          const int item = ray.tag, r = item / kFPix, idx = item - r * kFPix;
-          fs.col[r][idx] = make_float4((acc.x * 1.f) * inv_spp, (acc.y * 1.f) * inv_spp,
-                                       (acc.z * 1.f) * inv_spp, (acc.w * 1.f) * inv_spp);
-          if (ns) atomicAdd(&fs.ns[idx], ns);
-        });
+        [&](int k) { return fs.recs[k]; }, store);
     __syncthreads();
     if (t < kFPix)
       for (int r = 0; r < nc; ++r) {
