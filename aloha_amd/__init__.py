@@ -310,6 +310,20 @@ class Context:
         self._check(lib().aloha_render_ranks(self._h, C.byref(params), W, H, len(ranks), offs, ids,
                                              C.byref(imgs)))
 
+    def render_frame(self, params: RenderParams, W: int, H: int, ranks: Sequence[Sequence[int]],
+                     images: RankImages, order: Sequence[int], out_rgba=None,
+                     out_rgba8=None) -> None:
+        """One frame: render_ranks into `images` (float RGBA required) and the
+        ordered 'over' of their RGBA into the outputs, as render_ranks followed
+        by composite_over with rank_rects' rectangles (bit for bit); the
+        pixels no ray reaches are written on a second stream meanwhile."""
+        offs, ids = _rank_table(ranks)
+        imgs = images.as_c()
+        o = (C.c_int32 * len(order))(*order)
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        self._check(lib().aloha_render_frame(self._h, C.byref(params), W, H, len(ranks), offs, ids,
+                                             C.byref(imgs), o, p(out_rgba), p(out_rgba8)))
+
     def render_composite(self, params: RenderParams, W: int, H: int,
                          ranks: Sequence[Sequence[int]], order: Sequence[int], out_rgba=None,
                          out_rgba8=None, nsamples=None) -> None:

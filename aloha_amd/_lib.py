@@ -104,6 +104,8 @@ _SIGS = [
     ("aloha_render_ranks", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, C.POINTER(Images)]),
     ("aloha_render_composite", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP,
                                          _P, _P, _P]),
+    ("aloha_render_frame", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP,
+                                     C.POINTER(Images), _IP, _P, _P]),
     ("aloha_tf_set_device", C.c_int, [_P, _P, _I]),
     ("aloha_brick_update", C.c_int, [_P, _I, _P, _I]),
     ("aloha_diff_forward", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _FP, _I, _IP, _P, _P]),

@@ -48,8 +48,9 @@ __global__ __launch_bounds__(kBlock) void over_kernel(const float4 *__restrict__
 // rectangle c.rect[s]).  Blending a transparent-black pixel leaves C exactly
 // unchanged (C + w * 0 = C, w >= 0), so skipping those reads gives the dense
 // result bit for bit while reading only the pixels inside the rectangles.
-// Grid: x covers a row in runs of kSparsePix pixels (each thread two pixels
-// kBlock apart), y = row.  The output goes out with streaming stores.
+// Grid: the output box [x0, x1) x [y0, y0 + gridDim.y): x covers a row in
+// runs of kSparsePix pixels (each thread two pixels kBlock apart), y = row.
+// The output goes out with streaming stores.
 constexpr int kSparsePix = 2 * kBlock;
 
 __device__ __forceinline__ void store_nt(float4 *p, float4 v) {
@@ -60,13 +61,14 @@ __device__ __forceinline__ void store_nt(float4 *p, float4 v) {
 
 __global__ __launch_bounds__(kBlock) void over_sparse_kernel(const float4 *__restrict__ imgs,
                                                              const CompositeArgs c, int W,
+                                                             int x0, int y0, int x1,
                                                              float4 *__restrict__ out,
                                                              uint32_t *__restrict__ out8) {
-  const int py = (int)blockIdx.y;
+  const int py = y0 + (int)blockIdx.y;
 #pragma unroll
   for (int h = 0; h < kSparsePix / kBlock; ++h) {
-    const int px = (int)(blockIdx.x * kSparsePix + h * kBlock + threadIdx.x);
-    if (px >= W) return;
+    const int px = x0 + (int)(blockIdx.x * kSparsePix + h * kBlock + threadIdx.x);
+    if (px >= x1) return;
     const int64_t i = px + (int64_t)W * py;
     float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int k = 0; k < c.P; ++k) {
@@ -134,9 +136,11 @@ hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, flo
 }
 
 hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs &c, int32_t W,
-                                        int32_t H, float4 *out, uint32_t *out8, hipStream_t s) {
-  const dim3 grid((unsigned)((W + kSparsePix - 1) / kSparsePix), (unsigned)H);
-  over_sparse_kernel<<<grid, kBlock, 0, s>>>(imgs, c, W, out, out8);
+                                        int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                                        float4 *out, uint32_t *out8, hipStream_t s) {
+  if (x0 >= x1 || y0 >= y1) return hipSuccess;
+  const dim3 grid((unsigned)((x1 - x0 + kSparsePix - 1) / kSparsePix), (unsigned)(y1 - y0));
+  over_sparse_kernel<<<grid, kBlock, 0, s>>>(imgs, c, W, x0, y0, x1, out, out8);
   return hipGetLastError();
 }
 

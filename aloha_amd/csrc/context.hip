@@ -413,6 +413,9 @@ int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, i
       return fail(ctx, ALOHA_ERR_LIMIT, "render: rank rectangles too large");
     q.run_off[s + 1] = (int32_t)total;
   }
+  q.frame = nullptr;
+  q.frame8 = nullptr;
+  q.box = make_int4(0, 0, 0, 0);
   // persistent grid: as many one-wave blocks as the GPU holds at once
   const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
   const size_t lds = ranks_lds_bytes(a);
@@ -448,6 +451,64 @@ extern "C" int aloha_render_ranks(aloha_ctx *ctx, const aloha_render_params *par
   int waves = 0;
   if (int rc = prepare_ranks(ctx, a, ctx->march_lanes, q, waves)) return rc;
   HIP_CHECK(ctx, launch_march_ranks(a, q, ctx->march_lanes, waves, ctx->stream));
+  return ALOHA_OK;
+}
+
+namespace {
+// validates order[] as a permutation of the P slots and copies it into c (defined below)
+int check_order(aloha_ctx *ctx, const int32_t *order, int32_t P, CompositeArgs &c);
+}  // namespace
+
+// Renderer::render (waikikiRenderer.cpp:1015-1083): the raygen launch for
+// every rank, then IceT's ordered blend of their images.  For a plain frame
+// the per-rank launch also writes the composite outputs transparent outside
+// the rectangles' bounding box (in its fill phase, which overlaps the march
+// of other waves), and the sparse composite then covers only that box.  The
+// results equal aloha_render_ranks followed by aloha_composite_over_sparse
+// over the rectangles of aloha_rank_rects, bit for bit.
+extern "C" int aloha_render_frame(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                  int32_t H, int32_t P, const int32_t *rank_offsets,
+                                  const int32_t *brick_ids, const aloha_images *images,
+                                  const int32_t *order, float *d_out_rgba,
+                                  uint32_t *d_out_rgba8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_frame: ctx is NULL");
+  if (!images || !images->rgba)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_render_frame: the rank images need float RGBA");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  if (params->accum_id > 0 && !images->accum)
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_render_frame: accum_id > 0 needs the accum buffer");
+  CompositeArgs c;
+  if (int rc = check_order(ctx, order, P, c)) return rc;
+  c.npix = (int64_t)W * H;
+  a.rgba = (float4 *)images->rgba;
+  a.rgba8 = images->rgba8;
+  a.depth = images->depth;
+  a.accum = (float4 *)images->accum;
+  a.nsamples = images->nsamples;
+  RanksArgs q;
+  int waves = 0;
+  if (int rc = prepare_ranks(ctx, a, ctx->march_lanes, q, waves)) return rc;
+  // as aloha_rank_rects: no pixel is known transparent with a heat map or
+  // accumulation (then the composite covers the whole image)
+  const bool whole = (params->flags & ALOHA_FLAG_HEATMAP) || params->accum_id > 0;
+  int x0 = W, y0 = H, x1 = 0, y1 = 0;  // bounding box of the rectangles
+  for (int s = 0; s < P; ++s) {
+    const int32_t full[4] = {0, 0, W, H};
+    for (int k = 0; k < 4; ++k) c.rect[s][k] = whole ? full[k] : a.rect[s][k];
+    const int32_t *rc = c.rect[s];
+    if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
+    x0 = std::min(x0, (int)rc[0]), y0 = std::min(y0, (int)rc[1]);
+    x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
+  }
+  if (x0 >= x1 || y0 >= y1) x0 = y0 = x1 = y1 = 0;  // no rectangle: no composite launch
+  q.frame = (float4 *)d_out_rgba;
+  q.frame8 = d_out_rgba8;
+  q.box = make_int4(x0, y0, x1, y1);
+  HIP_CHECK(ctx, launch_march_ranks(a, q, ctx->march_lanes, waves, ctx->stream));
+  HIP_CHECK(ctx, launch_composite_over_sparse((const float4 *)images->rgba, c, W, x0, y0, x1, y1,
+                                              (float4 *)d_out_rgba, d_out_rgba8, ctx->stream));
   return ALOHA_OK;
 }
 
@@ -655,7 +716,7 @@ extern "C" int aloha_composite_over_sparse(aloha_ctx *ctx, const float *d_rank_r
   for (int s = 0; s < P; ++s)
     for (int k = 0; k < 4; ++k) c.rect[s][k] = rects[4 * s + k];
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  HIP_CHECK(ctx, launch_composite_over_sparse((const float4 *)d_rank_rgba, c, W, H,
+  HIP_CHECK(ctx, launch_composite_over_sparse((const float4 *)d_rank_rgba, c, W, 0, 0, W, H,
                                               (float4 *)d_out_rgba, d_out_rgba8, ctx->stream));
   return ALOHA_OK;
 }

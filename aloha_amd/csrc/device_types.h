@@ -62,6 +62,12 @@ constexpr int kRankParts = 16;
 constexpr int kRankRun = 16;
 struct RanksArgs {
   int32_t run_off[ALOHA_MAX_RANKS + 1];
+  // A frame's composite outputs (W x H; may be NULL): the launch writes them
+  // transparent outside the box [box.x, box.z) x [box.y, box.w), which the
+  // composite then covers.
+  float4 *frame;
+  uint32_t *frame8;
+  int4 box;
 };
 
 // Fused frame tiles (single fused frame and every batched view): 16x4 pixels.
@@ -163,9 +169,12 @@ hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], floa
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
 hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, float4 *out,
                                  uint32_t *out8, hipStream_t s);
-// sparse form: reads rank pixels only inside c.rect (W x H images)
+// sparse form: reads rank pixels only inside c.rect (W x H images); writes
+// the output pixels of the box [x0, x1) x [y0, y1) only
 hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs &c, int32_t W,
-                                        int32_t H, float4 *out, uint32_t *out8, hipStream_t s);
+                                        int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                                        float4 *out, uint32_t *out8, hipStream_t s);
+
 hipError_t launch_composite_over_rgba8(const uint32_t *imgs, const CompositeArgs &c,
                                        uint32_t *out8, hipStream_t s);
 hipError_t launch_composite_zmin(const uint32_t *imgs, const float *depth, int32_t P,

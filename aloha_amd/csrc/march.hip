@@ -743,11 +743,25 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
                       [&](int k) { return recs[k]; }, store);
 
   // fill: the miss result of every pixel outside the rectangles, rows
-  // w, w + nw, ... of the P x H rank image rows
+  // w, w + nw, ... of the P x H rank image rows, then (for a frame) the H
+  // rows of the composite outputs outside the composite's box
   const bool heat = a.flags & ALOHA_FLAG_HEATMAP;
   const bool plain = !(a.flags & (ALOHA_FLAG_HEATMAP | ALOHA_FLAG_CROSSHAIR)) &&
                      !(a.accum && a.accum_id > 0);
-  for (int r = w; r < a.P * a.H; r += nw) {
+  const int rank_rows = a.P * a.H;
+  const int rows = rank_rows + (q.frame || q.frame8 ? a.H : 0);
+  for (int r = w; r < rows; r += nw) {
+    if (r >= rank_rows) {
+      const int y = r - rank_rows;
+      const bool band = y >= q.box.y && y < q.box.w;
+      for (int x = lane; x < a.W; x += 64) {
+        if (band && x >= q.box.x && x < q.box.z) continue;  // composited
+        const long long o = x + (long long)a.W * y;
+        if (q.frame) store_nt(q.frame + o, make_float4(0.f, 0.f, 0.f, 0.f));
+        if (q.frame8) store_nt(q.frame8 + o, 0u);
+      }
+      continue;
+    }
     const int s = r / a.H, y = r - s * a.H;
     const int32_t *rc = tb.rect + 4 * s;
     const bool band = rc[0] < rc[2] && y >= rc[1] && y < rc[3];

@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the per-brick render-and-composite path on one MI355X.
 
-One step = one frame: every virtual rank ray-marches its bricks into its
-float RGBA + RGBA8 + depth image (one launch for all ranks), then the ranks'
-images are 'over'-composited in kd visibility order (one launch, reading each
-rank image only inside the pixel rectangle its bricks project to -- outside
-it the image is transparent black; the dense composite is timed beside it
-and gives the same frame).  Inputs (bricks, TF, camera) are resident in HBM
-before timing starts.
+One step = one frame (aloha_render_frame): every virtual rank ray-marches
+its bricks into its float RGBA + RGBA8 + depth image, and the ranks' images
+are 'over'-composited in kd visibility order into the frame, reading each
+rank image only inside the pixel rectangle its bricks project to (outside
+it the image is transparent black).  Two launches: the per-rank launch
+(which also clears the frame outside the rectangles' bounding box) and the
+composite of that box.  The same frame as two calls (render_ranks, then the
+sparse composite of the whole image), the dense composite and the fused
+one-launch form are timed beside it, each checked equal.  Inputs (bricks, TF, camera) are resident in
+HBM before timing starts.
 
 Default workload: 512^3 float32 fBm volume, chopSuey-split into 8 bricks
 (2x2x2, ghost voxels), one brick per virtual rank, 1024x1024, dt = 0.5.
@@ -101,6 +104,8 @@ def main():
     imgs = RankImages.alloc(P, W, H, rgba=True, rgba8=True, depth=True)
     out = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
     out8 = torch.empty((H, W), dtype=torch.int32, device=dev)
+    seq = torch.empty_like(out)
+    seq8 = torch.empty_like(out8)
     log(f"[bench] {len(ids)} bricks uploaded ({ctx.brick_bytes() / 1e6:.0f} MB padded) "
         f"in {upload_s:.2f}s incl. PCIe")
 
@@ -110,8 +115,11 @@ def main():
     rects = ctx.rank_rects(params, W, H, ranks)
 
     def frame():
+        ctx.render_frame(params, W, H, ranks, imgs, wl.order, out, out8)
+
+    def two_calls():
         ctx.render_ranks(params, W, H, ranks, imgs)
-        ctx.composite_over(imgs.rgba, wl.order, W, H, out, out8, rects=rects)
+        ctx.composite_over(imgs.rgba, wl.order, W, H, seq, seq8, rects=rects)
 
     # lanes-per-ray schedule of the per-rank and fused launches, chosen by
     # timing every choice on these exact launches (untimed; results are
@@ -144,6 +152,12 @@ def main():
     torch.cuda.synchronize()
     ms_frame = timed(frame, args.steps)
     log(f"[bench] {args.steps} frames: {ms_frame:.4f} ms/frame")
+    for _ in range(args.warmup):
+        two_calls()
+    ms_two = timed(two_calls, args.steps)
+    torch.cuda.synchronize()
+    two_equal = bool(torch.equal(seq, out) and torch.equal(seq8, out8))
+    log(f"[bench] as two calls (render_ranks, composite): {ms_two:.4f} ms/frame (equal {two_equal})")
 
     # per-kernel averages over the same number of launches
     ms_render = timed(lambda: ctx.render_ranks(params, W, H, ranks, imgs), args.steps)
@@ -191,20 +205,20 @@ def main():
                        if not k.startswith("_")}
     dominant = "render" if ms_render >= ms_comp else "composite"
     roof = {
-        "kernel": f"march_persist_kernel<{schedule[0]}>" if dominant == "render" else "over_sparse_kernel",
+        "kernel": f"march_persist_kernel<{schedule[0]}, true>" if dominant == "render" else "over_sparse_kernel",
         "bound": "hbm",
         "achieved": round(render_gbs if dominant == "render" else comp_gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round((render_gbs if dominant == "render" else comp_gbs) / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get(f"march_persist_kernel<{schedule[0]}>" if dominant == "render"
+        "traffic": traffic.get(f"march_persist_kernel<{schedule[0]}, true>" if dominant == "render"
                                else "over_sparse_kernel"),
         "traffic_source": "profiles/traffic.json (rocprofv3 --pmc, calibrated)" if traffic else None,
         "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
         "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
     }
     kernels = {
-        f"march_persist_kernel<{schedule[0]}>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
+        f"march_persist_kernel<{schedule[0]}, true>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
                          "GB/s": round(render_gbs, 1), "frac_hbm": round(render_gbs / HBM_PEAK_GBS, 4),
                          "Gsamples/s": round(samples / (ms_render * 1e-3) / 1e9, 3)},
         "over_sparse_kernel": {"ms": round(ms_comp, 5), "bytes": int(comp_bytes),
@@ -251,8 +265,11 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "forms": {
-            "plain_ranks_then_composite": {"ms_per_frame": round(ms_frame, 5),
-                                           "Gsamples/s": round(gsamples, 4)},
+            "render_frame": {"ms_per_frame": round(ms_frame, 5), "Gsamples/s": round(gsamples, 4)},
+            "render_ranks_then_sparse_composite": {
+                "ms_per_frame": round(ms_two, 5),
+                "Gsamples/s": round(samples / (ms_two * 1e-3) / 1e9, 4),
+                "bit_identical": two_equal},
             "fused_render_composite": {"ms_per_frame": round(ms_fused, 5),
                                        "Gsamples/s": round(samples / (ms_fused * 1e-3) / 1e9, 4),
                                        "bit_identical_to_plain": fused_equal},

@@ -218,6 +218,18 @@ int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params *params, in
                            const int32_t *brick_ids, const int32_t *order, float *d_out_rgba,
                            uint32_t *d_out_rgba8, uint32_t *d_nsamples);
 
+/* One frame of Renderer::render (waikikiRenderer.cpp:1015-1083): the
+ * raygen launch for every rank slot into `images` (as aloha_render_ranks;
+ * float RGBA required) and IceT's ordered blend of their float RGBA images
+ * in order[] into the outputs (as aloha_composite_over_sparse over the
+ * rectangles of aloha_rank_rects) -- the same results bit for bit, in two
+ * launches: the per-rank launch also writes the outputs transparent outside
+ * the rectangles' bounding box, and the composite covers only that box. */
+int aloha_render_frame(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
+                       int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
+                       const aloha_images *images, const int32_t *order, float *d_out_rgba,
+                       uint32_t *d_out_rgba8);
+
 /* Schedule of the per-rank (aloha_render_ranks) and fused
  * (aloha_render_composite) march launches: lanes per ray, 1, 2, 4 or 8.  With
  * n > 1 the n lanes of a group evaluate n consecutive samples of one ray at

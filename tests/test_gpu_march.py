@@ -190,6 +190,52 @@ def test_sparse_composite_equals_dense(which, c0, c1):
                 assert not rgba[s][m].any()
 
 
+@pytest.mark.parametrize("which", ["c0", "c1", "odd", "crosshair", "accum"])
+def test_render_frame_equals_ranks_then_composite(which, c0, c1):
+    """render_frame (the per-rank launch also clears the outputs outside the
+    rectangles' bounding box; the composite covers only the box) gives
+    render_ranks' images and the dense composite of them bit for bit, for
+    plain frames, the crosshair and accumulation (whole-image composite; the
+    heat map's clock-based colours are not reproducible); two frames in a
+    row on the same context."""
+    import torch
+    if which == "odd":
+        raw = WL.synth_blobs_u8((72, 60, 54), 12, 0x5EED7, 3.0, 10.0, 0.3, 1.0)
+        wl = WL.make_volume_workload("blobs_odd", raw, 27, 13, 203, 117, cam_seed=0x5EED8)
+    else:
+        wl = c1 if which == "c1" else c0
+    _case(f"{wl.name} render_frame vs render_ranks + composite ({which})")
+    kw = {"crosshair": dict(flags=A.FLAG_CROSSHAIR), "accum": dict(accum_id=1, spp=2)}.get(which, {})
+    params = wl.params(**kw)
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ctx.set_march_schedule(4, 4)
+        ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
+        ctx.set_tf(wl.tf)
+        ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+        ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
+        accum = which == "accum"
+        ref = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, accum=accum, nsamples=True)
+        got = poison(A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, accum=accum, nsamples=True))
+        if accum:  # the same earlier frame in both
+            prev = torch.rand_like(ref.accum)
+            ref.accum.copy_(prev)
+            got.accum.copy_(prev)
+        ctx.render_ranks(params, wl.W, wl.H, ranks, ref)
+        dense = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
+        dense8 = torch.empty((wl.H, wl.W), dtype=torch.int32, device="cuda")
+        ctx.composite_over(ref.rgba, wl.order, wl.W, wl.H, dense, dense8)
+        out = torch.full_like(dense, float("nan"))
+        out8 = torch.full_like(dense8, -1)
+        for _ in range(2):
+            if accum:
+                got.accum.copy_(prev)
+            ctx.render_frame(params, wl.W, wl.H, ranks, got, wl.order, out, out8)
+            torch.cuda.synchronize()
+            for name in ("rgba", "rgba8", "depth", "nsamples") + (("accum",) if accum else ()):
+                assert torch.equal(getattr(got, name), getattr(ref, name)), name
+            assert torch.equal(out, dense) and torch.equal(out8, dense8)
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_lane_schedules_identical(c0, lanes):
     """Every lanes-per-ray schedule gives the 1-lane images bit for bit, with
