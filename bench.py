@@ -205,20 +205,20 @@ def main():
                        if not k.startswith("_")}
     dominant = "render" if ms_render >= ms_comp else "composite"
     roof = {
-        "kernel": f"march_persist_kernel<{schedule[0]}, true>" if dominant == "render" else "over_sparse_kernel",
+        "kernel": f"march_persist_kernel<{schedule[0]}>" if dominant == "render" else "over_sparse_kernel",
         "bound": "hbm",
         "achieved": round(render_gbs if dominant == "render" else comp_gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round((render_gbs if dominant == "render" else comp_gbs) / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get(f"march_persist_kernel<{schedule[0]}, true>" if dominant == "render"
+        "traffic": traffic.get(f"march_persist_kernel<{schedule[0]}>" if dominant == "render"
                                else "over_sparse_kernel"),
         "traffic_source": "profiles/traffic.json (rocprofv3 --pmc, calibrated)" if traffic else None,
         "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
         "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
     }
     kernels = {
-        f"march_persist_kernel<{schedule[0]}, true>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
+        f"march_persist_kernel<{schedule[0]}>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
                          "GB/s": round(render_gbs, 1), "frac_hbm": round(render_gbs / HBM_PEAK_GBS, 4),
                          "Gsamples/s": round(samples / (ms_render * 1e-3) / 1e9, 3)},
         "over_sparse_kernel": {"ms": round(ms_comp, 5), "bytes": int(comp_bytes),
