@@ -1,7 +1,10 @@
-// march.hip -- the raygen/ray-march kernel of the per-brick render path
+// march.hip -- the raygen/ray-march kernels of the per-brick render path
 // (renderFrame + integrateBrick, deviceCode.cpp:340-416 and :256-298),
-// written for gfx950: one 64-lane wavefront per 8x8 pixel tile, four tiles
-// (16x16 pixels) per 256-thread workgroup, grid.z = virtual rank slot.
+// written for gfx950: lane groups of 1-8 lanes per ray (march_list) over
+// rays set up one per lane; the per-rank launch is a persistent grid of
+// one-wave workgroups over the ranks' pixel rectangles, the fused and
+// batched launches a 256-thread workgroup per 16x4 frame tile, and the
+// footprint pass a 256-thread workgroup per 16x16 tile of one rank slot.
 //
 // Per sample the reference issues 7 trilinear texture fetches (value + 6
 // central-difference taps at +-0.5 voxel) and one TF fetch.  Here the 7 taps
