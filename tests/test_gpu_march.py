@@ -190,7 +190,7 @@ def test_sparse_composite_equals_dense(which, c0, c1):
                 assert not rgba[s][m].any()
 
 
-@pytest.mark.parametrize("which", ["c0", "c1", "odd", "crosshair", "accum"])
+@pytest.mark.parametrize("which", ["c0", "c1", "odd", "crosshair", "accum", "away"])
 def test_render_frame_equals_ranks_then_composite(which, c0, c1):
     """render_frame (the per-rank launch also clears the outputs outside the
     rectangles' bounding box; the composite covers only the box) gives
@@ -207,11 +207,14 @@ def test_render_frame_equals_ranks_then_composite(which, c0, c1):
     _case(f"{wl.name} render_frame vs render_ranks + composite ({which})")
     kw = {"crosshair": dict(flags=A.FLAG_CROSSHAIR), "accum": dict(accum_id=1, spp=2)}.get(which, {})
     params = wl.params(**kw)
+    cam = dict(wl.cam)
+    if which == "away":  # image plane turned off the volume: no rectangle, an empty box
+        cam["dir_00"] = np.asarray(cam["dir_00"]) - 2.0 * wl.W * np.asarray(cam["dir_du"])
     with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
         ctx.set_march_schedule(4, 4)
         ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
         ctx.set_tf(wl.tf)
-        ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+        ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
         ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
         accum = which == "accum"
         ref = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, accum=accum, nsamples=True)
@@ -234,6 +237,10 @@ def test_render_frame_equals_ranks_then_composite(which, c0, c1):
             for name in ("rgba", "rgba8", "depth", "nsamples") + (("accum",) if accum else ()):
                 assert torch.equal(getattr(got, name), getattr(ref, name)), name
             assert torch.equal(out, dense) and torch.equal(out8, dense8)
+        if which == "away":
+            rects = ctx.rank_rects(params, wl.W, wl.H, ranks)
+            assert all(x0 >= x1 or y0 >= y1 for x0, y0, x1, y1 in rects)
+            assert not out.any() and not out8.any()
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8])
