@@ -243,6 +243,36 @@ def test_render_frame_equals_ranks_then_composite(which, c0, c1):
             assert not out.any() and not out8.any()
 
 
+def test_render_errors(c0):
+    """Argument errors of the volume render entry points come back as
+    AlohaError (the C status and message), and the context stays usable."""
+    import torch
+    wl = c0
+    _case("render entry point errors")
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
+        ctx.set_tf(wl.tf)
+        ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+        ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
+        imgs = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H)
+        out = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
+        with pytest.raises(A.AlohaError):  # a brick id that was never uploaded
+            ctx.render_ranks(wl.params(), wl.W, wl.H, [[len(ids) + 5]], imgs)
+        with pytest.raises(A.AlohaError):  # accumulation without the accum buffer
+            ctx.render_ranks(wl.params(accum_id=1), wl.W, wl.H, ranks, imgs)
+        no_rgba = A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, rgba=False)
+        with pytest.raises(A.AlohaError):  # render_frame composites the float images
+            ctx.render_frame(wl.params(), wl.W, wl.H, ranks, no_rgba, wl.order, out)
+        with pytest.raises(A.AlohaError):  # order is not a permutation of the slots
+            ctx.render_frame(wl.params(), wl.W, wl.H, ranks, imgs, [0] * wl.num_ranks, out)
+        with pytest.raises(A.AlohaError):
+            ctx.render_frame(wl.params(spp=0), wl.W, wl.H, ranks, imgs, wl.order, out)
+        # still usable after the errors
+        ctx.render_frame(wl.params(), wl.W, wl.H, ranks, imgs, wl.order, out)
+        torch.cuda.synchronize()
+        assert torch.isfinite(out).all()
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_lane_schedules_identical(c0, lanes):
     """Every lanes-per-ray schedule gives the 1-lane images bit for bit, with
