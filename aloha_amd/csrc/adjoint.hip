@@ -20,7 +20,10 @@
 // per touched entry; dDensity with global float atomics on the brick's
 // unpadded voxel grid (clamped corners receive the weight of the clamped-away
 // neighbour, exactly as the forward reads them), summed per voxel of the
-// shared 20-voxel footprint before the atomics where it fits.
+// shared 20-voxel footprint and then per workgroup in an LDS hash before the
+// atomics.  The forward marches a ray with a lane group (4 lanes, 4 lattice
+// samples a step); the adjoint keeps one lane per ray (its reverse walk is a
+// sequential recurrence).
 #include <hip/hip_runtime.h>
 
 #include "device_types.h"
@@ -136,65 +139,149 @@ __device__ void load_tf(const DiffArgs &a, float4 *s_tf) {
     s_tf[i] = (i >= 2 && i < a.tf_n + 2) ? a.tf[i - 2] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-__global__ __launch_bounds__(kBlock) void diff_forward_kernel(const DiffArgs a) {
+// The forward with lane groups: G lanes per ray evaluate G lattice samples
+// of its segment at once (t_k depends on k alone, so the lanes need no
+// chain), then every lane of the group applies them in order with the
+// sequential recurrence and the loop test (t < t1 && alpha < .99) before
+// each: the restated march's arithmetic in its order (integrateBrick with the
+// lattice t and the opacity clamp), checked bit for bit against the oracle.  Each wave owns one 8x8 tile of
+// the 16x16 block; a group takes the tile's next pixel as soon as its ray is
+// done.  States: kIdle (no ray), kPick (closest brick hit from tmin next),
+// kMarch (stepping the segment), kOver (the tile is done).
+template <int G>
+__global__ __launch_bounds__(kBlock) void diff_forward_group_kernel(const DiffArgs a) {
+  enum : int { kIdle = 0, kPick = 1, kMarch = 2, kOver = 3 };
   extern __shared__ float4 s_tf[];
   load_tf(a, s_tf);
   __syncthreads();
   const int view = blockIdx.z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
-  const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
-  if (px >= a.W || py >= a.H) return;
-  const int pixel = px + a.W * py;
-  const size_t out = (size_t)view * a.W * a.H + pixel;
+  const int gl = lane % G, gbase = lane - gl;
+  const int tx = blockIdx.x * kTile + (wave & 1) * 8, ty = blockIdx.y * kTile + (wave >> 1) * 8;
   const float *org = a.cam[view];
-  float d[3];
-  ray_dir(a, view, px, py, d);
+  const float dt = a.dt;
+  int state = kIdle, next = 0, px = 0, py = 0;  // next: wave-uniform
+  float d[3] = {1.f, 0.f, 0.f};
   float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t ns = 0;
-  uint32_t rng = rng_seed((uint32_t)pixel, 0u);
-  float tmin = 0.f;
-  for (;;) {
-    int hit = -1;
-    float best = 0.f, bt0 = 0.f, bt1 = 0.f;
-    for (int k = 0; k < a.nb; ++k) {
-      const int bid = a.brick_ids[k];
-      const DevBrick &db = a.bricks[bid];
-      float t0, t1, rep;
-      if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, t0, t1)) continue;
-      if (t0 > tmin) rep = t0;
-      else if (t1 > tmin) rep = t1;
-      else continue;
-      if (hit < 0 || rep < best) {
-        hit = bid;
-        best = rep;
-        bt0 = t0;
-        bt1 = t1;
+  uint32_t ns = 0, rng = 0;
+  float tmin = 0.f, t1 = 0.f, j = 0.f, k0 = 0.f;
+  int kk = 0;  // the segment's next lattice step
+  BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
+  float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
+  while (__any(state != kOver)) {
+    // a group without a ray takes the tile's next pixel
+    const bool want = state == kIdle && gl == 0;
+    const uint64_t m = __ballot(want);
+    int k = next + __popcll(m & ((1ull << lane) - 1ull));
+    next += __popcll(m);
+    k = group_bcast_i<G, 0>(k, gbase);
+    if (state == kIdle) {
+      if (k >= 64) {
+        state = kOver;
+      } else {
+        px = tx + (k & 7);
+        py = ty + (k >> 3);
+        if (px < a.W && py < a.H) {
+          ray_dir(a, view, px, py, d);
+          rng = rng_seed((uint32_t)(px + a.W * py), 0u);
+          c = make_float4(0.f, 0.f, 0.f, 0.f);
+          ns = 0;
+          tmin = 0.f;
+          state = kPick;
+        }
       }
     }
-    if (hit < 0) break;
-    const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
-    const float dt = a.dt;
-    const float segs = floorf((bt0 - dt * j) / dt);
-    const float tfirst = dt * (j + segs);
-    const Visit v{hit, 0, nullptr, bt0, bt1, j, tfirst < bt0 ? segs + 1.f : segs};
-    const DevBrick db = a.bricks[hit];
-    for (int k = 0;; ++k) {
-      const float t = lattice_t(a, v, k);
-      if (!(t < bt1 && c.w < .99f)) break;
-      ++ns;
-      const Shaded r = shade_sample(a, s_tf, db, org, d, t);
-      const float w = (1.f - c.w) * r.tf.w;
-      c.x = c.x + (w * r.tf.x) * r.shade;
-      c.y = c.y + (w * r.tf.y) * r.shade;
-      c.z = c.z + (w * r.tf.z) * r.shade;
-      c.w = c.w + w;
+    if (state == kPick) {  // closest reported hit from tmin, ties to the lower slot
+      int hit = -1;
+      float best = 0.f, bt0 = 0.f, bt1 = 0.f;
+      for (int kb = 0; kb < a.nb; ++kb) {
+        const DevBrick &db = a.bricks[a.brick_ids[kb]];  // kb is wave-uniform: scalar loads
+        float s0, s1, rep;
+        if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, s0, s1)) continue;
+        if (s0 > tmin) rep = s0;
+        else if (s1 > tmin) rep = s1;
+        else continue;
+        if (hit < 0 || rep < best) {
+          hit = a.brick_ids[kb];
+          best = rep;
+          bt0 = s0;
+          bt1 = s1;
+        }
+      }
+      if (hit < 0) {
+        if (gl == 0) {
+          const size_t out = (size_t)view * a.W * a.H + px + (size_t)a.W * py;
+          a.rgba[out] = c;
+          a.nsamples[out] = ns;
+        }
+        state = kIdle;
+      } else {
+        j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
+        const float segs = floorf((bt0 - dt * j) / dt);
+        const float tfirst = dt * (j + segs);
+        k0 = tfirst < bt0 ? segs + 1.f : segs;
+        t1 = bt1;
+        kk = 0;
+        const DevBrick &db = a.bricks[hit];
+        bv = brick_view(db);
+        lo0 = db.vox_lo[0];
+        lo1 = db.vox_lo[1];
+        lo2 = db.vox_lo[2];
+        tmin = bt1 + 1e-20f;
+        state = kMarch;
+      }
     }
-    tmin = bt1 + 1e-20f;
-    if (!(c.w < .99f)) break;
+    // kMarch: lattice samples kk .. kk + G - 1 of the segment
+    const bool go = state == kMarch;
+    const float t = dt * (j + (k0 + (float)(kk + gl)));
+    float4 tf = make_float4(0.f, 0.f, 0.f, 0.f);
+    float shade = -1.f;  // < 0: no sample (an evaluated shade is >= 0.1 or NaN)
+    if (go && t < t1) {
+      const float x = (org[0] + t * d[0]) - lo0;
+      const float y = (org[1] + t * d[1]) - lo1;
+      const float z = (org[2] + t * d[2]) - lo2;
+      Sample sm;
+      if (!sample_fast(bv, x, y, z, sm)) sm = sample_exact<false>(bv, nullptr, a.bricks[0], x, y, z);
+      const float ex = sm.gx + 1e-4f, ey = sm.gy + 1e-4f, ez = sm.gz + 1e-4f;
+      const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+      const float nx = sm.gx / len, ny = sm.gy / len, nz = sm.gz / len;
+      tf = tf_lookup(s_tf, a.tf_n, sm.val);
+      tf.w = fminf(tf.w, a.alpha_max);
+      shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
+    }
+    bool live = go;
+    static_for<0, G>([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      const float tx_ = group_bcast<G, J>(tf.x, gbase), ty_ = group_bcast<G, J>(tf.y, gbase);
+      const float tz_ = group_bcast<G, J>(tf.z, gbase), tw_ = group_bcast<G, J>(tf.w, gbase);
+      const float sh = group_bcast<G, J>(shade, gbase);
+      live = live && !(sh < 0.f) && c.w < .99f;
+      if (live) {
+        ++ns;
+        const float w = (1.f - c.w) * tw_;
+        c.x = c.x + (w * tx_) * sh;
+        c.y = c.y + (w * ty_) * sh;
+        c.z = c.z + (w * tz_) * sh;
+        c.w = c.w + w;
+      }
+    });
+    if (go) {
+      kk += G;
+      const float tn = dt * (j + (k0 + (float)kk));
+      if (!(tn < t1 && c.w < .99f)) {  // segment over
+        if (c.w < .99f) {
+          state = kPick;
+        } else {
+          if (gl == 0) {
+            const size_t out = (size_t)view * a.W * a.H + px + (size_t)a.W * py;
+            a.rgba[out] = c;
+            a.nsamples[out] = ns;
+          }
+          state = kIdle;
+        }
+      }
+    }
   }
-  a.rgba[out] = c;
-  a.nsamples[out] = ns;
 }
 
 // Where a brick visit's density gradient goes.  Global float atomics run at
@@ -543,7 +630,7 @@ hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s) {
                                (sizeof(uint32_t) + sizeof(float)) * (size_t)kCacheSlots,
                            s>>>(a);
   else
-    diff_forward_kernel<<<grid, kBlock, tf_lds, s>>>(a);
+    diff_forward_group_kernel<4><<<grid, kBlock, tf_lds, s>>>(a);
   return hipGetLastError();
 }
 

@@ -128,36 +128,6 @@ __device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *s
   return MarchTables{smem, sb, roff, rect};
 }
 
-// Compile-time loop: f(std::integral_constant<int, J>) for J in [J0, N).
-template <int J0, int N, class F>
-__device__ __forceinline__ void static_for(const F &f) {
-  if constexpr (J0 < N) {
-    f(std::integral_constant<int, J0>{});
-    static_for<J0 + 1, N>(f);
-  }
-}
-
-// Value of lane gbase + J of the caller's lane group (groups of G aligned
-// lanes), on every lane of the group.  Pairs and quads use a DPP quad
-// permutation (one VALU move, no LDS round trip); groups of 8 use
-// ds_bpermute.  Every lane of the wave must be active.
-template <int G, int J>
-__device__ __forceinline__ float group_bcast(float v, int gbase) {
-  if constexpr (G == 1) {
-    return v;
-  } else if constexpr (G == 2 || G == 4) {
-    constexpr int ctrl = G == 4 ? J * 0x55 : (J | (J << 2) | ((2 + J) << 4) | ((2 + J) << 6));
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
-  } else {
-    return __shfl(v, gbase + J);
-  }
-}
-
-template <int G, int J>
-__device__ __forceinline__ int group_bcast_i(int v, int gbase) {
-  return __float_as_int(group_bcast<G, J>(__int_as_float(v), gbase));
-}
-
 // One ray of a work list: rank slot (-1: the ray can reach no brick, its
 // result is the miss result), pixel, and the caller's tag for it.
 struct ListRay {
