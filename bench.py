@@ -171,6 +171,15 @@ def main():
     sparse_equal = bool(torch.equal(dense, out) and torch.equal(dense8, out8))
     log(f"[bench] render {ms_render:.4f} ms, composite {ms_comp:.4f} ms (sparse; dense "
         f"{ms_dense:.4f} ms, equal {sparse_equal})")
+    # the reference's own data path: the same rank images, IceT's ordered blend
+    # of their RGBA8 (ubyte colour format) -- 8-bit, so its own result
+    out8b = torch.empty_like(out8)
+    rgba8_frame = lambda: (ctx.render_ranks(params, W, H, ranks, imgs),
+                           ctx.composite_over_rgba8(imgs.rgba8, wl.order, W, H, out8b))
+    for _ in range(args.warmup):
+        rgba8_frame()
+    ms_rgba8 = timed(rgba8_frame, args.steps)
+    log(f"[bench] RGBA8 composite form (render_ranks + 8-bit ordered blend): {ms_rgba8:.4f} ms/frame")
     # fused form: the same frame (bit-identical result) in one launch
     fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
     for _ in range(args.warmup):
@@ -270,6 +279,10 @@ def main():
                 "ms_per_frame": round(ms_two, 5),
                 "Gsamples/s": round(samples / (ms_two * 1e-3) / 1e9, 4),
                 "bit_identical": two_equal},
+            "render_ranks_then_rgba8_composite": {
+                "ms_per_frame": round(ms_rgba8, 5),
+                "Gsamples/s": round(samples / (ms_rgba8 * 1e-3) / 1e9, 4),
+                "note": "IceT's ubyte path: the ranks' RGBA8 images blended in 8 bits"},
             "fused_render_composite": {"ms_per_frame": round(ms_fused, 5),
                                        "Gsamples/s": round(samples / (ms_fused * 1e-3) / 1e9, 4),
                                        "bit_identical_to_plain": fused_equal},
