@@ -4,8 +4,9 @@
 //
 // Layout: one 64-lane wavefront per 8x8 pixel tile, 2x2 tiles per 256-thread
 // workgroup, grid.z = rank slot (per-rank images).  Each lane keeps its
-// traversal stack in LDS (32 entries x 4 bytes, lane-interleaved so that the
-// 64 lanes of a push or pop hit 64 different banks); the BVH depth bound
+// traversal stack in LDS (4-byte entries, lane-interleaved so that the 64
+// lanes of a push or pop hit 64 different banks; the per-rank launch keeps
+// the first 16 there and deeper ones in private memory); the BVH depth bound
 // of the builder makes 32 entries always enough.  A node fetch is 64 bytes
 // (both children's boxes) and a triangle 48 bytes.
 //
@@ -34,6 +35,12 @@ constexpr int kTileW = 16, kTileH = 16;
 constexpr int kLeafBits = 3;
 constexpr int32_t kEmptyRoot = INT32_MIN;
 constexpr float kTMax = 1e10f;
+// LDS entries of the per-rank launch's traversal stack: 16 (16 KB per
+// workgroup) lets 8 waves share a SIMD instead of 5 with all 32 in LDS
+// (2.95 vs 3.42 ms on the 9.75 M-triangle scene); the rare deeper entries go
+// to private memory.  The fused launch keeps all 32 in LDS (measured faster
+// there).
+constexpr int kRankStackLds = 16;
 // slack on the far end of each slab interval: covers the rounding of
 // (plane - org) * (1/dir) (a few ulps) so that culling stays conservative
 constexpr float kSlabSlack = 1.000001f;
@@ -118,14 +125,18 @@ __device__ __forceinline__ bool slab(const Ray &r, float x0, float x1, float y0,
 
 // Closest hit of one geometry below best_t (in: the limit, out: the hit t).
 // Returns the hit's triangle slot, or -1.  stk: this lane's LDS stack
-// (stride kBlock).  nn / nt count node fetches and triangle tests.
-template <bool kStats>
+// (stride kBlock) of kLds entries; entries kLds .. kTriStack-1 of deeper
+// walks go to a private (scratch) array.  nn / nt count node fetches and
+// triangle tests.
+template <bool kStats, int kLds>
 __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best_t,
                                      int32_t *stk, uint32_t &nn, uint32_t &nt) {
+  static_assert(kLds >= 1 && kLds <= kTriStack, "LDS part of the traversal stack");
   int best_slot = -1, best_prim = -1;
   int32_t ref = g.root;
   if (ref == kEmptyRoot) return -1;
   int sp = 0;
+  int32_t deep[kTriStack - kLds > 0 ? kTriStack - kLds : 1];
   while (true) {
     if (ref >= 0) {
       const float4 *n = g.nodes + 4 * (int64_t)ref;
@@ -137,7 +148,9 @@ __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best
       const bool hr = slab(r, rxy.x, rxy.y, rxy.z, rxy.w, z.z, z.w, best_t, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
-        stk[sp * kBlock] = left_first ? kids.y : kids.x;
+        const int32_t far = left_first ? kids.y : kids.x;
+        if (kLds == kTriStack || sp < kLds) stk[sp * kBlock] = far;
+        else deep[sp - kLds] = far;
         ++sp;
         ref = left_first ? kids.x : kids.y;
         continue;
@@ -168,7 +181,7 @@ __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best
     }
     if (sp == 0) break;
     --sp;
-    ref = stk[sp * kBlock];
+    ref = (kLds == kTriStack || sp < kLds) ? stk[sp * kBlock] : deep[sp - kLds];
   }
   return best_slot;
 }
@@ -217,7 +230,7 @@ __device__ __forceinline__ uint32_t pack_rgba(const float4 &q, float s) {
 // Per-rank images: slot = blockIdx.z traces geometry a.geo[slot].
 template <bool kStats>
 __global__ __launch_bounds__(kBlock) void tris_rank_kernel(const TrisArgs a) {
-  __shared__ int32_t s_stack[kTriStack * kBlock];
+  __shared__ int32_t s_stack[kRankStackLds * kBlock];
   const int slot = blockIdx.z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = blockIdx.x * kTileW + (wave & 1) * 8 + (lane & 7);
@@ -236,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void tris_rank_kernel(const TrisArgs a) {
     Ray r;
     make_ray(a, px, py, r);
     float t = kTMax;
-    const int hit = trace<kStats>(a.geo[slot], r, t, s_stack + threadIdx.x, nn, nt);
+    const int hit = trace<kStats, kRankStackLds>(a.geo[slot], r, t, s_stack + threadIdx.x, nn, nt);
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
     if (hit >= 0) {
       c = shade(a.geo[slot], hit, r);
@@ -310,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void tris_composite_kernel(const TrisArgs a
     float t_win = kTMax, t_lim = kTMax;
     for (int k = 0; k < a.P; ++k) {
       float t = t_lim;
-      const int s = trace<kStats>(a.geo[k], r, t, s_stack + threadIdx.x, nn, nt);
+      const int s = trace<kStats, kTriStack>(a.geo[k], r, t, s_stack + threadIdx.x, nn, nt);
       const float zk = s >= 0 ? t / a.max_depth : 1.f;
       if (k == 0 || zk < z_best) {
         winner = k;
