@@ -124,11 +124,18 @@ def test_many_ranks_odd_image(lanes):
         assert torch.equal(out, ref)
 
 
-@pytest.mark.parametrize("which", ["c0", "c1"])
+@pytest.mark.parametrize("which", ["c0", "c1", "c0spp", "c1one", "c0isl"])
 def test_fused_render_composite_bit_exact(which, c0, c1):
+    """The fused launch (rays set up per thread into LDS records, lane groups,
+    in-tile blend) equals render_ranks + composite bit for bit, also with
+    spp > 1 (records that start in a later spp sample), one rank over all
+    bricks (re-selection inside the march) and the island mask."""
     import torch
-    wl = c0 if which == "c0" else c1
-    _case(f"{wl.name} fused render+composite vs ranks+composite")
+    wl = c1 if which.startswith("c1") else c0
+    _case(f"{wl.name} fused render+composite vs ranks+composite ({which})")
+    kw = {"c0spp": dict(spp=3), "c1one": dict(spp=2),
+          "c0isl": dict(spp=2, num_islands=3, island_id=1)}.get(which, {})
+    params = wl.params(**kw)
     out = torch.empty((wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
     out8 = torch.empty((wl.H, wl.W), dtype=torch.int32, device="cuda")
     ns = torch.empty((wl.H, wl.W), dtype=torch.int32, device="cuda")
@@ -139,10 +146,13 @@ def test_fused_render_composite_bit_exact(which, c0, c1):
         ctx.set_tf(wl.tf)
         ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
         ranks = [[ids[i] for i in r] for r in wl.rank_bricks]
-        imgs = poison(A.RankImages.alloc(wl.num_ranks, wl.W, wl.H, nsamples=True))
-        ctx.render_ranks(wl.params(), wl.W, wl.H, ranks, imgs)
-        ctx.composite_over(imgs.rgba, wl.order, wl.W, wl.H, ref, ref8)
-        ctx.render_composite(wl.params(), wl.W, wl.H, ranks, wl.order, out, out8, ns)
+        order = wl.order
+        if which == "c1one":  # one rank marching every brick
+            ranks, order = [ids], [0]
+        imgs = poison(A.RankImages.alloc(len(ranks), wl.W, wl.H, nsamples=True))
+        ctx.render_ranks(params, wl.W, wl.H, ranks, imgs)
+        ctx.composite_over(imgs.rgba, order, wl.W, wl.H, ref, ref8)
+        ctx.render_composite(params, wl.W, wl.H, ranks, order, out, out8, ns)
         torch.cuda.synchronize()
         assert torch.equal(out, ref)
         assert torch.equal(out8, ref8)
