@@ -65,6 +65,7 @@ class Partitioner:
 
     def __init__(self, handle: C.c_void_p, num_ranks: int):
         self._h = handle
+        self._free = lib().aloha_partitioner_destroy  # held: module globals are gone at exit
         self.num_ranks = num_ranks
 
     @classmethod
@@ -82,7 +83,7 @@ class Partitioner:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().aloha_partitioner_destroy(self._h)
+            self._free(self._h)
             self._h = None
 
     def rank_bricks(self, rank: int) -> list[int]:
@@ -245,13 +246,14 @@ class Context:
         h = C.c_void_p()
         check(lib().aloha_ctx_create(int(device), C.byref(h)))
         self._h = h
+        self._free = lib().aloha_ctx_destroy  # held: module globals are gone at exit
         self.device = device
         if stream is not None:
             self.set_stream(stream)
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().aloha_ctx_destroy(self._h)
+            self._free(self._h)
             self._h = None
 
     __del__ = close
@@ -475,6 +477,7 @@ class Mesh:
 
     def __init__(self, handle: C.c_void_p):
         self._h = handle
+        self._free = lib().aloha_mesh_destroy  # held: module globals are gone at exit
 
     @classmethod
     def from_arrays(cls, verts: np.ndarray, tris: np.ndarray) -> "Mesh":
@@ -509,7 +512,7 @@ class Mesh:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().aloha_mesh_destroy(self._h)
+            self._free(self._h)
             self._h = None
 
     __del__ = close
@@ -595,11 +598,12 @@ class Batch:
                                             offs, ids, ords.ctypes.data_as(C.POINTER(C.c_int32)),
                                             p(out_rgba), p(out_rgba8), p(nsamples), C.byref(h)))
         self._h = h
+        self._free = lib().aloha_batch_destroy  # held: module globals are gone at exit
         self._keep = (out_rgba, out_rgba8, nsamples)
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().aloha_batch_destroy(self._h)
+            self._free(self._h)
             self._h = None
 
     __del__ = close
