@@ -83,3 +83,37 @@ def test_batch_view_matches_oracle(wl):
     assert ref[..., 3].max() > 0.3
     b.close()
     ctx.close()
+
+
+def test_batch_every_output_spp_islands(wl):
+    """RGBA, RGBA8 and sample counts of every view, with two samples per
+    pixel and the island mask, equal the single-view fused frames bit for bit
+    under every lanes-per-ray schedule."""
+    import torch
+    print("\n[case] batch 8 bricks x 4 views, spp 2, island 1 of 2, all outputs", flush=True)
+    ctx, ranks = _setup(wl, torch)
+    prm = wl.params(spp=2, num_islands=2, island_id=1)
+    V, H, W = len(wl.cams), wl.H, wl.W
+    ref = {"rgba": torch.empty((V, H, W, 4), dtype=torch.float32, device="cuda"),
+           "rgba8": torch.empty((V, H, W), dtype=torch.int32, device="cuda"),
+           "ns": torch.empty((V, H, W), dtype=torch.int32, device="cuda")}
+    for v, (cam, order) in enumerate(zip(wl.cams, wl.orders)):
+        ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+        ctx.render_composite(prm, W, H, ranks, order, ref["rgba"][v], ref["rgba8"][v],
+                             ref["ns"][v])
+    torch.cuda.synchronize()
+    assert int(ref["ns"].sum()) > 0 and float(ref["rgba"][..., 3].max()) > 0.3
+    got = {k: torch.empty_like(t) for k, t in ref.items()}
+    b = A.Batch(ctx, prm, W, H, wl.cams, ranks, wl.orders, out_rgba=got["rgba"],
+                out_rgba8=got["rgba8"], nsamples=got["ns"])
+    for lanes in (1, 2, 4, 8):
+        for blocks in (0, 64):
+            for t in got.values():
+                t.fill_(-1)
+            b.set_schedule(lanes, blocks)
+            b.launch()
+            torch.cuda.synchronize()
+            for k in ref:
+                assert torch.equal(got[k], ref[k]), (k, lanes, blocks)
+    b.close()
+    ctx.close()
