@@ -337,3 +337,46 @@ def test_lane_schedules_brick_loop_and_fused(c1):
                                                outs[0][0], outs[0][1], reps=2)
         print("autotune", r_l, f_l, table)
         assert r_l in (1, 2, 4, 8) and f_l in (1, 2, 4, 8) and len(table) == 4
+
+
+def test_empty_rank_slots(c0):
+    """Rank slots without bricks (more ranks than bricks) render transparent
+    -- depth 1, no samples -- in the per-rank launch; the other slots equal
+    the launch without the empty ones bit for bit; render_frame and the
+    fused launch still equal the dense composite of the rank images."""
+    import torch
+    _case("blobs128: 4 rank slots, two of them empty, order 3 0 2 1")
+    W, H = c0.W, c0.H
+    params = c0.params()
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ctx.set_march_schedule(4, 4)
+        ids = [ctx.upload_brick(d, v) for d, v in zip(c0.bricks, c0.voxels)]
+        ctx.set_tf(c0.tf)
+        ctx.set_camera(c0.cam["org"], c0.cam["dir_00"], c0.cam["dir_du"], c0.cam["dir_dv"])
+        ranks = [[ids[0]], [], [ids[1]], []]
+        order = [3, 0, 2, 1]
+        two = A.RankImages.alloc(2, W, H, nsamples=True)
+        ctx.render_ranks(params, W, H, [[ids[0]], [ids[1]]], two)
+        ref = poison(A.RankImages.alloc(4, W, H, nsamples=True))
+        ctx.render_ranks(params, W, H, ranks, ref)
+        dense = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        dense8 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        ctx.composite_over(ref.rgba, order, W, H, dense, dense8)
+        got = poison(A.RankImages.alloc(4, W, H, nsamples=True))
+        out = torch.full_like(dense, float("nan"))
+        out8 = torch.full_like(dense8, -1)
+        ctx.render_frame(params, W, H, ranks, got, order, out, out8)
+        fout = torch.full_like(dense, float("nan"))
+        fout8 = torch.full_like(dense8, -1)
+        ctx.render_composite(params, W, H, ranks, order, fout, fout8)
+        torch.cuda.synchronize()
+        for s in (1, 3):
+            assert not ref.rgba[s].any() and not ref.rgba8[s].any() and not ref.nsamples[s].any()
+            assert bool((ref.depth[s] == 1.0).all())
+        for name in ("rgba", "rgba8", "depth", "nsamples"):
+            r, t = getattr(ref, name), getattr(two, name)
+            assert torch.equal(r[0], t[0]) and torch.equal(r[2], t[1]), name
+            assert torch.equal(getattr(got, name), r), name
+        assert int(ref.nsamples.sum()) > 0
+        assert torch.equal(out, dense) and torch.equal(out8, dense8)
+        assert torch.equal(fout, dense) and torch.equal(fout8, dense8)
