@@ -117,3 +117,36 @@ def test_batch_every_output_spp_islands(wl):
                 assert torch.equal(got[k], ref[k]), (k, lanes, blocks)
     b.close()
     ctx.close()
+
+
+def test_batch_multi_brick_and_empty_ranks(wl):
+    """Rank slots of several bricks (the brick loop inside a slot) and an
+    empty slot, with a different visibility order per view: every view
+    equals its single-view fused frame bit for bit, persistent or not."""
+    import torch
+    print("\n[case] batch 8 bricks as slots of 3 / 0 / 5 bricks x 4 views", flush=True)
+    ctx, ranks1 = _setup(wl, torch)
+    ids = [i for r in ranks1 for i in r]
+    assert len(ids) == 8
+    ranks = [ids[:3], [], ids[3:]]
+    orders = [[0, 1, 2], [2, 0, 1], [1, 2, 0], [2, 1, 0]][:len(wl.cams)]
+    V, H, W = len(wl.cams), wl.H, wl.W
+    ref = torch.empty((V, H, W, 4), dtype=torch.float32, device="cuda")
+    ref_ns = torch.empty((V, H, W), dtype=torch.int32, device="cuda")
+    for v, (cam, order) in enumerate(zip(wl.cams, orders)):
+        ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+        ctx.render_composite(wl.params(), W, H, ranks, order, ref[v], None, ref_ns[v])
+    torch.cuda.synchronize()
+    assert float(ref[..., 3].max()) > 0.3
+    out = torch.empty_like(ref)
+    ns = torch.empty_like(ref_ns)
+    b = A.Batch(ctx, wl.params(), W, H, wl.cams, ranks, orders, out_rgba=out, nsamples=ns)
+    for lanes, blocks in ((1, 0), (4, 0), (4, 64), (8, 8)):
+        out.fill_(float("nan"))
+        ns.fill_(-1)
+        b.set_schedule(lanes, blocks)
+        b.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref) and torch.equal(ns, ref_ns), (lanes, blocks)
+    b.close()
+    ctx.close()
