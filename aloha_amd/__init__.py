@@ -196,6 +196,22 @@ def max_depth(bounds: Box3f, org) -> float:
 # ----------------------------------------------------------------------------
 # device context
 # ----------------------------------------------------------------------------
+def _dev(t, nbytes: int, name: str):
+    """Device pointer of a contiguous GPU tensor of at least nbytes (None
+    passes through).  The kernels write through raw pointers, so a host,
+    strided or short buffer is refused here rather than faulting the GPU."""
+    if t is None:
+        return None
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name}: expected a GPU tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise ValueError(f"{name}: {have} bytes, the call writes or reads {nbytes}")
+    return C.c_void_p(t.data_ptr())
+
+
 @dataclass
 class RankImages:
     """Device tensors of one render launch (P rank slots); any may be None."""
@@ -205,10 +221,12 @@ class RankImages:
     accum: object = None
     nsamples: object = None
 
-    def as_c(self) -> Images:
-        def p(t):
-            return C.c_void_p(t.data_ptr()) if t is not None else None
-        return Images(p(self.rgba), p(self.rgba8), p(self.depth), p(self.accum), p(self.nsamples))
+    def as_c(self, P: int, W: int, H: int) -> Images:
+        """The C struct for a launch of P rank slots at W x H (sizes checked)."""
+        n = max(0, P) * max(0, W) * max(0, H)
+        return Images(_dev(self.rgba, 16 * n, "rgba"), _dev(self.rgba8, 4 * n, "rgba8"),
+                      _dev(self.depth, 4 * n, "depth"), _dev(self.accum, 16 * n, "accum"),
+                      _dev(self.nsamples, 4 * n, "nsamples"))
 
     @classmethod
     def alloc(cls, P: int, W: int, H: int, device="cuda", rgba=True, rgba8=True, depth=True,
@@ -308,7 +326,7 @@ class Context:
     def render_ranks(self, params: RenderParams, W: int, H: int,
                      ranks: Sequence[Sequence[int]], out: RankImages) -> None:
         offs, ids = _rank_table(ranks)
-        imgs = out.as_c()
+        imgs = out.as_c(len(ranks), W, H)
         self._check(lib().aloha_render_ranks(self._h, C.byref(params), W, H, len(ranks), offs, ids,
                                              C.byref(imgs)))
 
@@ -317,14 +335,16 @@ class Context:
                      out_rgba8=None) -> None:
         """One frame: render_ranks into `images` (float RGBA required) and the
         ordered 'over' of their RGBA into the outputs, as render_ranks followed
-        by composite_over with rank_rects' rectangles (bit for bit); the
-        pixels no ray reaches are written on a second stream meanwhile."""
+        by composite_over with rank_rects' rectangles (bit for bit), in two
+        launches: the per-rank launch also clears the outputs outside the
+        rectangles' bounding box, the composite covers the box."""
         offs, ids = _rank_table(ranks)
-        imgs = images.as_c()
+        imgs = images.as_c(len(ranks), W, H)
         o = (C.c_int32 * len(order))(*order)
-        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        n = max(0, W) * max(0, H)
         self._check(lib().aloha_render_frame(self._h, C.byref(params), W, H, len(ranks), offs, ids,
-                                             C.byref(imgs), o, p(out_rgba), p(out_rgba8)))
+                                             C.byref(imgs), o, _dev(out_rgba, 16 * n, "out_rgba"),
+                                             _dev(out_rgba8, 4 * n, "out_rgba8")))
 
     def render_composite(self, params: RenderParams, W: int, H: int,
                          ranks: Sequence[Sequence[int]], order: Sequence[int], out_rgba=None,
@@ -332,9 +352,11 @@ class Context:
         """Fused raygen + ordered 'over' composite (one launch)."""
         offs, ids = _rank_table(ranks)
         o = (C.c_int32 * len(order))(*order)
-        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        n = max(0, W) * max(0, H)
         self._check(lib().aloha_render_composite(self._h, C.byref(params), W, H, len(ranks), offs,
-                                                 ids, o, p(out_rgba), p(out_rgba8), p(nsamples)))
+                                                 ids, o, _dev(out_rgba, 16 * n, "out_rgba"),
+                                                 _dev(out_rgba8, 4 * n, "out_rgba8"),
+                                                 _dev(nsamples, 4 * n, "nsamples")))
 
     def set_march_schedule(self, ranks_lanes: int, fused_lanes: int) -> None:
         self._check(lib().aloha_set_march_schedule(self._h, ranks_lanes, fused_lanes))
@@ -351,12 +373,13 @@ class Context:
         the chosen (ranks_lanes, fused_lanes) and the timing table."""
         offs, ids = _rank_table(ranks)
         o = (C.c_int32 * len(order))(*order) if order is not None else None
-        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
-        imgs = out.as_c()
+        imgs = out.as_c(len(ranks), W, H)
+        n = max(0, W) * max(0, H)
         table = (C.c_float * 12)()
         rows = C.c_int32()
         self._check(lib().aloha_autotune_march(self._h, C.byref(params), W, H, len(ranks), offs, ids,
-                                               o, C.byref(imgs), p(out_rgba), p(out_rgba8), reps,
+                                               o, C.byref(imgs), _dev(out_rgba, 16 * n, "out_rgba"),
+                                               _dev(out_rgba8, 4 * n, "out_rgba8"), reps,
                                                table, C.byref(rows)))
         t = [(int(table[3 * i]), float(table[3 * i + 1]), float(table[3 * i + 2]))
              for i in range(rows.value)]
@@ -376,18 +399,18 @@ class Context:
         from rank_rects): the sparse form, reading each rank only inside its
         rectangle (the images must be transparent outside it)."""
         o = (C.c_int32 * len(order))(*order)
-        outs = (C.c_void_p(out_rgba.data_ptr()) if out_rgba is not None else None,
-                C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None)
+        n = max(0, W) * max(0, H)
+        outs = (_dev(out_rgba, 16 * n, "out_rgba"), _dev(out_rgba8, 4 * n, "out_rgba8"))
+        src = _dev(rank_rgba, 16 * n * len(order), "rank_rgba")
         if rects is None:
-            self._check(lib().aloha_composite_over(
-                self._h, C.c_void_p(rank_rgba.data_ptr()), len(order), o, W, H, *outs))
+            self._check(lib().aloha_composite_over(self._h, src, len(order), o, W, H, *outs))
         else:
             flat = [int(v) for r in rects for v in r]
             if len(flat) != 4 * len(order):
                 raise ValueError("rects: need 4 integers per rank")
             rc = (C.c_int32 * len(flat))(*flat)
-            self._check(lib().aloha_composite_over_sparse(
-                self._h, C.c_void_p(rank_rgba.data_ptr()), len(order), o, rc, W, H, *outs))
+            self._check(lib().aloha_composite_over_sparse(self._h, src, len(order), o, rc, W, H,
+                                                          *outs))
 
     def rank_rects(self, params: RenderParams, W: int, H: int,
                    ranks: Sequence[Sequence[int]]) -> list[tuple[int, int, int, int]]:
@@ -402,16 +425,18 @@ class Context:
     def composite_over_rgba8(self, rank_rgba8, order: Sequence[int], W: int, H: int,
                              out_rgba8) -> None:
         o = (C.c_int32 * len(order))(*order)
+        n = max(0, W) * max(0, H)
         self._check(lib().aloha_composite_over_rgba8(
-            self._h, C.c_void_p(rank_rgba8.data_ptr()), len(order), o, W, H,
-            C.c_void_p(out_rgba8.data_ptr())))
+            self._h, _dev(rank_rgba8, 4 * n * len(order), "rank_rgba8"), len(order), o, W, H,
+            _dev(out_rgba8, 4 * n, "out_rgba8")))
 
     def composite_zmin(self, rank_rgba8, rank_depth, P: int, W: int, H: int, out_rgba8=None,
                        out_depth=None) -> None:
+        n = max(0, W) * max(0, H)
         self._check(lib().aloha_composite_zmin(
-            self._h, C.c_void_p(rank_rgba8.data_ptr()), C.c_void_p(rank_depth.data_ptr()), P, W, H,
-            C.c_void_p(out_rgba8.data_ptr()) if out_rgba8 is not None else None,
-            C.c_void_p(out_depth.data_ptr()) if out_depth is not None else None))
+            self._h, _dev(rank_rgba8, 4 * n * max(0, P), "rank_rgba8"),
+            _dev(rank_depth, 4 * n * max(0, P), "rank_depth"), P, W, H,
+            _dev(out_rgba8, 4 * n, "out_rgba8"), _dev(out_depth, 4 * n, "out_depth")))
 
 
     # ---- surface path ---------------------------------------------------
@@ -448,17 +473,20 @@ class Context:
     def render_tris(self, params: RenderParams, W: int, H: int, geoms: Sequence[int],
                     out: RankImages, stats=None) -> None:
         g = (C.c_int32 * len(geoms))(*geoms)
-        imgs = out.as_c()
+        imgs = out.as_c(len(geoms), W, H)
         self._check(lib().aloha_render_tris(self._h, C.byref(params), W, H, len(geoms), g,
                                             C.byref(imgs),
-                                            C.c_void_p(stats.data_ptr()) if stats is not None else None))
+                                            _dev(stats, 8 * len(geoms) * max(0, W) * max(0, H),
+                                                 "stats")))
 
     def render_tris_composite(self, params: RenderParams, W: int, H: int, geoms: Sequence[int],
                               out_rgba8=None, out_depth=None, stats=None) -> None:
         g = (C.c_int32 * len(geoms))(*geoms)
-        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        n = max(0, W) * max(0, H)
         self._check(lib().aloha_render_tris_composite(self._h, C.byref(params), W, H, len(geoms), g,
-                                                      p(out_rgba8), p(out_depth), p(stats)))
+                                                      _dev(out_rgba8, 4 * n, "out_rgba8"),
+                                                      _dev(out_depth, 4 * n, "out_depth"),
+                                                      _dev(stats, 8 * n, "stats")))
 
 
 # ----------------------------------------------------------------------------
@@ -591,12 +619,14 @@ class Batch:
                               for c in cams]).astype(np.float32)
         offs, ids = _rank_table(ranks)
         ords = np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(V, len(ranks)))
-        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        n = V * max(0, W) * max(0, H)
         h = C.c_void_p()
         ctx._check(lib().aloha_batch_create(ctx._h, C.byref(params), W, H, V,
                                             cam.ctypes.data_as(C.POINTER(C.c_float)), len(ranks),
                                             offs, ids, ords.ctypes.data_as(C.POINTER(C.c_int32)),
-                                            p(out_rgba), p(out_rgba8), p(nsamples), C.byref(h)))
+                                            _dev(out_rgba, 16 * n, "out_rgba"),
+                                            _dev(out_rgba8, 4 * n, "out_rgba8"),
+                                            _dev(nsamples, 4 * n, "nsamples"), C.byref(h)))
         self._h = h
         self._free = lib().aloha_batch_destroy  # held: module globals are gone at exit
         self._keep = (out_rgba, out_rgba8, nsamples)

@@ -255,7 +255,9 @@ def test_render_frame_equals_ranks_then_composite(which, c0, c1):
 
 def test_render_errors(c0):
     """Argument errors of the volume render entry points come back as
-    AlohaError (the C status and message), and the context stays usable."""
+    AlohaError (the C status and message), output buffers that are host,
+    strided or too small as ValueError from the Python layer, and the
+    context stays usable."""
     import torch
     wl = c0
     _case("render entry point errors")
@@ -277,6 +279,13 @@ def test_render_errors(c0):
             ctx.render_frame(wl.params(), wl.W, wl.H, ranks, imgs, [0] * wl.num_ranks, out)
         with pytest.raises(A.AlohaError):
             ctx.render_frame(wl.params(spp=0), wl.W, wl.H, ranks, imgs, wl.order, out)
+        with pytest.raises(ValueError):  # an output smaller than W x H: refused before launch
+            ctx.render_frame(wl.params(), wl.W, wl.H, ranks, imgs, wl.order, out[:-1])
+        with pytest.raises(ValueError):  # rank images for fewer slots than the table has
+            ctx.render_ranks(wl.params(), wl.W, wl.H, ranks + [[]],
+                             A.RankImages.alloc(wl.num_ranks, wl.W, wl.H))
+        with pytest.raises(ValueError):  # a host tensor
+            ctx.render_composite(wl.params(), wl.W, wl.H, ranks, wl.order, out.cpu())
         # still usable after the errors
         ctx.render_frame(wl.params(), wl.W, wl.H, ranks, imgs, wl.order, out)
         torch.cuda.synchronize()
