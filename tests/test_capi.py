@@ -96,3 +96,15 @@ def test_max_depth_formula():
     b = A.Box3f((0, 0, 0), (2, 2, 2))
     md = A.max_depth(b, [3.0, 0.0, 0.0])
     assert abs(md - (np.linalg.norm([4, 1, 1]) + np.linalg.norm([3, 3, 3]))) < 1e-5
+
+
+def test_device_buffer_checks_on_host():
+    """The Python layer passes None through and refuses host tensors before
+    any call reaches the C ABI (the kernels write through raw pointers)."""
+    import torch
+    assert A._dev(None, 1 << 20, "out") is None
+    with pytest.raises(ValueError, match="GPU tensor"):
+        A._dev(torch.zeros(16), 4, "out")
+    with pytest.raises(ValueError, match="GPU tensor"):
+        A.RankImages(rgba=torch.zeros((1, 2, 2, 4))).as_c(1, 2, 2)
+    assert A.RankImages().as_c(4, 8, 8).rgba is None
