@@ -59,6 +59,7 @@ class DifferentiableVolume:
     def __init__(self, shape_zyx, cams, W: int, H: int, dt: float = 0.5,
                  jitter_value: float | None = 0.5):
         self.ctx = Context(0, stream=torch.cuda.current_stream())
+        self.shape = tuple(int(v) for v in shape_zyx)
         self.desc = _box_brick(shape_zyx)
         self.brick = self.ctx.upload_brick(self.desc, np.zeros(shape_zyx, np.float32))
         self.V = len(cams)
@@ -82,6 +83,13 @@ class DifferentiableVolume:
             return self._forward(tf, density)
 
     def _set_inputs(self, tf, density):
+        # the TF and the grid are read through raw device pointers
+        if not (tf.is_cuda and density.is_cuda):
+            raise ValueError("tf and density must be GPU tensors")
+        if tf.dim() != 2 or tf.shape[1] != 4:
+            raise ValueError(f"tf: expected (n, 4), got {tuple(tf.shape)}")
+        if tuple(density.shape) != self.shape:
+            raise ValueError(f"density: expected {self.shape}, got {tuple(density.shape)}")
         tf = tf.detach().contiguous().float()
         density = density.detach().contiguous().float()
         self.ctx._check(lib().aloha_tf_set_device(self.ctx._h, C.c_void_p(tf.data_ptr()),

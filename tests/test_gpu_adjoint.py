@@ -229,3 +229,19 @@ def test_multi_brick_forward_and_adjoint():
     errs = [np.linalg.norm(g - r) / np.linalg.norm(r) for g, r in zip(g_dd, ref_dd)]
     print(f"relative L2 error dTF {e_tf:.2e}, dDensity per brick {['%.2e' % e for e in errs]}")
     assert e_tf < 1e-3 and max(errs) < 1e-3
+
+
+def test_input_checks(small):
+    """A density grid of another shape, a TF that is not (n, 4) or host
+    tensors are refused before the device copy."""
+    import torch
+    print("\n[case] diff input checks", flush=True)
+    vol = _vol(small)
+    tf = torch.from_numpy(small.tf).cuda()
+    den = torch.from_numpy(small.density).cuda()
+    for bad_tf, bad_den in ((tf, den[:-1]), (tf[:, :3], den), (tf.cpu(), den)):
+        with pytest.raises(ValueError):
+            vol.forward_only(bad_tf, bad_den)
+    rgba, ns = vol.forward_only(tf, den)
+    torch.cuda.synchronize()
+    assert int(ns.sum()) > 0
