@@ -294,18 +294,32 @@ class Context:
         self._check(lib().aloha_sync(self._h))
 
     def upload_brick(self, desc: BrickDesc, voxels, dtype: int | None = None) -> int:
-        """voxels: numpy (z,y,x) u8/u16/f32 on the host, or a cuda torch tensor."""
+        """voxels: numpy (z,y,x) u8/u16/f32 on the host, or a cuda torch tensor,
+        holding at least the brick's voxel-range count of elements (checked:
+        the copy reads through a raw pointer)."""
         bid = C.c_int32()
+        lo, hi = desc.voxel_range.lower, desc.voxel_range.upper
+        count = 1
+        for a in range(3):
+            count *= max(0, hi[a] - lo[a])
+
+        def need(code, nbytes):
+            want = count * {U8: 1, U16: 2, F32: 4}.get(code, 4)
+            if nbytes < want:
+                raise ValueError(f"voxels: {nbytes} bytes, the brick's voxel range needs {want}")
+
         if hasattr(voxels, "data_ptr") and getattr(voxels, "is_cuda", False):
             import torch
             t = voxels.contiguous()
             code = {torch.uint8: U8, torch.int16: U16, torch.float32: F32}[t.dtype]
+            need(dtype or code, t.numel() * t.element_size())
             self._check(lib().aloha_brick_upload(self._h, C.byref(desc), C.c_void_p(t.data_ptr()),
                                                  dtype or code, 1, C.byref(bid)))
             return bid.value
         a = np.ascontiguousarray(voxels)
         code = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
                 np.dtype(np.float32): F32}[a.dtype]
+        need(dtype or code, a.nbytes)
         self._check(lib().aloha_brick_upload(self._h, C.byref(desc), a.ctypes.data, dtype or code,
                                              0, C.byref(bid)))
         return bid.value
@@ -317,6 +331,8 @@ class Context:
 
     def set_tf(self, tf: np.ndarray) -> None:
         tf = np.ascontiguousarray(tf, dtype=np.float32)
+        if tf.ndim != 2 or tf.shape[1] != 4:
+            raise ValueError(f"tf: expected (n, 4) RGBA entries, got {tf.shape}")
         self._check(lib().aloha_tf_set(self._h, tf.ctypes.data, tf.shape[0]))
 
     def set_camera(self, org, dir_00, dir_du, dir_dv) -> None:

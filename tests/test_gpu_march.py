@@ -286,6 +286,10 @@ def test_render_errors(c0):
                              A.RankImages.alloc(wl.num_ranks, wl.W, wl.H))
         with pytest.raises(ValueError):  # a host tensor
             ctx.render_composite(wl.params(), wl.W, wl.H, ranks, wl.order, out.cpu())
+        with pytest.raises(ValueError):  # fewer voxels than the brick's voxel range
+            ctx.upload_brick(wl.bricks[0], wl.voxels[0][:-1])
+        with pytest.raises(ValueError):  # TF entries that are not RGBA
+            ctx.set_tf(wl.tf[:, :3])
         # still usable after the errors
         ctx.render_frame(wl.params(), wl.W, wl.H, ranks, imgs, wl.order, out)
         torch.cuda.synchronize()
