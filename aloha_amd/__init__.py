@@ -178,6 +178,21 @@ def synth_camera(bounds: Box3f, W: int, H: int, fovy_deg=45.0, dist_factor=2.5, 
     return {"org": v[0:3], "dir_00": v[3:6], "dir_du": v[6:9], "dir_dv": v[9:12]}
 
 
+def project_boxes(cam: dict, boxes, W: int, H: int):
+    """Pixel rectangle (x0, y0, x1, y1) outside of which no primary ray of cam
+    reaches any of boxes (pairs of lower/upper corners); host only."""
+    arr = (Box3f * max(1, len(boxes)))()
+    for k, (lo, hi) in enumerate(boxes):
+        arr[k].lower[:] = [float(v) for v in lo]
+        arr[k].upper[:] = [float(v) for v in hi]
+    c = np.concatenate([np.asarray(cam[k], np.float32) for k in ("org", "dir_00", "dir_du",
+                                                                  "dir_dv")])
+    out = (C.c_int32 * 4)()
+    check(lib().aloha_project_boxes(c.ctypes.data_as(C.POINTER(C.c_float)), arr, len(boxes), W,
+                                    H, out))
+    return tuple(out)
+
+
 def max_depth(bounds: Box3f, org) -> float:
     """handleCameraEvent's maxDepth (waikikiRenderer.cpp:858-860), in float32:
     length(org + center) + length(center + upper)."""
@@ -257,8 +272,22 @@ def _rank_table(ranks: Sequence[Sequence[int]]):
     return (C.c_int32 * len(offs))(*offs), (C.c_int32 * max(1, len(ids)))(*ids)
 
 
+def _order(order: Sequence[int], P: int) -> C.Array:
+    """Visibility order of P rank slots as a C array; the C side reads P
+    entries, so a shorter or longer sequence is refused here."""
+    if len(order) != P:
+        raise ValueError(f"order: {len(order)} entries for {P} rank slots")
+    return (C.c_int32 * max(1, P))(*[int(v) for v in order])
+
+
 class Context:
-    """Device context (bricks in HBM, TF, camera) on one GPU."""
+    """Device context (bricks in HBM, TF, camera) on one GPU.
+
+    stream: a torch.cuda.Stream or raw hipStream_t for all of the context's
+    work.  Left out, the context launches on torch's current stream of the
+    device at construction (so torch's allocations, fills and copies of the
+    tensors it reads and writes are ordered with its kernels); without torch,
+    on a non-blocking stream of its own."""
 
     def __init__(self, device: int = 0, stream=None):
         h = C.c_void_p()
@@ -266,6 +295,11 @@ class Context:
         self._h = h
         self._free = lib().aloha_ctx_destroy  # held: module globals are gone at exit
         self.device = device
+        if stream is None:
+            import sys
+            torch = sys.modules.get("torch")
+            if torch is not None and torch.cuda.is_available():
+                stream = torch.cuda.current_stream(device)
         if stream is not None:
             self.set_stream(stream)
 
@@ -356,7 +390,7 @@ class Context:
         rectangles' bounding box, the composite covers the box."""
         offs, ids = _rank_table(ranks)
         imgs = images.as_c(len(ranks), W, H)
-        o = (C.c_int32 * len(order))(*order)
+        o = _order(order, len(ranks))
         n = max(0, W) * max(0, H)
         self._check(lib().aloha_render_frame(self._h, C.byref(params), W, H, len(ranks), offs, ids,
                                              C.byref(imgs), o, _dev(out_rgba, 16 * n, "out_rgba"),
@@ -367,7 +401,7 @@ class Context:
                          out_rgba8=None, nsamples=None) -> None:
         """Fused raygen + ordered 'over' composite (one launch)."""
         offs, ids = _rank_table(ranks)
-        o = (C.c_int32 * len(order))(*order)
+        o = _order(order, len(ranks))
         n = max(0, W) * max(0, H)
         self._check(lib().aloha_render_composite(self._h, C.byref(params), W, H, len(ranks), offs,
                                                  ids, o, _dev(out_rgba, 16 * n, "out_rgba"),
@@ -388,7 +422,7 @@ class Context:
         """Pick the fastest lanes-per-ray schedule for these launches; returns
         the chosen (ranks_lanes, fused_lanes) and the timing table."""
         offs, ids = _rank_table(ranks)
-        o = (C.c_int32 * len(order))(*order) if order is not None else None
+        o = _order(order, len(ranks)) if order is not None else None
         imgs = out.as_c(len(ranks), W, H)
         n = max(0, W) * max(0, H)
         table = (C.c_float * 12)()

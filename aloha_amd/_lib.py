@@ -128,6 +128,7 @@ _SIGS = [
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),
     ("aloha_composite_over_sparse", C.c_int, [_P, _P, _I, _IP, _IP, _I, _I, _P, _P]),
     ("aloha_rank_rects", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP]),
+    ("aloha_project_boxes", C.c_int, [_FP, C.POINTER(Box3f), _I, _I, _I, _IP]),
     ("aloha_composite_over_rgba8", C.c_int, [_P, _P, _I, _IP, _I, _I, _P]),
     ("aloha_composite_zmin", C.c_int, [_P, _P, _P, _I, _I, _I, _P, _P]),
     ("aloha_mesh_create", C.c_int, [_P, C.c_int64, _P, C.c_int64, C.POINTER(_P)]),

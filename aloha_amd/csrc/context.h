@@ -43,6 +43,12 @@ struct aloha_ctx {
   // dynamic LDS size they were computed for)
   int q_waves[4] = {0, 0, 0, 0};
   size_t q_waves_lds[4] = {0, 0, 0, 0};
+  // bumped whenever the device address of the brick table or the TF, or the
+  // TF's entry count, changes: a batch launch compares it with the value its
+  // launch arguments were made with
+  uint64_t gen = 0;
+  // batches made on this context (detached when the context goes first)
+  std::vector<aloha_batch *> batches;
   std::string err;
 };
 

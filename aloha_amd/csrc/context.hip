@@ -11,6 +11,7 @@
 // capture them in a hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -48,6 +49,7 @@ int sync_bricks(aloha_ctx *ctx) {
     const size_t cap = std::max<size_t>(16, 2 * ctx->bricks.size());
     HIP_CHECK(ctx, hipMalloc(&ctx->d_bricks, cap * sizeof(DevBrick)));
     ctx->d_bricks_cap = cap;
+    ++ctx->gen;
   }
   if (!ctx->bricks.empty())
     HIP_CHECK(ctx, hipMemcpy(ctx->d_bricks, ctx->bricks.data(),
@@ -78,62 +80,19 @@ int aloha::check_params(aloha_ctx *ctx, const aloha_render_params *p) {
 
 namespace {
 
-// Conservative pixel rectangle of the bricks bricks[ids[0..n)] as seen from
-// the camera: a primary ray through pixel (x,y) has direction
-// dir_00 + (x+.5)*du + (y+.5)*dv, so solving P - org = c0*dir_00 + c1*du +
-// c2*dv for each box corner P gives its pixel coordinate (c1/c0 - .5,
-// c2/c0 - .5).  With every corner in front (c0 > 0) the box's image is the
-// convex hull of these points; a 2-pixel margin absorbs float rounding of the
-// rays (relative 1e-6) and the 1e-5 direction clamp.  Otherwise (a box
-// reaching the camera plane) the rectangle is the whole image.
+// Pixel rectangle of the bricks bricks[ids[0..n)] (project_boxes over their
+// space boxes): no primary ray of another pixel reaches them.
 void rank_rect(const float cam[12], const std::vector<DevBrick> &bricks, const int32_t *ids,
                int n, int W, int H, int32_t out[4]) {
-  out[0] = 0, out[1] = 0, out[2] = W, out[3] = H;
-  if (n == 0) {
-    out[2] = out[3] = 0;
-    return;
-  }
-  const double O[3] = {cam[0], cam[1], cam[2]};
-  const double D[3] = {cam[3], cam[4], cam[5]};
-  const double U[3] = {cam[6], cam[7], cam[8]};
-  const double V[3] = {cam[9], cam[10], cam[11]};
-  auto cross = [](const double a[3], const double b[3], double r[3]) {
-    r[0] = a[1] * b[2] - a[2] * b[1];
-    r[1] = a[2] * b[0] - a[0] * b[2];
-    r[2] = a[0] * b[1] - a[1] * b[0];
-  };
-  auto dot = [](const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-  double uv[3], vd[3], du[3];
-  cross(U, V, uv);
-  cross(V, D, vd);
-  cross(D, U, du);
-  const double det = dot(D, uv);
-  if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) return;
-  double smin = 1e300, smax = -1e300, tmin = 1e300, tmax = -1e300;
+  std::vector<aloha_box3f> boxes((size_t)std::max(n, 0));
   for (int k = 0; k < n; ++k) {
     const DevBrick &b = bricks[ids[k]];
-    for (int c = 0; c < 8; ++c) {
-      double Q[3], q2 = 0.0;
-      for (int a = 0; a < 3; ++a) {
-        Q[a] = ((c >> a) & 1 ? b.space_hi[a] : b.space_lo[a]) - O[a];
-        q2 += Q[a] * Q[a];
-      }
-      const double c0 = dot(uv, Q) / det, c1 = dot(vd, Q) / det, c2 = dot(du, Q) / det;
-      // corners on or behind the plane through org spanned by du, dv
-      if (!(c0 > 1e-6 * std::sqrt(q2) / std::sqrt(dot(D, D)))) return;
-      const double s = c1 / c0 - 0.5, t = c2 / c0 - 0.5;
-      if (!std::isfinite(s) || !std::isfinite(t)) return;
-      smin = std::min(smin, s), smax = std::max(smax, s);
-      tmin = std::min(tmin, t), tmax = std::max(tmax, t);
+    for (int a = 0; a < 3; ++a) {
+      boxes[k].lower[a] = b.space_lo[a];
+      boxes[k].upper[a] = b.space_hi[a];
     }
   }
-  const double m = 2.0;
-  const double x0 = std::floor(smin - m), x1 = std::ceil(smax + m) + 1.0;
-  const double y0 = std::floor(tmin - m), y1 = std::ceil(tmax + m) + 1.0;
-  out[0] = (int32_t)std::min<double>(W, std::max(0.0, x0));
-  out[2] = (int32_t)std::min<double>(W, std::max(0.0, x1));
-  out[1] = (int32_t)std::min<double>(H, std::max(0.0, y0));
-  out[3] = (int32_t)std::min<double>(H, std::max(0.0, y1));
+  project_boxes(cam, boxes.data(), n, W, H, out);
 }
 
 // The march advances t by repeated addition t += dt, so every t it reaches
@@ -248,9 +207,17 @@ extern "C" int aloha_ctx_create(int32_t device, aloha_ctx **out) {
   return ALOHA_OK;
 }
 
+namespace aloha {
+void batch_detach(aloha_batch *bt);
+}
+
 extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
   if (!ctx) return ALOHA_OK;
   hipSetDevice(ctx->device);
+  // batches still alive lose their device resources now; destroying them
+  // later only frees the host record
+  for (aloha_batch *bt : ctx->batches) batch_detach(bt);
+  ctx->batches.clear();
   hipStreamSynchronize(ctx->stream);
   for (float *m : ctx->brick_mem) hipFree(m);
   if (ctx->d_bricks) hipFree(ctx->d_bricks);
@@ -373,15 +340,17 @@ extern "C" int aloha_tf_set(aloha_ctx *ctx, const float *rgba, int32_t n) {
     if (!std::isfinite(rgba[i]))
       return fail(ctx, ALOHA_ERR_ARG, "aloha_tf_set: entry %d is not finite", i / 4);
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // ordered after earlier launches on the stream that read the old table
+  HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (n > ctx->tf_cap) {
     if (ctx->d_tf) HIP_CHECK(ctx, hipFree(ctx->d_tf));
     ctx->d_tf = nullptr;
     HIP_CHECK(ctx, hipMalloc(&ctx->d_tf, sizeof(float4) * (size_t)n));
     ctx->tf_cap = n;
+    ++ctx->gen;
   }
-  // ordered after earlier launches on the stream that read the old table
-  HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   HIP_CHECK(ctx, hipMemcpy(ctx->d_tf, rgba, sizeof(float4) * (size_t)n, hipMemcpyHostToDevice));
+  if (n != ctx->tf_n) ++ctx->gen;
   ctx->tf_n = n;
   return ALOHA_OK;
 }
@@ -778,9 +747,11 @@ extern "C" int aloha_tf_set_device(aloha_ctx *ctx, const float *d_rgba, int32_t 
     ctx->d_tf = nullptr;
     HIP_CHECK(ctx, hipMalloc(&ctx->d_tf, sizeof(float4) * (size_t)n));
     ctx->tf_cap = n;
+    ++ctx->gen;
   }
   HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_tf, d_rgba, sizeof(float4) * (size_t)n,
                                 hipMemcpyDeviceToDevice, ctx->stream));
+  if (n != ctx->tf_n) ++ctx->gen;
   ctx->tf_n = n;
   return ALOHA_OK;
 }
@@ -886,7 +857,8 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
 // batched form: many views of one rank table in one (graph-captured) launch
 // ---------------------------------------------------------------------------
 struct aloha_batch {
-  aloha_ctx *ctx = nullptr;
+  aloha_ctx *ctx = nullptr;  // NULL once the context is destroyed
+  uint64_t gen = 0;          // ctx->gen the launch arguments were made with
   BatchArgs b;
   float *d_cams = nullptr;
   int32_t *d_orders = nullptr, *d_rects = nullptr, *d_boxes = nullptr, *d_box_off = nullptr;
@@ -909,7 +881,35 @@ void batch_drop_graph(aloha_batch *bt) {
 }
 
 bool lanes_valid(int n) { return n == 1 || n == 2 || n == 4 || n == 8; }
+
+// Device resources of a batch (the context's device is current).
+void batch_release(aloha_batch *bt) {
+  if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+  batch_drop_graph(bt);
+  for (void *p : {(void *)bt->d_cams, (void *)bt->d_orders, (void *)bt->d_rects, (void *)bt->d_boxes,
+                  (void *)bt->d_box_off, (void *)bt->d_counter})
+    if (p) (void)hipFree(p);
+  bt->d_cams = nullptr, bt->d_orders = bt->d_rects = bt->d_boxes = bt->d_box_off = nullptr;
+  bt->d_counter = nullptr;
+  if (bt->ev_in) (void)hipEventDestroy(bt->ev_in);
+  if (bt->ev_out) (void)hipEventDestroy(bt->ev_out);
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  bt->ev_in = bt->ev_out = nullptr;
+  bt->stream = nullptr;
+}
+
+// The context's TF or brick table moved (or the TF changed size) since the
+// batch's launch arguments were made: take the current ones and drop a
+// captured graph, which holds the old ones (capture again if there was one).
+int batch_refresh(aloha_batch *bt);
 }  // namespace
+
+namespace aloha {
+void batch_detach(aloha_batch *bt) {
+  batch_release(bt);
+  bt->ctx = nullptr;
+}
+}  // namespace aloha
 
 extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
                                   int32_t H, int32_t V, const float *cams, int32_t P,
@@ -1013,21 +1013,20 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   bt->b.total_active = box_off[V];
   bt->b.fill_blocks = (int)std::min<int64_t>((int64_t)V * H, 1024);
   bt->lanes = ctx->fused_lanes;
+  bt->gen = ctx->gen;
+  ctx->batches.push_back(bt);
   *out = bt;
   return ALOHA_OK;
 }
 
 extern "C" int aloha_batch_destroy(aloha_batch *bt) {
   if (!bt) return ALOHA_OK;
-  (void)hipSetDevice(bt->ctx->device);
-  if (bt->stream) (void)hipStreamSynchronize(bt->stream);
-  batch_drop_graph(bt);
-  for (void *p : {(void *)bt->d_cams, (void *)bt->d_orders, (void *)bt->d_rects, (void *)bt->d_boxes,
-                  (void *)bt->d_box_off, (void *)bt->d_counter})
-    if (p) (void)hipFree(p);
-  if (bt->ev_in) (void)hipEventDestroy(bt->ev_in);
-  if (bt->ev_out) (void)hipEventDestroy(bt->ev_out);
-  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  if (aloha_ctx *ctx = bt->ctx) {
+    (void)hipSetDevice(ctx->device);
+    batch_release(bt);
+    auto &v = ctx->batches;
+    v.erase(std::remove(v.begin(), v.end(), bt), v.end());
+  }
   delete bt;
   return ALOHA_OK;
 }
@@ -1035,6 +1034,7 @@ extern "C" int aloha_batch_destroy(aloha_batch *bt) {
 extern "C" int aloha_batch_set_schedule(aloha_batch *bt, int32_t lanes_per_ray,
                                         int32_t persistent_blocks) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_set_schedule: NULL");
+  if (!bt->ctx) return fail(nullptr, ALOHA_ERR_STATE, "aloha_batch_set_schedule: context destroyed");
   if (!lanes_valid(lanes_per_ray) || persistent_blocks < 0 || persistent_blocks > (1 << 20))
     return fail(bt->ctx, ALOHA_ERR_ARG,
                 "aloha_batch_set_schedule: lanes per ray 1, 2, 4 or 8; blocks >= 0");
@@ -1057,8 +1057,11 @@ extern "C" int aloha_batch_get_schedule(const aloha_batch *bt, int32_t *lanes_pe
 // and before later work on it.
 extern "C" int aloha_batch_launch(aloha_batch *bt) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_launch: NULL");
+  if (!bt->ctx) return fail(nullptr, ALOHA_ERR_STATE, "aloha_batch_launch: context destroyed");
   aloha_ctx *ctx = bt->ctx;
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (bt->gen != ctx->gen)
+    if (int rc = batch_refresh(bt)) return rc;
   HIP_CHECK(ctx, hipEventRecord(bt->ev_in, ctx->stream));
   HIP_CHECK(ctx, hipStreamWaitEvent(bt->stream, bt->ev_in, 0));
   if (bt->exec)
@@ -1072,8 +1075,11 @@ extern "C" int aloha_batch_launch(aloha_batch *bt) {
 
 extern "C" int aloha_batch_capture(aloha_batch *bt) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_capture: NULL");
+  if (!bt->ctx) return fail(nullptr, ALOHA_ERR_STATE, "aloha_batch_capture: context destroyed");
   aloha_ctx *ctx = bt->ctx;
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (bt->gen != ctx->gen)
+    if (int rc = batch_refresh(bt)) return rc;
   batch_drop_graph(bt);
   HIP_CHECK(ctx, hipStreamSynchronize(bt->stream));
   HIP_CHECK(ctx, hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
@@ -1094,6 +1100,23 @@ extern "C" int aloha_batch_capture(aloha_batch *bt) {
   return ALOHA_OK;
 }
 
+namespace {
+int batch_refresh(aloha_batch *bt) {
+  aloha_ctx *ctx = bt->ctx;
+  if (!ctx->d_tf || ctx->tf_n < 1) return fail(ctx, ALOHA_ERR_STATE, "batch: no transfer function set");
+  if (int rc = sync_bricks(ctx)) return rc;
+  const bool was_captured = bt->exec != nullptr;
+  // the old graph may still run on the batch stream
+  HIP_CHECK(ctx, hipStreamSynchronize(bt->stream));
+  batch_drop_graph(bt);
+  bt->b.m.tf = ctx->d_tf;
+  bt->b.m.tf_n = ctx->tf_n;
+  bt->b.m.bricks = ctx->d_bricks;
+  bt->gen = ctx->gen;
+  return was_captured ? aloha_batch_capture(bt) : ALOHA_OK;
+}
+}  // namespace
+
 // Times every schedule (1/2/4/8 lanes per ray x {one block per item,
 // persistent with 2/4/8 blocks per CU}) over `reps` launches each and keeps
 // the fastest.  The
@@ -1102,9 +1125,12 @@ extern "C" int aloha_batch_capture(aloha_batch *bt) {
 extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_ms,
                                     float *table, int32_t table_cap, int32_t *table_n) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_autotune: NULL");
+  if (!bt->ctx) return fail(nullptr, ALOHA_ERR_STATE, "aloha_batch_autotune: context destroyed");
   aloha_ctx *ctx = bt->ctx;
   if (reps < 1) return fail(ctx, ALOHA_ERR_ARG, "aloha_batch_autotune: reps < 1");
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (bt->gen != ctx->gen)
+    if (int rc = batch_refresh(bt)) return rc;
   batch_drop_graph(bt);
   hipEvent_t e0, e1;
   HIP_CHECK(ctx, hipEventCreate(&e0));

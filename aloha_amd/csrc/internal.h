@@ -23,6 +23,10 @@ inline int format_error(std::string &s, int code, const char *fmt, ...) {
   return code;
 }
 
+// Pixel bounds of the rays that can reach the boxes (aloha_project_boxes).
+void project_boxes(const float cam[12], const aloha_box3f *boxes, int n, int W, int H,
+                   int32_t out[4]);
+
 }  // namespace aloha
 
 #define ALOHA_FAIL_GLOBAL(code, ...) ::aloha::format_error(::aloha::global_error(), code, __VA_ARGS__)

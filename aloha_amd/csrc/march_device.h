@@ -22,19 +22,24 @@ __device__ __forceinline__ float lerp_(float a, float b, float w) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
-// --- RNG: murmur3 fmix32 seed, Numerical-Recipes LCG, 24-bit draws -------
+// --- RNG: owl::common::LCG<4> (deviceCode.cpp:12, :354, :386) --------------
+// The published construction (Zafar, Olano & Curtis, HPG 2010): 4 rounds of
+// TEA over (pixelID, accumID) with the key {a341316c, c8013ea4, ad90777d,
+// 7e95761e} give the state; each draw steps the Numerical-Recipes LCG and
+// returns its low 24 bits / 2^24.
 __device__ __forceinline__ uint32_t rng_seed(uint32_t pixel, uint32_t frame) {
-  uint32_t h = (pixel * 0x9E3779B1u) ^ (frame * 0x85EBCA77u + 0x165667B1u);
-  h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  h *= 0xC2B2AE35u;
-  h ^= h >> 16;
-  return h;
+  uint32_t v0 = pixel, v1 = frame, sum = 0;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    sum += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
 }
 __device__ __forceinline__ float rng_next(uint32_t &s) {
   s = s * 1664525u + 1013904223u;
-  return (float)(s >> 8) * (1.0f / 16777216.0f);
+  return (float)(s & 0x00FFFFFFu) * (1.0f / 16777216.0f);
 }
 
 // Voxel reads go through global-address-space pointers, so the compiler

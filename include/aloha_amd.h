@@ -275,6 +275,14 @@ int aloha_composite_over_sparse(aloha_ctx *ctx, const float *d_rank_rgba, int32_
 int aloha_rank_rects(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
                      int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
                      int32_t *rects);
+/* Host only: pixel rectangle (x0, y0, x1, y1) outside of which no primary
+ * ray of the camera (org, dir_00, dir_du, dir_dv; generateRay,
+ * deviceCode.cpp:94-107, including its 1e-5 direction clamp) reaches any of
+ * the n boxes; the whole image when no bound follows (a box reaching the
+ * camera plane).  aloha_rank_rects applies it to each rank's bricks.  No
+ * counterpart in the reference. */
+int aloha_project_boxes(const float cam[12], const aloha_box3f *boxes, int32_t n, int32_t W,
+                        int32_t H, int32_t rect[4]);
 /* Same over P RGBA8 rank images (the reference's ubyte path, :908, :915). */
 int aloha_composite_over_rgba8(aloha_ctx *ctx, const uint32_t *d_rank_rgba8, int32_t P,
                                const int32_t *order, int32_t W, int32_t H,

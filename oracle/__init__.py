@@ -178,6 +178,17 @@ def slab(o, d, tmin, tmax, lo, hi):
     return bool(hit), t[0], t[1]
 
 
+def tea(v0: int, v1: int, key, rounds: int):
+    """TEA encryption of the block (v0, v1) under the 4-word key, `rounds` cycles."""
+    v = (C.c_uint32 * 2)(v0, v1)
+    lib().orc_tea(v, (C.c_uint32 * 4)(*key), int(rounds))
+    return int(v[0]), int(v[1])
+
+
+def rng_seed(pixel: int, frame: int) -> int:
+    return int(lib().orc_rng_seed(pixel, frame))
+
+
 def rng_stream(pixel: int, frame: int, count: int):
     s = C.c_uint32(lib().orc_rng_seed(pixel, frame))
     return [lib().orc_rng_draw(C.byref(s)) for _ in range(count)]

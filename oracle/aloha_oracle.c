@@ -29,9 +29,9 @@
  *     x first, then y, then z;
  *   - TF: x = v*N - 0.5, border entries T[-1] = T[N] = 0, same lerp;
  *   - normalize(v) = v / sqrt(dot(v,v)); dot = (x*x + y*y) + z*z;
- *   - jitter RNG: murmur3 fmix32 seed of (pixelID, accumID), then the
- *     Numerical-Recipes LCG, 24-bit output (parity unpinned against OWL's
- *     LCG<4>: its source is not in the reference);
+ *   - jitter RNG: OWL's LCG<4> -- 4 TEA rounds over (pixelID, accumID)
+ *     seed the Numerical-Recipes LCG, draws (state & 0xFFFFFF)/2^24 (the
+ *     published construction; OWL's source is not in the reference);
  *   - 8-bit pack: min(255, max(0, int(f*256))), R in the low byte;
  *   - composite: premultiplied 'over' onto transparent black.
  * Compile with -ffp-contract=off: every float operation below is rounded
@@ -80,20 +80,37 @@ typedef struct {
 } orc_kdnode;
 
 /* ------------------------------------------------------------------ */
-/* RNG                                                                 */
+/* RNG: owl::common::LCG<4> (deviceCode.cpp:12, seeded per pixel at :354, */
+/* one draw per brick visit at :386).  OWL is not vendored; its LCG<N> is  */
+/* the published construction of Zafar, Olano & Curtis, "GPU random       */
+/* numbers via the tiny encryption algorithm" (HPG 2010), as the OptiX SDK */
+/* samples ship it (random.h, tea<N> + lcg + rnd):                         */
+/*   seed  = v0 after N rounds of TEA (Wheeler & Needham 1994) over        */
+/*           (v0, v1) = (pixelID, accumID) with the key                    */
+/*           {0xa341316c, 0xc8013ea4, 0xad90777d, 0x7e95761e};             */
+/*   draw  = state = 1664525*state + 1013904223 (Numerical Recipes),       */
+/*           (state & 0xFFFFFF) / 2^24.                                    */
 /* ------------------------------------------------------------------ */
+/* TEA encryption of (v[0], v[1]) under key k, `rounds` cycles. */
+void orc_tea(uint32_t v[2], const uint32_t k[4], int32_t rounds) {
+  uint32_t v0 = v[0], v1 = v[1], sum = 0;
+  for (int32_t i = 0; i < rounds; ++i) {
+    sum += 0x9e3779b9u;
+    v0 += ((v1 << 4) + k[0]) ^ (v1 + sum) ^ ((v1 >> 5) + k[1]);
+    v1 += ((v0 << 4) + k[2]) ^ (v0 + sum) ^ ((v0 >> 5) + k[3]);
+  }
+  v[0] = v0;
+  v[1] = v1;
+}
+static const uint32_t kLcgKey[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
 static inline uint32_t rng_seed(uint32_t pixel, uint32_t frame) {
-  uint32_t h = (pixel * 0x9E3779B1u) ^ (frame * 0x85EBCA77u + 0x165667B1u);
-  h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  h *= 0xC2B2AE35u;
-  h ^= h >> 16;
-  return h;
+  uint32_t v[2] = {pixel, frame};
+  orc_tea(v, kLcgKey, 4);
+  return v[0];
 }
 static inline float rng_next(uint32_t *s) {
   *s = *s * 1664525u + 1013904223u;
-  return (float)(*s >> 8) * (1.0f / 16777216.0f);
+  return (float)(*s & 0x00FFFFFFu) / (float)0x01000000;
 }
 uint32_t orc_rng_seed(uint32_t pixel, uint32_t frame) { return rng_seed(pixel, frame); }
 float orc_rng_draw(uint32_t *s) { return rng_next(s); }

@@ -150,3 +150,55 @@ def test_batch_multi_brick_and_empty_ranks(wl):
         assert torch.equal(out, ref) and torch.equal(ns, ref_ns), (lanes, blocks)
     b.close()
     ctx.close()
+
+
+def test_batch_follows_tf_and_brick_table_changes(wl):
+    """A TF of another size (a new device buffer) and more bricks (a new
+    device brick table) after the batch was made and captured: the next
+    launch takes the current ones and equals the single-view frames."""
+    import torch
+    print("\n[case] batch after TF resize and brick upload", flush=True)
+    ctx, ranks = _setup(wl, torch)
+    V = len(wl.cams)
+    out = torch.empty((V, wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
+    b = A.Batch(ctx, wl.params(), wl.W, wl.H, wl.cams, ranks, wl.orders, out_rgba=out)
+    b.capture()
+    b.launch()
+    torch.cuda.synchronize()
+    # a 3x longer TF through a larger allocation, and 40 extra bricks, which
+    # move the context's brick table to a larger allocation
+    tf2 = np.repeat(wl.tf[::-1], 3, axis=0).copy()
+    ctx.set_tf(tf2)
+    for _ in range(40):
+        ctx.upload_brick(wl.bricks[0], wl.voxels[0])
+    ref = _single_views(ctx, wl, ranks, torch)
+    for rep in range(2):
+        out.fill_(-1.0)
+        torch.cuda.synchronize()
+        b.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+    assert b.schedule()["captured"]
+    # a smaller TF in the same allocation changes only the entry count
+    ctx.set_tf(wl.tf[:64])
+    ref = _single_views(ctx, wl, ranks, torch)
+    out.fill_(-1.0)
+    b.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    b.close()
+    ctx.close()
+
+
+def test_batch_outliving_its_context_is_refused(wl):
+    import torch
+    print("\n[case] batch after its context is closed", flush=True)
+    ctx, ranks = _setup(wl, torch)
+    V = len(wl.cams)
+    out = torch.empty((V, wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
+    b = A.Batch(ctx, wl.params(), wl.W, wl.H, wl.cams, ranks, wl.orders, out_rgba=out)
+    b.launch()
+    ctx.close()
+    with pytest.raises(A.AlohaError):
+        b.launch()
+    b.close()

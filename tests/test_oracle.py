@@ -121,6 +121,47 @@ def test_slab_closed_form():
 
 
 # ---------------------------------------------------------------- RNG / pack
+# TEA (Wheeler & Needham 1994) known-answer vectors at 32 cycles, blocks and
+# keys read big-endian (the published test vectors of the cipher): an all-zero
+# key and block, and a key whose four words differ, so a swapped key word, a
+# wrong shift or a misplaced round sum fails.
+TEA_KAT = [
+    ((0x00000000, 0x00000000), (0, 0, 0, 0), (0x41EA3A0A, 0x94BAA940)),
+    ((0x01020304, 0x05060708), (0, 0, 0, 0), (0x6A2F9CF3, 0xFCCF3C55)),
+    ((0x00000000, 0x00000000), (0x01234567, 0x12345678, 0x23456789, 0x3456789A),
+     (0x34E943B0, 0x900F5DCB)),
+    ((0x01020304, 0x05060708), (0x01234567, 0x12345678, 0x23456789, 0x3456789A),
+     (0x773DC179, 0x878A81C0)),
+]
+# OWL's LCG<4> key (the OptiX SDK's tea<N>, Zafar, Olano & Curtis 2010)
+LCG_KEY = (0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E)
+
+
+def test_tea_known_answers():
+    for block, key, want in TEA_KAT:
+        assert O.tea(*block, key, 32) == want
+
+
+def test_rng_seed_is_four_tea_rounds_of_pixel_and_accum_id():
+    # Random random(pixelID, lp.accumID) (deviceCode.cpp:354): state = v0 of
+    # TEA<4>(pixelID, accumID) under the LCG key
+    for pixel, frame in [(0, 0), (1, 0), (0, 1), (512 * 300 + 17, 3), (2 ** 20 - 1, 7)]:
+        assert O.rng_seed(pixel, frame) == O.tea(pixel, frame, LCG_KEY, 4)[0]
+    # 4 rounds differ from the full cipher and from other round counts
+    assert O.rng_seed(5, 2) != O.tea(5, 2, LCG_KEY, 32)[0]
+    assert O.rng_seed(5, 2) != O.tea(5, 2, LCG_KEY, 3)[0]
+
+
+def test_rng_draws_are_low_24_bits_of_the_numerical_recipes_lcg():
+    for pixel, frame in [(0, 0), (777, 1), (1048575, 9)]:
+        s = O.rng_seed(pixel, frame)
+        want = []
+        for _ in range(6):
+            s = (1664525 * s + 1013904223) % 2 ** 32
+            want.append((s & 0xFFFFFF) / 2.0 ** 24)
+        assert O.rng_stream(pixel, frame, 6) == want
+
+
 def test_rng_is_24bit_uniform():
     draws = np.array([d for p in range(200) for d in O.rng_stream(p, 3, 50)], np.float64)
     assert draws.min() >= 0 and draws.max() < 1
@@ -181,10 +222,14 @@ def test_constant_field_alpha_recurrence():
         assert np.allclose(rgba[:3], 0.1 * np.array([0.8, 0.4, 0.2]) * alpha, atol=1e-6)
 
 
-def _brute_force(cam, tf, bricks, W, H, dt, jitter, max_depth):
+def _brute_force(cam, tf, bricks, W, H, dt, jitter, max_depth, closed_lattice=False, amax=None):
     """float64 re-derivation of renderFrame for fixed jitter: float32 ray
     setup and sample lattice (the reference's control), float64 data with
-    explicit corner weights.  Pure Python; tiny inputs only."""
+    explicit corner weights.  Pure Python; tiny inputs only.
+
+    closed_lattice / amax give the differentiable form instead: samples at
+    t_k = dt*(j + (k0 + k)) in float32 (k0 the first lattice index at or
+    past the entry), each sample's opacity clamped to amax."""
     n_tf = tf.shape[0]
 
     def tfl(v):
@@ -251,9 +296,15 @@ def _brute_force(cam, tf, bricks, W, H, dt, jitter, max_depth):
                     break
                 _, tn, t0, t1, desc, vox = best
                 tn = f32(tn + tmin)
-                t = f32(dt) * (f32(jitter) + np.floor((t0 - f32(dt) * f32(jitter)) / f32(dt)))
+                segs = np.floor((t0 - f32(dt) * f32(jitter)) / f32(dt))
+                t = f32(dt) * (f32(jitter) + segs)
+                k0 = segs
                 if t < t0:
                     t = f32(t + f32(dt))
+                    k0 = f32(segs + f32(1))
+                k = 0
+                if closed_lattice:
+                    t = f32(dt) * (f32(jitter) + (f32(k0) + f32(k)))
                 vl = desc["voxel"][0]
                 n = [desc["voxel"][1][a] - vl[a] for a in range(3)]
                 while t < t1 and c[3] < 0.99:
@@ -269,10 +320,16 @@ def _brute_force(cam, tf, bricks, W, H, dt, jitter, max_depth):
                     L = math.sqrt(sum(x * x for x in e))
                     shade = 0.1 + 0.9 * abs(sum(g[a] / L * float(d[a]) for a in range(3)))
                     s = tfl(tri(vox, n, p))
+                    if amax is not None:
+                        s[3] = min(s[3], float(f32(amax)))
                     w = (1 - c[3]) * s[3]
                     c[:3] += w * s[:3] * shade
                     c[3] += w
-                    t = f32(t + f32(dt))
+                    k += 1
+                    if closed_lattice:
+                        t = f32(dt) * (f32(jitter) + (f32(k0) + f32(k)))
+                    else:
+                        t = f32(t + f32(dt))
                 z = min(z, f32(tn / f32(max_depth)))
                 tmin = f32(t1 + f32(1e-20))
                 if not c[3] < 0.99:
@@ -310,6 +367,41 @@ def test_brute_force_two_bricks(alpha_scale, dt):
     assert np.abs(o["rgba"] - rgba).max() < 2e-4
     # premultiplied: rgb <= alpha
     assert np.all(o["rgba"][..., :3] <= o["rgba"][..., 3:] + 1e-6)
+
+
+@pytest.mark.parametrize("alpha_scale,dt", [(0.05, 0.5), (1.0, 0.7)])
+def test_diff_forward64_against_brute_force(alpha_scale, dt):
+    """orc_diff_render64 (the float64 forward the adjoint's finite differences
+    use) and orc_diff_render (float) against the pure-Python float64 march of
+    the differentiable form: closed-form lattice, opacity clamped to .9999."""
+    rng = np.random.default_rng(6)
+    raw = rng.random((6, 5, 8), dtype=np.float32)
+    bricks = []
+    for dsc in A.split_volume((8, 5, 6), 2):
+        dd = dsc.as_dict()
+        lo, hi = dd["voxel"]
+        bricks.append(({"cell": dd["cell"], "voxel": dd["voxel"], "space": dd["space"]},
+                       np.ascontiguousarray(raw[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]])))
+    tf = rng.random((32, 4), dtype=np.float32)
+    tf[:, 3] *= alpha_scale
+    tf[4:9, 3] = 1.0  # opaque entries: the .9999 clamp is taken
+    W, H = 7, 6
+    cam = {"org": np.array([-4.0, 1.0, -3.0], np.float32),
+           "dir_00": np.array([0.7, -0.1, 0.35], np.float32),
+           "dir_du": np.array([0.05, 0.0, 0.02], np.float32),
+           "dir_dv": np.array([0.0, 0.06, 0.01], np.float32)}
+    kw = dict(dt=dt, jitter_mode=1, jitter_value=0.3)
+    rgba32, ns = O.diff_render(cam, tf, bricks, W, H, **kw)
+    rgba64 = O.diff_render64(cam, tf, bricks, W, H, **kw)
+    ref, ref_ns, _ = _brute_force(cam, tf, bricks, W, H, dt, 0.3, 50.0, closed_lattice=True,
+                                  amax=0.9999)
+    assert ns.sum() > 20
+    assert np.array_equal(ns, ref_ns)
+    assert np.abs(rgba64 - ref).max() < 1e-9
+    assert np.abs(rgba32 - rgba64).max() < 1e-5
+    # the clamp is exercised: without it some pixel's colour changes
+    unclamped, _, _ = _brute_force(cam, tf, bricks, W, H, dt, 0.3, 50.0, closed_lattice=True)
+    assert np.abs(unclamped - ref).max() > 1e-6
 
 
 def test_rank_images_composite_to_single_march():
