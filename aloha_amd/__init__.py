@@ -14,6 +14,8 @@ from typing import Sequence
 
 import numpy as np
 
+from ._lib import (ERR_ALLOC, ERR_ARG, ERR_HIP, ERR_IO, ERR_LIMIT, ERR_STATE,  # noqa: F401
+                   NOALLOC_SCRATCH_BYTES)
 from ._lib import (F32, FLAG_CROSSHAIR, FLAG_HEATMAP, JITTER_FIXED, JITTER_RANDOM, MAX_RANKS,
                    MAX_RANK_BRICKS, MAX_TF, U8, U16, AlohaError, Box3f, Box3i, BrickDesc, Images,
                    KDNode, Range1f, RenderParams, check, lib)
@@ -395,6 +397,19 @@ class Context:
         self._check(lib().aloha_render_frame(self._h, C.byref(params), W, H, len(ranks), offs, ids,
                                              C.byref(imgs), o, _dev(out_rgba, 16 * n, "out_rgba"),
                                              _dev(out_rgba8, 4 * n, "out_rgba8")))
+
+    def render_frame_noalloc(self, params: RenderParams, W: int, H: int,
+                             ranks: Sequence[Sequence[int]], order: Sequence[int],
+                             out_rgba=None, out_rgba8=None) -> None:
+        """render_frame's outputs without caller-held rank images: the ranks'
+        images go to a workspace the library holds on the GPU (compact, only
+        each rank's rectangle).  Allocates nothing on the device."""
+        offs, ids = _rank_table(ranks)
+        o = _order(order, len(ranks))
+        n = max(0, W) * max(0, H)
+        self._check(lib().aloha_render_frame_noalloc(
+            self._h, C.byref(params), W, H, len(ranks), offs, ids, o,
+            _dev(out_rgba, 16 * n, "out_rgba"), _dev(out_rgba8, 4 * n, "out_rgba8")))
 
     def render_composite(self, params: RenderParams, W: int, H: int,
                          ranks: Sequence[Sequence[int]], order: Sequence[int], out_rgba=None,

@@ -16,6 +16,7 @@ U8, U16, F32 = 1, 2, 4
 JITTER_RANDOM, JITTER_FIXED = 0, 1
 FLAG_CROSSHAIR, FLAG_HEATMAP = 1, 2
 MAX_RANKS, MAX_RANK_BRICKS, MAX_TF = 64, 256, 4096
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class Box3i(C.Structure):
@@ -128,6 +129,7 @@ _SIGS = [
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),
     ("aloha_composite_over_sparse", C.c_int, [_P, _P, _I, _IP, _IP, _I, _I, _P, _P]),
     ("aloha_rank_rects", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP]),
+    ("aloha_render_frame_noalloc", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP, _P, _P]),
     ("aloha_project_boxes", C.c_int, [_FP, C.POINTER(Box3f), _I, _I, _I, _IP]),
     ("aloha_composite_over_rgba8", C.c_int, [_P, _P, _I, _IP, _I, _I, _P]),
     ("aloha_composite_zmin", C.c_int, [_P, _P, _P, _I, _I, _I, _P, _P]),

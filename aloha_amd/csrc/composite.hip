@@ -59,6 +59,7 @@ __device__ __forceinline__ void store_nt(float4 *p, float4 v) {
   __builtin_nontemporal_store(u, reinterpret_cast<f4 *>(p));
 }
 
+template <bool kCompact>
 __global__ __launch_bounds__(kBlock) void over_sparse_kernel(const float4 *__restrict__ imgs,
                                                              const CompositeArgs c, int W,
                                                              int x0, int y0, int x1,
@@ -75,7 +76,9 @@ __global__ __launch_bounds__(kBlock) void over_sparse_kernel(const float4 *__res
       const int s = c.order[k];  // k is uniform: scalar loads
       const int32_t *rc = c.rect[s];
       if (px < rc[0] || px >= rc[2] || py < rc[1] || py >= rc[3]) continue;
-      const float4 v = imgs[(int64_t)s * c.npix + i];
+      const int64_t src = kCompact ? c.cbase[s] + (int64_t)(py - rc[1]) * (rc[2] - rc[0]) + (px - rc[0])
+                                   : (int64_t)s * c.npix + i;
+      const float4 v = imgs[src];
       const float w = 1.f - C.w;
       C.x = C.x + w * v.x;
       C.y = C.y + w * v.y;
@@ -140,7 +143,16 @@ hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs 
                                         float4 *out, uint32_t *out8, hipStream_t s) {
   if (x0 >= x1 || y0 >= y1) return hipSuccess;
   const dim3 grid((unsigned)((x1 - x0 + kSparsePix - 1) / kSparsePix), (unsigned)(y1 - y0));
-  over_sparse_kernel<<<grid, kBlock, 0, s>>>(imgs, c, W, x0, y0, x1, out, out8);
+  over_sparse_kernel<false><<<grid, kBlock, 0, s>>>(imgs, c, W, x0, y0, x1, out, out8);
+  return hipGetLastError();
+}
+
+hipError_t launch_composite_over_compact(const float4 *imgs, const CompositeArgs &c, int32_t W,
+                                         int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                                         float4 *out, uint32_t *out8, hipStream_t s) {
+  if (x0 >= x1 || y0 >= y1) return hipSuccess;
+  const dim3 grid((unsigned)((x1 - x0 + kSparsePix - 1) / kSparsePix), (unsigned)(y1 - y0));
+  over_sparse_kernel<true><<<grid, kBlock, 0, s>>>(imgs, c, W, x0, y0, x1, out, out8);
   return hipGetLastError();
 }
 

@@ -49,10 +49,18 @@ struct aloha_ctx {
   uint64_t gen = 0;
   // batches made on this context (detached when the context goes first)
   std::vector<aloha_batch *> batches;
+  // this device's workspace of aloha_render_frame_noalloc (NULL if its
+  // address could not be had: scratch_err says why)
+  void *scratch = nullptr;
+  std::string scratch_err;
   std::string err;
 };
 
 namespace aloha {
+
+constexpr int kMaxDevices = 64;
+// Device address of the noalloc workspace on `device` (current).
+int scratch_on_device(int device, void **out);
 
 // Launch arguments of the persistent per-rank kernel for march arguments
 // `a`; sets waves to its grid for `lanes` lanes per ray.

@@ -15,6 +15,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -197,6 +198,9 @@ extern "C" int aloha_ctx_create(int32_t device, aloha_ctx **out) {
   if (e != hipSuccess) return fail(nullptr, ALOHA_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
   auto *ctx = new aloha_ctx;
   ctx->device = device;
+  // the workspace of aloha_render_frame_noalloc, looked up once per device
+  if (hipError_t se = (hipError_t)scratch_on_device(device, &ctx->scratch); se != hipSuccess)
+    ctx->scratch_err = hipGetErrorString(se);
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;
@@ -316,6 +320,9 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
   ctx->descs.push_back(*desc);
   ctx->brick_bytes += (uint64_t)padded * sizeof(float);
   ctx->bricks_dirty = true;
+  // the device brick table follows at once, so render calls find it current
+  // (and aloha_render_frame_noalloc never has to grow it)
+  if (int rc = sync_bricks(ctx)) return rc;
   if (brick_id) *brick_id = (int32_t)ctx->bricks.size() - 1;
   return ALOHA_OK;
 }
@@ -385,6 +392,7 @@ int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, i
   q.frame = nullptr;
   q.frame8 = nullptr;
   q.box = make_int4(0, 0, 0, 0);
+  q.compact = 0;
   // persistent grid: as many one-wave blocks as the GPU holds at once
   const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
   const size_t lds = ranks_lds_bytes(a);
@@ -478,6 +486,99 @@ extern "C" int aloha_render_frame(aloha_ctx *ctx, const aloha_render_params *par
   HIP_CHECK(ctx, launch_march_ranks(a, q, ctx->march_lanes, waves, ctx->stream));
   HIP_CHECK(ctx, launch_composite_over_sparse((const float4 *)images->rgba, c, W, x0, y0, x1, y1,
                                               (float4 *)d_out_rgba, d_out_rgba8, ctx->stream));
+  return ALOHA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free frame: the rank images live in a workspace the library
+// holds, not in caller memory.
+// ---------------------------------------------------------------------------
+// The ranks' compact float RGBA images of aloha_render_frame_noalloc.  Each
+// GPU holds its own copy for the life of the process.  In host code the name
+// is not a device address: the address of the current device's copy comes
+// from hipGetSymbolAddress (scratch_on_device) and only that is used.
+static __device__ __align__(256) float4 scratch[ALOHA_NOALLOC_SCRATCH_BYTES / sizeof(float4)];
+
+namespace aloha {
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+// Address of `device`'s scratch array; looked up once per device (with that
+// device current: the caller has made it so).
+int scratch_on_device(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace aloha
+
+extern "C" int aloha_render_frame_noalloc(aloha_ctx *ctx, const aloha_render_params *params,
+                                          int32_t W, int32_t H, int32_t P,
+                                          const int32_t *rank_offsets, const int32_t *brick_ids,
+                                          const int32_t *order, float *d_out_rgba,
+                                          uint32_t *d_out_rgba8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_frame_noalloc: ctx is NULL");
+  if (!ctx->scratch)
+    return fail(ctx, ALOHA_ERR_STATE, "aloha_render_frame_noalloc: no workspace on device %d (%s)",
+                ctx->device, ctx->scratch_err.c_str());
+  if (!params) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_frame_noalloc: params is NULL");
+  if (params->accum_id != 0 || (params->flags & ALOHA_FLAG_HEATMAP))
+    return fail(ctx, ALOHA_ERR_ARG,
+                "aloha_render_frame_noalloc: accum_id 0 and no heat map only (no accum buffer)");
+  // build_march would grow a stale device brick table; uploads keep it current
+  if (ctx->bricks_dirty)
+    return fail(ctx, ALOHA_ERR_STATE, "aloha_render_frame_noalloc: brick table not on the device");
+  HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MarchArgs a;
+  if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
+  CompositeArgs c;
+  if (int rc = check_order(ctx, order, P, c)) return rc;
+  c.npix = (int64_t)W * H;
+  // compact layout: the rectangles one after the other, each row-major
+  int64_t total = 0;
+  int x0 = W, y0 = H, x1 = 0, y1 = 0;  // bounding box of the rectangles
+  for (int s = 0; s < P; ++s) {
+    const int32_t *rc = a.rect[s];
+    for (int k = 0; k < 4; ++k) c.rect[s][k] = rc[k];
+    c.cbase[s] = (int32_t)std::min<int64_t>(total, INT32_MAX);
+    if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
+    total += (int64_t)(rc[2] - rc[0]) * (rc[3] - rc[1]);
+    x0 = std::min(x0, (int)rc[0]), y0 = std::min(y0, (int)rc[1]);
+    x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
+  }
+  if (x0 >= x1 || y0 >= y1) x0 = y0 = x1 = y1 = 0;  // no rectangle: no composite launch
+  if (total * (int64_t)sizeof(float4) > (int64_t)ALOHA_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, ALOHA_ERR_LIMIT,
+                "aloha_render_frame_noalloc: the ranks' rectangles hold %lld pixels (%lld bytes); "
+                "the workspace holds %lld bytes",
+                (long long)total, (long long)(total * (int64_t)sizeof(float4)),
+                (long long)ALOHA_NOALLOC_SCRATCH_BYTES);
+  a.rgba = (float4 *)ctx->scratch;
+  a.rgba8 = nullptr;
+  a.depth = nullptr;
+  a.accum = nullptr;
+  a.nsamples = nullptr;
+  RanksArgs q;
+  int waves = 0;
+  if (int rc = prepare_ranks(ctx, a, ctx->march_lanes, q, waves)) return rc;
+  q.compact = 1;
+  for (int s = 0; s < P; ++s) q.cbase[s] = c.cbase[s];
+  q.frame = (float4 *)d_out_rgba;
+  q.frame8 = d_out_rgba8;
+  q.box = make_int4(x0, y0, x1, y1);
+  HIP_CHECK(ctx, launch_march_ranks(a, q, ctx->march_lanes, waves, ctx->stream));
+  HIP_CHECK(ctx, launch_composite_over_compact((const float4 *)ctx->scratch, c, W, x0, y0, x1, y1,
+                                               (float4 *)d_out_rgba, d_out_rgba8, ctx->stream));
   return ALOHA_OK;
 }
 

@@ -62,6 +62,11 @@ constexpr int kRankParts = 16;
 constexpr int kRankRun = 16;
 struct RanksArgs {
   int32_t run_off[ALOHA_MAX_RANKS + 1];
+  // compact != 0: the rank images hold only the pixels of each slot's
+  // rectangle, slot s's row y at element cbase[s] + (y - y0) * (x1 - x0)
+  // (x from x0), and nothing is written outside the rectangles
+  int32_t compact;
+  int32_t cbase[ALOHA_MAX_RANKS];
   // A frame's composite outputs (W x H; may be NULL): the launch writes them
   // transparent outside the box [box.x, box.z) x [box.y, box.w), which the
   // composite then covers.
@@ -111,6 +116,9 @@ struct CompositeArgs {
   int32_t order[ALOHA_MAX_RANKS];
   // sparse form: rank s's image is transparent outside [x0,x1) x [y0,y1)
   int32_t rect[ALOHA_MAX_RANKS][4];
+  // compact sparse images (RanksArgs::compact): slot s's rectangle row y at
+  // element cbase[s] + (y - y0) * (x1 - x0)
+  int32_t cbase[ALOHA_MAX_RANKS];
 };
 
 // Surface path: one rank's triangle geometry (BVH nodes of 4 float4, see
@@ -153,11 +161,11 @@ __host__ __device__ inline size_t march_tables_bytes(const MarchArgs &a) {
 }
 // The persistent per-rank launch adds a list of 64 ray records (48 B each)
 // after the tables, at the next 16-byte boundary past them, then the run
-// offsets run_off[0..P].
+// offsets run_off[0..P] and the compact image bases cbase[0..P).
 constexpr size_t kRayRecBytes = 48;
 __host__ __device__ inline size_t ranks_lds_bytes(const MarchArgs &a) {
   return (march_tables_bytes(a) / 16 + 1) * 16 + 64 * kRayRecBytes +
-         sizeof(int32_t) * (size_t)(a.P + 1);
+         sizeof(int32_t) * (size_t)(2 * a.P + 1);
 }
 // resident one-wave blocks per CU of the persistent per-rank kernel
 hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu);
@@ -174,6 +182,10 @@ hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, flo
 hipError_t launch_composite_over_sparse(const float4 *imgs, const CompositeArgs &c, int32_t W,
                                         int32_t x0, int32_t y0, int32_t x1, int32_t y1,
                                         float4 *out, uint32_t *out8, hipStream_t s);
+// the same over compact rank images (c.cbase, RanksArgs::compact)
+hipError_t launch_composite_over_compact(const float4 *imgs, const CompositeArgs &c, int32_t W,
+                                         int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                                         float4 *out, uint32_t *out8, hipStream_t s);
 
 hipError_t launch_composite_over_rgba8(const uint32_t *imgs, const CompositeArgs &c,
                                        uint32_t *out8, hipStream_t s);

@@ -651,8 +651,11 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
   const MarchTables tb = load_tables(a, smem);
   RayRec *recs = reinterpret_cast<RayRec *>(smem + march_tables_bytes(a) / sizeof(float4) + 1);
   int32_t *roff = reinterpret_cast<int32_t *>(recs + 64);
+  int32_t *cbase = roff + a.P + 1;
   const int lane = (int)threadIdx.x, w = (int)blockIdx.x, nw = (int)gridDim.x;
   for (int i = lane; i <= a.P; i += 64) roff[i] = q.run_off[i];
+  if (q.compact)
+    for (int i = lane; i < a.P; i += 64) cbase[i] = q.cbase[i];
   __syncthreads();
   const int runs = q.run_off[a.P];
   // this wave's part [r0, r1) of the run list and its place among the
@@ -669,7 +672,14 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
     const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
                              ? (float)(clock64() - clock_begin) * heat_scale
                              : 0.f;
-    write_rank_pixel(a, rank_index(a, r.tag, r.px, r.py), r.px, r.py, acc, z, ns, heat_t);
+    long long o;
+    if (q.compact) {  // the pixel's place in its slot's rectangle
+      const int32_t *rc = tb.rect + 4 * r.tag;
+      o = cbase[r.tag] + (long long)(r.py - rc[1]) * (rc[2] - rc[0]) + (r.px - rc[0]);
+    } else {
+      o = rank_index(a, r.tag, r.px, r.py);
+    }
+    write_rank_pixel(a, o, r.px, r.py, acc, z, ns, heat_t);
   };
   // set-up pass p: lane l takes pixel l % kRankRun of the wave's run
   // kRunsPerPass p + l / kRankRun; returns the number of records (wave-uniform)
@@ -721,7 +731,7 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
   const bool heat = a.flags & ALOHA_FLAG_HEATMAP;
   const bool plain = !(a.flags & (ALOHA_FLAG_HEATMAP | ALOHA_FLAG_CROSSHAIR)) &&
                      !(a.accum && a.accum_id > 0);
-  const int rank_rows = a.P * a.H;
+  const int rank_rows = q.compact ? 0 : a.P * a.H;  // compact images: nothing outside
   const int rows = rank_rows + (q.frame || q.frame8 ? a.H : 0);
   for (int r = w; r < rows; r += nw) {
     if (r >= rank_rows) {

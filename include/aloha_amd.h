@@ -230,6 +230,23 @@ int aloha_render_frame(aloha_ctx *ctx, const aloha_render_params *params, int32_
                        const aloha_images *images, const int32_t *order, float *d_out_rgba,
                        uint32_t *d_out_rgba8);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The frame of aloha_render_frame (same outputs, bit for
+ * bit) without caller-held rank images: each rank's float RGBA image is kept
+ * only inside its pixel rectangle (aloha_rank_rects), in a workspace of
+ * ALOHA_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends.  Allocates nothing.  ALOHA_ERR_LIMIT when the
+ * rectangles' pixels x 16 bytes exceed the workspace; accum_id must be 0 and
+ * the heat map off (ALOHA_ERR_ARG).  Every call on one GPU uses the same
+ * workspace, which keeps the last call's contents: calls that overlap on one
+ * GPU (other streams or contexts) race; calls on one stream do not. */
+#define ALOHA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int aloha_render_frame_noalloc(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                               int32_t H, int32_t P, const int32_t *rank_offsets,
+                               const int32_t *brick_ids, const int32_t *order, float *d_out_rgba,
+                               uint32_t *d_out_rgba8);
+
 /* Schedule of the per-rank (aloha_render_ranks) and fused
  * (aloha_render_composite) march launches: lanes per ray, 1, 2, 4 or 8.  With
  * n > 1 the n lanes of a group evaluate n consecutive samples of one ray at
