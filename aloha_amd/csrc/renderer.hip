@@ -357,7 +357,7 @@ void Renderer::render(uint32_t *fb) {
   p.accum_id = accumID++;
   p.num_islands = numIslands;
   p.jitter_mode = ALOHA_JITTER_RANDOM;
-  p.flags = heatMapEnabled ? ALOHA_FLAG_HEATMAP : 0;
+  p.flags = (heatMapEnabled ? ALOHA_FLAG_HEATMAP : 0) | (crossHairs ? ALOHA_FLAG_CROSSHAIR : 0);
   p.heat_map_scale = heatMapScale;
   const int W = fbSize.x, H = fbSize.y;
   const size_t npix = (size_t)W * H;
@@ -392,9 +392,14 @@ void Renderer::render(uint32_t *fb) {
     check(aloha_rank_rects(ctx_, &p, W, H, islandSize_, offs.data(), ids.data(), rects.data()));
     std::vector<int32_t> all;
     for (int i = 0; i < numIslands; ++i) all.insert(all.end(), rects.begin(), rects.end());
+    const bool display = commWorld.rank == displayRank;
     check(aloha_composite_over_sparse(ctx_, d_rank_rgba_, commWorld.size, order.data(),
                                       all.data(), W, H, d_comp_,
-                                      commWorld.rank == displayRank ? fb : nullptr));
+                                      display && !ubyteComposite ? fb : nullptr));
+    // IceT's RGBA_UBYTE blend of the ranks' 8-bit frames (:908-916, :1066-1077)
+    if (display && ubyteComposite)
+      check(aloha_composite_over_rgba8(ctx_, d_rank_rgba8_, commWorld.size, order.data(), W, H,
+                                       fb));
   }
   check(aloha_sync(ctx_));
   if (commWorld.rank != displayRank) {

@@ -5,7 +5,7 @@
 //   aloha_headless in.vol|in.tri -ranks P [-islands I] [-rank R] [-size W H]
 //                  [-camera ox oy oz d00x d00y d00z dux duy duz dvx dvy dvz]
 //                  [-tf tf.f32 (n x RGBA float; volumes)] [-dt dt] [-spp n]
-//                  [-frames F] [-o out]
+//                  [-frames F] [-float-composite] [-no-crosshair] [-o out]
 //     -> out.rgba8 (W*H u32) and out.rgba (W*H*4 f32) for volumes,
 //        out.rgba8 and out.depth (W*H f32) for meshes
 #include <hip/hip_runtime.h>
@@ -22,6 +22,7 @@ int main(int argc, char **argv) {
   std::string in, out = "frame", tf_path;
   int P = 1, islands = 1, rank = 0, frames = 1, W = 512, H = 512;
   float dt = 2.f;
+  bool ubyte = true, crosshair = true;  // the reference's frame (Renderer defaults)
   std::vector<float> cam;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -33,6 +34,8 @@ int main(int argc, char **argv) {
     else if (a == "-spp") maui::Renderer::spp = std::atoi(argv[++i]);
     else if (a == "-dt") dt = std::atof(argv[++i]);
     else if (a == "-tf") tf_path = argv[++i];
+    else if (a == "-float-composite") ubyte = false;
+    else if (a == "-no-crosshair") crosshair = false;
     else if (a == "-o") out = argv[++i];
     else if (a == "-size") {
       W = std::atoi(argv[++i]);
@@ -53,6 +56,8 @@ int main(int argc, char **argv) {
   try {
     maui::Renderer r(rank, P, islands, in);
     r.setDt(dt);
+    r.ubyteComposite = ubyte;
+    r.crossHairs = crosshair;
     r.resize({W, H});
     if (!tf_path.empty()) {
       std::ifstream tf(tf_path, std::ios::binary | std::ios::ate);

@@ -129,6 +129,15 @@ struct Renderer : mpi::DistributedRenderer {
   int islandID = 0;
   int numIslands = 1;
   int displayRank = 0;
+  // The reference's data path (the defaults): each rank's frame is packed to
+  // RGBA8 with the crosshair inverted on row H/2 and column W/2
+  // (deviceCode.cpp:410-416), and IceT blends those bytes (RGBA_UBYTE,
+  // waikikiRenderer.cpp:908-916).  ubyteComposite = false blends the ranks'
+  // float images and packs once (up to 2/255 apart from the reference's
+  // bytes); crossHairs = false leaves the rank images without the crosshair.
+  // compositeRGBA() is the float blend either way.
+  bool ubyteComposite = true;
+  bool crossHairs = true;
 
   static int spp;
   static bool heatMapEnabled;

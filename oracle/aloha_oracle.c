@@ -776,15 +776,50 @@ int orc_composite_zmin(const uint32_t *imgs, const float *depths, int P, long lo
 /* The HIP differentiable forward marches the closed-form lattice
  * t_k = dt*(j + (k0+k)) with opacity clamped to DIFF_AMAX; restated here in
  * float.  The adjoint is restated in float64 data arithmetic on the same
- * float control (same samples): it stores the samples of a ray and computes
+ * float control (same samples, same discrete decisions of the data path:
+ * at a kink -- N.d = 0, a TF texel centre, the opacity clamp -- the float64
+ * functions take the side the float evaluation took, so a float32 adjoint of
+ * the same samples is compared with the derivative on its own branch): it
+ * stores the samples of a ray and computes
  * transmittance forward as a product, then back-propagates with suffix
  * sums -- a different method from the HIP kernel's backward inversion. */
 #define DIFF_AMAX 0.9999f
 
+/* One sample of the float control: its brick, t, and the data path's
+ * discrete decisions as the float march took them -- the sign of N.d (the
+ * side of the |N.d| kink), the TF segment i0 and whether the opacity clamp
+ * applied -- which the float64 functions below follow (as they follow the
+ * float sample lattice), so that they and their derivatives sit on the same
+ * branches as any float32 evaluation of the same samples. */
 typedef struct {
   int brick;
   float t;
+  int i0;
+  signed char sgn;
+  signed char clamped;
 } diff_sample;
+
+/* TF segment index of value v as tf_f takes it. */
+static inline int tf_segment_f(const orc_params *p, float v) {
+  const int n = p->tf_n;
+  const float xt = v * (float)n - 0.5f;
+  const float fi = floorf(xt);
+  float fc = fi < -2.f ? -2.f : (fi > (float)n ? (float)n : fi);
+  if (!(fi == fi)) fc = -2.f;
+  return (int)fc;
+}
+
+/* TF lookup in double on a given segment i (w may leave [0,1] by rounding:
+ * the segment's line is extended). */
+static inline void tf_d_seg(const orc_params *p, double v, int i, double out[4]) {
+  const int n = p->tf_n;
+  const double w = (v * (double)n - 0.5) - (double)i;
+  for (int c = 0; c < 4; ++c) {
+    const double e0 = (i >= 0 && i < n) ? p->tf[4 * i + c] : 0.;
+    const double e1 = (i + 1 >= 0 && i + 1 < n) ? p->tf[4 * (i + 1) + c] : 0.;
+    out[c] = lerpd_(e0, e1, w);
+  }
+}
 
 /* float control of one pixel: appends its samples (brick, t) to out (cap),
  * returns the count; fills the float colour. */
@@ -817,7 +852,8 @@ static int diff_march_pixel(const orc_params *p, const bview *bv, int nb, const 
     for (int k = 0;; ++k) {
       const float t = dt * (j + (k0 + (float)k));
       if (!(t < bt1 && c[3] < .99f)) break;
-      if (n < cap) { out[n].brick = hit; out[n].t = t; }
+      diff_sample *rec = n < cap ? &out[n] : NULL;
+      if (rec) { rec->brick = hit; rec->t = t; }
       ++n;
       float x = (o[0] + t * d[0]) - vl[0];
       float y = (o[1] + t * d[1]) - vl[1];
@@ -829,9 +865,17 @@ static int diff_march_pixel(const orc_params *p, const bview *bv, int nb, const 
       float len = sqrtf(ex * ex + ey * ey + ez * ez);
       float nx = gx / len, ny = gy / len, nz = gz / len;
       float s[4];
-      tf_f(p, tri_f(&bv[hit], x, y, z), s);
+      const float val = tri_f(&bv[hit], x, y, z);
+      tf_f(p, val, s);
+      const int clamped = !(s[3] < DIFF_AMAX);
       s[3] = fminf(s[3], DIFF_AMAX);
-      float shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
+      const float ndf = nx * d[0] + ny * d[1] + nz * d[2];
+      float shade = .1f + .9f * fabsf(ndf);
+      if (rec) {
+        rec->i0 = tf_segment_f(p, val);
+        rec->sgn = (signed char)(ndf > 0.f ? 1 : (ndf < 0.f ? -1 : 0));
+        rec->clamped = (signed char)clamped;
+      }
       float w = (1.f - c[3]) * s[3];
       c[0] = c[0] + (w * s[0]) * shade;
       c[1] = c[1] + (w * s[1]) * shade;
@@ -945,11 +989,11 @@ int orc_diff_adjoint(const orc_params *p, const orc_brick *bricks, int nb, const
                           (gz + 1e-4) * (gz + 1e-4));
           double val = tri_d(v, X, Y, Z);
           double s[4];
-          tf_d(p, val, s);
+          tf_d_seg(p, val, smp[k].i0, s);
           double araw = s[3];
-          double a = araw < DIFF_AMAX ? araw : DIFF_AMAX;
+          double a = smp[k].clamped ? DIFF_AMAX : araw;
           double nd = (gx / L) * d[0] + (gy / L) * d[1] + (gz / L) * d[2];
-          double sh = .1 + .9 * fabs(nd);
+          double sh = .1 + .9 * (smp[k].sgn * nd);
           double *r = A + 12 * (size_t)k;
           r[0] = a; r[1] = s[0]; r[2] = s[1]; r[3] = s[2]; r[4] = sh; r[5] = T;
           r[6] = val; r[7] = gx; r[8] = gy; r[9] = gz; r[10] = L; r[11] = araw;
@@ -971,12 +1015,10 @@ int orc_diff_adjoint(const orc_params *p, const orc_brick *bricks, int nb, const
           const double dshade = Ta * Gc;
           double dalpha = Tk * Gcs - (GS - G[3] * TN) / (1.0 - a);
           for (int q = 0; q < 3; ++q) S[q] += Ta * cc[q] * sh;
-          if (!(r[11] < DIFF_AMAX)) dalpha = 0.0;
-          /* TF entries */
-          const double xt = val * n_tf - 0.5;
-          const double fi = floor(xt);
-          const double w = xt - fi;
-          const int i0 = (int)(fi < -2 ? -2 : (fi > n_tf ? n_tf : fi));
+          if (smp[k].clamped) dalpha = 0.0;
+          /* TF entries, on the float march's segment */
+          const int i0 = smp[k].i0;
+          const double w = (val * n_tf - 0.5) - (double)i0;
           const double g4[4] = {dc[0], dc[1], dc[2], dalpha};
           for (int ch = 0; ch < 4; ++ch) {
             if (i0 >= 0 && i0 < n_tf) {
@@ -1002,8 +1044,7 @@ int orc_diff_adjoint(const orc_params *p, const orc_brick *bricks, int nb, const
           double dv = 0;
           for (int ch = 0; ch < 4; ++ch) dv += g4[ch] * (e1[ch] - e0[ch]);
           dv *= n_tf;
-          const double nd = (g3[0] / L) * d[0] + (g3[1] / L) * d[1] + (g3[2] / L) * d[2];
-          const double sg = nd > 0 ? .9 : (nd < 0 ? -.9 : 0.0);
+          const double sg = .9 * smp[k].sgn;
           double dN[3], dNg = 0;
           for (int q = 0; q < 3; ++q) { dN[q] = dshade * sg * d[q]; dNg += dN[q] * g3[q]; }
           double dg[3];
@@ -1027,8 +1068,10 @@ int orc_diff_adjoint(const orc_params *p, const orc_brick *bricks, int nb, const
   return 0;
 }
 
-/* float64 colour of the differentiable forward on the float control (the
- * function whose derivatives orc_diff_adjoint computes); out: H*W*4. */
+/* float64 colour of the differentiable forward on the float control -- the
+ * float march's samples and its discrete decisions (sign of N.d, TF
+ * segment, opacity clamp) -- the function whose derivatives orc_diff_adjoint
+ * computes; out: H*W*4. */
 int orc_diff_render64(const orc_params *p, const orc_brick *bricks, int nb, double *out,
                       int nthreads) {
   bview bv[256];
@@ -1063,9 +1106,9 @@ int orc_diff_render64(const orc_params *p, const orc_brick *bricks, int nb, doub
           double L = sqrt((gx + 1e-4) * (gx + 1e-4) + (gy + 1e-4) * (gy + 1e-4) +
                           (gz + 1e-4) * (gz + 1e-4));
           double s[4];
-          tf_d(p, tri_d(v, X, Y, Z), s);
-          double a = s[3] < DIFF_AMAX ? s[3] : DIFF_AMAX;
-          double sh = .1 + .9 * fabs((gx / L) * d[0] + (gy / L) * d[1] + (gz / L) * d[2]);
+          tf_d_seg(p, tri_d(v, X, Y, Z), smp[k].i0, s);
+          double a = smp[k].clamped ? DIFF_AMAX : s[3];
+          double sh = .1 + .9 * (smp[k].sgn * ((gx / L) * d[0] + (gy / L) * d[1] + (gz / L) * d[2]));
           for (int q = 0; q < 3; ++q) C[q] += T * a * s[q] * sh;
           T *= 1.0 - a;
         }

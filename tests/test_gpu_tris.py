@@ -213,7 +213,7 @@ def test_headless_tri_renderer_api(tmp_path):
     wl.mesh.write_tri(str(path))
     cam = [str(float(x)) for k in ("org", "dir_00", "dir_du", "dir_dv") for x in wl.cam[k]]
     exe = Path(A.__file__).resolve().parent / "bin" / "aloha_headless"
-    refs = [_oracle_rank(wl, r) for r in range(4)]
+    refs = [_oracle_rank(wl, r, crosshair=True) for r in range(4)]  # the Renderer default
     o8, oz = O.composite_zmin(np.stack([x["rgba8"] for x in refs]), np.stack([x["depth"] for x in refs]))
     for rank in (0, 1):
         out = tmp_path / f"f{rank}"

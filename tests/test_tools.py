@@ -43,9 +43,15 @@ def test_chopsuey_usage_errors(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("P,islands,rank", [(2, 1, 0), (4, 2, 0), (2, 1, 1)])
-def test_headless_renderer_api_matches_oracle(tmp_path, P, islands, rank):
-    print(f"\n[case] aloha_headless P={P} islands={islands} rank={rank}", flush=True)
+@pytest.mark.parametrize("P,islands,rank,mode", [(2, 1, 0, "ubyte"), (4, 2, 0, "ubyte"),
+                                                 (2, 1, 1, "ubyte"), (2, 1, 0, "float"),
+                                                 (4, 2, 0, "float")])
+def test_headless_renderer_api_matches_oracle(tmp_path, P, islands, rank, mode):
+    """maui::Renderer through aloha_headless against the oracle.  "ubyte" is
+    the reference's frame (its defaults): rank frames packed to RGBA8 with the
+    crosshair, blended in 8 bits; "float" blends the float rank images and
+    packs once, without the crosshair."""
+    print(f"\n[case] aloha_headless P={P} islands={islands} rank={rank} {mode}", flush=True)
     from aloha_amd import workloads as WL
     wl = WL.blobs128(num_ranks=P // islands, W=96, H=80, size=48)
     vols = [A.volume_extract(wl.raw, 1, b.voxel_range)[0] for b in wl.bricks]
@@ -56,10 +62,11 @@ def test_headless_renderer_api_matches_oracle(tmp_path, P, islands, rank):
     wl.tf.tofile(tmp_path / "tf.f32")
     cam = [str(float(x)) for k in ("org", "dir_00", "dir_du", "dir_dv") for x in wl.cam[k]]
     out = tmp_path / "frame"
+    extra = [] if mode == "ubyte" else ["-float-composite", "-no-crosshair"]
     r = subprocess.run([str(BIN / "aloha_headless"), str(path), "-ranks", str(P), "-islands",
                         str(islands), "-rank", str(rank), "-size", str(wl.W), str(wl.H),
                         "-camera", *cam, "-tf", str(tmp_path / "tf.f32"), "-dt", "0.5",
-                        "-frames", "2", "-o", str(out)], capture_output=True, text=True,
+                        "-frames", "2", *extra, "-o", str(out)], capture_output=True, text=True,
                        timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
@@ -71,15 +78,22 @@ def test_headless_renderer_api_matches_oracle(tmp_path, P, islands, rank):
     for i in range(islands):
         for r_ in range(isz):
             bl = [(wl.bricks[b], vols[b]) for b in wl.rank_bricks[r_]]
+            cross = mode == "ubyte"
             f0 = O.render_rank(wl.cam, wl.tf, bl, wl.W, wl.H, dt=0.5, max_depth=wl.max_depth,
-                               island_id=i, num_islands=islands)
+                               island_id=i, num_islands=islands, crosshair=cross)
             f1 = O.render_rank(wl.cam, wl.tf, bl, wl.W, wl.H, dt=0.5, max_depth=wl.max_depth,
-                               island_id=i, num_islands=islands, accum_id=1, accum=f0["accum"])
+                               island_id=i, num_islands=islands, accum_id=1, accum=f0["accum"],
+                               crosshair=cross)
             imgs.append(f1)
     order = [i * isz + r_ for i in range(islands) for r_ in wl.order]
     ref, ref8 = O.composite_over(np.stack([o["rgba"] for o in imgs]), order)
     assert np.abs(got - ref).max() <= 1e-4
-    if rank == 0:
-        assert np.abs(got8.view(np.uint8).astype(int) - ref8.view(np.uint8).astype(int)).max() <= 1
-    else:
+    if rank != 0:
         assert np.array_equal(got8, imgs[rank]["rgba8"])
+    elif mode == "ubyte":
+        # IceT's 8-bit blend of the ranks' RGBA8 frames
+        ref8u = O.composite_over_u8(np.stack([o["rgba8"] for o in imgs]), order)
+        assert np.abs(got8.view(np.uint8).astype(int) - ref8u.view(np.uint8).astype(int)).max() <= 1
+        assert (got8 == ref8u).mean() > 0.999
+    else:
+        assert np.abs(got8.view(np.uint8).astype(int) - ref8.view(np.uint8).astype(int)).max() <= 1
