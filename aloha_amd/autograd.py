@@ -90,14 +90,24 @@ class DifferentiableVolume:
             raise ValueError(f"tf: expected (n, 4), got {tuple(tf.shape)}")
         if tuple(density.shape) != self.shape:
             raise ValueError(f"density: expected {self.shape}, got {tuple(density.shape)}")
-        tf = tf.detach().contiguous().float()
-        density = density.detach().contiguous().float()
-        self.ctx._check(lib().aloha_tf_set_device(self.ctx._h, C.c_void_p(tf.data_ptr()),
-                                                  tf.shape[0]))
+        # the context already holds these exact tensors (the backward re-marches
+        # the forward's inputs): no new copy.  A tensor changed in place since
+        # (an optimizer step) has a new version counter; the last inputs are
+        # kept referenced, so no other tensor can take their addresses.
+        key = (tf.data_ptr(), tf._version, tuple(tf.shape), density.data_ptr(),
+               density._version)
+        if key == getattr(self, "_last_key", None):
+            return
+        tf_c = tf.detach().contiguous().float()
+        den_c = density.detach().contiguous().float()
+        self.ctx._check(lib().aloha_tf_set_device(self.ctx._h, C.c_void_p(tf_c.data_ptr()),
+                                                  tf_c.shape[0]))
         self.ctx._check(lib().aloha_brick_update(self.ctx._h, self.brick,
-                                                 C.c_void_p(density.data_ptr()), F32))
-        self._last_tf = tf  # keep alive until the stream has consumed them
-        self._last_density = density
+                                                 C.c_void_p(den_c.data_ptr()), F32))
+        self._last_tf = tf_c  # keep alive until the stream has consumed them
+        self._last_density = den_c
+        self._last_key = key
+        self._last_src = (tf, density)
 
     def _forward(self, tf, density):
         self._set_inputs(tf, density)

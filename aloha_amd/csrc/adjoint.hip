@@ -25,6 +25,7 @@
 // atomics.  The forward marches a ray with a lane group (4 lanes, 4 lattice
 // samples a step); the adjoint keeps one lane per ray (its reverse walk is a
 // sequential recurrence).
+#include <algorithm>
 #include <climits>
 #include <hip/hip_runtime.h>
 
@@ -50,8 +51,8 @@ struct Visit {
 
 // The brick visit sequence of renderFrame's do/while for one ray: closest
 // reported hit among the rank's bricks, ties to the lower slot.  Returns the
-// number of visits (<= max_visits) and fills vis[] if non-null.
-__device__ int visits(const DiffArgs &a, const float org[3], const float d[3], uint32_t pixel,
+// number of visits found (<= max_visits) and sets *vis to visit `want`.
+__device__ __forceinline__ int visits(const DiffArgs &a, const float org[3], const float d[3], uint32_t pixel,
                       int max_visits, Visit *vis, int want) {
   uint32_t rng = rng_seed(pixel, 0u);
   float tmin = 0.f;
@@ -79,12 +80,11 @@ __device__ int visits(const DiffArgs &a, const float org[3], const float d[3], u
     }
     if (hit < 0) break;
     const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
-    if (nv == want || want < 0) {
+    if (nv == want && vis) {
       const float dt = a.dt;
       const float segs = floorf((bt0 - dt * j) / dt);
       const float tf = dt * (j + segs);
-      Visit v{hit, hslot, hgrad, bt0, bt1, j, tf < bt0 ? segs + 1.f : segs};
-      if (vis) vis[want < 0 ? nv : 0] = v;
+      *vis = Visit{hit, hslot, hgrad, bt0, bt1, j, tf < bt0 ? segs + 1.f : segs};
     }
     ++nv;
     tmin = bt1 + 1e-20f;
@@ -114,6 +114,7 @@ __device__ __forceinline__ BrickView view_of(const DevBrick &db) { return brick_
 struct Shaded {
   Sample s;
   float4 tf;  // TF colour, opacity clamped
+  float raw_a;  // TF opacity before the clamp
   float shade;
   float len;
 };
@@ -131,6 +132,7 @@ __device__ __forceinline__ Shaded shade_sample(const DiffArgs &a, const float4 *
   r.len = sqrtf(ex * ex + ey * ey + ez * ez);
   const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
   r.tf = tf_lookup(s_tf, a.tf_n, r.s.val);
+  r.raw_a = r.tf.w;
   r.tf.w = fminf(r.tf.w, a.alpha_max);
   r.shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
   return r;
@@ -286,54 +288,42 @@ __global__ __launch_bounds__(kBlock) void diff_forward_group_kernel(const DiffAr
   }
 }
 
-// Where a brick visit's density gradient goes.  Global float atomics run at
-// about 20 G/s on MI355X whatever the address pattern, and neighbouring rays
-// and samples hit the same voxels many times, so each workgroup first sums
-// into an LDS hash table of (brick slot, voxel) -> partial gradient and, at
-// its end, issues one global atomic per distinct voxel.  A brick whose voxel
-// index does not fit the 24-bit key, and any update whose short probe run is
-// full, goes straight to a global atomic.  Sums are reassociated either way
-// (float atomics are unordered).  Sized by measurement on the 256^3 x 8
-// views workload: 8192 slots (64 KB, two workgroups per CU) with runs of two
-// probes beat 4096 slots with eight (fewer global fall-throughs, shorter
-// chains of dependent LDS compare-and-swaps): 5.2 vs 6.7 ms forward+adjoint.
-constexpr int kCacheBits = 13;
-constexpr int kCacheSlots = 1 << kCacheBits;
-constexpr int kCacheProbes = 2;
-constexpr uint32_t kCacheEmpty = 0xFFFFFFFFu;
+// Where a density-gradient contribution goes: a sink takes voxel (x, y, z)
+// of the brick's unpadded grid (already clamped to it) and a value.
+//
+// FixedTile: the bin's 18^3 LDS tile of 64-bit fixed-point sums.  Integer
+// addition is associative, so the sums -- and the gradient -- do not depend
+// on the order the lanes add in: the adjoint is bit-reproducible.  (LDS
+// float atomics are also several times slower than integer ones on this
+// GPU.)  Each value is rounded to a multiple of 2^-scale once; the bin's
+// scale keeps the largest possible sum below 2^61.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 
-struct GradSink {
-  float *grad;     // the brick's unpadded gradient grid
-  uint32_t *keys;  // LDS table keys (null: no caching for this brick)
-  float *vals;     // LDS table partial sums
-  uint32_t tag;    // brick slot << 24
-};
-
-// LDS and global views of the table and the grid, so each add is an LDS or a
-// global atomic (a generic pointer would make both flat atomics)
-typedef __attribute__((address_space(3))) float lds_float;
-typedef __attribute__((address_space(1))) float glb_float;
-
-__device__ __forceinline__ void sink_add(const GradSink &s, uint32_t idx, float v) {
-  if (s.keys) {
-    const uint32_t key = s.tag | idx;
-    uint32_t h = (key * 2654435761u) >> (32 - kCacheBits);
-    for (int p = 0; p < kCacheProbes; ++p) {
-      const uint32_t k = atomicCAS(&s.keys[h], kCacheEmpty, key);
-      if (k == kCacheEmpty || k == key) {
-        __hip_atomic_fetch_add((lds_float *)s.vals + h, v, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        return;
-      }
-      h = (h + 1u) & (uint32_t)(kCacheSlots - 1);
-    }
-  }
-  __hip_atomic_fetch_add((glb_float *)s.grad + idx, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ long long to_fixed(float v, double scale) {
+  return __double2ll_rn((double)v * scale);
 }
 
+struct FixedTile {
+  unsigned long long *tile;  // LDS, kTileVox
+  int x0, y0, z0;            // the tile's first voxel per axis
+  double scale;              // 2^S of the bin
+  __device__ __forceinline__ void add(int x, int y, int z, float v) const {
+    const int lx = x - x0, ly = y - y0, lz = z - z0;
+    // every voxel a bin's sample reads lies in its tile (the bin is keyed
+    // by the sample's lowest corner); anything else is dropped, not written
+    // out of bounds
+    if ((unsigned)lx < (unsigned)kTileEdge && (unsigned)ly < (unsigned)kTileEdge &&
+        (unsigned)lz < (unsigned)kTileEdge)
+      __hip_atomic_fetch_add((lds_u64 *)tile + (lz * kTileEdge + ly) * kTileEdge + lx,
+                             (unsigned long long)to_fixed(v, scale), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+};
+
 // Scatter d(loss)/d(tap value) onto the 8 corners of one trilinear tap.
-__device__ __forceinline__ void scatter_tap(const GradSink &sink, const DevBrick &db, float x,
-                                            float y, float z, float g) {
+template <class Sink>
+__device__ __forceinline__ void scatter_tap(const Sink &sink, const DevBrick &db, float x, float y,
+                                            float z, float g) {
   if (g == 0.f) return;
   const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
   const float wx = x - fx, wy = y - fy, wz = z - fz;
@@ -351,7 +341,7 @@ __device__ __forceinline__ void scatter_tap(const GradSink &sink, const DevBrick
       for (int i = 0; i < 2; ++i) {
         const int xx = clampi(ix + i, 0, nx - 1);
         const float w = (i ? wx : 1.f - wx) * w_yz;
-        sink_add(sink, (uint32_t)(((size_t)zz * ny + yy) * nx + xx), g * w);
+        sink.add(xx, yy, zz, g * w);
       }
     }
   }
@@ -369,13 +359,11 @@ __device__ __forceinline__ void add_x(float row[3], int scx, float v) {
   }
 }
 
-__device__ __forceinline__ void atomic_voxel(const GradSink &sink, const DevBrick &db, int x, int y,
-                                             int z, float v) {
+template <class Sink>
+__device__ __forceinline__ void add_voxel(const Sink &sink, const DevBrick &db, int x, int y, int z,
+                                          float v) {
   if (v == 0.f) return;
-  x = clampi(x, 0, db.n[0] - 1);
-  y = clampi(y, 0, db.n[1] - 1);
-  z = clampi(z, 0, db.n[2] - 1);
-  sink_add(sink, (uint32_t)(((size_t)z * db.n[1] + y) * db.n[0] + x), v);
+  sink.add(clampi(x, 0, db.n[0] - 1), clampi(y, 0, db.n[1] - 1), clampi(z, 0, db.n[2] - 1), v);
 }
 
 // Scatter of all seven taps of one sample through the shared 20-voxel
@@ -386,8 +374,9 @@ __device__ __forceinline__ void atomic_voxel(const GradSink &sink, const DevBric
 // to float reassociation.  gV: d/d(value tap); gd: d/d(gradient component),
 // i.e. +gd for the +0.5 tap and -gd for the -0.5 tap of each axis.  Returns
 // false, scattering nothing, when the footprint does not fit.
-__device__ __forceinline__ bool scatter_fast(const GradSink &sink, const DevBrick &db, float x,
-                                             float y, float z, float gV, const float gd[3]) {
+template <class Sink>
+__device__ __forceinline__ bool scatter_fast(const Sink &sink, const DevBrick &db, float x, float y,
+                                             float z, float gV, const float gd[3]) {
   const float xm = x - .5f, xp = x + .5f;
   const float ym = y - .5f, yp = y + .5f;
   const float zm = z - .5f, zp = z + .5f;
@@ -469,196 +458,495 @@ __device__ __forceinline__ bool scatter_fast(const GradSink &sink, const DevBric
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) atomic_voxel(sink, db, ax + i, cy + jj, cz + kk, A[kk][jj][i]);
+      for (int i = 0; i < 3; ++i) add_voxel(sink, db, ax + i, cy + jj, cz + kk, A[kk][jj][i]);
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii) atomic_voxel(sink, db, cx + ii, yn, cz + kk, YN[kk][ii]);
+    for (int ii = 0; ii < 2; ++ii) add_voxel(sink, db, cx + ii, yn, cz + kk, YN[kk][ii]);
 #pragma unroll
   for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii) atomic_voxel(sink, db, cx + ii, cy + jj, zn, ZN[jj][ii]);
+    for (int ii = 0; ii < 2; ++ii) add_voxel(sink, db, cx + ii, cy + jj, zn, ZN[jj][ii]);
   return true;
 }
 
-// Adds a run's TF gradient (entries pi and pi + 1; pi == INT_MIN: none) to
-// the workgroup's LDS sums.  Consecutive samples of a ray mostly fall
-// between the same two TF entries, so the backward sums them in registers
-// and adds once per run: 8 LDS float atomics per run instead of per sample
-// (measured on 256^3, 8 views of 512^2: fwd+adjoint 5.09 -> 4.92 ms, the
-// same as with no TF atomics at all).
-__device__ __forceinline__ void tf_flush(float *s_dtf, int n, int pi, const float p0[4],
-                                         const float p1[4]) {
-  if (pi == INT_MIN) return;
-#pragma unroll
-  for (int ch = 0; ch < 4; ++ch) {
-    if (pi >= 0 && pi < n) atomicAdd(&s_dtf[4 * pi + ch], p0[ch]);
-    if (pi + 1 >= 0 && pi + 1 < n) atomicAdd(&s_dtf[4 * (pi + 1) + ch], p1[ch]);
-  }
+// Samples of visit v on the lattice before its exit t1, at most limit.
+__device__ __forceinline__ uint32_t visit_count(const DiffArgs &a, const Visit &v, uint32_t limit) {
+  uint32_t cnt = 0;
+  while (cnt < limit && lattice_t(a, v, (int)cnt) < v.t1) ++cnt;
+  return cnt;
 }
 
-__global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a) {
-  extern __shared__ float4 s_mem[];
-  float4 *s_tf = s_mem;
-  float *s_dtf = (float *)(s_mem + a.tf_n + 4);  // tf_n x 4
-  uint32_t *s_keys = (uint32_t *)(s_dtf + 4 * a.tf_n);
-  float *s_vals = (float *)(s_keys + kCacheSlots);
-  load_tf(a, s_tf);
-  for (int i = threadIdx.x; i < 4 * a.tf_n; i += kBlock) s_dtf[i] = 0.f;
-  for (int i = threadIdx.x; i < kCacheSlots; i += kBlock) {
-    s_keys[i] = kCacheEmpty;
-    s_vals[i] = 0.f;
+// The bin of a sample at brick-texel position (x, y, z) of brick slot `slot`.
+__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const DevBrick &db, float x,
+                                          float y, float z) {
+  const int ax = clampi((int)floorf(x - .5f), -1, db.n[0]);
+  const int ay = clampi((int)floorf(y - .5f), -1, db.n[1]);
+  const int az = clampi((int)floorf(z - .5f), -1, db.n[2]);
+  const int bx = (ax + 1) / kBinEdge, by = (ay + 1) / kBinEdge, bz = (az + 1) / kBinEdge;
+  return b.base[slot] + (bz * b.nby[slot] + by) * b.nbx[slot] + bx;
+}
+
+__device__ __forceinline__ void sample_pos(const float org[3], const float d[3], const DevBrick &db,
+                                           float t, float &x, float &y, float &z) {
+  x = (org[0] + t * d[0]) - db.vox_lo[0];
+  y = (org[1] + t * d[1]) - db.vox_lo[1];
+  z = (org[2] + t * d[2]) - db.vox_lo[2];
+}
+
+// Count pass: every sample the forward took (nsamples per pixel), counted
+// into its bin.  A ray's consecutive samples
+// mostly share a bin, so each run of one bin costs one atomic.
+__global__ __launch_bounds__(kBlock) void diff_count_kernel(const DiffArgs a, const DiffBins b) {
+  const int view = blockIdx.z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+  const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+  if (px >= a.W || py >= a.H) return;
+  const size_t out = (size_t)view * a.W * a.H + (px + (size_t)a.W * py);
+  const uint32_t N = a.nsamples[out];
+  if (N == 0) return;
+  const int pixel = px + a.W * py;
+  const float *org = a.cam[view];
+  float d[3];
+  ray_dir(a, view, px, py, d);
+  uint32_t acc = 0, run = 0;
+  int run_bin = -1;
+  for (int vi = 0; acc < N && vi < kMaxVisits; ++vi) {
+    Visit v;
+    if (visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi) <= vi) break;
+    const uint32_t cnt = visit_count(a, v, N - acc);
+    const DevBrick db = a.bricks[v.bid];
+    for (uint32_t k = 0; k < cnt; ++k) {
+      float x, y, z;
+      sample_pos(org, d, db, lattice_t(a, v, (int)k), x, y, z);
+      const int bin = sample_bin(b, v.slot, db, x, y, z);
+      if (bin != run_bin) {
+        if (run) atomicAdd(&b.count[run_bin], run);
+        run_bin = bin;
+        run = 0;
+      }
+      ++run;
+    }
+    acc += cnt;
   }
+  if (run) atomicAdd(&b.count[run_bin], run);
+}
+
+// Exclusive prefix of the bin counts (one workgroup): off[] and the record
+// cursors; off[nbins] is the record total.
+__global__ __launch_bounds__(1024) void diff_scan_kernel(const DiffBins b) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x, n = b.nbins;
+  const int per = (n + 1023) / 1024;
+  const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+  uint32_t sum = 0;
+  for (int i = i0; i < i1; ++i) sum += b.count[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = t ? part[t - 1] : 0u;
+  for (int i = i0; i < i1; ++i) {
+    b.off[i] = run;
+    b.cursor[i] = run;
+    run += b.count[i];
+  }
+  if (t == 1023) b.off[n] = part[1023];
+}
+
+// The adjoint march: per pixel, the forward's samples in reverse (T_k =
+// T_{k+1}/(1-a_k)), back-propagated through compositing, shading and the TF
+// lookup.  Each sample becomes a record in its bin: its position, the
+// derivative of the loss with respect to its value tap and to its gradient
+// (through N = g/|g+1e-4|), its value (which fixes the two TF entries it
+// read) and the derivative with respect to the TF colour and opacity it got;
+// diff_bin_kernel scatters them.
+//
+// One lane per ray, 8x8 pixels per wave.  Rays are short at these
+// opacities (~13 samples per ray that reaches the volume), so the per-ray
+// set-up weighs as much as the samples.  Measured on 256^3 x 8 views of
+// 512^2: 1.36 ms; a lane group per ray, which repeats the set-up on every
+// lane of the group, 1.53 ms (4 lanes); persistent lanes taking pixels of a
+// 16x16 region per wave as their rays end, whose loop trips mix set-up and
+// sample code and which leave a quarter of the waves, 2.5 ms.  Records of
+// one bin run (consecutive samples of a visit in one bin) are reserved with
+// one atomic.
+__global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a, const DiffBins b) {
+  extern __shared__ float4 s_tf[];
+  load_tf(a, s_tf);
   __syncthreads();
 
   const int view = blockIdx.z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
   const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
-  const bool live = px < a.W && py < a.H;
-  uint32_t N = 0;
-  size_t out = 0;
-  // TF gradient of the current run of samples on one entry pair (pi, pi+1)
-  int pi = INT_MIN;
-  float p0[4] = {0.f, 0.f, 0.f, 0.f}, p1[4] = {0.f, 0.f, 0.f, 0.f};
-  if (live) {
-    out = (size_t)view * a.W * a.H + (px + a.W * py);
-    N = a.nsamples[out];
-  }
-  if (N > 0) {
-    const int pixel = px + a.W * py;
-    const float *org = a.cam[view];
-    float d[3];
-    ray_dir(a, view, px, py, d);
-    const float4 C = a.rgba[out];
-    const float4 G = a.dldc[out];
-    // visits and their sample counts: all but the last run to t1
-    const int nv = visits(a, org, d, (uint32_t)pixel, kMaxVisits, nullptr, -1);
-    int last = -1;
-    uint32_t before_last = 0, acc = 0;
-    for (int vi = 0; vi < nv && last < 0; ++vi) {
-      Visit v;
-      visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
-      uint32_t cnt = 0;
-      while (lattice_t(a, v, (int)cnt) < v.t1) ++cnt;
-      if (acc + cnt >= N) {
-        last = vi;
-        before_last = acc;
-      }
-      acc += cnt;
+  if (px >= a.W || py >= a.H) return;
+  const size_t out = (size_t)view * a.W * a.H + (px + a.W * py);
+  const uint32_t N = a.nsamples[out];
+  if (N == 0) return;
+  const int pixel = px + a.W * py;
+  const float *org = a.cam[view];
+  float d[3];
+  ray_dir(a, view, px, py, d);
+  const float4 C = a.rgba[out];
+  const float4 G = a.dldc[out];
+  // the visit holding the last sample, and the samples before it
+  int last = -1;
+  uint32_t before_last = 0, acc = 0;
+  for (int vi = 0; vi < kMaxVisits && last < 0; ++vi) {
+    Visit v;
+    if (visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi) <= vi) break;
+    const uint32_t cnt = visit_count(a, v, N - acc);
+    if (acc + cnt >= N) {
+      last = vi;
+      before_last = acc;
     }
-    float T = 1.f - C.w;  // transmittance after the last sample
-    const float TN = T;
-    float Sr = 0.f, Sg = 0.f, Sb = 0.f;  // suffix sums of later contributions
-    for (int vi = last; vi >= 0; --vi) {
-      Visit v;
-      visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
-      uint32_t cnt;
-      if (vi == last) {
-        cnt = N - before_last;
-      } else {
-        cnt = 0;
-        while (lattice_t(a, v, (int)cnt) < v.t1) ++cnt;
+    acc += cnt;
+  }
+  float T = 1.f - C.w;  // transmittance after the last sample
+  const float TN = T;
+  float Sr = 0.f, Sg = 0.f, Sb = 0.f;  // suffix sums of later contributions
+  uint32_t rec = 0, rec_end = 0, run_left = 0;  // records of the current bin run
+  const int n_tf = a.tf_n;
+  for (int vi = last; vi >= 0; --vi) {
+    Visit v;
+    visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
+    const uint32_t cnt = vi == last ? N - before_last : visit_count(a, v, 0xFFFFFFFFu);
+    const DevBrick db = a.bricks[v.bid];
+    for (int k = (int)cnt - 1; k >= 0; --k) {
+      const float t = lattice_t(a, v, k);
+      const Shaded r = shade_sample(a, s_tf, db, org, d, t);
+      const float am = r.tf.w;
+      const float one_m = 1.f - am;
+      const float inv_om = 1.f / one_m;
+      const float Tk = T * inv_om;
+      const float cs_r = r.tf.x * r.shade, cs_g = r.tf.y * r.shade, cs_b = r.tf.z * r.shade;
+      const float Gc = G.x * r.tf.x + G.y * r.tf.y + G.z * r.tf.z;
+      // d/d(colour), d/d(shade), d/d(opacity)
+      const float Ta = Tk * am;
+      const float dcr = G.x * Ta * r.shade, dcg = G.y * Ta * r.shade, dcb = G.z * Ta * r.shade;
+      const float dshade = Ta * Gc;
+      const float dalpha = Tk * (G.x * cs_r + G.y * cs_g + G.z * cs_b) -
+                           ((G.x * Sr + G.y * Sg + G.z * Sb) - G.w * TN) * inv_om;
+      Sr += Ta * cs_r;
+      Sg += Ta * cs_g;
+      Sb += Ta * cs_b;
+      T = Tk;
+      const float da = r.raw_a < a.alpha_max ? dalpha : 0.f;
+      // value tap through the TF slope (the entries tf_lookup read)
+      const float xt = r.s.val * (float)n_tf - 0.5f;
+      const int i0 = (int)fminf(fmaxf(floorf(xt), -2.f), (float)n_tf);
+      const float4 e0 = s_tf[i0 + 2], e1 = s_tf[i0 + 3];
+      const float dv = (dcr * (e1.x - e0.x) + dcg * (e1.y - e0.y) + dcb * (e1.z - e0.z) +
+                        da * (e1.w - e0.w)) * (float)n_tf;
+      // shading: s = .1 + .9|N.d|, N = g/L, L = |g + 1e-4|; the side of the
+      // |N.d| kink as the forward took it
+      const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
+      const float nd = nx * d[0] + ny * d[1] + nz * d[2];
+      const float sg = nd > 0.f ? .9f : (nd < 0.f ? -.9f : 0.f);
+      const float dN[3] = {dshade * sg * d[0], dshade * sg * d[1], dshade * sg * d[2]};
+      const float gv[3] = {r.s.gx, r.s.gy, r.s.gz};
+      const float dNg = dN[0] * gv[0] + dN[1] * gv[1] + dN[2] * gv[2];
+      const float il = 1.f / r.len, il3 = il * il * il;
+      float dg[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dg[c] = dN[c] * il - dNg * (gv[c] + 1e-4f) * il3;
+      float x, y, z;
+      sample_pos(org, d, db, t, x, y, z);
+      if (run_left == 0) {
+        // a new run: this sample and the earlier ones of the visit in its bin
+        // get consecutive records of the bin
+        const int bin = sample_bin(b, v.slot, db, x, y, z);
+        uint32_t len = 1;
+        for (int j = k - 1; j >= 0; --j) {
+          float xj, yj, zj;
+          sample_pos(org, d, db, lattice_t(a, v, j), xj, yj, zj);
+          if (sample_bin(b, v.slot, db, xj, yj, zj) != bin) break;
+          ++len;
+        }
+        rec = atomicAdd(&b.cursor[bin], len);
+        rec_end = min(b.off[bin + 1], b.capacity);
+        run_left = len;
       }
-      const DevBrick db = a.bricks[v.bid];
-      float *grad = v.grad;
-      const bool keyed = (int64_t)db.n[0] * db.n[1] * db.n[2] <= (1 << 24);
-      const GradSink sink{grad, keyed ? s_keys : nullptr, s_vals, (uint32_t)v.slot << 24};
-      for (int k = (int)cnt - 1; k >= 0; --k) {
-        const float t = lattice_t(a, v, k);
-        const Shaded r = shade_sample(a, s_tf, db, org, d, t);
-        const float am = r.tf.w;
-        const float one_m = 1.f - am;
-        const float Tk = T / one_m;
-        const float cs_r = r.tf.x * r.shade, cs_g = r.tf.y * r.shade, cs_b = r.tf.z * r.shade;
-        const float Gc = G.x * r.tf.x + G.y * r.tf.y + G.z * r.tf.z;
-        // d/d(colour), d/d(shade), d/d(opacity)
-        const float Ta = Tk * am;
-        const float dcr = G.x * Ta * r.shade, dcg = G.y * Ta * r.shade, dcb = G.z * Ta * r.shade;
-        const float dshade = Ta * Gc;
-        const float dalpha = Tk * (G.x * cs_r + G.y * cs_g + G.z * cs_b) -
-                             ((G.x * Sr + G.y * Sg + G.z * Sb) - G.w * TN) / one_m;
-        Sr += Ta * cs_r;
-        Sg += Ta * cs_g;
-        Sb += Ta * cs_b;
-        T = Tk;
-        // TF entries (border entries are constants)
-        const int n = a.tf_n;
-        const float xt = r.s.val * (float)n - 0.5f;
+      if (rec < rec_end) {
+        float4 *o = b.records + (size_t)kRecWords * rec;
+        o[0] = make_float4(x, y, z, dv);
+        o[1] = make_float4(dg[0], dg[1], dg[2], r.s.val);
+        o[2] = make_float4(dcr, dcg, dcb, da);
+      }
+      ++rec;
+      --run_left;
+    }
+  }
+}
+
+// Scale 2^S of a bin's fixed-point sums: at most `count` contributions of at
+// most `bound` each stay below 2^61 in magnitude.  0 when there is nothing
+// to add, -1 when the bound is not finite (then the sums are NaN).
+__device__ __forceinline__ double fixed_scale(float bound, uint32_t count) {
+  if (!(bound <= 3.0e38f)) return -1.0;
+  if (bound == 0.f) return 0.0;
+  int e;
+  frexp((double)bound * (double)count, &e);  // bound * count < 2^e
+  return ldexp(1.0, 61 - e);
+}
+
+// Block-wide maximum of non-negative floats (NaN wins); every thread gets it.
+__device__ __forceinline__ float block_max(float v, float *red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const float u = __shfl_xor(v, o);
+    v = (u > v || u != u) ? u : v;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float m = red[0];
+  for (int w = 1; w < kBlock / 64; ++w) m = (red[w] > m || red[w] != red[w]) ? red[w] : m;
+  __syncthreads();
+  return m;
+}
+
+// Adds a run's fixed-point TF sums (entries i and i + 1 of the chunk
+// [e0, e1); i == INT_MIN: no run) to the LDS sums.
+__device__ __forceinline__ void tf_run_flush(unsigned long long *tfs, int i, int e0, int e1,
+                                             const long long q0[4], const long long q1[4]) {
+  if (i == INT_MIN) return;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (i >= e0 && i < e1 && q0[c] != 0)
+      __hip_atomic_fetch_add((lds_u64 *)tfs + 4 * (i - e0) + c, (unsigned long long)q0[c],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (i + 1 >= e0 && i + 1 < e1 && q1[c] != 0)
+      __hip_atomic_fetch_add((lds_u64 *)tfs + 4 * (i + 1 - e0) + c, (unsigned long long)q1[c],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
+// One bin: its records' density gradient scattered onto an 18^3 tile (each
+// sample's 20 footprint voxels, or the 56 corners of its seven taps near a
+// brick face) and their TF gradient onto the TF entries, both as 64-bit
+// fixed-point LDS sums; then the tile goes out as floats to b.tiles and the
+// TF sums as doubles to b.tf_part.  Order-independent sums: bit-reproducible.
+__global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, const DiffBins b) {
+  extern __shared__ unsigned long long s_fix[];
+  unsigned long long *tile = s_fix;               // kTileVox
+  unsigned long long *tfs = s_fix + kTileVox;     // 4 x min(tf_n, kLdsTfChunk)
+  __shared__ float red[kBlock / 64];
+  const int bin = blockIdx.x;
+  const uint32_t r0 = b.off[bin], r1 = min(b.off[bin + 1], b.capacity);
+  if (r0 >= r1) return;  // block-uniform
+  int slot = 0;  // the brick slot holding this bin
+  while (b.base[slot + 1] <= bin) ++slot;
+  const DevBrick db = a.bricks[a.brick_ids[slot]];
+  const bool dens = a.dvox[slot] != nullptr;
+  const bool want_tf = a.dtf != nullptr;
+  const int n = a.tf_n;
+  // a bin's records come in runs of one ray's consecutive samples (half a
+  // voxel apart, footprints overlapping): each thread takes a contiguous
+  // chunk, so the lanes of one instruction rarely add to the same word
+  const uint32_t per = (r1 - r0 + kBlock - 1) / kBlock;
+  const uint32_t c0 = r0 + per * threadIdx.x, c1 = min(r1, c0 + per);
+  // bounds of one record's contributions: density |dv| + 2 (|dgx|+|dgy|+|dgz|)
+  // (the weights of a tap sum to 1), TF the largest of |dc|, |da|
+  float md = 0.f, mt = 0.f;
+  for (uint32_t r = c0; r < c1; ++r) {
+    const float4 *p = b.records + (size_t)kRecWords * r;
+    const float4 p0 = p[0], p1 = p[1], p2 = p[2];
+    const float bd = fabsf(p0.w) + 2.f * (fabsf(p1.x) + fabsf(p1.y) + fabsf(p1.z));
+    const float bt = fmaxf(fmaxf(fabsf(p2.x), fabsf(p2.y)), fmaxf(fabsf(p2.z), fabsf(p2.w)));
+    md = (bd > md || bd != bd) ? bd : md;
+    mt = (bt > mt || bt != bt) ? bt : mt;
+  }
+  md = block_max(md, red);
+  mt = block_max(mt, red);
+  const uint32_t cnt = r1 - r0;
+  const double sd = dens ? fixed_scale(md, 8u * cnt) : 0.0;  // 8 corners of a tap share a voxel
+  const double st = want_tf ? fixed_scale(mt, cnt) : 0.0;
+  const int l = bin - b.base[slot], nbx = b.nbx[slot], nby = b.nby[slot];
+  const int bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
+  const FixedTile sink{tile, bx * kBinEdge - 1, by * kBinEdge - 1, bz * kBinEdge - 1, sd};
+  const int chunks = want_tf ? (n + kLdsTfChunk - 1) / kLdsTfChunk : 1;
+  for (int ch = 0; ch < chunks; ++ch) {
+    const int e0 = ch * kLdsTfChunk, e1 = min(n, e0 + kLdsTfChunk);
+    const bool do_dens = ch == 0 && sd > 0.0;
+    if (do_dens)
+      for (int i = threadIdx.x; i < kTileVox; i += kBlock) tile[i] = 0ull;
+    for (int i = threadIdx.x; i < 4 * (e1 - e0); i += kBlock) tfs[i] = 0ull;
+    __syncthreads();
+    int ri = INT_MIN;  // the TF entry pair of the current run, its sums
+    long long q0[4] = {0, 0, 0, 0}, q1[4] = {0, 0, 0, 0};
+    for (uint32_t r = c0; r < c1; ++r) {
+      const float4 *p = b.records + (size_t)kRecWords * r;
+      const float4 p0 = p[0], p1 = p[1];
+      if (do_dens) {
+        const float gd[3] = {p1.x, p1.y, p1.z};
+        if (p0.w != 0.f || gd[0] != 0.f || gd[1] != 0.f || gd[2] != 0.f) {
+          if (!scatter_fast(sink, db, p0.x, p0.y, p0.z, p0.w, gd)) {
+            scatter_tap(sink, db, p0.x, p0.y, p0.z, p0.w);
+            scatter_tap(sink, db, p0.x + .5f, p0.y, p0.z, gd[0]);
+            scatter_tap(sink, db, p0.x - .5f, p0.y, p0.z, -gd[0]);
+            scatter_tap(sink, db, p0.x, p0.y + .5f, p0.z, gd[1]);
+            scatter_tap(sink, db, p0.x, p0.y - .5f, p0.z, -gd[1]);
+            scatter_tap(sink, db, p0.x, p0.y, p0.z + .5f, gd[2]);
+            scatter_tap(sink, db, p0.x, p0.y, p0.z - .5f, -gd[2]);
+          }
+        }
+      }
+      if (st > 0.0) {
+        // the two TF entries of the lookup (tf_lookup's texel and weight);
+        // consecutive records of a thread are neighbouring samples of one
+        // ray and mostly read the same entry pair: their fixed-point values
+        // are summed in registers and added to LDS once per run
+        const float4 g4 = p[2];
+        const float xt = p1.w * (float)n - 0.5f;
         const float fi = floorf(xt);
         const float w = xt - fi;
         const int i0 = (int)fminf(fmaxf(fi, -2.f), (float)n);
-        const float4 raw = tf_lookup(s_tf, n, r.s.val);
-        const float da = raw.w < a.alpha_max ? dalpha : 0.f;
-        const float g4[4] = {dcr, dcg, dcb, da};
-        if (i0 != pi) {
-          tf_flush(s_dtf, n, pi, p0, p1);
-          pi = i0;
+        if (i0 != ri) {
+          tf_run_flush(tfs, ri, e0, e1, q0, q1);
+          ri = i0;
 #pragma unroll
-          for (int ch = 0; ch < 4; ++ch) p0[ch] = p1[ch] = 0.f;
+          for (int c = 0; c < 4; ++c) q0[c] = q1[c] = 0;
         }
+        const float gs[4] = {g4.x, g4.y, g4.z, g4.w};
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) {
-          p0[ch] += (1.f - w) * g4[ch];
-          p1[ch] += w * g4[ch];
-        }
-        if (grad) {
-          // value tap through the TF slope
-          const float4 e0 = s_tf[i0 + 2], e1 = s_tf[i0 + 3];
-          const float dv = (dcr * (e1.x - e0.x) + dcg * (e1.y - e0.y) + dcb * (e1.z - e0.z) +
-                            da * (e1.w - e0.w)) * (float)n;
-          // shading: s = .1 + .9|N.d|, N = g/L, L = |g + 1e-4|
-          const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
-          const float nd = nx * d[0] + ny * d[1] + nz * d[2];
-          const float sg = nd > 0.f ? .9f : (nd < 0.f ? -.9f : 0.f);
-          const float dN[3] = {dshade * sg * d[0], dshade * sg * d[1], dshade * sg * d[2]};
-          const float gv[3] = {r.s.gx, r.s.gy, r.s.gz};
-          const float dNg = dN[0] * gv[0] + dN[1] * gv[1] + dN[2] * gv[2];
-          const float L3 = r.len * r.len * r.len;
-          float dg[3];
-#pragma unroll
-          for (int q = 0; q < 3; ++q) dg[q] = dN[q] / r.len - dNg * (gv[q] + 1e-4f) / L3;
-          const float x = (org[0] + t * d[0]) - db.vox_lo[0];
-          const float y = (org[1] + t * d[1]) - db.vox_lo[1];
-          const float z = (org[2] + t * d[2]) - db.vox_lo[2];
-          if (dv == 0.f && dg[0] == 0.f && dg[1] == 0.f && dg[2] == 0.f) continue;
-          if (scatter_fast(sink, db, x, y, z, dv, dg)) continue;
-          scatter_tap(sink, db, x, y, z, dv);
-          scatter_tap(sink, db, x + .5f, y, z, dg[0]);
-          scatter_tap(sink, db, x - .5f, y, z, -dg[0]);
-          scatter_tap(sink, db, x, y + .5f, z, dg[1]);
-          scatter_tap(sink, db, x, y - .5f, z, -dg[1]);
-          scatter_tap(sink, db, x, y, z + .5f, dg[2]);
-          scatter_tap(sink, db, x, y, z - .5f, -dg[2]);
+        for (int c = 0; c < 4; ++c) {
+          q0[c] += to_fixed((1.f - w) * gs[c], st);
+          q1[c] += to_fixed(w * gs[c], st);
         }
       }
     }
+    tf_run_flush(tfs, ri, e0, e1, q0, q1);
+    __syncthreads();
+    if (ch == 0 && dens) {
+      float *dst = b.tiles + (size_t)bin * kTileVox;
+      const double inv = sd > 0.0 ? 1.0 / sd : 0.0;
+      for (int i = threadIdx.x; i < kTileVox; i += kBlock)
+        dst[i] = sd > 0.0 ? (float)((double)(long long)tile[i] * inv)
+                          : (sd < 0.0 ? __int_as_float(0x7fc00000) : 0.f);
+    }
+    if (want_tf) {
+      double *dst = b.tf_part + (size_t)bin * 4 * n;
+      const double inv = st > 0.0 ? 1.0 / st : 0.0;
+      for (int i = 4 * e0 + threadIdx.x; i < 4 * e1; i += kBlock)
+        dst[i] = st > 0.0 ? (double)(long long)tfs[i - 4 * e0] * inv
+                          : (st < 0.0 ? __longlong_as_double(0x7ff8000000000000ll) : 0.0);
+    }
+    __syncthreads();
   }
-  tf_flush(s_dtf, a.tf_n, pi, p0, p1);
-  __syncthreads();
-  if (a.dtf)
-    for (int i = threadIdx.x; i < 4 * a.tf_n; i += kBlock)
-      if (s_dtf[i] != 0.f) atomicAdd(&a.dtf[i], s_dtf[i]);
-  // one global atomic per distinct cached voxel
-  for (int i = threadIdx.x; i < kCacheSlots; i += kBlock) {
-    const uint32_t key = s_keys[i];
-    const float v = s_vals[i];
-    if (key != kCacheEmpty && v != 0.f) atomicAdd(&a.dvox[key >> 24][key & 0xFFFFFFu], v);
-  }
+}
+
+// The TF gradient in two fixed-order levels: group g of kTfGroupBins
+// consecutive bins sums its bins' partials in bin order (doubles), then each
+// entry sums the groups in order and adds the result into dtf.
+__global__ __launch_bounds__(kBlock) void diff_tf_group_kernel(const DiffArgs a, const DiffBins b,
+                                                               double *groups) {
+  const int i = blockIdx.x * kBlock + threadIdx.x, g = blockIdx.y;
+  const int m = 4 * a.tf_n;
+  if (i >= m) return;
+  const int b0 = g * kTfGroupBins, b1 = min(b.nbins, b0 + kTfGroupBins);
+  double s = 0.0;
+  for (int bin = b0; bin < b1; ++bin)
+    if (b.off[bin] != b.off[bin + 1]) s += b.tf_part[(size_t)bin * m + i];
+  groups[(size_t)g * m + i] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void diff_tf_reduce_kernel(const DiffArgs a, const DiffBins b,
+                                                                const double *groups, int ngroups) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int m = 4 * a.tf_n;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int g = 0; g < ngroups; ++g) s += groups[(size_t)g * m + i];
+  a.dtf[i] += (float)s;
+}
+
+// Tiles into the gradient grid: voxel v of an axis lies in the tile of block
+// (v+1)/16 (at (v+1)%16) and, when (v+1)%16 < 2, also in the previous
+// block's (at (v+1)%16 + 16): up to 8 tiles, added in a fixed order.
+__global__ __launch_bounds__(kBlock) void diff_fold_kernel(const DiffArgs a, const DiffBins b,
+                                                           int nz_max) {
+  // grid: x = 64-voxel row pieces, y = rows of 4, z = slot * nz_max + z
+  const int slot = blockIdx.z / nz_max, z = blockIdx.z - slot * nz_max;
+  float *grad = a.dvox[slot];
+  if (!grad) return;
+  const DevBrick db = a.bricks[a.brick_ids[slot]];
+  const int nx = db.n[0], ny = db.n[1], nz = db.n[2];
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= nx || y >= ny || z >= nz) return;
+  int bxs[2], lxs[2], bys[2], lys[2], bzs[2], lzs[2], cx = 0, cy = 0, cz = 0;
+  auto cands = [](int v, int *bs, int *ls, int &c) {
+    const int u = v + 1, bh = u / kBinEdge, lh = u % kBinEdge;
+    if (lh < 2 && bh >= 1) {
+      bs[c] = bh - 1;
+      ls[c++] = lh + kBinEdge;
+    }
+    bs[c] = bh;
+    ls[c++] = lh;
+  };
+  cands(x, bxs, lxs, cx);
+  cands(y, bys, lys, cy);
+  cands(z, bzs, lzs, cz);
+  const int nbx = b.nbx[slot], nby = b.nby[slot], base = b.base[slot];
+  float sum = 0.f;
+  for (int kz = 0; kz < cz; ++kz)
+    for (int ky = 0; ky < cy; ++ky)
+      for (int kx = 0; kx < cx; ++kx) {
+        const int bin = base + (bzs[kz] * nby + bys[ky]) * nbx + bxs[kx];
+        if (b.off[bin] == b.off[bin + 1]) continue;  // no samples: tile not written
+        sum += b.tiles[(size_t)bin * kTileVox + (lzs[kz] * kTileEdge + lys[ky]) * kTileEdge +
+                       lxs[kx]];
+      }
+  if (sum != 0.f) grad[((size_t)z * ny + y) * nx + x] += sum;
 }
 
 }  // namespace
 
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s) {
+  if (backward) return hipErrorInvalidValue;  // the backward runs in stages (launch_diff_*)
   const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
-  const size_t tf_lds = sizeof(float4) * (size_t)(a.tf_n + 4);
-  if (backward)
-    diff_backward_kernel<<<grid, kBlock,
-                           tf_lds + sizeof(float) * 4 * (size_t)a.tf_n +
-                               (sizeof(uint32_t) + sizeof(float)) * (size_t)kCacheSlots,
-                           s>>>(a);
-  else
-    diff_forward_group_kernel<4><<<grid, kBlock, tf_lds, s>>>(a);
+  diff_forward_group_kernel<4><<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_count(const DiffArgs &a, const DiffBins &b, hipStream_t s) {
+  const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
+  if (b.nbins > 0) {
+    diff_count_kernel<<<grid, kBlock, 0, s>>>(a, b);
+    diff_scan_kernel<<<1, 1024, 0, s>>>(b);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_backward(const DiffArgs &a, const DiffBins &b, hipStream_t s) {
+  const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
+  diff_backward_kernel<<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_bins(const DiffArgs &a, const DiffBins &b, const int max_dims[3],
+                            hipStream_t s) {
+  if (b.nbins == 0) return hipSuccess;
+  const size_t lds =
+      sizeof(unsigned long long) * ((size_t)kTileVox + 4 * (size_t)std::min(a.tf_n, kLdsTfChunk));
+  diff_bin_kernel<<<b.nbins, kBlock, lds, s>>>(a, b);
+  if (max_dims[0] > 0) {
+    const dim3 grid((unsigned)((max_dims[0] + 63) / 64), (unsigned)((max_dims[1] + 3) / 4),
+                    (unsigned)(max_dims[2] * a.nb));
+    diff_fold_kernel<<<grid, kBlock, 0, s>>>(a, b, max_dims[2]);
+  }
+  if (a.dtf) {
+    // the group sums go behind the bins' partials in b.tf_part
+    const int m = 4 * a.tf_n, ngroups = (b.nbins + kTfGroupBins - 1) / kTfGroupBins;
+    double *groups = b.tf_part + (size_t)b.nbins * m;
+    const unsigned bx = (unsigned)((m + kBlock - 1) / kBlock);
+    diff_tf_group_kernel<<<dim3(bx, (unsigned)ngroups), kBlock, 0, s>>>(a, b, groups);
+    diff_tf_reduce_kernel<<<bx, kBlock, 0, s>>>(a, b, groups, ngroups);
+  }
   return hipGetLastError();
 }
 

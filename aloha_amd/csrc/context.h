@@ -53,6 +53,17 @@ struct aloha_ctx {
   // address could not be had: scratch_err says why)
   void *scratch = nullptr;
   std::string scratch_err;
+  // workspace of aloha_diff_backward, grown on demand: bin counts, offsets
+  // and cursors; sample records; bin tiles; the record total read back
+  uint32_t *diff_bins = nullptr;
+  size_t diff_bins_cap = 0;
+  float4 *diff_records = nullptr;
+  size_t diff_records_cap = 0;  // records (2 float4 each)
+  float *diff_tiles = nullptr;
+  size_t diff_tiles_cap = 0;  // floats
+  double *diff_tf_part = nullptr;
+  size_t diff_tf_part_cap = 0;  // doubles
+  uint32_t *h_total = nullptr;  // pinned host word
   std::string err;
 };
 

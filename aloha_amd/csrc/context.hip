@@ -226,6 +226,11 @@ extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
   for (float *m : ctx->brick_mem) hipFree(m);
   if (ctx->d_bricks) hipFree(ctx->d_bricks);
   if (ctx->d_tf) hipFree(ctx->d_tf);
+  if (ctx->diff_bins) hipFree(ctx->diff_bins);
+  if (ctx->diff_records) hipFree(ctx->diff_records);
+  if (ctx->diff_tiles) hipFree(ctx->diff_tiles);
+  if (ctx->diff_tf_part) hipFree(ctx->diff_tf_part);
+  if (ctx->h_total) hipHostFree(ctx->h_total);
   for (TriGeom &g : ctx->geoms) {
     if (g.d_nodes) hipFree(g.d_nodes);
     if (g.d_tris) hipFree(g.d_tris);
@@ -950,7 +955,85 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
   a.dldc = (const float4 *)d_dl_drgba;
   a.dtf = d_dtf;
   for (int k = 0; k < nb; ++k) a.dvox[k] = d_dvoxels ? d_dvoxels[k] : nullptr;
-  HIP_CHECK(ctx, launch_diff(a, true, ctx->stream));
+  // bins of the bricks with a gradient grid: 16^3 blocks over the corners
+  // -1 .. n of each axis
+  DiffBins b;
+  std::memset(&b, 0, sizeof b);
+  int64_t nbins = 0, max_voxels = 0;
+  int max_dims[3] = {0, 0, 0};
+  for (int k = 0; k < nb; ++k) {
+    b.base[k] = (int32_t)nbins;
+    const DevBrick &db = ctx->bricks[brick_ids[k]];
+    b.nbx[k] = (db.n[0] + 1) / kBinEdge + 1;
+    b.nby[k] = (db.n[1] + 1) / kBinEdge + 1;
+    b.nbz[k] = (db.n[2] + 1) / kBinEdge + 1;
+    nbins += (int64_t)b.nbx[k] * b.nby[k] * b.nbz[k];
+    if (a.dvox[k]) {
+      max_voxels = std::max<int64_t>(max_voxels, (int64_t)db.n[0] * db.n[1] * db.n[2]);
+      for (int ax = 0; ax < 3; ++ax) max_dims[ax] = std::max(max_dims[ax], db.n[ax]);
+    }
+  }
+  b.base[nb] = (int32_t)nbins;
+  if ((int64_t)max_dims[2] * nb > 65535)  // the fold's grid.z
+    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %d bricks of depth %d", nb, max_dims[2]);
+  if (nbins > (1 << 24))
+    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %lld gradient bins", (long long)nbins);
+  b.nbins = (int32_t)nbins;
+  if (nbins > 0) {
+    // count pass and scan, then the record total (read back: it sizes the records)
+    const size_t words = 3 * (size_t)nbins + 1;
+    if (words > ctx->diff_bins_cap) {
+      if (ctx->diff_bins) HIP_CHECK(ctx, hipFree(ctx->diff_bins));
+      ctx->diff_bins = nullptr;
+      ctx->diff_bins_cap = 0;
+      HIP_CHECK(ctx, hipMalloc(&ctx->diff_bins, words * sizeof(uint32_t)));
+      ctx->diff_bins_cap = words;
+    }
+    if (!ctx->h_total) HIP_CHECK(ctx, hipHostMalloc(&ctx->h_total, sizeof(uint32_t)));
+    b.count = ctx->diff_bins;
+    b.off = b.count + nbins;
+    b.cursor = b.off + nbins + 1;
+    HIP_CHECK(ctx, hipMemsetAsync(b.count, 0, sizeof(uint32_t) * (size_t)nbins, ctx->stream));
+    HIP_CHECK(ctx, launch_diff_count(a, b, ctx->stream));
+    HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_total, b.off + nbins, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t total = *ctx->h_total;
+    if (total > ((size_t)1 << 31))
+      return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %zu samples", total);
+    if (total > ctx->diff_records_cap) {
+      if (ctx->diff_records) HIP_CHECK(ctx, hipFree(ctx->diff_records));
+      ctx->diff_records = nullptr;
+      ctx->diff_records_cap = 0;
+      const size_t cap = total + total / 4 + 1024;
+      HIP_CHECK(ctx, hipMalloc(&ctx->diff_records, cap * kRecWords * sizeof(float4)));
+      ctx->diff_records_cap = cap;
+    }
+    const size_t tile_floats = max_voxels > 0 ? (size_t)nbins * kTileVox : 0;
+    if (tile_floats > ctx->diff_tiles_cap) {
+      if (ctx->diff_tiles) HIP_CHECK(ctx, hipFree(ctx->diff_tiles));
+      ctx->diff_tiles = nullptr;
+      ctx->diff_tiles_cap = 0;
+      HIP_CHECK(ctx, hipMalloc(&ctx->diff_tiles, tile_floats * sizeof(float)));
+      ctx->diff_tiles_cap = tile_floats;
+    }
+    // per-bin TF partials, then the per-group sums of the reduction
+    const size_t groups = ((size_t)nbins + kTfGroupBins - 1) / kTfGroupBins;
+    const size_t part = d_dtf ? ((size_t)nbins + groups) * 4 * (size_t)a.tf_n : 0;
+    if (part > ctx->diff_tf_part_cap) {
+      if (ctx->diff_tf_part) HIP_CHECK(ctx, hipFree(ctx->diff_tf_part));
+      ctx->diff_tf_part = nullptr;
+      ctx->diff_tf_part_cap = 0;
+      HIP_CHECK(ctx, hipMalloc(&ctx->diff_tf_part, part * sizeof(double)));
+      ctx->diff_tf_part_cap = part;
+    }
+    b.records = ctx->diff_records;
+    b.capacity = (uint32_t)std::min<size_t>(ctx->diff_records_cap, 0xFFFFFFFFu);
+    b.tiles = ctx->diff_tiles;
+    b.tf_part = ctx->diff_tf_part;
+  }
+  HIP_CHECK(ctx, launch_diff_backward(a, b, ctx->stream));
+  HIP_CHECK(ctx, launch_diff_bins(a, b, max_dims, ctx->stream));
   return ALOHA_OK;
 }
 

@@ -110,6 +110,32 @@ struct DiffArgs {
   float *dtf;                   // backward: dLoss/dTF, tf_n x 4 (accumulated)
 };
 
+// Backward of the differentiable form: the density gradient goes through
+// spatial bins.  Bin = one 16^3 block of a brick's voxels, keyed by the
+// lowest corner a sample's seven taps can read, floor(x - .5) per axis
+// (clamped to [-1, n]); the block of bin (bx, by, bz) spans corners
+// [16 b - 1, 16 b + 15), its taps' voxels [16 b - 1, 16 b + 17): an 18^3 tile.
+constexpr int kBinEdge = 16;
+constexpr int kTileEdge = kBinEdge + 2;
+constexpr int kTileVox = kTileEdge * kTileEdge * kTileEdge;
+// Records are 3 float4 per sample: (x, y, z, d/d(value tap)),
+// (d/d(gradient) x, y, z, value), (d/d(TF colour) r, g, b, d/d(opacity)).
+constexpr int kRecWords = 3;
+constexpr int kLdsTfChunk = 1024;  // TF entries whose gradient a bin sums in LDS at once
+constexpr int kTfGroupBins = 32;   // bins per group of the TF-gradient reduction
+struct DiffBins {
+  int32_t base[kMaxDiffBricks + 1];  // first bin of each brick slot
+  int32_t nbx[kMaxDiffBricks], nby[kMaxDiffBricks], nbz[kMaxDiffBricks];
+  int32_t nbins;
+  uint32_t *count;     // samples per bin (count pass)
+  uint32_t *off;       // exclusive prefix of count, nbins + 1 entries
+  uint32_t *cursor;    // next free record of each bin (starts at off)
+  float4 *records;     // kRecWords float4 per sample
+  uint32_t capacity;   // records allocated
+  float *tiles;        // nbins x kTileVox density-gradient partial sums
+  double *tf_part;     // nbins x 4 tf_n TF-gradient partial sums
+};
+
 struct CompositeArgs {
   int32_t P;
   int64_t npix;
@@ -175,6 +201,14 @@ hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hi
 hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
                             int32_t px, int32_t py, int32_t pz, hipStream_t s);
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
+// backward in stages: count samples per bin (and scan), then the march that
+// writes records and the TF gradient, then bins into tiles and tiles into
+// the gradient grids
+hipError_t launch_diff_count(const DiffArgs &a, const DiffBins &b, hipStream_t s);
+hipError_t launch_diff_backward(const DiffArgs &a, const DiffBins &b, hipStream_t s);
+// max_dims: the largest extent per axis over the bricks with a gradient grid
+hipError_t launch_diff_bins(const DiffArgs &a, const DiffBins &b, const int max_dims[3],
+                            hipStream_t s);
 hipError_t launch_composite_over(const float4 *imgs, const CompositeArgs &c, float4 *out,
                                  uint32_t *out8, hipStream_t s);
 // sparse form: reads rank pixels only inside c.rect (W x H images); writes

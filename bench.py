@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--adam-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     ap.add_argument("--no-surface", action="store_true", help="skip the triangle surface path")
+    ap.add_argument("--only", choices=["adjoint", "batched", "surface"], default=None,
+                    help="run only this section (for profiling its kernels)")
     return ap.parse_args()
 
 
@@ -79,6 +81,8 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+    if args.only:
+        return only_section(args)
 
     t0 = time.time()
     if args.workload == "fbm512":
@@ -297,6 +301,28 @@ def main():
     ctx.close()
 
 
+def _timer():
+    import torch
+
+    def timed(fn, n):
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+    return timed
+
+
+def only_section(args):
+    """One section alone (its JSON object on stdout), for profiling runs."""
+    fn = {"adjoint": adjoint_bench, "batched": batch_bench, "surface": surface_bench}[args.only]
+    print(json.dumps({args.only: fn(args, _timer())}), flush=True)
+
+
 def adjoint_bench(args, timed):
     """Differentiable form at 256^3, 8 views of 512^2: forward-only vs
     forward+adjoint ms, and an Adam fit of TF + density to targets rendered
@@ -344,13 +370,15 @@ def adjoint_bench(args, timed):
         with torch.no_grad():
             tfp.clamp_(0, 1)
         losses.append(float(loss.detach()))
-    log(f"[bench] Adam {args.adam_steps} steps: loss {losses[0]:.3f} -> {losses[-1]:.3f} "
-        f"in {time.time() - t0:.1f}s")
+    if losses:
+        log(f"[bench] Adam {args.adam_steps} steps: loss {losses[0]:.3f} -> {losses[-1]:.3f} "
+            f"in {time.time() - t0:.1f}s")
     return {"workload": "256^3 f32 blob density, 256-entry TF, 8 views 512x512, dt 0.5",
             "samples_per_forward": samples, "fwd_ms": round(ms_f, 4),
             "fwd_adjoint_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
-            "adam_steps": args.adam_steps, "loss_first": round(losses[0], 4),
-            "loss_last": round(losses[-1], 4)}
+            "adam_steps": args.adam_steps,
+            "loss_first": round(losses[0], 4) if losses else None,
+            "loss_last": round(losses[-1], 4) if losses else None}
 
 
 def batch_bench(args, timed):
