@@ -16,7 +16,7 @@ import numpy as np
 
 from ._lib import (ERR_ALLOC, ERR_ARG, ERR_HIP, ERR_IO, ERR_LIMIT, ERR_STATE,  # noqa: F401
                    NOALLOC_SCRATCH_BYTES)
-from ._lib import (F32, FLAG_CROSSHAIR, FLAG_HEATMAP, JITTER_FIXED, JITTER_RANDOM, MAX_RANKS,
+from ._lib import (F32, FLAG_CROSSHAIR, FLAG_HEATMAP, FLAG_HEAT_CLOCK, JITTER_FIXED, JITTER_RANDOM, MAX_RANKS,
                    MAX_RANK_BRICKS, MAX_TF, U8, U16, AlohaError, Box3f, Box3i, BrickDesc, Images,
                    KDNode, Range1f, RenderParams, check, lib)
 
@@ -24,7 +24,7 @@ __all__ = [
     "AlohaError", "BrickDesc", "Box3i", "Box3f", "RenderParams", "Context", "Partitioner",
     "split_volume", "volume_extract", "synth_blobs_u8", "synth_blobs_f32", "synth_fbm_f32",
     "synth_tf", "synth_camera", "max_depth", "U8", "U16", "F32", "JITTER_RANDOM",
-    "JITTER_FIXED", "FLAG_CROSSHAIR", "FLAG_HEATMAP", "MAX_RANKS", "MAX_RANK_BRICKS", "MAX_TF",
+    "JITTER_FIXED", "FLAG_CROSSHAIR", "FLAG_HEATMAP", "FLAG_HEAT_CLOCK", "MAX_RANKS", "MAX_RANK_BRICKS", "MAX_TF",
     "vol_write", "vol_read", "Mesh", "mesh_color_ids",
 ]
 
@@ -260,9 +260,10 @@ class RankImages:
 
 
 def render_params(dt=0.5, max_depth=1.0, spp=1, accum_id=0, island_id=0, num_islands=1,
-                  jitter_mode=JITTER_RANDOM, jitter_value=0.0, flags=0, heat_map_scale=1e-5):
+                  jitter_mode=JITTER_RANDOM, jitter_value=0.0, flags=0, heat_map_scale=1e-5,
+                  heat_clock=0):
     return RenderParams(dt, max_depth, spp, accum_id, island_id, num_islands, jitter_mode,
-                        jitter_value, flags, heat_map_scale)
+                        jitter_value, flags, heat_map_scale, heat_clock)
 
 
 def _rank_table(ranks: Sequence[Sequence[int]]):

@@ -14,7 +14,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_IO, ERR_LIMIT = -1, -2, -3, -4, -5, -6
 U8, U16, F32 = 1, 2, 4
 JITTER_RANDOM, JITTER_FIXED = 0, 1
-FLAG_CROSSHAIR, FLAG_HEATMAP = 1, 2
+FLAG_CROSSHAIR, FLAG_HEATMAP, FLAG_HEAT_CLOCK = 1, 2, 4
 MAX_RANKS, MAX_RANK_BRICKS, MAX_TF = 64, 256, 4096
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -51,7 +51,7 @@ class RenderParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("max_depth", C.c_float), ("spp", C.c_int32),
                 ("accum_id", C.c_int32), ("island_id", C.c_int32), ("num_islands", C.c_int32),
                 ("jitter_mode", C.c_int32), ("jitter_value", C.c_float), ("flags", C.c_int32),
-                ("heat_map_scale", C.c_float)]
+                ("heat_map_scale", C.c_float), ("heat_clock", C.c_uint32)]
 
 
 class Images(C.Structure):

@@ -157,6 +157,7 @@ int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t
   a.max_depth = p->max_depth;
   a.jitter_value = p->jitter_value;
   a.heat_scale = p->heat_map_scale;
+  a.heat_clock = p->heat_clock;
   a.W = W;
   a.H = H;
   a.spp = p->spp;

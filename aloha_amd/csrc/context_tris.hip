@@ -165,6 +165,7 @@ int build_tris(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t 
   std::memcpy(a.dv, ctx->cam + 9, sizeof a.dv);
   a.max_depth = p->max_depth;
   a.heat_scale = p->heat_map_scale;
+  a.heat_clock = p->heat_clock;
   a.W = W;
   a.H = H;
   a.spp = p->spp;

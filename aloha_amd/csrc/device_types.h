@@ -36,6 +36,8 @@ struct MarchArgs {
   int32_t W, H, spp, accum_id;
   int32_t island_id, num_islands, jitter_mode, flags;
   int32_t P, tf_n;
+  uint32_t heat_clock;  // clock delta of every pixel under ALOHA_FLAG_HEAT_CLOCK
+  int32_t pad0;
   const DevBrick *bricks;
   const float4 *tf;
   float4 *rgba;
@@ -162,6 +164,7 @@ struct TrisArgs {
   float max_depth, heat_scale;
   int32_t W, H, spp, accum_id;
   int32_t island_id, num_islands, flags, P;
+  uint32_t heat_clock, pad0;  // clock delta of every pixel under ALOHA_FLAG_HEAT_CLOCK
   float4 *rgba;
   uint32_t *rgba8;
   float *depth;

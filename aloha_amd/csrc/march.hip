@@ -551,6 +551,16 @@ __device__ __forceinline__ int compact_records(int n, RayRec *recs, int *wcnt, c
   return total;
 }
 
+// heatMap's argument (deviceCode.cpp:399-401): the pixel's clock delta times
+// heatMapScale / spp (`scale`), the delta measured since clock_begin or,
+// under ALOHA_FLAG_HEAT_CLOCK, the host-given a.heat_clock.
+__device__ __forceinline__ float heat_time(const MarchArgs &a, long long clock_begin, float scale) {
+  const unsigned long long delta = (a.flags & ALOHA_FLAG_HEAT_CLOCK)
+                                       ? (unsigned long long)a.heat_clock
+                                       : (unsigned long long)(clock64() - clock_begin);
+  return (float)delta * scale;
+}
+
 // Per-pixel epilogue of a rank image (renderFrame :395-415), as march_kernel.
 __device__ __forceinline__ void write_rank_pixel(const MarchArgs &a, long long out_idx, int px,
                                                  int py, float4 acc, float z, uint32_t ns,
@@ -669,9 +679,7 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
   const float heat_scale = a.heat_scale / (float)a.spp;
   auto store = [&](const ListRay &r, float4 acc, float z, uint32_t ns, long long clock_begin) {
     if (r.tag < 0) return;
-    const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP)
-                             ? (float)(clock64() - clock_begin) * heat_scale
-                             : 0.f;
+    const float heat_t = (a.flags & ALOHA_FLAG_HEATMAP) ? heat_time(a, clock_begin, heat_scale) : 0.f;
     long long o;
     if (q.compact) {  // the pixel's place in its slot's rectangle
       const int32_t *rc = tb.rect + 4 * r.tag;
@@ -761,7 +769,7 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
       } else {
         long long clock_begin = 0;
         if (heat) clock_begin = clock64();
-        const float heat_t = heat ? (float)(clock64() - clock_begin) * heat_scale : 0.f;
+        const float heat_t = heat ? heat_time(a, clock_begin, heat_scale) : 0.f;
         write_rank_pixel(a, o, x, y, make_float4(0.f, 0.f, 0.f, 0.f), 1.f, 0u, heat_t);
       }
     }

@@ -268,8 +268,10 @@ __global__ __launch_bounds__(kBlock) void tris_rank_kernel(const TrisArgs a) {
     a.stats[2 * out_idx + 1] = nt;
   }
   if (a.flags & ALOHA_FLAG_HEATMAP) {
-    const long long clock_end = clock64();
-    const float t = (float)(clock_end - clock_begin) * (a.heat_scale / (float)a.spp);
+    const unsigned long long delta = (a.flags & ALOHA_FLAG_HEAT_CLOCK)
+                                         ? (unsigned long long)a.heat_clock
+                                         : (unsigned long long)(clock64() - clock_begin);
+    const float t = (float)delta * (a.heat_scale / (float)a.spp);
     const float3 h = heat_map(t);
     acc = make_float4(h.x + .5f * acc.x, h.y + .5f * acc.y, h.z + .5f * acc.z, .5f + .5f * acc.w);
   }

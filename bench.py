@@ -134,23 +134,10 @@ def main():
                                                reps=10)
     log(f"[bench] march schedule (lanes per ray: ranks, fused) = {schedule}")
 
-    def timed(fn, n):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        for _ in range(n):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / n
+    timed = _timer()
 
     if args.profile:
-        # calibration for HBM-traffic counters: one copy of exactly 256 MiB
-        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibration_copy()
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
@@ -183,7 +170,16 @@ def main():
     for _ in range(args.warmup):
         rgba8_frame()
     ms_rgba8 = timed(rgba8_frame, args.steps)
-    log(f"[bench] RGBA8 composite form (render_ranks + 8-bit ordered blend): {ms_rgba8:.4f} ms/frame")
+    torch.cuda.synchronize()
+    # 8-bit levels of the float path's frame (packed from its float 'over')
+    # against the 8-bit blend of the ranks' RGBA8 images, per channel
+    lv = lambda t: t.view(torch.uint8).reshape(H, W, 4).to(torch.int16)
+    level_diff = (lv(out8) - lv(out8b)).abs()
+    rgba8_levels = {"max": int(level_diff.max()),
+                    "pixels_differing": int((level_diff.amax(dim=2) > 0).sum()),
+                    "histogram": [int((level_diff.amax(dim=2) == k).sum()) for k in range(4)]}
+    log(f"[bench] RGBA8 composite form (render_ranks + 8-bit ordered blend): {ms_rgba8:.4f} ms/frame, "
+        f"levels vs float path {rgba8_levels}")
     # fused form: the same frame (bit-identical result) in one launch
     fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
     for _ in range(args.warmup):
@@ -244,8 +240,10 @@ def main():
 
     cpu = None
     parity = None
+    fp64_c0 = None
     if not args.no_cpu:
         cpu, parity = cpu_baseline(wl, imgs, args.cpu_seconds)
+        fp64_c0 = fp64_blobs128()
     adjoint = None
     if not args.no_adjoint and not args.profile:
         adjoint = adjoint_bench(args, timed)
@@ -286,7 +284,8 @@ def main():
             "render_ranks_then_rgba8_composite": {
                 "ms_per_frame": round(ms_rgba8, 5),
                 "Gsamples/s": round(samples / (ms_rgba8 * 1e-3) / 1e9, 4),
-                "note": "IceT's ubyte path: the ranks' RGBA8 images blended in 8 bits"},
+                "note": "IceT's ubyte path: the ranks' RGBA8 images blended in 8 bits",
+                "levels_vs_float_path": rgba8_levels},
             "fused_render_composite": {"ms_per_frame": round(ms_fused, 5),
                                        "Gsamples/s": round(samples / (ms_fused * 1e-3) / 1e9, 4),
                                        "bit_identical_to_plain": fused_equal},
@@ -296,12 +295,21 @@ def main():
         "surface": surface,
         "cpu_baseline": cpu,
         "parity_sample": parity,
+        "fp64_error": {
+            "C0 blobs128 2 bricks 512^2 (frame)": fp64_c0,
+            "C1 fbm512 8 bricks 1024^2 (rank images, band)": parity,
+            "C2 surface 1920x1080 (hit pixels, band)": (surface or {}).get("cpu_baseline", {}).get("fp64"),
+            "C3 batched view 0 (frame, band)": (batched or {}).get("fp64"),
+            "C4 differentiable forward view 0": (adjoint or {}).get("fp64"),
+        },
     }
     print(json.dumps(res), flush=True)
     ctx.close()
 
 
 def _timer():
+    """timed(fn, n): mean ms of n calls of fn, from HIP events recorded on
+    the stream current at the call (the stream the timed work runs on)."""
     import torch
 
     def timed(fn, n):
@@ -317,16 +325,29 @@ def _timer():
     return timed
 
 
+def calibration_copy():
+    """For HBM-traffic counters: one fill and one copy of exactly 256 MiB."""
+    import torch
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def only_section(args):
     """One section alone (its JSON object on stdout), for profiling runs."""
     fn = {"adjoint": adjoint_bench, "batched": batch_bench, "surface": surface_bench}[args.only]
+    if args.profile:
+        calibration_copy()
     print(json.dumps({args.only: fn(args, _timer())}), flush=True)
 
 
 def adjoint_bench(args, timed):
     """Differentiable form at 256^3, 8 views of 512^2: forward-only vs
-    forward+adjoint ms, and an Adam fit of TF + density to targets rendered
-    from a perturbed TF and density."""
+    forward+adjoint ms (and the adjoint alone), and an Adam fit of TF +
+    density to targets rendered from a perturbed TF and density; the timings
+    and samples per forward are taken again after the Adam steps."""
     import torch
     from aloha_amd import workloads as WL
     from aloha_amd.autograd import DifferentiableVolume
@@ -342,6 +363,7 @@ def adjoint_bench(args, timed):
     log(f"[bench] adjoint workload built in {time.time() - t0:.1f}s")
     tfp = tf.clone().requires_grad_(True)
     denp = den.clone().requires_grad_(True)
+    V, H, W = len(dw.cams), dw.H, dw.W
 
     def fwd():
         vol.forward_only(tfp, denp)
@@ -352,13 +374,29 @@ def adjoint_bench(args, timed):
         ((vol.render(tfp, denp) - target) ** 2).sum().backward()
 
     n = max(3, min(args.steps, 20))
-    for _ in range(2):
-        fwd_bwd()
-    ms_f = timed(fwd, n)
-    ms_fb = timed(fwd_bwd, n)
-    _, ns = vol.forward_only(tfp, denp)
-    samples = int(ns.to(torch.int64).sum())
-    log(f"[bench] adjoint: forward {ms_f:.3f} ms, forward+adjoint {ms_fb:.3f} ms")
+
+    def measure():
+        for _ in range(2):
+            fwd_bwd()
+        ms_f = timed(fwd, n)
+        ms_fb = timed(fwd_bwd, n)
+        # the adjoint launches alone (aloha_diff_backward on a fixed dL/dimage)
+        rgba, ns = vol.forward_only(tfp, denp)
+        grad = (2 * (rgba - target)).contiguous()
+        dtf = torch.zeros_like(tfp)
+        dd = torch.zeros_like(denp)
+        ms_b = timed(lambda: vol._backward(rgba, ns, grad, dtf, dd), n)
+        return ms_f, ms_fb, ms_b, int(ns.to(torch.int64).sum())
+
+    ms_f, ms_fb, ms_b, samples = measure()
+    log(f"[bench] adjoint: forward {ms_f:.3f} ms, forward+adjoint {ms_fb:.3f} ms, "
+        f"adjoint launches {ms_b:.3f} ms, {samples} samples")
+    # compulsory voxel reads: the footprint of the same views' march of this
+    # grid (the differentiable march samples the same lattice)
+    touched = 0
+    for cam in dw.cams:
+        vol.ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+        touched += vol.ctx.footprint(vol.params, W, H, [[vol.brick]])
     opt = torch.optim.Adam([tfp, denp], lr=2e-3)
     losses = []
     t0 = time.time()
@@ -370,30 +408,87 @@ def adjoint_bench(args, timed):
         with torch.no_grad():
             tfp.clamp_(0, 1)
         losses.append(float(loss.detach()))
+    adam_s = time.time() - t0
+    end = None
     if losses:
         log(f"[bench] Adam {args.adam_steps} steps: loss {losses[0]:.3f} -> {losses[-1]:.3f} "
-            f"in {time.time() - t0:.1f}s")
-    return {"workload": "256^3 f32 blob density, 256-entry TF, 8 views 512x512, dt 0.5",
-            "samples_per_forward": samples, "fwd_ms": round(ms_f, 4),
-            "fwd_adjoint_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
-            "adam_steps": args.adam_steps,
-            "loss_first": round(losses[0], 4) if losses else None,
-            "loss_last": round(losses[-1], 4) if losses else None}
+            f"in {adam_s:.1f}s")
+        e_f, e_fb, e_b, e_samples = measure()
+        log(f"[bench] adjoint after Adam: forward {e_f:.3f} ms, forward+adjoint {e_fb:.3f} ms, "
+            f"adjoint launches {e_b:.3f} ms, {e_samples} samples")
+        end = {"fwd_ms": round(e_f, 4), "fwd_adjoint_ms": round(e_fb, 4),
+               "adjoint_launches_ms": round(e_b, 4), "ratio": round(e_fb / e_f, 3),
+               "samples_per_forward": e_samples}
+    # algorithmic bytes: forward = compulsory voxel reads + the images and
+    # sample counts written; adjoint = the same voxel reads + the forward
+    # image and dL/dimage read + dDensity (whole grid) and dTF written
+    grid = int(np.prod(dw.density.shape))
+    img = V * H * W
+    fwd_bytes = touched * 4 + dw.tf.nbytes + img * (16 + 4)
+    bwd_bytes = touched * 4 + dw.tf.nbytes + img * (16 + 16 + 4) + grid * 4 + dw.tf.nbytes
+    roof = lambda b, ms: {"bound": "hbm", "achieved": round(b / (ms * 1e-3) / 1e9, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "algorithmic_bytes": int(b), "ms": round(ms, 4)}
+    res = {"workload": "256^3 f32 blob density, 256-entry TF, 8 views 512x512, dt 0.5",
+           "samples_per_forward": samples, "touched_voxels": touched,
+           "fwd_ms": round(ms_f, 4), "fwd_adjoint_ms": round(ms_fb, 4),
+           "adjoint_launches_ms": round(ms_b, 4), "ratio": round(ms_fb / ms_f, 3),
+           "roofline_forward": roof(fwd_bytes, ms_f),
+           "roofline_adjoint": roof(bwd_bytes, ms_b),
+           "adam_steps": args.adam_steps, "adam_s": round(adam_s, 2),
+           "loss_first": round(losses[0], 4) if losses else None,
+           "loss_last": round(losses[-1], 4) if losses else None,
+           "after_adam": end}
+    if not args.no_cpu:
+        res["fp64"] = fp64_adjoint_forward(dw, vol, tf, den)
+    return res
+
+
+def fp64_adjoint_forward(dw, vol, tf, den):
+    """CPU leg: the float64 re-render of view 0 at the start parameters (on
+    the float march's decisions) against the GPU forward and the oracle's
+    float32 forward."""
+    import torch
+    import oracle as O
+    threads, _ = host_threads()
+    rgba, ns = vol.forward_only(tf, den)
+    torch.cuda.synchronize()
+    g = rgba[0].cpu().numpy()
+    n = dw.density.shape[::-1]
+    blist = [({"cell": ((0, 0, 0), n), "voxel": ((0, 0, 0), n),
+               "space": ((0.0, 0.0, 0.0), tuple(float(v) for v in n))}, dw.density)]
+    kw = dict(dt=dw.dt, jitter_mode=1, jitter_value=0.5, threads=threads)
+    r32, _ = O.diff_render(dw.cams[0], dw.tf, blist, dw.W, dw.H, **kw)
+    r64 = O.diff_render64(dw.cams[0], dw.tf, blist, dw.W, dw.H, **kw)
+    return _fp64_line(g, r32, r64, ns[0].cpu().numpy() > 0, "view 0, all pixels")
+
+
+def _fp64_line(gpu, cpu32, ref64, mask, what):
+    e = np.abs(gpu - ref64).max(-1)
+    sel = e[mask] if mask.any() else e.ravel()
+    return {"sample": what, "max_abs_gpu_vs_fp64": float(e.max()),
+            "p999_abs_gpu_vs_fp64": float(np.quantile(sel, 0.999)),
+            "max_abs_cpu_fp32_vs_fp64": float(np.abs(cpu32 - ref64).max()),
+            "max_abs_gpu_vs_cpu_fp32": float(np.abs(gpu - cpu32).max())}
 
 
 def batch_bench(args, timed):
     """Batched form: 64 uneven u8/f32 bricks (axes 32..256) on a 4x4x4 lattice,
     one virtual rank each, 16 views of 768x768 with per-view sort-last
     compositing, in one autotuned, hipGraph-captured launch -- against the
-    plain per-view sequence (render_ranks over 64 slots + composite per view)
-    and the per-brick launch sequence (one render launch per brick per view)."""
+    per-view sequence (render_ranks over the 64 slots + sparse composite, per
+    view) and the per-brick sequence (one render launch per brick per view +
+    sparse composite per view), each captured as one graph too, so all three
+    time the device and not the host's launch loop."""
     import torch
     import aloha_amd as A
     from aloha_amd import RankImages
     from aloha_amd import workloads as WL
     t0 = time.time()
     bw = WL.batched()
-    ctx = A.Context(0, stream=torch.cuda.current_stream())
+    side = torch.cuda.Stream()  # graph capture needs a stream other than the null stream
+    ctx = A.Context(0, stream=side)
     ids = [ctx.upload_brick(d, v) for d, v in zip(bw.bricks, bw.voxels)]
     ctx.set_tf(bw.tf)
     ranks = [[ids[i] for i in r] for r in bw.rank_bricks]
@@ -411,49 +506,114 @@ def batch_bench(args, timed):
     b.capture()
     sched = b.schedule()  # the timed schedule, after the graph capture
     n = max(3, min(args.steps, 10))
-    ms_batch = timed(b.launch, n)
-    # plain per-view sequence
+    with torch.cuda.stream(side):
+        ms_batch = timed(b.launch, n)
+    # the baselines: per-view camera, rectangles and order fixed at capture
     imgs = RankImages.alloc(P, W, H, rgba=True, rgba8=False, depth=False)
-    ref = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    ref = torch.empty((V, H, W, 4), dtype=torch.float32, device="cuda")
     p = bw.params()
+    rects = []
+    for cam in bw.cams:
+        ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+        rects.append(ctx.rank_rects(p, W, H, ranks))
 
     def per_view():
-        for cam, order in zip(bw.cams, bw.orders):
+        for v, (cam, order) in enumerate(zip(bw.cams, bw.orders)):
             ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
             ctx.render_ranks(p, W, H, ranks, imgs)
-            ctx.composite_over(imgs.rgba, order, W, H, ref, None)
-
-    per_view()
-    ms_views = timed(per_view, max(2, n // 3))
-    equal = bool(torch.equal(ref, out[V - 1]))
-    # per-brick launch sequence: one render launch per (view, brick)
-    one = RankImages.alloc(1, W, H, rgba=True, rgba8=False, depth=False)
+            ctx.composite_over(imgs.rgba, order, W, H, ref[v], None, rects=rects[v])
 
     def per_brick():
-        for cam, order in zip(bw.cams, bw.orders):
+        for v, (cam, order) in enumerate(zip(bw.cams, bw.orders)):
             ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
             for s in range(P):
-                sub = RankImages(rgba=imgs.rgba[s:s + 1])
-                ctx.render_ranks(p, W, H, [ranks[s]], sub)
-            ctx.composite_over(imgs.rgba, order, W, H, ref, None)
+                ctx.render_ranks(p, W, H, [ranks[s]], RankImages(rgba=imgs.rgba[s:s + 1]))
+            ctx.composite_over(imgs.rgba, order, W, H, ref[v], None, rects=rects[v])
 
-    per_brick()
-    ms_bricks = timed(per_brick, 2)
-    del one
-    log(f"[bench] batched: graph {ms_batch:.3f} ms ({sched}), per-view sequence {ms_views:.3f} ms, "
-        f"per-brick sequence {ms_bricks:.3f} ms, equal {equal}")
+    def graph_of(fn):
+        with torch.cuda.stream(side):
+            fn()  # warm-up outside the capture (lazy state, occupancy queries)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side, capture_error_mode="thread_local"):
+            fn()
+        return g
+
+    res_seq = {}
+    for name, fn in (("per_view", per_view), ("per_brick", per_brick)):
+        ref.fill_(float("nan"))
+        g = graph_of(fn)
+        with torch.cuda.stream(side):
+            ms_g = timed(g.replay, max(2, n // 2))
+            ms_h = timed(fn, 2)  # the same launches issued from Python
+        torch.cuda.synchronize()
+        res_seq[name] = {"ms_graph": round(ms_g, 4), "ms_host_loop": round(ms_h, 4),
+                         "launches": V * (1 + (1 if name == "per_view" else P)),
+                         "equal_to_batch": bool(torch.equal(ref, out))}
+        del g
+    log(f"[bench] batched: graph {ms_batch:.3f} ms ({sched}); captured per-view sequence "
+        f"{res_seq['per_view']['ms_graph']:.3f} ms, per-brick sequence "
+        f"{res_seq['per_brick']['ms_graph']:.3f} ms; {res_seq}")
+    # compulsory voxel reads of every view (the footprint pass)
+    touched = 0
+    for cam in bw.cams:
+        ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+        touched += ctx.footprint(p, W, H, ranks)
+    # algorithmic bytes: voxels read (4 B: bricks are stored decoded to f32) +
+    # the TF + the V frames written (float RGBA, RGBA8, sample counts)
+    bbytes = touched * 4 + bw.tf.nbytes + V * H * W * (16 + 4 + 4)
+    gbs = bbytes / (ms_batch * 1e-3) / 1e9
+    res = {"workload": "64 bricks (axes U[32,256], u8/f32), 16 views 768x768, per-view kd order",
+           "samples": samples, "touched_voxels": touched, "ms_graph_launch": round(ms_batch, 4),
+           "Gsamples/s": round(samples / (ms_batch * 1e-3) / 1e9, 3),
+           "roofline": {"kernel": f"batch_kernel<{sched['lanes_per_ray']}, "
+                                  f"{str(sched['persistent_blocks'] > 0).lower()}>",
+                        "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes": int(bbytes)},
+           "per_view_sequence": res_seq["per_view"], "per_brick_sequence": res_seq["per_brick"],
+           "speedup_vs_per_view_graph": round(res_seq["per_view"]["ms_graph"] / ms_batch, 2),
+           "speedup_vs_per_brick_graph": round(res_seq["per_brick"]["ms_graph"] / ms_batch, 2),
+           "autotune": {"chosen": sched, "best_ms": round(best, 4),
+                        "table": [{"lanes_per_ray": t, "persistent_blocks": pb, "ms": round(m, 4)}
+                                  for t, pb, m in table]}}
+    if not args.no_cpu:
+        res["fp64"] = fp64_batched(bw, out)
     b.close()
     ctx.close()
-    return {"workload": "64 bricks (axes U[32,256], u8/f32), 16 views 768x768, per-view kd order",
-            "samples": samples, "ms_graph_launch": round(ms_batch, 4),
-            "Gsamples/s": round(samples / (ms_batch * 1e-3) / 1e9, 3),
-            "ms_per_view_sequence": round(ms_views, 4), "ms_per_brick_sequence": round(ms_bricks, 4),
-            "speedup_vs_per_view": round(ms_views / ms_batch, 2),
-            "speedup_vs_per_brick": round(ms_bricks / ms_batch, 2),
-            "autotune": {"chosen": sched, "best_ms": round(best, 4),
-                         "table": [{"lanes_per_ray": t, "persistent_blocks": pb, "ms": round(m, 4)}
-                                   for t, pb, m in table]},
-            "last_view_equal_to_per_view_sequence": equal}
+    return res
+
+
+def fp64_batched(bw, out):
+    """CPU leg: view 0 of the batch on a band of 8 rows through the centre:
+    each rank rendered by the oracle in float32 and float64, composited in
+    the view's order, against the GPU frame."""
+    import torch
+    import aloha_amd as A
+    import oracle as O
+    threads, _ = host_threads()
+    torch.cuda.synchronize()
+    W, H = bw.W, bw.H
+    rows = (H // 2 - 4, H // 2 + 4)
+    sl = slice(*rows)
+    cam, order = bw.cams[0], bw.orders[0]
+    C = np.zeros((H, W, 4), np.float32)
+    C64 = np.zeros((H, W, 4), np.float64)
+    cnt = np.zeros((H, W), np.uint32)
+    for s in order:
+        (bi,) = bw.rank_bricks[s]
+        d = bw.bricks[bi]
+        x0, y0, x1, y1 = A.project_boxes(
+            cam, [(tuple(d.space_range.lower), tuple(d.space_range.upper))], W, H)
+        if x0 >= x1 or max(y0, rows[0]) >= min(y1, rows[1]):
+            continue
+        o = O.render_rank(cam, bw.tf, [(d, bw.voxels[bi])], W, H, dt=bw.dt, rows=rows,
+                          want64=True, threads=threads)
+        C[sl] = C[sl] + (np.float32(1.0) - C[sl, :, 3:4]) * o["rgba"][sl]
+        C64[sl] = C64[sl] + (1.0 - C64[sl, :, 3:4]) * o["rgba64"][sl]
+        cnt[sl] += o["nsamples"][sl]
+    g = out[0, rows[0]:rows[1]].cpu().numpy()
+    return _fp64_line(g, C[sl], C64[sl], cnt[sl] > 0, f"view 0, rows {rows[0]}..{rows[1]}")
 
 
 def surface_bench(args, timed):
@@ -508,6 +668,12 @@ def surface_bench(args, timed):
     ctx.render_tris_composite(p, W, H, geoms, f8, fz, stats=fst)
     torch.cuda.synchronize()
     tot = st.to(torch.int64).sum(dim=(0, 1, 2)).tolist()
+    stn = st.cpu().numpy().reshape(-1, 2)
+    tail = {"nodes_per_ray_p99": float(np.percentile(stn[:, 0], 99)),
+            "nodes_per_ray_max": int(stn[:, 0].max()),
+            "tris_per_ray_p99": float(np.percentile(stn[:, 1], 99)),
+            "tris_per_ray_max": int(stn[:, 1].max())}
+    del stn
     ftot = fst.to(torch.int64).sum(dim=(0, 1)).tolist()
     rays = P * W * H
     hit_frac = float((outz < 1).float().mean())
@@ -526,7 +692,15 @@ def surface_bench(args, timed):
            "render_ms": round(ms_render, 4), "zmin_ms": round(ms_zmin, 4),
            "render_Mrays/s": round(rays / (ms_render * 1e-3) / 1e6, 1),
            "nodes_per_ray": round(tot[0] / rays, 2), "tris_per_ray": round(tot[1] / rays, 2),
+           **tail,
            "render_algorithmic_GB/s": round(bytes_render / (ms_render * 1e-3) / 1e9, 1),
+           "roofline": {"kernel": "tris_rank_kernel<false>", "bound": "hbm",
+                        "achieved": round(bytes_render / (ms_render * 1e-3) / 1e9, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(bytes_render / (ms_render * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes": int(bytes_render),
+                        "note": "bytes of the traversal this BVH needs (64 B per node visit, 48 B "
+                                "per triangle test, not deduplicated) + the images written"},
            "fused_ms": round(ms_fused, 4), "fused_Mrays/s": round(rays / (ms_fused * 1e-3) / 1e6, 1),
            "fused_nodes_per_pixel": round(ftot[0] / (W * H), 2),
            "fused_bit_identical": equal}
@@ -540,7 +714,7 @@ def surface_cpu_baseline(ctx, geoms, sw, imgs, seconds):
     """The oracle's CPU walk of the same BVHs (OpenMP) on a band of rows of
     every rank; also checks those rows against the GPU images."""
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
     exported = [ctx.tris_export(g) for g in geoms]
     d8 = imgs.rgba8.cpu().numpy().view(np.uint32)
     dz = imgs.depth.cpu().numpy()
@@ -558,10 +732,104 @@ def surface_cpu_baseline(ctx, geoms, sw, imgs, seconds):
     sl = slice(rows[0], rows[1])
     mism = sum(int((o["rgba8"][sl] != d8[r, sl]).sum() + (o["depth"][sl] != dz[r, sl]).sum())
                for r, o in enumerate(outs))
+    fp64 = fp64_surface(sw, exported, outs, d8, dz, rows)
     return {"value": round(rays / el / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"rows {rows[0]}..{rows[1]} of all {len(geoms)} ranks ({rays} rays, {el:.1f}s), "
                       f"oracle BVH walk, OpenMP",
-            "gpu_mismatched_values_in_sample": mism}
+            "gpu_mismatched_values_in_sample": mism, "host": host, "fp64": fp64}
+
+
+def fp64_surface(sw, exported, outs, d8, dz, rows):
+    """CPU leg: for the band's hit pixels, the closest hit's distance and
+    headlight colour recomputed in float64 (Moller-Trumbore on the hit
+    triangle the oracle's walk found, ray direction and normal in float64)
+    against the GPU's depth and 8-bit colour."""
+    import oracle as O
+    cam = {k: np.asarray(v, np.float64) for k, v in sw.cam.items()}
+    sl = slice(*rows)
+    ys, xs = np.mgrid[sl, 0:sw.W]
+    d = (cam["dir_00"][None, None] + (xs[..., None] + .5) * cam["dir_du"] +
+         (ys[..., None] + .5) * cam["dir_dv"])
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    d = np.where(np.abs(d) < 1e-5, 1e-5, d)
+    worst_z, worst_rel, worst_lv, hits = 0.0, 0.0, 0, 0
+    for r, ((nodes, tris, root), o) in enumerate(zip(exported, outs)):
+        prim = o["prim"][sl]
+        hit = prim >= 0
+        if not hit.any():
+            continue
+        t12 = np.asarray(tris, np.float32).reshape(-1, 12)
+        ids = t12[:, 3].view(np.int32)
+        order = np.argsort(ids)
+        row = order[np.searchsorted(ids[order], prim[hit])]
+        v0 = t12[row, 0:3].astype(np.float64)
+        e1 = t12[row, 4:7].astype(np.float64) - v0
+        e2 = t12[row, 8:11].astype(np.float64) - v0
+        dd = d[hit]
+        pv = np.cross(dd, e2)
+        det = (e1 * pv).sum(-1)
+        tv = cam["org"] - v0
+        qv = np.cross(tv, e1)
+        t = (e2 * qv).sum(-1) / det
+        z64 = t / sw.max_depth
+        gn = np.cross(e1, e2)
+        gn /= np.linalg.norm(gn, axis=-1, keepdims=True)
+        lam = np.maximum(0.0, -(dd * gn).sum(-1))
+        cid = t12[row, 7].view(np.uint32)
+        uc, inv = np.unique(cid, return_inverse=True)
+        rc = np.array([O.random_color(int(c)) for c in uc], np.float64)[inv]
+        col = np.clip(np.floor((.2 + rc * lam[:, None]) * 256), 0, 255)
+        g8 = d8[r, sl][hit]
+        lv = max(int(np.abs(((g8 >> (8 * k)) & 255) - col[:, k]).max()) for k in range(3))
+        ez = np.abs(dz[r, sl][hit] - z64)
+        worst_z = max(worst_z, float(ez.max()))
+        worst_rel = max(worst_rel, float((ez / np.abs(z64)).max()))
+        worst_lv = max(worst_lv, lv)
+        hits += int(hit.sum())
+    return {"sample": f"hit pixels of rows {rows[0]}..{rows[1]}, all ranks ({hits})",
+            "max_abs_depth_gpu_vs_fp64": worst_z, "max_rel_depth_gpu_vs_fp64": worst_rel,
+            "max_rgb8_levels_gpu_vs_fp64": worst_lv}
+
+
+def host_threads():
+    """Worker threads of the CPU baselines: every CPU this process may run
+    on, at most 16 (one GPU's share of the host's cores), and the counts it
+    was chosen from."""
+    nproc = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    return min(16, allowed), {"nproc": nproc, "affinity_cpus": allowed,
+                              "cap": "16 threads per GPU (the host's CPUs are shared per GPU)"}
+
+
+def fp64_blobs128():
+    """CPU leg: the 128^3 u8 two-brick volume at 512x512 (the CPU-runnable
+    configuration): the GPU frame against the oracle's float32 and float64
+    re-renders of both ranks composited in visibility order."""
+    import torch
+    import aloha_amd as A
+    from aloha_amd import RankImages
+    from aloha_amd import workloads as WL
+    import oracle as O
+    threads, _ = host_threads()
+    wl = WL.blobs128()
+    P, W, H = wl.num_ranks, wl.W, wl.H
+    with A.Context(0, stream=torch.cuda.current_stream()) as ctx:
+        ids = [ctx.upload_brick(d, v) for d, v in zip(wl.bricks, wl.voxels)]
+        ctx.set_tf(wl.tf)
+        ctx.set_camera(wl.cam["org"], wl.cam["dir_00"], wl.cam["dir_du"], wl.cam["dir_dv"])
+        imgs = RankImages.alloc(P, W, H, rgba=True, rgba8=False, depth=False)
+        out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        ctx.render_frame(wl.params(), W, H, [[ids[i] for i in r] for r in wl.rank_bricks], imgs,
+                         wl.order, out, None)
+        torch.cuda.synchronize()
+        g = out.cpu().numpy()
+    outs = [O.render_rank(wl.cam, wl.tf, [(wl.bricks[i], wl.voxels[i]) for i in wl.rank_bricks[r]],
+                          W, H, dt=wl.dt, max_depth=wl.max_depth, want64=True, threads=threads)
+            for r in range(P)]
+    c32, _ = O.composite_over(np.stack([o["rgba"] for o in outs]), wl.order)
+    c64 = O.composite_over_d(np.stack([o["rgba64"] for o in outs]), wl.order)
+    mask = sum(o["nsamples"] for o in outs) > 0
+    return _fp64_line(g, c32, c64, mask, "whole frame")
 
 
 def cpu_baseline(wl, imgs, seconds):
@@ -569,7 +837,7 @@ def cpu_baseline(wl, imgs, seconds):
     every rank, on this host's cores; also the FP64 error of both paths on
     that band."""
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
     H = wl.H
     gpu_rgba = imgs.rgba.cpu().numpy()
     # band through the image centre, grown until it holds ~`seconds` of work
@@ -610,7 +878,8 @@ def cpu_baseline(wl, imgs, seconds):
         max_ulp = max(max_ulp, float(np.abs(gpu_rgba[r, sl] - o["rgba"][sl]).max()))
     cpu = {"value": round(cpu_gs, 5), "unit": "Gsamples/s", "cores": threads, "kind": "port",
            "sample": f"{reps} x rows {rows[0]}..{rows[1]} of all {wl.num_ranks} ranks ({ns} "
-                     f"samples, {el:.1f}s), FP32 oracle, OpenMP"}
+                     f"samples, {el:.1f}s), FP32 oracle, OpenMP",
+           "host": host}
     parity = {"rows": f"{r0[0]}..{r0[1]}", "max_abs_gpu_vs_cpu_fp32": max_ulp,
               "max_abs_gpu_vs_fp64": err_gpu, "max_abs_cpu_fp32_vs_fp64": err_cpu}
     return cpu, parity

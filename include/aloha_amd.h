@@ -39,6 +39,9 @@ extern "C" {
 /* render flags */
 #define ALOHA_FLAG_CROSSHAIR 1 /* invert row H/2 and column W/2 in rgba8 (deviceCode.cpp:410-414) */
 #define ALOHA_FLAG_HEATMAP 2   /* blend a per-pixel clock heat map (deviceCode.cpp:399-403) */
+#define ALOHA_FLAG_HEAT_CLOCK 4 /* with HEATMAP: every pixel's clock delta is params.heat_clock
+                                   (instead of clock64() around its march), a reproducible
+                                   heat map for checking the ramp and blend */
 
 /* limits of one render launch (kernel-argument tables) */
 #define ALOHA_MAX_RANKS 64
@@ -74,6 +77,7 @@ typedef struct {
   float jitter_value;
   int32_t flags;
   float heat_map_scale;
+  uint32_t heat_clock; /* clock delta of every pixel under ALOHA_FLAG_HEAT_CLOCK */
 } aloha_render_params;
 
 /* Per-rank outputs (device pointers, each may be NULL).  Rank slot s of a

@@ -27,7 +27,9 @@ class OrcParams(C.Structure):
                 ("W", C.c_int32), ("H", C.c_int32), ("spp", C.c_int32), ("accum_id", C.c_int32),
                 ("island_id", C.c_int32), ("num_islands", C.c_int32),
                 ("jitter_mode", C.c_int32), ("jitter_value", C.c_float),
-                ("crosshair", C.c_int32), ("tf_n", C.c_int32), ("tf", C.POINTER(C.c_float))]
+                ("crosshair", C.c_int32), ("tf_n", C.c_int32), ("tf", C.POINTER(C.c_float)),
+                ("heat", C.c_int32), ("heat_scale", C.c_float), ("heat_clock", C.c_uint32),
+                ("pad_", C.c_int32)]
 
 
 class OrcDomain(C.Structure):
@@ -204,7 +206,7 @@ _DT = {np.dtype(np.uint8): 1, np.dtype(np.uint16): 2, np.dtype(np.float32): 4}
 def render_rank(cam: dict, tf: np.ndarray, bricks, W: int, H: int, dt=0.5, max_depth=1.0,
                 spp=1, accum_id=0, island_id=0, num_islands=1, jitter_mode=0, jitter_value=0.0,
                 crosshair=False, rows=None, accum=None, want64=False, touched=False,
-                threads=0):
+                threads=0, heat_clock=None, heat_scale=1e-5):
     """bricks: list of (desc, voxels) where desc has cell_range/voxel_range/
     space_range (aloha_amd.BrickDesc or dict) and voxels is (z,y,x) u8/u16/f32.
     Returns dict with rgba (H,W,4), rgba8 (H,W) u32, depth, nsamples, accum,
@@ -234,6 +236,7 @@ def render_rank(cam: dict, tf: np.ndarray, bricks, W: int, H: int, dt=0.5, max_d
     p.jitter_mode, p.jitter_value, p.crosshair = jitter_mode, jitter_value, int(crosshair)
     p.tf_n = tf.shape[0]
     p.tf = tf.ctypes.data_as(C.POINTER(C.c_float))
+    _heat(p, heat_clock, heat_scale)
     row0, row1 = (0, H) if rows is None else rows
     out = {
         "rgba": np.zeros((H, W, 4), np.float32),
@@ -257,6 +260,12 @@ def render_rank(cam: dict, tf: np.ndarray, bricks, W: int, H: int, dt=0.5, max_d
     if rc != 0:
         raise ValueError(f"orc_render_rank failed ({rc})")
     return out
+
+
+def _heat(p, heat_clock, heat_scale):
+    """Heat map with every pixel's clock delta = heat_clock (None: off)."""
+    if heat_clock is not None:
+        p.heat, p.heat_scale, p.heat_clock = 1, heat_scale, int(heat_clock)
 
 
 def composite_over(imgs: np.ndarray, order):
@@ -364,13 +373,15 @@ def diff_render64(cam, tf, bricks, W, H, dt=0.5, jitter_mode=0, jitter_value=0.0
 # ----------------------------------------------------------------------------
 # surface path
 # ----------------------------------------------------------------------------
-def _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair):
+def _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair, heat_clock=None,
+                heat_scale=1e-5):
     p = OrcParams()
     p.org, p.dir00, p.du, p.dv = (_f3(cam[k]) for k in ("org", "dir_00", "dir_du", "dir_dv"))
     p.dt, p.max_depth, p.W, p.H = 0.5, max_depth, W, H
     p.spp, p.accum_id, p.island_id, p.num_islands = spp, 0, island_id, num_islands
     p.jitter_mode, p.jitter_value, p.crosshair = 0, 0.0, int(crosshair)
     p.tf_n, p.tf = 0, None
+    _heat(p, heat_clock, heat_scale)
     return p
 
 
@@ -385,7 +396,8 @@ def _tri_ptrs(out):
 
 
 def tri_render(cam, tri9: np.ndarray, colors: np.ndarray, W: int, H: int, max_depth=1.0, spp=1,
-               island_id=0, num_islands=1, crosshair=False, rows=None, threads=0):
+               island_id=0, num_islands=1, crosshair=False, rows=None, threads=0,
+               heat_clock=None, heat_scale=1e-5):
     """One rank's surface image by brute-force closest hit (deviceCode.cpp:
     340-416 with ModelCH :176-188).  tri9: (n,9) float32 vertices in the
     rank's list order (primitive id = row); colors: (n,) colour ids.
@@ -393,7 +405,8 @@ def tri_render(cam, tri9: np.ndarray, colors: np.ndarray, W: int, H: int, max_de
     tri9 = np.ascontiguousarray(tri9, dtype=np.float32).reshape(-1, 9)
     colors = np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
     assert len(colors) == len(tri9)
-    p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair)
+    p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair, heat_clock,
+                    heat_scale)
     out = _tri_out(H, W)
     row0, row1 = (0, H) if rows is None else rows
     rc = lib().orc_tri_render(C.byref(p), C.c_void_p(tri9.ctypes.data),
@@ -406,13 +419,14 @@ def tri_render(cam, tri9: np.ndarray, colors: np.ndarray, W: int, H: int, max_de
 
 def tri_render_bvh(cam, nodes: np.ndarray, tris: np.ndarray, root: int, W: int, H: int,
                    max_depth=1.0, spp=1, island_id=0, num_islands=1, crosshair=False, rows=None,
-                   threads=0):
+                   threads=0, heat_clock=None, heat_scale=1e-5):
     """The same image through a BVH in the library's device layout (nodes
     (n,16) 4-byte words, tris (m,12) float32), for meshes too large for brute
     force.  prim holds primitive ids."""
     nodes = np.ascontiguousarray(nodes).view(np.uint32).reshape(-1, 16)
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
-    p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair)
+    p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair, heat_clock,
+                    heat_scale)
     out = _tri_out(H, W)
     row0, row1 = (0, H) if rows is None else rows
     f = lib().orc_tri_render_bvh

@@ -71,6 +71,10 @@ typedef struct {
   int32_t crosshair; /* 1: invert row H/2 and column W/2 in the 8-bit output */
   int32_t tf_n;
   const float *tf; /* tf_n x RGBA */
+  int32_t heat;         /* 1: blend the heat map of every pixel's clock delta heat_clock */
+  float heat_scale;     /* heatMapScale */
+  uint32_t heat_clock;  /* the clock delta (clock_end - clock_begin) of every pixel */
+  int32_t pad_;
 } orc_params;
 
 typedef struct {
@@ -628,6 +632,36 @@ static void render_pixel(const orc_params *p, const bview *bv, int nb, int px, i
   *zout = z;
 }
 
+/* heatMap = temperature_to_rgb (deviceCode.cpp:300-320) */
+static float sat_(float f) { return fminf(fmaxf(f, 0.f), 1.f); }
+static void heat_rgb(float t, float out[3]) {
+  const float K = 4.0f / 6.0f;
+  const float h = K - K * t;
+  const float v = .5f + 0.5f * t;
+  const float s = sat_(h) * 6.0f; /* hue_to_rgb(h) */
+  const float r = sat_(fabsf(s - 3.f) - 1.0f);
+  const float g = sat_(2.0f - fabsf(s - 2.0f));
+  const float b = sat_(2.0f - fabsf(s - 4.0f));
+  out[0] = v * r, out[1] = v * g, out[2] = v * b;
+}
+
+/* renderFrame's heat-map blend (deviceCode.cpp:399-403) with the pixel's
+ * clock delta given: t = delta * (heatMapScale / spp) (the uint64 delta
+ * converted to float), then accum = over((heatMap(t), .5), accum) with
+ * over(A, B) = A + (1 - A.w) B (:36-39). */
+static void heat_blend(const orc_params *p, float acc[4], double acc64[4]) {
+  const float t = (float)(uint64_t)p->heat_clock * (p->heat_scale / (float)p->spp);
+  float h[3];
+  heat_rgb(t, h);
+  const float k = 1.f - .5f;
+  for (int c = 0; c < 3; ++c) acc[c] = h[c] + k * acc[c];
+  acc[3] = .5f + k * acc[3];
+  if (acc64) {
+    for (int c = 0; c < 3; ++c) acc64[c] = (double)h[c] + .5 * acc64[c];
+    acc64[3] = .5 + .5 * acc64[3];
+  }
+}
+
 static inline uint32_t pack8(float f) {
   float g = f * 256.f;
   g = fmaxf(g, 0.f);
@@ -672,6 +706,7 @@ int orc_render_rank(const orc_params *p, const orc_brick *bricks, int nb, int ro
       render_pixel(p, bv, nb, px, py, acc, acc64, &z, &ns, rgba64 != NULL);
       if (depth) depth[pix] = z;
       if (nsamples) nsamples[pix] = ns;
+      if (p->heat) heat_blend(p, acc, acc64);
       if (accum) {
         if (p->accum_id > 0)
           for (int c = 0; c < 4; ++c) acc[c] = acc[c] + accum[4 * pix + c];
@@ -1219,6 +1254,7 @@ static void tri_finish(const orc_params *p, int px, int py, const float c[4], fl
   for (int s = 0; s < p->spp; ++s)
     for (int k = 0; k < 4; ++k) acc[k] = acc[k] + c[k];
   if (depth) depth[pix] = z;
+  if (p->heat) heat_blend(p, acc, NULL);
   if (rgba)
     for (int k = 0; k < 4; ++k) rgba[4 * pix + k] = (acc[k] * 1.f) * inv_spp;
   if (rgba8) {
@@ -1242,7 +1278,7 @@ static int island_pixel_of(const orc_params *p, int px, int py) {
 /* One rank's surface image by brute force over its triangles (tri9:
  * ntris x 9 floats in the rank's list order, primitive id = index;
  * color: colour id per triangle).  Rows [row0,row1); every output may be
- * NULL; hit_prim/hit_t: -1 / 0 for a miss.  accum_id 0, no heat map. */
+ * NULL; hit_prim/hit_t: -1 / 0 for a miss.  accum_id 0. */
 int orc_tri_render(const orc_params *p, const float *tri9, const uint32_t *color, long long ntris,
                    int row0, int row1, float *rgba, uint32_t *rgba8, float *depth,
                    int32_t *hit_prim, float *hit_t, int nthreads) {

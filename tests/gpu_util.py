@@ -57,6 +57,10 @@ def oracle_render(wl, ranks=None, *, accum=None, want64=False, **params):
     p = dict(dt=wl.dt, max_depth=wl.max_depth)
     p.update(params)
     flags = p.pop("flags", 0)
+    heat_clock, heat_scale = p.pop("heat_clock", 0), p.pop("heat_map_scale", 1e-5)
+    if flags & A.FLAG_HEATMAP:  # only a host-given clock delta is reproducible
+        assert flags & A.FLAG_HEAT_CLOCK
+        p.update(heat_clock=heat_clock, heat_scale=heat_scale)
     for r, bl in enumerate(ranks):
         o = O.render_rank(wl.cam, wl.tf, [(wl.bricks[i], wl.voxels[i]) for i in bl], wl.W, wl.H,
                           crosshair=bool(flags & A.FLAG_CROSSHAIR),

@@ -53,6 +53,18 @@ def test_c0_fixed_jitter_dt2_crosshair(c0):
     print(compare(gpu_render(c0, **kw), oracle_render(c0, **kw)))
 
 
+def test_c0_heat_map_host_clock(c0):
+    _case("blobs128 heat map with a host-given clock delta, spp=2, crosshair")
+    kw = dict(flags=A.FLAG_HEATMAP | A.FLAG_HEAT_CLOCK | A.FLAG_CROSSHAIR, heat_clock=123457,
+              heat_map_scale=2e-6, spp=2)
+    g = gpu_render(c0, **kw)
+    r = oracle_render(c0, **kw)
+    info = compare(g, r)
+    print(info)
+    assert info["exact_fraction"] == 1.0
+    assert (r["rgba"][..., 3] >= 0.25).all()  # every pixel carries the heat blend
+
+
 def test_c0_spp_accum_islands(c0):
     _case("blobs128 spp=3 accum_id=2 islands 1/3")
     acc = np.random.default_rng(3).random((2, c0.H, c0.W, 4), dtype=np.float32)

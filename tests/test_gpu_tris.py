@@ -74,6 +74,24 @@ def test_rank_images_match_brute_force(small, variant):
     assert hits > 2000
 
 
+def test_heat_map_host_clock(small):
+    print(f"\n[case] tris heat map, host clock delta, {small.W}x{small.H}", flush=True)
+    torch = _torch()
+    ctx, geoms = _ctx(small)
+    P, W, H = len(geoms), small.W, small.H
+    out = _images(P, W, H)
+    params = small.params(spp=2, flags=A.FLAG_HEATMAP | A.FLAG_HEAT_CLOCK, heat_clock=777777,
+                          heat_map_scale=1e-6)
+    ctx.render_tris(params, W, H, geoms, out)
+    torch.cuda.synchronize()
+    for r in range(P):
+        ref = _oracle_rank(small, r, spp=2, heat_clock=777777, heat_scale=1e-6)
+        assert np.array_equal(out.depth[r].cpu().numpy(), ref["depth"])
+        assert np.array_equal(out.rgba[r].cpu().numpy(), ref["rgba"])
+        assert np.array_equal(_u32(out.rgba8[r]), ref["rgba8"])
+    ctx.close()
+
+
 def test_fused_equals_ranks_plus_zmin(small):
     print("\n[case] tris fused composite vs ranks + zmin", flush=True)
     torch = _torch()

@@ -178,6 +178,46 @@ def test_make_rgba():
     assert O.make_rgba([-1, 2, 0.999, 1 / 256]) == (0 | 255 << 8 | 255 << 16 | 1 << 24)
 
 
+# ---------------------------------------------------------------- heat map
+_HEAT_CLOCK = 1 << 20  # a power of two: heat_scale = t / clock is exact
+
+
+@pytest.mark.parametrize("t, rgb", [(0.0, (0, 0, .5)), (.25, (0, .625, .625)), (.5, (0, .75, 0)),
+                                    (.75, (.875, .875, 0)), (1.0, (1, 0, 0))])
+def test_heat_map_ramp_closed_forms(t, rgb):
+    # temperature_to_rgb (deviceCode.cpp:300-320): hue h = 4/6 (1 - t) through
+    # hue_to_rgb's three ramps, value .5 + .5 t.  Over an empty pixel the blend
+    # over((heat, .5), 0) leaves (heat, .5).
+    cam = _cam_looking_down_z(4, 3)
+    tf = np.zeros((4, 4), np.float32)
+    o = O.render_rank(cam, tf, [], 4, 3, heat_clock=_HEAT_CLOCK, heat_scale=t / _HEAT_CLOCK)
+    assert np.allclose(o["rgba"][..., :3], rgb, atol=1e-6)
+    assert (o["rgba"][..., 3] == 0.5).all()
+    # spp divides the clock before the ramp, and the spp sum after the blend
+    o2 = O.render_rank(cam, tf, [], 4, 3, spp=2, heat_clock=_HEAT_CLOCK,
+                       heat_scale=2 * t / _HEAT_CLOCK)
+    assert np.allclose(o2["rgba"][..., :3], np.asarray(rgb) / 2, atol=1e-6)
+    assert (o2["rgba"][..., 3] == 0.25).all()
+
+
+def test_heat_map_blends_over_the_pixel():
+    # accumColor = over((heat, .5), accumColor) = heat + .5 accumColor (:36-39, :402)
+    vox = np.full((12, 6, 7), 0.5, np.float32)
+    b = _brick((0, 0, 0), (7, 6, 12))
+    tf = np.tile(np.array([[0.8, 0.4, 0.2, 0.3]], np.float32), (8, 1))
+    cam = _cam_looking_down_z(6, 5, org=(3.0, 2.5, -6.0), fov=0.2)
+    plain = O.render_rank(cam, tf, [(b, vox)], 6, 5, jitter_mode=1, jitter_value=0.5)
+    hot = O.render_rank(cam, tf, [(b, vox)], 6, 5, jitter_mode=1, jitter_value=0.5,
+                        heat_clock=_HEAT_CLOCK, heat_scale=0.5 / _HEAT_CLOCK)
+    assert plain["rgba"][..., 3].min() > 0.1  # every ray crosses the brick
+    heat = np.array([0, .75, 0], np.float32)
+    want = np.empty_like(plain["rgba"])
+    want[..., :3] = heat + f32(0.5) * plain["rgba"][..., :3]
+    want[..., 3] = f32(0.5) + f32(0.5) * plain["rgba"][..., 3]
+    assert np.allclose(hot["rgba"], want, atol=1e-6)
+    assert np.array_equal(hot["depth"], plain["depth"])
+
+
 # ---------------------------------------------------------------- composite
 def test_over_associativity_and_opaque_front():
     rng = np.random.default_rng(4)
