@@ -117,6 +117,7 @@ struct Shaded {
   float raw_a;  // TF opacity before the clamp
   float shade;
   float len;
+  float nd;  // N . d as the shade evaluates it (its sign: the side of the |N.d| kink)
 };
 
 __device__ __forceinline__ Shaded shade_sample(const DiffArgs &a, const float4 *s_tf,
@@ -134,7 +135,8 @@ __device__ __forceinline__ Shaded shade_sample(const DiffArgs &a, const float4 *
   r.tf = tf_lookup(s_tf, a.tf_n, r.s.val);
   r.raw_a = r.tf.w;
   r.tf.w = fminf(r.tf.w, a.alpha_max);
-  r.shade = .1f + .9f * fabsf(nx * d[0] + ny * d[1] + nz * d[2]);
+  r.nd = nx * d[0] + ny * d[1] + nz * d[2];
+  r.shade = .1f + .9f * fabsf(r.nd);
   return r;
 }
 
@@ -142,6 +144,65 @@ __device__ void load_tf(const DiffArgs &a, float4 *s_tf) {
   for (int i = threadIdx.x; i < a.tf_n + 4; i += kBlock)
     s_tf[i] = (i >= 2 && i < a.tf_n + 2) ? a.tf[i - 2] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
+
+// The bin of a sample at brick-texel position (x, y, z) of brick slot `slot`.
+__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, int n0, int n1, int n2,
+                                          float x, float y, float z) {
+  const int ax = clampi((int)floorf(x - .5f), -1, n0);
+  const int ay = clampi((int)floorf(y - .5f), -1, n1);
+  const int az = clampi((int)floorf(z - .5f), -1, n2);
+  const int bx = (ax + 1) / kBinEdge, by = (ay + 1) / kBinEdge, bz = (az + 1) / kBinEdge;
+  return b.base[slot] + (bz * b.nby[slot] + by) * b.nbx[slot] + bx;
+}
+__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const DevBrick &db, float x,
+                                          float y, float z) {
+  return sample_bin(b, slot, db.n[0], db.n[1], db.n[2], x, y, z);
+}
+
+// Sample counts of a lane in up to kTallySlots bins (a lane's samples --
+// its rays' consecutive lattice points -- fall in few bins), summed over the
+// wave per bin at the end: one global atomic per distinct bin of the wave
+// instead of one per run of one lane, many of them to the same counter.  A
+// sample of a further bin goes to its counter at once.
+constexpr int kTallySlots = 4;
+struct BinTally {
+  int bin[kTallySlots];
+  uint32_t n[kTallySlots];
+  __device__ BinTally() {
+#pragma unroll
+    for (int k = 0; k < kTallySlots; ++k) bin[k] = -1, n[k] = 0u;
+  }
+  __device__ void add(uint32_t *count, int b, uint32_t c) {
+    bool done = false;
+#pragma unroll
+    for (int k = 0; k < kTallySlots; ++k)
+      if (!done && bin[k] == b) n[k] += c, done = true;
+#pragma unroll
+    for (int k = 0; k < kTallySlots; ++k)
+      if (!done && bin[k] < 0) bin[k] = b, n[k] = c, done = true;
+    if (!done) atomicAdd(&count[b], c);
+  }
+  // Every lane of the wave calls it (wave-uniform control flow).
+  __device__ void flush(uint32_t *count) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < kTallySlots; ++k) {
+      while (true) {
+        const uint64_t pm = __ballot(bin[k] >= 0);
+        if (!pm) break;
+        const int leader = __ffsll((unsigned long long)pm) - 1;
+        const int lb = __shfl(bin[k], leader);
+        uint32_t v = 0u;
+#pragma unroll
+        for (int j = k; j < kTallySlots; ++j)
+          if (bin[j] == lb) v += n[j], bin[j] = -1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+        if (lane == leader) atomicAdd(&count[lb], v);
+      }
+    }
+  }
+};
 
 // The forward with lane groups: G lanes per ray evaluate G lattice samples
 // of its segment at once (t_k depends on k alone, so the lanes need no
@@ -477,16 +538,6 @@ __device__ __forceinline__ uint32_t visit_count(const DiffArgs &a, const Visit &
   return cnt;
 }
 
-// The bin of a sample at brick-texel position (x, y, z) of brick slot `slot`.
-__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const DevBrick &db, float x,
-                                          float y, float z) {
-  const int ax = clampi((int)floorf(x - .5f), -1, db.n[0]);
-  const int ay = clampi((int)floorf(y - .5f), -1, db.n[1]);
-  const int az = clampi((int)floorf(z - .5f), -1, db.n[2]);
-  const int bx = (ax + 1) / kBinEdge, by = (ay + 1) / kBinEdge, bz = (az + 1) / kBinEdge;
-  return b.base[slot] + (bz * b.nby[slot] + by) * b.nbx[slot] + bx;
-}
-
 __device__ __forceinline__ void sample_pos(const float org[3], const float d[3], const DevBrick &db,
                                            float t, float &x, float &y, float &z) {
   x = (org[0] + t * d[0]) - db.vox_lo[0];
@@ -495,42 +546,46 @@ __device__ __forceinline__ void sample_pos(const float org[3], const float d[3],
 }
 
 // Count pass: every sample the forward took (nsamples per pixel), counted
-// into its bin.  A ray's consecutive samples
-// mostly share a bin, so each run of one bin costs one atomic.
+// into its bin: a ray's consecutive samples mostly share a bin, so runs of
+// one bin go to the lane's tally, and the wave's tallies to the counters.
 __global__ __launch_bounds__(kBlock) void diff_count_kernel(const DiffArgs a, const DiffBins b) {
   const int view = blockIdx.z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
   const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
-  if (px >= a.W || py >= a.H) return;
+  BinTally tally;
+  // no early exit: every lane reaches the wave's flush
+  const bool in = px < a.W && py < a.H;
   const size_t out = (size_t)view * a.W * a.H + (px + (size_t)a.W * py);
-  const uint32_t N = a.nsamples[out];
-  if (N == 0) return;
-  const int pixel = px + a.W * py;
-  const float *org = a.cam[view];
-  float d[3];
-  ray_dir(a, view, px, py, d);
-  uint32_t acc = 0, run = 0;
-  int run_bin = -1;
-  for (int vi = 0; acc < N && vi < kMaxVisits; ++vi) {
-    Visit v;
-    if (visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi) <= vi) break;
-    const uint32_t cnt = visit_count(a, v, N - acc);
-    const DevBrick db = a.bricks[v.bid];
-    for (uint32_t k = 0; k < cnt; ++k) {
-      float x, y, z;
-      sample_pos(org, d, db, lattice_t(a, v, (int)k), x, y, z);
-      const int bin = sample_bin(b, v.slot, db, x, y, z);
-      if (bin != run_bin) {
-        if (run) atomicAdd(&b.count[run_bin], run);
-        run_bin = bin;
-        run = 0;
+  const uint32_t N = in ? a.nsamples[out] : 0u;
+  if (N > 0) {
+    const int pixel = px + a.W * py;
+    const float *org = a.cam[view];
+    float d[3];
+    ray_dir(a, view, px, py, d);
+    uint32_t acc = 0, run = 0;
+    int run_bin = -1;
+    for (int vi = 0; acc < N && vi < kMaxVisits; ++vi) {
+      Visit v;
+      if (visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi) <= vi) break;
+      const uint32_t cnt = visit_count(a, v, N - acc);
+      const DevBrick db = a.bricks[v.bid];
+      for (uint32_t k = 0; k < cnt; ++k) {
+        float x, y, z;
+        sample_pos(org, d, db, lattice_t(a, v, (int)k), x, y, z);
+        const int bin = sample_bin(b, v.slot, db, x, y, z);
+        if (bin != run_bin) {
+          if (run) tally.add(b.count, run_bin, run);
+          run_bin = bin;
+          run = 0;
+        }
+        ++run;
       }
-      ++run;
+      acc += cnt;
     }
-    acc += cnt;
+    if (run) tally.add(b.count, run_bin, run);
   }
-  if (run) atomicAdd(&b.count[run_bin], run);
+  tally.flush(b.count);
 }
 
 // Exclusive prefix of the bin counts (one workgroup): off[] and the record
@@ -612,6 +667,16 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
   const float TN = T;
   float Sr = 0.f, Sg = 0.f, Sb = 0.f;  // suffix sums of later contributions
   uint32_t rec = 0, rec_end = 0, run_left = 0;  // records of the current bin run
+  // the run's bin and the largest bounds of its records (diff_bin_kernel's
+  // fixed-point scales), raised into the bin's maxima once per run
+  int run_bin = -1;
+  float rmd = 0.f, rmt = 0.f;
+  auto flush_run = [&]() {
+    if (run_bin >= 0 && (rmd != 0.f || rmt != 0.f)) {
+      atomicMax(&b.maxd[run_bin], __float_as_uint(rmd));
+      atomicMax(&b.maxt[run_bin], __float_as_uint(rmt));
+    }
+  };
   const int n_tf = a.tf_n;
   for (int vi = last; vi >= 0; --vi) {
     Visit v;
@@ -646,8 +711,7 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
                         da * (e1.w - e0.w)) * (float)n_tf;
       // shading: s = .1 + .9|N.d|, N = g/L, L = |g + 1e-4|; the side of the
       // |N.d| kink as the forward took it
-      const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
-      const float nd = nx * d[0] + ny * d[1] + nz * d[2];
+      const float nd = r.nd;
       const float sg = nd > 0.f ? .9f : (nd < 0.f ? -.9f : 0.f);
       const float dN[3] = {dshade * sg * d[0], dshade * sg * d[1], dshade * sg * d[2]};
       const float gv[3] = {r.s.gx, r.s.gy, r.s.gz};
@@ -672,17 +736,29 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
         rec = atomicAdd(&b.cursor[bin], len);
         rec_end = min(b.off[bin + 1], b.capacity);
         run_left = len;
+        flush_run();
+        run_bin = bin;
+        rmd = rmt = 0.f;
       }
       if (rec < rec_end) {
         float4 *o = b.records + (size_t)kRecWords * rec;
         o[0] = make_float4(x, y, z, dv);
         o[1] = make_float4(dg[0], dg[1], dg[2], r.s.val);
         o[2] = make_float4(dcr, dcg, dcb, da);
+        // bounds of the record's contributions: density |dv| + 2 (|dgx| +
+        // |dgy| + |dgz|) (the weights of a tap sum to 1), TF the largest of
+        // |dc|, |da|; NaN wins (as unsigned bits it is above every finite
+        // bound and infinity)
+        const float bd = fabsf(dv) + 2.f * (fabsf(dg[0]) + fabsf(dg[1]) + fabsf(dg[2]));
+        const float bt = fmaxf(fmaxf(fabsf(dcr), fabsf(dcg)), fmaxf(fabsf(dcb), fabsf(da)));
+        rmd = (bd > rmd || bd != bd) ? bd : rmd;
+        rmt = (bt > rmt || bt != bt) ? bt : rmt;
       }
       ++rec;
       --run_left;
     }
   }
+  flush_run();
 }
 
 // Scale 2^S of a bin's fixed-point sums: at most `count` contributions of at
@@ -694,20 +770,6 @@ __device__ __forceinline__ double fixed_scale(float bound, uint32_t count) {
   int e;
   frexp((double)bound * (double)count, &e);  // bound * count < 2^e
   return ldexp(1.0, 61 - e);
-}
-
-// Block-wide maximum of non-negative floats (NaN wins); every thread gets it.
-__device__ __forceinline__ float block_max(float v, float *red) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const float u = __shfl_xor(v, o);
-    v = (u > v || u != u) ? u : v;
-  }
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  float m = red[0];
-  for (int w = 1; w < kBlock / 64; ++w) m = (red[w] > m || red[w] != red[w]) ? red[w] : m;
-  __syncthreads();
-  return m;
 }
 
 // Adds a run's fixed-point TF sums (entries i and i + 1 of the chunk
@@ -735,7 +797,6 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
   extern __shared__ unsigned long long s_fix[];
   unsigned long long *tile = s_fix;               // kTileVox
   unsigned long long *tfs = s_fix + kTileVox;     // 4 x min(tf_n, kLdsTfChunk)
-  __shared__ float red[kBlock / 64];
   const int bin = blockIdx.x;
   const uint32_t r0 = b.off[bin], r1 = min(b.off[bin + 1], b.capacity);
   if (r0 >= r1) return;  // block-uniform
@@ -750,19 +811,8 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
   // chunk, so the lanes of one instruction rarely add to the same word
   const uint32_t per = (r1 - r0 + kBlock - 1) / kBlock;
   const uint32_t c0 = r0 + per * threadIdx.x, c1 = min(r1, c0 + per);
-  // bounds of one record's contributions: density |dv| + 2 (|dgx|+|dgy|+|dgz|)
-  // (the weights of a tap sum to 1), TF the largest of |dc|, |da|
-  float md = 0.f, mt = 0.f;
-  for (uint32_t r = c0; r < c1; ++r) {
-    const float4 *p = b.records + (size_t)kRecWords * r;
-    const float4 p0 = p[0], p1 = p[1], p2 = p[2];
-    const float bd = fabsf(p0.w) + 2.f * (fabsf(p1.x) + fabsf(p1.y) + fabsf(p1.z));
-    const float bt = fmaxf(fmaxf(fabsf(p2.x), fabsf(p2.y)), fmaxf(fabsf(p2.z), fabsf(p2.w)));
-    md = (bd > md || bd != bd) ? bd : md;
-    mt = (bt > mt || bt != bt) ? bt : mt;
-  }
-  md = block_max(md, red);
-  mt = block_max(mt, red);
+  // the largest bounds of the bin's records (raised by the adjoint march)
+  const float md = __uint_as_float(b.maxd[bin]), mt = __uint_as_float(b.maxt[bin]);
   const uint32_t cnt = r1 - r0;
   const double sd = dens ? fixed_scale(md, 8u * cnt) : 0.0;  // 8 corners of a tap share a voxel
   const double st = want_tf ? fixed_scale(mt, cnt) : 0.0;

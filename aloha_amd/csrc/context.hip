@@ -923,6 +923,36 @@ int build_diff(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t 
 }
 }  // namespace
 
+namespace {
+// Bins of the brick slots: 16^3 blocks over the corners -1 .. n of each axis
+// (every slot, with or without a gradient grid: the counts cover them all).
+// Returns the bin total; max_dims / max_voxels over the slots with a grid.
+int64_t diff_layout(const aloha_ctx *ctx, int32_t nb, const int32_t *brick_ids,
+                    float *const *dvox, DiffBins &b, int max_dims[3], int64_t &max_voxels) {
+  std::memset(&b, 0, sizeof b);
+  int64_t nbins = 0;
+  max_voxels = 0;
+  max_dims[0] = max_dims[1] = max_dims[2] = 0;
+  for (int k = 0; k < nb; ++k) {
+    b.base[k] = (int32_t)std::min<int64_t>(nbins, INT32_MAX);
+    const DevBrick &db = ctx->bricks[brick_ids[k]];
+    b.nbx[k] = (db.n[0] + 1) / kBinEdge + 1;
+    b.nby[k] = (db.n[1] + 1) / kBinEdge + 1;
+    b.nbz[k] = (db.n[2] + 1) / kBinEdge + 1;
+    nbins += (int64_t)b.nbx[k] * b.nby[k] * b.nbz[k];
+    if (dvox && dvox[k]) {
+      max_voxels = std::max<int64_t>(max_voxels, (int64_t)db.n[0] * db.n[1] * db.n[2]);
+      for (int ax = 0; ax < 3; ++ax) max_dims[ax] = std::max(max_dims[ax], db.n[ax]);
+    }
+  }
+  b.base[nb] = (int32_t)std::min<int64_t>(nbins, INT32_MAX);
+  b.nbins = (int32_t)std::min<int64_t>(nbins, INT32_MAX);
+  return nbins;
+}
+
+constexpr int64_t kMaxDiffBins = 1 << 24;
+}  // namespace
+
 extern "C" int aloha_diff_forward(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
                                   int32_t H, int32_t V, const float *cams, int32_t nb,
                                   const int32_t *brick_ids, float *d_rgba,
@@ -956,33 +986,18 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
   a.dldc = (const float4 *)d_dl_drgba;
   a.dtf = d_dtf;
   for (int k = 0; k < nb; ++k) a.dvox[k] = d_dvoxels ? d_dvoxels[k] : nullptr;
-  // bins of the bricks with a gradient grid: 16^3 blocks over the corners
-  // -1 .. n of each axis
   DiffBins b;
-  std::memset(&b, 0, sizeof b);
-  int64_t nbins = 0, max_voxels = 0;
-  int max_dims[3] = {0, 0, 0};
-  for (int k = 0; k < nb; ++k) {
-    b.base[k] = (int32_t)nbins;
-    const DevBrick &db = ctx->bricks[brick_ids[k]];
-    b.nbx[k] = (db.n[0] + 1) / kBinEdge + 1;
-    b.nby[k] = (db.n[1] + 1) / kBinEdge + 1;
-    b.nbz[k] = (db.n[2] + 1) / kBinEdge + 1;
-    nbins += (int64_t)b.nbx[k] * b.nby[k] * b.nbz[k];
-    if (a.dvox[k]) {
-      max_voxels = std::max<int64_t>(max_voxels, (int64_t)db.n[0] * db.n[1] * db.n[2]);
-      for (int ax = 0; ax < 3; ++ax) max_dims[ax] = std::max(max_dims[ax], db.n[ax]);
-    }
-  }
-  b.base[nb] = (int32_t)nbins;
+  int max_dims[3];
+  int64_t max_voxels = 0;
+  const int64_t nbins = diff_layout(ctx, nb, brick_ids, a.dvox, b, max_dims, max_voxels);
   if ((int64_t)max_dims[2] * nb > 65535)  // the fold's grid.z
     return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %d bricks of depth %d", nb, max_dims[2]);
-  if (nbins > (1 << 24))
+  if (nbins > kMaxDiffBins)
     return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %lld gradient bins", (long long)nbins);
-  b.nbins = (int32_t)nbins;
   if (nbins > 0) {
-    // count pass and scan, then the record total (read back: it sizes the records)
-    const size_t words = 3 * (size_t)nbins + 1;
+    // counts, per-bin maxima, scan, then the record total (read back: it
+    // sizes the records)
+    const size_t words = 5 * (size_t)nbins + 1;
     if (words > ctx->diff_bins_cap) {
       if (ctx->diff_bins) HIP_CHECK(ctx, hipFree(ctx->diff_bins));
       ctx->diff_bins = nullptr;
@@ -991,10 +1006,12 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       ctx->diff_bins_cap = words;
     }
     if (!ctx->h_total) HIP_CHECK(ctx, hipHostMalloc(&ctx->h_total, sizeof(uint32_t)));
-    b.count = ctx->diff_bins;
-    b.off = b.count + nbins;
+    b.count = ctx->diff_bins;  // count | maxd | maxt | off (n + 1) | cursor
+    b.maxd = b.count + nbins;
+    b.maxt = b.maxd + nbins;
+    b.off = b.maxt + nbins;
     b.cursor = b.off + nbins + 1;
-    HIP_CHECK(ctx, hipMemsetAsync(b.count, 0, sizeof(uint32_t) * (size_t)nbins, ctx->stream));
+    HIP_CHECK(ctx, hipMemsetAsync(b.count, 0, sizeof(uint32_t) * 3 * (size_t)nbins, ctx->stream));
     HIP_CHECK(ctx, launch_diff_count(a, b, ctx->stream));
     HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_total, b.off + nbins, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, ctx->stream));
@@ -1038,6 +1055,7 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
   return ALOHA_OK;
 }
 
+
 // ---------------------------------------------------------------------------
 // batched form: many views of one rank table in one (graph-captured) launch
 // ---------------------------------------------------------------------------
@@ -1048,6 +1066,7 @@ struct aloha_batch {
   float *d_cams = nullptr;
   int32_t *d_orders = nullptr, *d_rects = nullptr, *d_boxes = nullptr, *d_box_off = nullptr;
   uint32_t *d_counter = nullptr;
+  int32_t *d_xoff = nullptr;
   int lanes = 2;
   int persistent_blocks = 0;
   int cus = 0;
@@ -1072,10 +1091,11 @@ void batch_release(aloha_batch *bt) {
   if (bt->stream) (void)hipStreamSynchronize(bt->stream);
   batch_drop_graph(bt);
   for (void *p : {(void *)bt->d_cams, (void *)bt->d_orders, (void *)bt->d_rects, (void *)bt->d_boxes,
-                  (void *)bt->d_box_off, (void *)bt->d_counter})
+                  (void *)bt->d_box_off, (void *)bt->d_counter, (void *)bt->d_xoff})
     if (p) (void)hipFree(p);
   bt->d_cams = nullptr, bt->d_orders = bt->d_rects = bt->d_boxes = bt->d_box_off = nullptr;
   bt->d_counter = nullptr;
+  bt->d_xoff = nullptr;
   if (bt->ev_in) (void)hipEventDestroy(bt->ev_in);
   if (bt->ev_out) (void)hipEventDestroy(bt->ev_out);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);
@@ -1160,6 +1180,15 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
     box_off[v + 1] = (int32_t)next;
   }
   if ((int64_t)V * H > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many rows");
+  // the persistent schedule's per-XCD lists: part x of view v is its items
+  // [n x / kXcds, n (x + 1) / kXcds), n = the view's item count
+  std::vector<int32_t> xoff((size_t)kXcds * (V + 1), 0);
+  for (int x = 0; x < kXcds; ++x)
+    for (int v = 0; v < V; ++v) {
+      const int64_t n = (int64_t)box_off[v + 1] - box_off[v];
+      const int64_t part = n * (x + 1) / kXcds - n * x / kXcds;
+      xoff[(size_t)x * (V + 1) + v + 1] = xoff[(size_t)x * (V + 1) + v] + (int32_t)part;
+    }
   auto *bt = new aloha_batch;
   bt->ctx = ctx;
   std::memset(&bt->b, 0, sizeof bt->b);
@@ -1176,7 +1205,9 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   if (e == hipSuccess) e = hipMalloc(&bt->d_rects, sizeof(int32_t) * rects.size());
   if (e == hipSuccess) e = hipMalloc(&bt->d_boxes, sizeof(int32_t) * boxes.size());
   if (e == hipSuccess) e = hipMalloc(&bt->d_box_off, sizeof(int32_t) * box_off.size());
-  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(uint32_t) * kXcds);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_xoff, sizeof(int32_t) * xoff.size());
+  if (e == hipSuccess) e = hipMemcpy(bt->d_xoff, xoff.data(), sizeof(int32_t) * xoff.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_cams, cams, sizeof(float) * 12 * (size_t)V, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_orders, orders, sizeof(int32_t) * (size_t)V * P, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_rects, rects.data(), sizeof(int32_t) * rects.size(), hipMemcpyHostToDevice);
@@ -1195,6 +1226,7 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   bt->b.boxes = bt->d_boxes;
   bt->b.box_off = bt->d_box_off;
   bt->b.counter = bt->d_counter;
+  bt->b.xoff = bt->d_xoff;
   bt->b.total_active = box_off[V];
   bt->b.fill_blocks = (int)std::min<int64_t>((int64_t)V * H, 1024);
   bt->lanes = ctx->fused_lanes;

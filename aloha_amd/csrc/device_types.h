@@ -81,6 +81,10 @@ struct RanksArgs {
 constexpr int kFusedTileW = 16, kFusedTileH = 4;
 
 // Batched form: V views of one rank table in one launch.
+// Workgroups are dealt to the GPU's XCDs (each with its own L2) in turn:
+// workgroup i runs on XCD i % kXcds.
+constexpr int kXcds = 8;
+
 struct BatchArgs {
   MarchArgs m;              // ranks, bricks, TF, params; m.rgba/rgba8/nsamples are V x H x W
   const float *cams;        // V x 12 (org, dir_00, dir_du, dir_dv)
@@ -88,7 +92,11 @@ struct BatchArgs {
   const int32_t *rects;     // V x P x 4 pixel rectangles
   const int32_t *boxes;     // V x 4: fused-tile bounding box of the view's rectangles (x0, y0, nx, ny)
   const int32_t *box_off;   // V + 1: prefix sums of nx * ny (active work items)
-  uint32_t *counter;        // persistent schedule: next work item
+  // persistent schedule: each view's work items cut into kXcds contiguous
+  // parts; xoff[x * (V + 1) + v] = the items of parts x of views < v, and
+  // counter[x] the next index into the list of parts x
+  const int32_t *xoff;
+  uint32_t *counter;        // kXcds counters
   int32_t V, total_active;  // views, active work items (box_off[V])
   int32_t fill_blocks;      // non-persistent schedule: blocks that write the rows outside the boxes
 };
@@ -130,6 +138,8 @@ struct DiffBins {
   int32_t nbx[kMaxDiffBricks], nby[kMaxDiffBricks], nbz[kMaxDiffBricks];
   int32_t nbins;
   uint32_t *count;     // samples per bin (count pass)
+  uint32_t *maxd;      // per bin: bits of the largest record density bound (>= 0 or NaN)
+  uint32_t *maxt;      // per bin: bits of the largest record TF bound
   uint32_t *off;       // exclusive prefix of count, nbins + 1 entries
   uint32_t *cursor;    // next free record of each bin (starts at off)
   float4 *records;     // kRecWords float4 per sample
@@ -204,8 +214,8 @@ hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hi
 hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
                             int32_t px, int32_t py, int32_t pz, hipStream_t s);
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
-// backward in stages: count samples per bin (and scan), then the march that
-// writes records and the TF gradient, then bins into tiles and tiles into
+// backward in stages: count samples per bin and scan, then the march that
+// writes records and the per-bin bounds, then bins into tiles and tiles into
 // the gradient grids
 hipError_t launch_diff_count(const DiffArgs &a, const DiffBins &b, hipStream_t s);
 hipError_t launch_diff_backward(const DiffArgs &a, const DiffBins &b, hipStream_t s);

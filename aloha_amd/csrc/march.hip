@@ -1005,22 +1005,39 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
     fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
     return;
   }
+  // XCD x works through part x of every view (a contiguous band of each
+  // view's tiles, so its L2 holds the bricks of that band), then helps the
+  // other XCDs with what is left of theirs
   const MarchTables tb = load_tables(b.m, s_tf);
-  for (int it = 0; it <= b.total_active; ++it) {  // bounded: each pass takes a fresh item
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter, 1u);
-    __syncthreads();
-    const int item = s_item;
-    __syncthreads();  // every wave has read s_item; the previous tile's epilogue is done
-    if (item >= b.total_active) break;
-    int view, bx, by;
-    batch_active(b, item, view, bx, by);
-    fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
+  const int x0 = (int)(blockIdx.x % kXcds);
+  for (int q = 0; q < kXcds; ++q) {
+    const int x = (x0 + q) % kXcds;
+    const int32_t *xo = b.xoff + x * (b.V + 1);
+    const int total = xo[b.V];
+    for (int it = 0; it <= total; ++it) {  // bounded: each pass takes a fresh index
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter + x, 1u);
+      __syncthreads();
+      const int i = s_item;
+      __syncthreads();  // every wave has read s_item; the previous tile's epilogue is done
+      if (i >= total) break;
+      int lo = 0, hi = b.V;  // the last view whose part starts at or before i
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (xo[mid] <= i) lo = mid;
+        else hi = mid;
+      }
+      const int n = b.box_off[lo + 1] - b.box_off[lo];
+      const int item = b.box_off[lo] + (int)((long long)n * x / kXcds) + (i - xo[lo]);
+      int view, bx, by;
+      batch_active(b, item, view, bx, by);
+      fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
+    }
   }
   batch_fill(b, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Work-counter reset as a kernel node (graph-capturable without a memset node).
-__global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
+__global__ void reset_counter_kernel(uint32_t *c) { c[threadIdx.x] = 0u; }
 
 // Decode + pad one brick: dst element (i,j,k), i in [-1, n+2], reads the
 // clamp-to-edge source voxel.  u8/u16 decode as Volume::get (u/255.f).
@@ -1078,7 +1095,7 @@ hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) 
 hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s) {
   const size_t lds = march_tables_bytes(b.m);
   if (persistent_blocks > 0) {
-    reset_counter_kernel<<<1, 1, 0, s>>>(b.counter);
+    reset_counter_kernel<<<1, kXcds, 0, s>>>(b.counter);
     const dim3 grid(persistent_blocks);
     switch (lanes) {
       case 1: batch_kernel<1, true><<<grid, kBlock, lds, s>>>(b); break;
