@@ -259,6 +259,15 @@ class RankImages:
         )
 
 
+def debug_alloc_fault(after: int = -1) -> int:
+    """Test hook: make the library's allocation `after` places further on fail
+    as out of memory, once (after < 0: disarm).  Returns the allocations made
+    so far."""
+    made = C.c_int64()
+    check(lib().aloha_debug_alloc_fault(int(after), C.byref(made)))
+    return int(made.value)
+
+
 def render_params(dt=0.5, max_depth=1.0, spp=1, accum_id=0, island_id=0, num_islands=1,
                   jitter_mode=JITTER_RANDOM, jitter_value=0.0, flags=0, heat_map_scale=1e-5,
                   heat_clock=0):

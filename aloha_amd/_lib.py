@@ -75,6 +75,7 @@ _FP = C.POINTER(C.c_float)
 _SIGS = [
     ("aloha_last_error", C.c_char_p, [_P]),
     ("aloha_version", C.c_char_p, []),
+    ("aloha_debug_alloc_fault", C.c_int, [C.c_int64, C.POINTER(C.c_int64)]),
     ("aloha_split_volume", C.c_int, [_IP, _I, C.POINTER(BrickDesc), _I, _IP]),
     ("aloha_volume_extract", C.c_int, [_P, _IP, _I, C.POINTER(Box3i), _FP, C.POINTER(Range1f)]),
     ("aloha_partitioner_create_kd", C.c_int, [C.POINTER(Box3f), _I, _I, C.POINTER(_P)]),

@@ -750,7 +750,11 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
         // |dc|, |da|; NaN wins (as unsigned bits it is above every finite
         // bound and infinity)
         const float bd = fabsf(dv) + 2.f * (fabsf(dg[0]) + fabsf(dg[1]) + fabsf(dg[2]));
-        const float bt = fmaxf(fmaxf(fabsf(dcr), fabsf(dcg)), fmaxf(fabsf(dcb), fabsf(da)));
+        // (a NaN must win here as well: fmaxf would drop it, and the bin's
+        // fixed-point scale would then turn the NaN into an integer)
+        const float a0 = fabsf(dcr), a1 = fabsf(dcg), a2 = fabsf(dcb), a3 = fabsf(da);
+        const float m01 = (a0 > a1 || a0 != a0) ? a0 : a1, m23 = (a2 > a3 || a2 != a2) ? a2 : a3;
+        const float bt = (m01 > m23 || m01 != m01) ? m01 : m23;
         rmd = (bd > rmd || bd != bd) ? bd : rmd;
         rmt = (bt > rmt || bt != bt) ? bt : rmt;
       }

@@ -79,6 +79,11 @@ int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, i
 
 // Record a message on ctx (and as the thread's global error); returns code.
 int fail(aloha_ctx *ctx, int code, const char *fmt, ...);
+// Every device and pinned-host allocation of the library goes through these
+// (hipMalloc / hipHostMalloc), so a test can make the n-th one fail
+// (aloha_debug_alloc_fault).
+hipError_t dev_malloc(void **p, size_t bytes);
+hipError_t host_malloc(void **p, size_t bytes);
 // Validate the render parameters every render entry point shares.
 int check_params(aloha_ctx *ctx, const aloha_render_params *p);
 
@@ -88,6 +93,6 @@ int check_params(aloha_ctx *ctx, const aloha_render_params *p);
   do {                                                                                    \
     hipError_t e_ = (call);                                                               \
     if (e_ != hipSuccess)                                                                 \
-      return ::aloha::fail(ctx, ALOHA_ERR_HIP, "%s: %s (%s)", __func__, #call,            \
-                           hipGetErrorString(e_));                                        \
+      return ::aloha::fail(ctx, e_ == hipErrorOutOfMemory ? ALOHA_ERR_ALLOC : ALOHA_ERR_HIP, \
+                           "%s: %s (%s)", __func__, #call, hipGetErrorString(e_));        \
   } while (0)

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -24,6 +25,32 @@
 using namespace aloha;
 
 namespace aloha {
+
+namespace {
+std::atomic<int64_t> g_allocs{0};    // allocations made through dev_malloc / host_malloc
+std::atomic<int64_t> g_fail_at{-1};  // index of the allocation that fails (-1: none)
+
+bool alloc_fault() {
+  const int64_t k = g_allocs.fetch_add(1);
+  int64_t at = k;
+  // one-shot: the armed allocation fails and disarms the fault
+  return g_fail_at.compare_exchange_strong(at, -1);
+}
+}  // namespace
+
+int64_t g_allocs_total() { return g_allocs.load(); }
+
+hipError_t dev_malloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (alloc_fault()) return hipErrorOutOfMemory;
+  return hipMalloc(p, bytes);
+}
+
+hipError_t host_malloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (alloc_fault()) return hipErrorOutOfMemory;
+  return hipHostMalloc(p, bytes);
+}
 
 int fail(aloha_ctx *ctx, int code, const char *fmt, ...) {
   char buf[512];
@@ -47,8 +74,9 @@ int sync_bricks(aloha_ctx *ctx) {
   if (ctx->bricks.size() > ctx->d_bricks_cap) {
     if (ctx->d_bricks) HIP_CHECK(ctx, hipFree(ctx->d_bricks));
     ctx->d_bricks = nullptr;
+    ctx->d_bricks_cap = 0;  // a failed allocation leaves no table (and grows again next time)
     const size_t cap = std::max<size_t>(16, 2 * ctx->bricks.size());
-    HIP_CHECK(ctx, hipMalloc(&ctx->d_bricks, cap * sizeof(DevBrick)));
+    HIP_CHECK(ctx, dev_malloc((void **)&ctx->d_bricks, cap * sizeof(DevBrick)));
     ctx->d_bricks_cap = cap;
     ++ctx->gen;
   }
@@ -261,6 +289,12 @@ extern "C" int aloha_sync(aloha_ctx *ctx) {
   return ALOHA_OK;
 }
 
+extern "C" int aloha_debug_alloc_fault(int64_t after, int64_t *made) {
+  if (made) *made = g_allocs_total();
+  g_fail_at.store(after < 0 ? -1 : g_allocs_total() + after);
+  return ALOHA_OK;
+}
+
 extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
                                   const void *voxels, int32_t dtype, int32_t src_on_device,
                                   int32_t *brick_id) {
@@ -290,16 +324,17 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
   const size_t src_bytes = (size_t)n[0] * n[1] * n[2] * (size_t)dtype;
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
   float *mem = nullptr;
-  HIP_CHECK(ctx, hipMalloc(&mem, (size_t)padded * sizeof(float)));
+  HIP_CHECK(ctx, dev_malloc((void **)&mem, (size_t)padded * sizeof(float)));
   const void *src = voxels;
   void *staging = nullptr;
   if (!src_on_device) {
-    hipError_t e = hipMalloc(&staging, src_bytes);
+    hipError_t e = dev_malloc((void **)&staging, src_bytes);
     if (e == hipSuccess) e = hipMemcpy(staging, voxels, src_bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       hipFree(mem);
       if (staging) hipFree(staging);
-      return fail(ctx, ALOHA_ERR_HIP, "aloha_brick_upload: staging copy: %s", hipGetErrorString(e));
+      return fail(ctx, e == hipErrorOutOfMemory ? ALOHA_ERR_ALLOC : ALOHA_ERR_HIP,
+                  "aloha_brick_upload: staging copy: %s", hipGetErrorString(e));
     }
     src = staging;
   }
@@ -327,8 +362,16 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
   ctx->brick_bytes += (uint64_t)padded * sizeof(float);
   ctx->bricks_dirty = true;
   // the device brick table follows at once, so render calls find it current
-  // (and aloha_render_frame_noalloc never has to grow it)
-  if (int rc = sync_bricks(ctx)) return rc;
+  // (and aloha_render_frame_noalloc never has to grow it); if it cannot,
+  // the brick is not added
+  if (int rc = sync_bricks(ctx)) {
+    ctx->bricks.pop_back();
+    ctx->brick_mem.pop_back();
+    ctx->descs.pop_back();
+    ctx->brick_bytes -= (uint64_t)padded * sizeof(float);
+    hipFree(mem);
+    return rc;
+  }
   if (brick_id) *brick_id = (int32_t)ctx->bricks.size() - 1;
   return ALOHA_OK;
 }
@@ -358,7 +401,9 @@ extern "C" int aloha_tf_set(aloha_ctx *ctx, const float *rgba, int32_t n) {
   if (n > ctx->tf_cap) {
     if (ctx->d_tf) HIP_CHECK(ctx, hipFree(ctx->d_tf));
     ctx->d_tf = nullptr;
-    HIP_CHECK(ctx, hipMalloc(&ctx->d_tf, sizeof(float4) * (size_t)n));
+    ctx->tf_cap = ctx->tf_n = 0;  // no table until the allocation succeeds
+    ++ctx->gen;
+    HIP_CHECK(ctx, dev_malloc((void **)&ctx->d_tf, sizeof(float4) * (size_t)n));
     ctx->tf_cap = n;
     ++ctx->gen;
   }
@@ -733,7 +778,7 @@ extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params 
   ctx->bricks_dirty = true;
   if (int rc = sync_bricks(ctx)) return rc;
   uint32_t *bits = nullptr;
-  HIP_CHECK(ctx, hipMalloc(&bits, words * 4));
+  HIP_CHECK(ctx, dev_malloc((void **)&bits, words * 4));
   hipError_t e = hipMemsetAsync(bits, 0, words * 4, ctx->stream);
   a.touched = bits;
   a.bricks = ctx->d_bricks;
@@ -852,7 +897,9 @@ extern "C" int aloha_tf_set_device(aloha_ctx *ctx, const float *d_rgba, int32_t 
     HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->d_tf) HIP_CHECK(ctx, hipFree(ctx->d_tf));
     ctx->d_tf = nullptr;
-    HIP_CHECK(ctx, hipMalloc(&ctx->d_tf, sizeof(float4) * (size_t)n));
+    ctx->tf_cap = ctx->tf_n = 0;  // no table until the allocation succeeds
+    ++ctx->gen;
+    HIP_CHECK(ctx, dev_malloc((void **)&ctx->d_tf, sizeof(float4) * (size_t)n));
     ctx->tf_cap = n;
     ++ctx->gen;
   }
@@ -1002,10 +1049,10 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       if (ctx->diff_bins) HIP_CHECK(ctx, hipFree(ctx->diff_bins));
       ctx->diff_bins = nullptr;
       ctx->diff_bins_cap = 0;
-      HIP_CHECK(ctx, hipMalloc(&ctx->diff_bins, words * sizeof(uint32_t)));
+      HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_bins, words * sizeof(uint32_t)));
       ctx->diff_bins_cap = words;
     }
-    if (!ctx->h_total) HIP_CHECK(ctx, hipHostMalloc(&ctx->h_total, sizeof(uint32_t)));
+    if (!ctx->h_total) HIP_CHECK(ctx, host_malloc((void **)&ctx->h_total, sizeof(uint32_t)));
     b.count = ctx->diff_bins;  // count | maxd | maxt | off (n + 1) | cursor
     b.maxd = b.count + nbins;
     b.maxt = b.maxd + nbins;
@@ -1024,7 +1071,7 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       ctx->diff_records = nullptr;
       ctx->diff_records_cap = 0;
       const size_t cap = total + total / 4 + 1024;
-      HIP_CHECK(ctx, hipMalloc(&ctx->diff_records, cap * kRecWords * sizeof(float4)));
+      HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_records, cap * kRecWords * sizeof(float4)));
       ctx->diff_records_cap = cap;
     }
     const size_t tile_floats = max_voxels > 0 ? (size_t)nbins * kTileVox : 0;
@@ -1032,7 +1079,7 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       if (ctx->diff_tiles) HIP_CHECK(ctx, hipFree(ctx->diff_tiles));
       ctx->diff_tiles = nullptr;
       ctx->diff_tiles_cap = 0;
-      HIP_CHECK(ctx, hipMalloc(&ctx->diff_tiles, tile_floats * sizeof(float)));
+      HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_tiles, tile_floats * sizeof(float)));
       ctx->diff_tiles_cap = tile_floats;
     }
     // per-bin TF partials, then the per-group sums of the reduction
@@ -1042,7 +1089,7 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       if (ctx->diff_tf_part) HIP_CHECK(ctx, hipFree(ctx->diff_tf_part));
       ctx->diff_tf_part = nullptr;
       ctx->diff_tf_part_cap = 0;
-      HIP_CHECK(ctx, hipMalloc(&ctx->diff_tf_part, part * sizeof(double)));
+      HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_tf_part, part * sizeof(double)));
       ctx->diff_tf_part_cap = part;
     }
     b.records = ctx->diff_records;
@@ -1200,13 +1247,13 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   hipDeviceProp_t prop;
   hipError_t e = hipGetDeviceProperties(&prop, ctx->device);
   bt->cus = e == hipSuccess ? prop.multiProcessorCount : 256;
-  if (e == hipSuccess) e = hipMalloc(&bt->d_cams, sizeof(float) * 12 * (size_t)V);
-  if (e == hipSuccess) e = hipMalloc(&bt->d_orders, sizeof(int32_t) * (size_t)V * P);
-  if (e == hipSuccess) e = hipMalloc(&bt->d_rects, sizeof(int32_t) * rects.size());
-  if (e == hipSuccess) e = hipMalloc(&bt->d_boxes, sizeof(int32_t) * boxes.size());
-  if (e == hipSuccess) e = hipMalloc(&bt->d_box_off, sizeof(int32_t) * box_off.size());
-  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(uint32_t) * kXcds);
-  if (e == hipSuccess) e = hipMalloc(&bt->d_xoff, sizeof(int32_t) * xoff.size());
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_cams, sizeof(float) * 12 * (size_t)V);
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_orders, sizeof(int32_t) * (size_t)V * P);
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_rects, sizeof(int32_t) * rects.size());
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_boxes, sizeof(int32_t) * boxes.size());
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_box_off, sizeof(int32_t) * box_off.size());
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_counter, sizeof(uint32_t) * kXcds);
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_xoff, sizeof(int32_t) * xoff.size());
   if (e == hipSuccess) e = hipMemcpy(bt->d_xoff, xoff.data(), sizeof(int32_t) * xoff.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_cams, cams, sizeof(float) * 12 * (size_t)V, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_orders, orders, sizeof(int32_t) * (size_t)V * P, hipMemcpyHostToDevice);
@@ -1218,7 +1265,8 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming);
   if (e != hipSuccess) {
     aloha_batch_destroy(bt);
-    return fail(ctx, ALOHA_ERR_HIP, "aloha_batch_create: %s", hipGetErrorString(e));
+    return fail(ctx, e == hipErrorOutOfMemory ? ALOHA_ERR_ALLOC : ALOHA_ERR_HIP,
+                "aloha_batch_create: %s", hipGetErrorString(e));
   }
   bt->b.cams = bt->d_cams;
   bt->b.orders = bt->d_orders;

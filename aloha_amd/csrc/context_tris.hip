@@ -90,14 +90,20 @@ extern "C" int aloha_tris_upload(aloha_ctx *ctx, const aloha_mesh *mesh, int32_t
       g.nnodes = (int64_t)b.nodes.size();
       const size_t nb = std::max<size_t>(1, b.nodes.size()) * sizeof(TriNode);
       const size_t tb = rb[r].dev_tris.size() * sizeof(float);
-      hipError_t e = hipMalloc(&g.d_nodes, nb);
-      if (e == hipSuccess) e = hipMalloc(&g.d_tris, tb);
+      hipError_t e = dev_malloc((void **)&g.d_nodes, nb);
+      if (e == hipSuccess) e = dev_malloc((void **)&g.d_tris, tb);
       if (e == hipSuccess && !b.nodes.empty())
         e = hipMemcpy(g.d_nodes, b.nodes.data(), b.nodes.size() * sizeof(TriNode), hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMemcpy(g.d_tris, rb[r].dev_tris.data(), tb, hipMemcpyHostToDevice);
       if (e != hipSuccess) {
         if (g.d_nodes) hipFree(g.d_nodes);
         if (g.d_tris) hipFree(g.d_tris);
+        // the ranks uploaded by this call go too: a failed call adds nothing
+        for (size_t k = first_id; k < ctx->geoms.size(); ++k) {
+          if (ctx->geoms[k].d_nodes) hipFree(ctx->geoms[k].d_nodes);
+          if (ctx->geoms[k].d_tris) hipFree(ctx->geoms[k].d_tris);
+        }
+        ctx->geoms.resize(first_id);
         return fail(ctx, e == hipErrorOutOfMemory ? ALOHA_ERR_ALLOC : ALOHA_ERR_HIP,
                     "aloha_tris_upload: %s", hipGetErrorString(e));
       }

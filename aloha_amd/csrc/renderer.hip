@@ -12,6 +12,12 @@
 
 #include "aloha_renderer.h"
 
+namespace aloha {
+// the library's allocation entry (context.hip): hipMalloc behind the
+// allocation-fault test hook
+hipError_t dev_malloc(void **p, size_t bytes);
+}  // namespace aloha
+
 namespace maui {
 
 int Renderer::spp = 1;
@@ -257,7 +263,7 @@ void Renderer::handleResizeEvent(const vec2i &newSize) {
   const size_t npix = (size_t)newSize.x * newSize.y;
   const size_t P = (size_t)commWorld.size;
   auto alloc = [&](void **p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = aloha::dev_malloc(p, bytes);
     if (e != hipSuccess) throw std::runtime_error(std::string("aloha: hipMalloc: ") + hipGetErrorString(e));
   };
   alloc((void **)&d_rank_rgba_, P * npix * 16);

@@ -106,6 +106,13 @@ typedef struct aloha_partitioner aloha_partitioner;
 const char *aloha_last_error(const aloha_ctx *ctx);
 /* library version string */
 const char *aloha_version(void);
+/* Test hook (no counterpart in the reference; for tests of the
+ * allocation-failure paths): *made (may be NULL) = the device and pinned-host
+ * allocations the library has made in this process so far; after >= 0 makes
+ * the allocation `after` places further on fail as out of memory (the call
+ * making it returns ALOHA_ERR_ALLOC and leaves its context as it was), once;
+ * after < 0 disarms. */
+int aloha_debug_alloc_fault(int64_t after, int64_t *made);
 
 /* ---------------------------------------------------------------------- */
 /* host-side bricking and partitioning (no GPU needed)                     */
