@@ -19,6 +19,7 @@ struct TriGeom {
   int32_t root = 0, depth = 0;
   int64_t ntris = 0, nnodes = 0;
   uint64_t bytes = 0;
+  aloha_box3f box{};  // the triangles' bounds widened like the node boxes
 };
 
 struct aloha_ctx {

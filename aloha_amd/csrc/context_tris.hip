@@ -87,6 +87,10 @@ extern "C" int aloha_tris_upload(aloha_ctx *ctx, const aloha_mesh *mesh, int32_t
         return fail(ctx, ALOHA_ERR_LIMIT, "aloha_tris_upload: BVH depth %d exceeds %d", b.depth, kTriStack);
       g.root = b.root;
       g.depth = b.depth;
+      for (int k = 0; k < 3; ++k) {
+        g.box.lower[k] = b.lo[k] - b.pad;
+        g.box.upper[k] = b.hi[k] + b.pad;
+      }
       g.nnodes = (int64_t)b.nodes.size();
       const size_t nb = std::max<size_t>(1, b.nodes.size()) * sizeof(TriNode);
       const size_t tb = rb[r].dev_tris.size() * sizeof(float);
@@ -164,6 +168,14 @@ int build_tris(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t 
     a.geo[r].nodes = t.d_nodes;
     a.geo[r].tris = t.d_tris;
     a.geo[r].root = t.root;
+    // the pixels whose ray can reach the geometry's (widened) bounds: every
+    // box of its BVH lies inside them, so the walk of any other pixel finds
+    // nothing
+    if (t.root == kEmptyRoot) {
+      for (int k = 0; k < 4; ++k) a.rect[r][k] = 0;
+    } else {
+      project_boxes(ctx->cam, &t.box, 1, W, H, a.rect[r]);
+    }
   }
   std::memcpy(a.org, ctx->cam, sizeof a.org);
   std::memcpy(a.dir00, ctx->cam + 3, sizeof a.dir00);

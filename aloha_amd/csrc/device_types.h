@@ -181,6 +181,7 @@ struct TrisArgs {
   float4 *accum;
   uint32_t *stats;  // 2 per pixel: nodes visited, triangles tested
   DevTris geo[ALOHA_MAX_RANKS];
+  int32_t rect[ALOHA_MAX_RANKS][4];  // per rank: the pixels its geometry can cover
 };
 
 // launchers (march.hip, composite.hip)

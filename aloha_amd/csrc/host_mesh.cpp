@@ -382,6 +382,7 @@ int need_levels(int64_t n) {
 
 bool build_tri_bvh(const float *tri9, int64_t ntris, float pad, TriBvh &out, std::string &err) {
   out = TriBvh{};
+  out.pad = pad;
   if (ntris <= 0) {
     err = "empty geometry";
     return false;

@@ -45,7 +45,8 @@ struct TriBvh {
   std::vector<uint32_t> order;  // BVH triangle slot -> input triangle
   int32_t root = 0;             // node 0, or a leaf ref when there is one leaf
   int32_t depth = 0;            // deepest leaf (root = 0)
-  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // the triangles' bounds
+  float pad = 0.f;                              // the node boxes' widening
 };
 
 // Binned-SAH BVH over ntris triangles given as 9 floats each (three

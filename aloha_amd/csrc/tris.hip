@@ -7,8 +7,11 @@
 // traversal stack in LDS (4-byte entries, lane-interleaved so that the 64
 // lanes of a push or pop hit 64 different banks; the per-rank launch keeps
 // the first 16 there and deeper ones in private memory); the BVH depth bound
-// of the builder makes 32 entries always enough.  A node fetch is 64 bytes
-// (both children's boxes) and a triangle 48 bytes.
+// of the builder makes 32 entries (and a sentinel) always enough.  A node
+// fetch is 64 bytes
+// (both children's boxes) and a triangle 48 bytes.  Pixels outside the
+// rectangle that a rank's geometry can project to (aloha_project_boxes of its
+// root's child boxes) take the miss result without a walk.
 //
 // Ray/triangle test: the watertight test of Woop, Benthin and Wald, "Watertight
 // Ray/Triangle Intersection", JCGT 2(1), 2013 (shear to ray space, edge
@@ -123,23 +126,45 @@ __device__ __forceinline__ bool slab(const Ray &r, float x0, float x1, float y0,
   return tn <= tf * kSlabSlack;
 }
 
-// Closest hit of one geometry below best_t (in: the limit, out: the hit t).
-// Returns the hit's triangle slot, or -1.  stk: this lane's LDS stack
-// (stride kBlock) of kLds entries; entries kLds .. kTriStack-1 of deeper
-// walks go to a private (scratch) array.  nn / nt count node fetches and
-// triangle tests.
+// Closest hit of one geometry below best_t (in: the limit, out: the hit t);
+// returns the hit's triangle slot, or -1.  stk: this lane's LDS stack
+// (stride kBlock) of kLds entries; deeper entries of a walk go to a private
+// (scratch) array.  nn / nt count node fetches and triangle tests.
+//
+// The walk is the "while-while" loop with postponed leaves of Aila & Laine, "Understanding the Efficiency of Ray Traversal on GPUs"
+// (HPG 2009): a lane that reaches a leaf keeps it and goes on through inner
+// nodes until every lane of the wave still walking holds a leaf, then the
+// lanes test their leaves together -- so a wave runs the costly leaf block
+// once for many lanes instead of once per lane that happens to reach a leaf
+// in a given trip.  The leaves are the same; the hit is the closest one with
+// ties to the smaller primitive id, which no visiting order changes.
+// (Culling by the best t found so far comes later, so more nodes may be
+// fetched.)  The stack's bottom holds a sentinel above every node index.
+constexpr int32_t kSentinel = INT32_MAX;
+
 template <bool kStats, int kLds>
 __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best_t,
-                                     int32_t *stk, uint32_t &nn, uint32_t &nt) {
-  static_assert(kLds >= 1 && kLds <= kTriStack, "LDS part of the traversal stack");
+                                        int32_t *stk, uint32_t &nn, uint32_t &nt) {
+  static_assert(kLds >= 1 && kLds <= kTriStack + 1, "LDS part of the traversal stack");
   int best_slot = -1, best_prim = -1;
-  int32_t ref = g.root;
-  if (ref == kEmptyRoot) return -1;
+  if (g.root == kEmptyRoot) return -1;
   int sp = 0;
-  int32_t deep[kTriStack - kLds > 0 ? kTriStack - kLds : 1];
+  int32_t deep[kTriStack + 1 - kLds > 0 ? kTriStack + 1 - kLds : 1];
+  auto push = [&](int32_t v) {
+    if (kLds == kTriStack + 1 || sp < kLds) stk[sp * kBlock] = v;
+    else deep[sp - kLds] = v;
+    ++sp;
+  };
+  auto pop = [&]() {
+    --sp;
+    return (kLds == kTriStack + 1 || sp < kLds) ? stk[sp * kBlock] : deep[sp - kLds];
+  };
+  push(kSentinel);
+  int32_t node = g.root, leaf = 0;  // leaf < 0: a postponed leaf
   while (true) {
-    if (ref >= 0) {
-      const float4 *n = g.nodes + 4 * (int64_t)ref;
+    bool searching = true;
+    while ((uint32_t)node < (uint32_t)kSentinel) {  // an inner node
+      const float4 *n = g.nodes + 4 * (int64_t)node;
       const float4 lxy = n[0], rxy = n[1], z = n[2];
       const int4 kids = *reinterpret_cast<const int4 *>(n + 3);
       if (kStats) ++nn;
@@ -148,24 +173,27 @@ __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best
       const bool hr = slab(r, rxy.x, rxy.y, rxy.z, rxy.w, z.z, z.w, best_t, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
-        const int32_t far = left_first ? kids.y : kids.x;
-        if (kLds == kTriStack || sp < kLds) stk[sp * kBlock] = far;
-        else deep[sp - kLds] = far;
-        ++sp;
-        ref = left_first ? kids.x : kids.y;
-        continue;
+        push(left_first ? kids.y : kids.x);
+        node = left_first ? kids.x : kids.y;
+      } else if (hl || hr) {
+        node = hl ? kids.x : kids.y;
+      } else {
+        node = pop();
       }
-      if (hl) {
-        ref = kids.x;
-        continue;
+      if (node < 0 && leaf >= 0) {  // the first leaf: keep it, walk on
+        searching = false;
+        leaf = node;
+        node = pop();
       }
-      if (hr) {
-        ref = kids.y;
-        continue;
-      }
-    } else {
-      const uint32_t leaf = ~(uint32_t)ref;
-      const int first = (int)(leaf >> kLeafBits), cnt = (int)(leaf & ((1u << kLeafBits) - 1)) + 1;
+      if (!__any(searching)) break;  // every lane still walking holds a leaf
+    }
+    if (node < 0 && leaf >= 0) {  // a leaf root
+      leaf = node;
+      node = pop();
+    }
+    while (leaf < 0) {
+      const uint32_t lf = ~(uint32_t)leaf;
+      const int first = (int)(lf >> kLeafBits), cnt = (int)(lf & ((1u << kLeafBits) - 1)) + 1;
       for (int i = 0; i < cnt; ++i) {
         const float4 *tp = g.tris + 3 * (int64_t)(first + i);
         const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
@@ -178,10 +206,10 @@ __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best
           best_slot = first + i;
         }
       }
+      leaf = node;  // another leaf reached as the walk broke off: it goes next
+      if (node < 0) node = pop();
     }
-    if (sp == 0) break;
-    --sp;
-    ref = (kLds == kTriStack || sp < kLds) ? stk[sp * kBlock] : deep[sp - kLds];
+    if (node == kSentinel) break;
   }
   return best_slot;
 }
@@ -245,7 +273,8 @@ __global__ __launch_bounds__(kBlock) void tris_rank_kernel(const TrisArgs a) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   float z = 1.f;
   uint32_t nn = 0, nt = 0;
-  if (island_of(a, px, py)) {
+  const int32_t *rc = a.rect[slot];
+  if (island_of(a, px, py) && px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3]) {
     Ray r;
     make_ray(a, px, py, r);
     float t = kTMax;
@@ -325,7 +354,9 @@ __global__ __launch_bounds__(kBlock) void tris_composite_kernel(const TrisArgs a
     float t_win = kTMax, t_lim = kTMax;
     for (int k = 0; k < a.P; ++k) {
       float t = t_lim;
-      const int s = trace<kStats, kTriStack>(a.geo[k], r, t, s_stack + threadIdx.x, nn, nt);
+      const int32_t *rc = a.rect[k];  // outside it, rank k misses the pixel
+      const bool in = px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
+      const int s = in ? trace<kStats, kTriStack>(a.geo[k], r, t, s_stack + threadIdx.x, nn, nt) : -1;
       const float zk = s >= 0 ? t / a.max_depth : 1.f;
       if (k == 0 || zk < z_best) {
         winner = k;

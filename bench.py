@@ -180,6 +180,18 @@ def main():
                     "histogram": [int((level_diff.amax(dim=2) == k).sum()) for k in range(4)]}
     log(f"[bench] RGBA8 composite form (render_ranks + 8-bit ordered blend): {ms_rgba8:.4f} ms/frame, "
         f"levels vs float path {rgba8_levels}")
+    # the frame with compact rank images in the library's workspace (only each
+    # rank's rectangle is stored; aloha_render_frame_noalloc), same result
+    nout = torch.empty_like(out)
+    nout8 = torch.empty_like(out8)
+    noalloc = lambda: ctx.render_frame_noalloc(params, W, H, ranks, wl.order, nout, nout8)
+    for _ in range(args.warmup):
+        noalloc()
+    ms_noalloc = timed(noalloc, args.steps)
+    torch.cuda.synchronize()
+    noalloc_equal = bool(torch.equal(nout, out) and torch.equal(nout8, out8))
+    log(f"[bench] frame with compact rank images (noalloc entry) {ms_noalloc:.4f} ms "
+        f"(equal {noalloc_equal})")
     # fused form: the same frame (bit-identical result) in one launch
     fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
     for _ in range(args.warmup):
@@ -286,6 +298,12 @@ def main():
                 "Gsamples/s": round(samples / (ms_rgba8 * 1e-3) / 1e9, 4),
                 "note": "IceT's ubyte path: the ranks' RGBA8 images blended in 8 bits",
                 "levels_vs_float_path": rgba8_levels},
+            "render_frame_compact_rank_images": {
+                "ms_per_frame": round(ms_noalloc, 5),
+                "Gsamples/s": round(samples / (ms_noalloc * 1e-3) / 1e9, 4),
+                "bit_identical": noalloc_equal,
+                "note": "rank images stored only inside their rectangles (IceT's sparse "
+                        "images), in the library's workspace: aloha_render_frame_noalloc"},
             "fused_render_composite": {"ms_per_frame": round(ms_fused, 5),
                                        "Gsamples/s": round(samples / (ms_fused * 1e-3) / 1e9, 4),
                                        "bit_identical_to_plain": fused_equal},
