@@ -391,7 +391,12 @@ bool build_tri_bvh(const float *tri9, int64_t ntris, float pad, TriBvh &out, std
     err = "too many triangles in one geometry for the leaf encoding";
     return false;
   }
-  constexpr int kBins = 16;
+  // 8 centroid bins per axis: on the 4096-sphere scene (1920x1080 primary
+  // rays, 8 ranks) the walk takes 1.66 ms with 8 bins against 2.25 with 16
+  // and 3.4 with 12 or 32 -- finer bins let the SAH peel thin slabs off
+  // clusters of overlapping spheres, which its uniform-ray cost model rates
+  // well and the camera's rays do not (same images in every case)
+  constexpr int kBins = 8;
   constexpr double kTrav = 1.0;  // cost of one node step relative to one triangle test
   std::vector<uint32_t> &order = out.order;
   order.resize(ntris);

@@ -44,6 +44,7 @@ constexpr float kTMax = 1e10f;
 // to private memory.  The fused launch keeps all 32 in LDS (measured faster
 // there).
 constexpr int kRankStackLds = 16;
+constexpr int kFusedStackLds = kTriStack;
 // slack on the far end of each slab interval: covers the rounding of
 // (plane - org) * (1/dir) (a few ulps) so that culling stays conservative
 constexpr float kSlabSlack = 1.000001f;
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void tris_rank_kernel(const TrisArgs a) {
 // is then shaded exactly as its own image would hold it (accum_id 0).
 template <bool kStats>
 __global__ __launch_bounds__(kBlock) void tris_composite_kernel(const TrisArgs a) {
-  __shared__ int32_t s_stack[kTriStack * kBlock];
+  __shared__ int32_t s_stack[kFusedStackLds * kBlock];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = blockIdx.x * kTileW + (wave & 1) * 8 + (lane & 7);
   const int py = blockIdx.y * kTileH + (wave >> 1) * 8 + (lane >> 3);
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void tris_composite_kernel(const TrisArgs a
       float t = t_lim;
       const int32_t *rc = a.rect[k];  // outside it, rank k misses the pixel
       const bool in = px >= rc[0] && px < rc[2] && py >= rc[1] && py < rc[3];
-      const int s = in ? trace<kStats, kTriStack>(a.geo[k], r, t, s_stack + threadIdx.x, nn, nt) : -1;
+      const int s = in ? trace<kStats, kFusedStackLds>(a.geo[k], r, t, s_stack + threadIdx.x, nn, nt) : -1;
       const float zk = s >= 0 ? t / a.max_depth : 1.f;
       if (k == 0 || zk < z_best) {
         winner = k;
