@@ -393,19 +393,21 @@ class Context:
                                              C.byref(imgs)))
 
     def render_frame(self, params: RenderParams, W: int, H: int, ranks: Sequence[Sequence[int]],
-                     images: RankImages, order: Sequence[int], out_rgba=None,
+                     images: RankImages | None, order: Sequence[int], out_rgba=None,
                      out_rgba8=None) -> None:
         """One frame: render_ranks into `images` (float RGBA required) and the
         ordered 'over' of their RGBA into the outputs, as render_ranks followed
         by composite_over with rank_rects' rectangles (bit for bit), in two
         launches: the per-rank launch also clears the outputs outside the
-        rectangles' bounding box, the composite covers the box."""
+        rectangles' bounding box, the composite covers the box.  images None:
+        the frame only, the rank images kept compact (only inside their
+        rectangles) in the context's workspace."""
         offs, ids = _rank_table(ranks)
-        imgs = images.as_c(len(ranks), W, H)
         o = _order(order, len(ranks))
         n = max(0, W) * max(0, H)
+        imgs = C.byref(images.as_c(len(ranks), W, H)) if images is not None else None
         self._check(lib().aloha_render_frame(self._h, C.byref(params), W, H, len(ranks), offs, ids,
-                                             C.byref(imgs), o, _dev(out_rgba, 16 * n, "out_rgba"),
+                                             imgs, o, _dev(out_rgba, 16 * n, "out_rgba"),
                                              _dev(out_rgba8, 4 * n, "out_rgba8")))
 
     def render_frame_noalloc(self, params: RenderParams, W: int, H: int,

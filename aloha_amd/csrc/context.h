@@ -65,6 +65,8 @@ struct aloha_ctx {
   double *diff_tf_part = nullptr;
   size_t diff_tf_part_cap = 0;  // doubles
   uint32_t *h_total = nullptr;  // pinned host word
+  float4 *frame_ws = nullptr;   // compact rank images of aloha_render_frame without images
+  size_t frame_ws_cap = 0;      // float4
   std::string err;
 };
 
@@ -78,6 +80,12 @@ int scratch_on_device(int device, void **out);
 // `a`; sets waves to its grid for `lanes` lanes per ray.
 int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, int &waves);
 
+// The frame with compact rank images (aloha_render_frame without images,
+// aloha_render_frame_noalloc), named `name` in its error messages.
+int frame_compact(aloha_ctx *ctx, const char *name, bool noalloc,
+                  const aloha_render_params *params, int32_t W, int32_t H, int32_t P,
+                  const int32_t *rank_offsets, const int32_t *brick_ids, const int32_t *order,
+                  float *d_out_rgba, uint32_t *d_out_rgba8);
 // Record a message on ctx (and as the thread's global error); returns code.
 int fail(aloha_ctx *ctx, int code, const char *fmt, ...);
 // Every device and pinned-host allocation of the library goes through these

@@ -259,6 +259,7 @@ extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
   if (ctx->diff_records) hipFree(ctx->diff_records);
   if (ctx->diff_tiles) hipFree(ctx->diff_tiles);
   if (ctx->diff_tf_part) hipFree(ctx->diff_tf_part);
+  if (ctx->frame_ws) hipFree(ctx->frame_ws);
   if (ctx->h_total) hipHostFree(ctx->h_total);
   for (TriGeom &g : ctx->geoms) {
     if (g.d_nodes) hipFree(g.d_nodes);
@@ -500,7 +501,10 @@ extern "C" int aloha_render_frame(aloha_ctx *ctx, const aloha_render_params *par
                                   const int32_t *order, float *d_out_rgba,
                                   uint32_t *d_out_rgba8) {
   if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_frame: ctx is NULL");
-  if (!images || !images->rgba)
+  if (!images)  // no rank images wanted: compact ones in the context's workspace
+    return frame_compact(ctx, "aloha_render_frame", false, params, W, H, P, rank_offsets,
+                         brick_ids, order, d_out_rgba, d_out_rgba8);
+  if (!images->rgba)
     return fail(ctx, ALOHA_ERR_ARG, "aloha_render_frame: the rank images need float RGBA");
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
   MarchArgs a;
@@ -573,22 +577,27 @@ int scratch_on_device(int device, void **out) {
 }
 }  // namespace aloha
 
-extern "C" int aloha_render_frame_noalloc(aloha_ctx *ctx, const aloha_render_params *params,
-                                          int32_t W, int32_t H, int32_t P,
-                                          const int32_t *rank_offsets, const int32_t *brick_ids,
-                                          const int32_t *order, float *d_out_rgba,
-                                          uint32_t *d_out_rgba8) {
-  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_frame_noalloc: ctx is NULL");
-  if (!ctx->scratch)
-    return fail(ctx, ALOHA_ERR_STATE, "aloha_render_frame_noalloc: no workspace on device %d (%s)",
-                ctx->device, ctx->scratch_err.c_str());
-  if (!params) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_frame_noalloc: params is NULL");
+namespace aloha {
+// The frame with compact rank images: the per-rank launch stores each rank's
+// pixels only inside its rectangle, the rectangles one after the other
+// (row-major each: IceT's sparse images, kept on the device), then the
+// composite of the rectangles' bounding box.  noalloc: the workspace is the
+// static per-device array (ALOHA_NOALLOC_SCRATCH_BYTES, nothing allocated,
+// the brick table must be current); else the context's own workspace, grown
+// when a frame needs more.
+int frame_compact(aloha_ctx *ctx, const char *name, bool noalloc,
+                  const aloha_render_params *params, int32_t W, int32_t H, int32_t P,
+                  const int32_t *rank_offsets, const int32_t *brick_ids, const int32_t *order,
+                  float *d_out_rgba, uint32_t *d_out_rgba8) {
+  if (noalloc && !ctx->scratch)
+    return fail(ctx, ALOHA_ERR_STATE, "%s: no workspace on device %d (%s)", name, ctx->device,
+                ctx->scratch_err.c_str());
+  if (!params) return fail(ctx, ALOHA_ERR_ARG, "%s: params is NULL", name);
   if (params->accum_id != 0 || (params->flags & ALOHA_FLAG_HEATMAP))
-    return fail(ctx, ALOHA_ERR_ARG,
-                "aloha_render_frame_noalloc: accum_id 0 and no heat map only (no accum buffer)");
+    return fail(ctx, ALOHA_ERR_ARG, "%s: accum_id 0 and no heat map only (no accum buffer)", name);
   // build_march would grow a stale device brick table; uploads keep it current
-  if (ctx->bricks_dirty)
-    return fail(ctx, ALOHA_ERR_STATE, "aloha_render_frame_noalloc: brick table not on the device");
+  if (noalloc && ctx->bricks_dirty)
+    return fail(ctx, ALOHA_ERR_STATE, "%s: brick table not on the device", name);
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
   MarchArgs a;
   if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
@@ -608,13 +617,29 @@ extern "C" int aloha_render_frame_noalloc(aloha_ctx *ctx, const aloha_render_par
     x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
   }
   if (x0 >= x1 || y0 >= y1) x0 = y0 = x1 = y1 = 0;  // no rectangle: no composite launch
-  if (total * (int64_t)sizeof(float4) > (int64_t)ALOHA_NOALLOC_SCRATCH_BYTES)
-    return fail(ctx, ALOHA_ERR_LIMIT,
-                "aloha_render_frame_noalloc: the ranks' rectangles hold %lld pixels (%lld bytes); "
-                "the workspace holds %lld bytes",
-                (long long)total, (long long)(total * (int64_t)sizeof(float4)),
-                (long long)ALOHA_NOALLOC_SCRATCH_BYTES);
-  a.rgba = (float4 *)ctx->scratch;
+  if (total > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "%s: rectangles too large", name);
+  float4 *ws = nullptr;
+  if (noalloc) {
+    if (total * (int64_t)sizeof(float4) > (int64_t)ALOHA_NOALLOC_SCRATCH_BYTES)
+      return fail(ctx, ALOHA_ERR_LIMIT,
+                  "%s: the ranks' rectangles hold %lld pixels (%lld bytes); the workspace holds "
+                  "%lld bytes",
+                  name, (long long)total, (long long)(total * (int64_t)sizeof(float4)),
+                  (long long)ALOHA_NOALLOC_SCRATCH_BYTES);
+    ws = (float4 *)ctx->scratch;
+  } else {
+    if ((size_t)total > ctx->frame_ws_cap) {
+      HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // earlier frames read the old one
+      if (ctx->frame_ws) HIP_CHECK(ctx, hipFree(ctx->frame_ws));
+      ctx->frame_ws = nullptr;
+      ctx->frame_ws_cap = 0;
+      const size_t cap = (size_t)total + (size_t)total / 4 + 1024;
+      HIP_CHECK(ctx, dev_malloc((void **)&ctx->frame_ws, cap * sizeof(float4)));
+      ctx->frame_ws_cap = cap;
+    }
+    ws = ctx->frame_ws;
+  }
+  a.rgba = ws;
   a.rgba8 = nullptr;
   a.depth = nullptr;
   a.accum = nullptr;
@@ -628,9 +653,20 @@ extern "C" int aloha_render_frame_noalloc(aloha_ctx *ctx, const aloha_render_par
   q.frame8 = d_out_rgba8;
   q.box = make_int4(x0, y0, x1, y1);
   HIP_CHECK(ctx, launch_march_ranks(a, q, ctx->march_lanes, waves, ctx->stream));
-  HIP_CHECK(ctx, launch_composite_over_compact((const float4 *)ctx->scratch, c, W, x0, y0, x1, y1,
-                                               (float4 *)d_out_rgba, d_out_rgba8, ctx->stream));
+  HIP_CHECK(ctx, launch_composite_over_compact(ws, c, W, x0, y0, x1, y1, (float4 *)d_out_rgba,
+                                               d_out_rgba8, ctx->stream));
   return ALOHA_OK;
+}
+}  // namespace aloha
+
+extern "C" int aloha_render_frame_noalloc(aloha_ctx *ctx, const aloha_render_params *params,
+                                          int32_t W, int32_t H, int32_t P,
+                                          const int32_t *rank_offsets, const int32_t *brick_ids,
+                                          const int32_t *order, float *d_out_rgba,
+                                          uint32_t *d_out_rgba8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_frame_noalloc: ctx is NULL");
+  return frame_compact(ctx, "aloha_render_frame_noalloc", true, params, W, H, P, rank_offsets,
+                       brick_ids, order, d_out_rgba, d_out_rgba8);
 }
 
 extern "C" int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params *params,

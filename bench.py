@@ -1,16 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the per-brick render-and-composite path on one MI355X.
 
-One step = one frame (aloha_render_frame): every virtual rank ray-marches
-its bricks into its float RGBA + RGBA8 + depth image, and the ranks' images
-are 'over'-composited in kd visibility order into the frame, reading each
-rank image only inside the pixel rectangle its bricks project to (outside
-it the image is transparent black).  Two launches: the per-rank launch
-(which also clears the frame outside the rectangles' bounding box) and the
-composite of that box.  The same frame as two calls (render_ranks, then the
-sparse composite of the whole image), the dense composite and the fused
-one-launch form are timed beside it, each checked equal.  Inputs (bricks, TF, camera) are resident in
-HBM before timing starts.
+One step = one frame (aloha_render_frame, the frame only): every virtual
+rank ray-marches its bricks into its rank image -- stored compact, only
+inside the pixel rectangle its bricks project to (IceT's sparse images; the
+rank image is transparent black outside it) -- and the ranks' images are
+'over'-composited in kd visibility order into the float RGBA + RGBA8 frame.
+Two launches: the per-rank launch (which also clears the frame outside the
+rectangles' bounding box) and the composite of that box.  Timed beside it,
+each checked bit-identical to it: the same frame with full-size rank images
+(float RGBA + RGBA8 + depth per rank, as the reference's ranks write their
+framebuffers), as two calls (render_ranks, then the sparse composite), with
+the dense composite, from the allocation-free entry point, and the fused
+one-launch form.  Inputs (bricks, TF, camera) are resident in HBM before
+timing starts.
 
 Default workload: 512^3 float32 fBm volume, chopSuey-split into 8 bricks
 (2x2x2, ghost voxels), one brick per virtual rank, 1024x1024, dt = 0.5.
@@ -118,8 +121,14 @@ def main():
     # images); the dense composite is timed beside it and gives the same frame
     rects = ctx.rank_rects(params, W, H, ranks)
 
-    def frame():
-        ctx.render_frame(params, W, H, ranks, imgs, wl.order, out, out8)
+    def frame():  # the frame only: compact rank images in the context's workspace
+        ctx.render_frame(params, W, H, ranks, None, wl.order, out, out8)
+
+    dout = torch.empty_like(out)
+    dout8 = torch.empty_like(out8)
+
+    def frame_with_images():  # the same frame, every rank's full image written too
+        ctx.render_frame(params, W, H, ranks, imgs, wl.order, dout, dout8)
 
     def two_calls():
         ctx.render_ranks(params, W, H, ranks, imgs)
@@ -143,6 +152,13 @@ def main():
     torch.cuda.synchronize()
     ms_frame = timed(frame, args.steps)
     log(f"[bench] {args.steps} frames: {ms_frame:.4f} ms/frame")
+    for _ in range(args.warmup):
+        frame_with_images()
+    ms_frame_img = timed(frame_with_images, args.steps)
+    torch.cuda.synchronize()
+    img_equal = bool(torch.equal(dout, out) and torch.equal(dout8, out8))
+    log(f"[bench] the frame with full-size rank images: {ms_frame_img:.4f} ms/frame "
+        f"(equal {img_equal})")
     for _ in range(args.warmup):
         two_calls()
     ms_two = timed(two_calls, args.steps)
@@ -180,8 +196,8 @@ def main():
                     "histogram": [int((level_diff.amax(dim=2) == k).sum()) for k in range(4)]}
     log(f"[bench] RGBA8 composite form (render_ranks + 8-bit ordered blend): {ms_rgba8:.4f} ms/frame, "
         f"levels vs float path {rgba8_levels}")
-    # the frame with compact rank images in the library's workspace (only each
-    # rank's rectangle is stored; aloha_render_frame_noalloc), same result
+    # the allocation-free entry point (the same compact rank images in a static
+    # per-GPU workspace), same result
     nout = torch.empty_like(out)
     nout8 = torch.empty_like(out8)
     noalloc = lambda: ctx.render_frame_noalloc(params, W, H, ranks, wl.order, nout, nout8)
@@ -190,7 +206,7 @@ def main():
     ms_noalloc = timed(noalloc, args.steps)
     torch.cuda.synchronize()
     noalloc_equal = bool(torch.equal(nout, out) and torch.equal(nout8, out8))
-    log(f"[bench] frame with compact rank images (noalloc entry) {ms_noalloc:.4f} ms "
+    log(f"[bench] allocation-free entry point {ms_noalloc:.4f} ms "
         f"(equal {noalloc_equal})")
     # fused form: the same frame (bit-identical result) in one launch
     fused = lambda: ctx.render_composite(params, W, H, ranks, wl.order, fout, fout8)
@@ -238,6 +254,16 @@ def main():
         "algorithmic_bytes": int(render_bytes if dominant == "render" else comp_bytes),
         "launch_ms": round(ms_render if dominant == "render" else ms_comp, 5),
     }
+    roof["note"] = ("the per-rank launch of render_ranks (full-size rank images), timed alone; "
+                    "the headline frame's two launches are frame_roofline")
+    # the headline frame: voxels read, compact rank pixels written and read
+    # back by the composite, the frame's float RGBA + RGBA8 written
+    frame_bytes = touched * 4 + wl.tf.nbytes + rect_pix * (16 + 16) + npix * (16 + 4)
+    frame_gbs = frame_bytes / (ms_frame * 1e-3) / 1e9
+    frame_roof = {"kernels": f"march_persist_kernel<{schedule[0]}> (compact) + over_compact",
+                  "bound": "hbm", "achieved": round(frame_gbs, 1), "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": round(frame_gbs / HBM_PEAK_GBS, 4),
+                  "algorithmic_bytes": int(frame_bytes), "ms": round(ms_frame, 5)}
     kernels = {
         f"march_persist_kernel<{schedule[0]}>": {"ms": round(ms_render, 5), "bytes": int(render_bytes),
                          "GB/s": round(render_gbs, 1), "frac_hbm": round(render_gbs / HBM_PEAK_GBS, 4),
@@ -286,9 +312,17 @@ def main():
                    "samples_per_frame": samples, "touched_voxels": touched,
                    "brick_bytes_padded": ctx.brick_bytes(), "upload_s_incl_pcie": round(upload_s, 3)},
         "roofline": roof,
+        "frame_roofline": frame_roof,
         "kernels": kernels,
         "forms": {
-            "render_frame": {"ms_per_frame": round(ms_frame, 5), "Gsamples/s": round(gsamples, 4)},
+            "render_frame": {"ms_per_frame": round(ms_frame, 5), "Gsamples/s": round(gsamples, 4),
+                             "note": "the headline step: compact rank images (only inside their "
+                                     "rectangles) in the context's workspace"},
+            "render_frame_with_full_rank_images": {
+                "ms_per_frame": round(ms_frame_img, 5),
+                "Gsamples/s": round(samples / (ms_frame_img * 1e-3) / 1e9, 4),
+                "bit_identical": img_equal,
+                "note": "every rank's 1024x1024 float RGBA + RGBA8 + depth image written too"},
             "render_ranks_then_sparse_composite": {
                 "ms_per_frame": round(ms_two, 5),
                 "Gsamples/s": round(samples / (ms_two * 1e-3) / 1e9, 4),
@@ -298,12 +332,12 @@ def main():
                 "Gsamples/s": round(samples / (ms_rgba8 * 1e-3) / 1e9, 4),
                 "note": "IceT's ubyte path: the ranks' RGBA8 images blended in 8 bits",
                 "levels_vs_float_path": rgba8_levels},
-            "render_frame_compact_rank_images": {
+            "render_frame_noalloc": {
                 "ms_per_frame": round(ms_noalloc, 5),
                 "Gsamples/s": round(samples / (ms_noalloc * 1e-3) / 1e9, 4),
                 "bit_identical": noalloc_equal,
-                "note": "rank images stored only inside their rectangles (IceT's sparse "
-                        "images), in the library's workspace: aloha_render_frame_noalloc"},
+                "note": "the allocation-free entry point: the same compact rank images in a "
+                        "static per-GPU workspace"},
             "fused_render_composite": {"ms_per_frame": round(ms_fused, 5),
                                        "Gsamples/s": round(samples / (ms_fused * 1e-3) / 1e9, 4),
                                        "bit_identical_to_plain": fused_equal},

@@ -235,7 +235,11 @@ int aloha_render_composite(aloha_ctx *ctx, const aloha_render_params *params, in
  * in order[] into the outputs (as aloha_composite_over_sparse over the
  * rectangles of aloha_rank_rects) -- the same results bit for bit, in two
  * launches: the per-rank launch also writes the outputs transparent outside
- * the rectangles' bounding box, and the composite covers only that box. */
+ * the rectangles' bounding box, and the composite covers only that box.
+ * images NULL: the caller wants the frame only; the rank images are then
+ * stored compact, each only inside its rectangle (IceT's sparse images), in
+ * a workspace the context holds (grown when a frame needs more; accum_id 0
+ * and no heat map), with the same frame as result. */
 int aloha_render_frame(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
                        int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
                        const aloha_images *images, const int32_t *order, float *d_out_rgba,

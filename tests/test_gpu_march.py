@@ -219,7 +219,8 @@ def test_render_frame_equals_ranks_then_composite(which, c0, c1):
     render_ranks' images and the dense composite of them bit for bit, for
     plain frames, the crosshair and accumulation (whole-image composite; the
     heat map's clock-based colours are not reproducible); two frames in a
-    row on the same context."""
+    row on the same context; and the frame alone (no caller rank images)
+    equal to them."""
     import torch
     if which == "odd":
         raw = WL.synth_blobs_u8((72, 60, 54), 12, 0x5EED7, 3.0, 10.0, 0.3, 1.0)
@@ -259,6 +260,18 @@ def test_render_frame_equals_ranks_then_composite(which, c0, c1):
             for name in ("rgba", "rgba8", "depth", "nsamples") + (("accum",) if accum else ()):
                 assert torch.equal(getattr(got, name), getattr(ref, name)), name
             assert torch.equal(out, dense) and torch.equal(out8, dense8)
+        # the frame only (no rank images from the caller: compact ones in the
+        # context's workspace), twice in a row
+        out_c = torch.full_like(dense, float("nan"))
+        out_c8 = torch.full_like(dense8, -1)
+        if accum:  # no accumulation buffer without rank images
+            with pytest.raises(A.AlohaError):
+                ctx.render_frame(params, wl.W, wl.H, ranks, None, wl.order, out_c, out_c8)
+        else:
+            for _ in range(2):
+                ctx.render_frame(params, wl.W, wl.H, ranks, None, wl.order, out_c, out_c8)
+                torch.cuda.synchronize()
+                assert torch.equal(out_c, dense) and torch.equal(out_c8, dense8)
         if which == "away":
             rects = ctx.rank_rects(params, wl.W, wl.H, ranks)
             assert all(x0 >= x1 or y0 >= y1 for x0, y0, x1, y1 in rects)
