@@ -109,8 +109,6 @@ __device__ __forceinline__ void ray_dir(const DiffArgs &a, int view, int px, int
   }
 }
 
-__device__ __forceinline__ BrickView view_of(const DevBrick &db) { return brick_view(db); }
-
 struct Shaded {
   Sample s;
   float4 tf;  // TF colour, opacity clamped
@@ -120,15 +118,12 @@ struct Shaded {
   float nd;  // N . d as the shade evaluates it (its sign: the side of the |N.d| kink)
 };
 
-__device__ __forceinline__ Shaded shade_sample(const DiffArgs &a, const float4 *s_tf,
-                                               const DevBrick &db, const float org[3],
-                                               const float d[3], float t) {
-  const BrickView b = view_of(db);
-  const float x = (org[0] + t * d[0]) - db.vox_lo[0];
-  const float y = (org[1] + t * d[1]) - db.vox_lo[1];
-  const float z = (org[2] + t * d[2]) - db.vox_lo[2];
+// Normal, TF colour and shade of a sample (value tap and gradient), in the
+// forward's arithmetic order.
+__device__ __forceinline__ Shaded shade_of(const DiffArgs &a, const float4 *s_tf, const Sample &s,
+                                           const float d[3]) {
   Shaded r;
-  if (!sample_fast(b, x, y, z, r.s)) r.s = sample_exact<false>(b, nullptr, db, x, y, z);
+  r.s = s;
   const float ex = r.s.gx + 1e-4f, ey = r.s.gy + 1e-4f, ez = r.s.gz + 1e-4f;
   r.len = sqrtf(ex * ex + ey * ey + ez * ez);
   const float nx = r.s.gx / r.len, ny = r.s.gy / r.len, nz = r.s.gz / r.len;
@@ -158,51 +153,6 @@ __device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const Dev
                                           float y, float z) {
   return sample_bin(b, slot, db.n[0], db.n[1], db.n[2], x, y, z);
 }
-
-// Sample counts of a lane in up to kTallySlots bins (a lane's samples --
-// its rays' consecutive lattice points -- fall in few bins), summed over the
-// wave per bin at the end: one global atomic per distinct bin of the wave
-// instead of one per run of one lane, many of them to the same counter.  A
-// sample of a further bin goes to its counter at once.
-constexpr int kTallySlots = 4;
-struct BinTally {
-  int bin[kTallySlots];
-  uint32_t n[kTallySlots];
-  __device__ BinTally() {
-#pragma unroll
-    for (int k = 0; k < kTallySlots; ++k) bin[k] = -1, n[k] = 0u;
-  }
-  __device__ void add(uint32_t *count, int b, uint32_t c) {
-    bool done = false;
-#pragma unroll
-    for (int k = 0; k < kTallySlots; ++k)
-      if (!done && bin[k] == b) n[k] += c, done = true;
-#pragma unroll
-    for (int k = 0; k < kTallySlots; ++k)
-      if (!done && bin[k] < 0) bin[k] = b, n[k] = c, done = true;
-    if (!done) atomicAdd(&count[b], c);
-  }
-  // Every lane of the wave calls it (wave-uniform control flow).
-  __device__ void flush(uint32_t *count) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < kTallySlots; ++k) {
-      while (true) {
-        const uint64_t pm = __ballot(bin[k] >= 0);
-        if (!pm) break;
-        const int leader = __ffsll((unsigned long long)pm) - 1;
-        const int lb = __shfl(bin[k], leader);
-        uint32_t v = 0u;
-#pragma unroll
-        for (int j = k; j < kTallySlots; ++j)
-          if (bin[j] == lb) v += n[j], bin[j] = -1;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
-        if (lane == leader) atomicAdd(&count[lb], v);
-      }
-    }
-  }
-};
 
 // The forward with lane groups: G lanes per ray evaluate G lattice samples
 // of its segment at once (t_k depends on k alone, so the lanes need no
@@ -545,51 +495,166 @@ __device__ __forceinline__ void sample_pos(const float org[3], const float d[3],
   z = (org[2] + t * d[2]) - db.vox_lo[2];
 }
 
-// Count pass: every sample the forward took (nsamples per pixel), counted
-// into its bin: a ray's consecutive samples mostly share a bin, so runs of
-// one bin go to the lane's tally, and the wave's tallies to the counters.
-__global__ __launch_bounds__(kBlock) void diff_count_kernel(const DiffArgs a, const DiffBins b) {
+// Slots of the waves: each wave of the 16x16-pixel blocks (an 8x8 tile of
+// one view) gets 64 x its longest sample count.  A block takes its four
+// waves' slots with one atomic on one of kSlotCounters counters (block id
+// mod kSlotCounters, so no counter serves more than a few hundred blocks);
+// counter c's slots start where the host puts its region once it has read
+// the counters.  wave_off[w] = c << 56 | the wave's offset in the region.
+// Which slots hold which samples does not change any sum.
+__device__ __forceinline__ int wave_id(const DiffArgs &a, int view, int bx, int by, int wave) {
+  const int gx = (a.W + kTile - 1) / kTile, gy = (a.H + kTile - 1) / kTile;
+  return ((view * gy + by) * gx + bx) * 4 + wave;
+}
+// The first slot of pixel (px, py)'s ray (its place in its wave's tile added).
+__device__ __forceinline__ unsigned long long ray_slot(const DiffArgs &a, const DiffBins &b,
+                                                      int view, int px, int py) {
+  const int w = wave_id(a, view, px / kTile, py / kTile, ((px >> 3) & 1) + 2 * ((py >> 3) & 1));
+  const unsigned long long wo = b.wave_off[w];
+  return b.region[wo >> 56] + (wo & ((1ull << 56) - 1ull)) + (unsigned)((px & 7) + 8 * (py & 7));
+}
+
+__global__ __launch_bounds__(kBlock) void diff_wave_slots_kernel(const DiffArgs a, const DiffBins b) {
+  __shared__ unsigned long long s_size[4];
   const int view = blockIdx.z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
   const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
-  BinTally tally;
-  // no early exit: every lane reaches the wave's flush
-  const bool in = px < a.W && py < a.H;
-  const size_t out = (size_t)view * a.W * a.H + (px + (size_t)a.W * py);
-  const uint32_t N = in ? a.nsamples[out] : 0u;
-  if (N > 0) {
-    const int pixel = px + a.W * py;
-    const float *org = a.cam[view];
-    float d[3];
-    ray_dir(a, view, px, py, d);
-    uint32_t acc = 0, run = 0;
-    int run_bin = -1;
-    for (int vi = 0; acc < N && vi < kMaxVisits; ++vi) {
-      Visit v;
-      if (visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi) <= vi) break;
-      const uint32_t cnt = visit_count(a, v, N - acc);
-      const DevBrick db = a.bricks[v.bid];
-      for (uint32_t k = 0; k < cnt; ++k) {
-        float x, y, z;
-        sample_pos(org, d, db, lattice_t(a, v, (int)k), x, y, z);
-        const int bin = sample_bin(b, v.slot, db, x, y, z);
-        if (bin != run_bin) {
-          if (run) tally.add(b.count, run_bin, run);
-          run_bin = bin;
-          run = 0;
-        }
-        ++run;
-      }
-      acc += cnt;
+  uint32_t n = 0;
+  if (px < a.W && py < a.H) n = a.nsamples[(size_t)view * a.W * a.H + px + (size_t)a.W * py];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) n = max(n, (uint32_t)__shfl_xor((int)n, o));
+  if (lane == 0) s_size[wave] = 64ull * n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long total = s_size[0] + s_size[1] + s_size[2] + s_size[3];
+    const int c = (int)((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) %
+                        kSlotCounters);
+    unsigned long long off = total ? atomicAdd(&b.slot_ctr[c], total) : 0ull;
+    for (int w = 0; w < 4; ++w) {
+      b.wave_off[wave_id(a, view, blockIdx.x, blockIdx.y, w)] = (unsigned long long)c << 56 | off;
+      off += s_size[w];
     }
-    if (run) tally.add(b.count, run_bin, run);
   }
-  tally.flush(b.count);
 }
 
-// Exclusive prefix of the bin counts (one workgroup): off[] and the record
-// cursors; off[nbins] is the record total.
+// Resample pass of the adjoint: every sample the forward took (the first
+// nsamples lattice points of the pixel's brick visits) is evaluated once
+// more -- value tap and gradient, the seven trilinear taps -- and stored in
+// its slot (sample k of N of the ray in tile place l: backward step N-1-k).  Nothing here
+// depends on the compositing recurrence (a sample's index in the ray fixes
+// it), so the G lanes of a group evaluate G consecutive lattice samples at
+// once and a ray is done after its nsamples samples; the reverse walk then
+// reads 16 bytes per sample instead of gathering its 20 footprint voxels.
+// The brick walk is the forward's (closest reported hit from tmin, ties to
+// the lower slot; jitter per visit), so the samples are the forward's bit
+// for bit.  Each wave owns one 4x4 tile of an 8x8 block (few rays per wave:
+// the rays with samples are a small part of the image, and more waves hide
+// the gathers' latency better); a group takes the tile's next pixel with
+// samples as soon as its ray is done.
+template <int G>
+__global__ __launch_bounds__(kBlock) void diff_resample_kernel(const DiffArgs a, const DiffBins b) {
+  enum : int { kIdle = 0, kPick = 1, kMarch = 2, kOver = 3 };
+  const int view = blockIdx.z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gl = lane % G, gbase = lane - gl;
+  const int tx = blockIdx.x * 8 + (wave & 1) * 4, ty = blockIdx.y * 8 + (wave >> 1) * 4;
+  const float *org = a.cam[view];
+  const float dt = a.dt;
+  int state = kIdle, next = 0;  // next: wave-uniform
+  float d[3] = {1.f, 0.f, 0.f};
+  uint32_t N = 0, done = 0, rng = 0;
+  float4 *out_s = b.samples;  // the slot of the ray's last sample
+  float tmin = 0.f, t1 = 0.f, j = 0.f, k0 = 0.f;
+  int kk = 0;
+  BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
+  float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
+  while (__any(state != kOver)) {
+    // a group without a ray takes the tile's next pixel
+    const bool want = state == kIdle && gl == 0;
+    const uint64_t m = __ballot(want);
+    int k = next + __popcll(m & ((1ull << lane) - 1ull));
+    next += __popcll(m);
+    k = group_bcast_i<G, 0>(k, gbase);
+    if (state == kIdle) {
+      if (k >= 16) {
+        state = kOver;
+      } else {
+        const int px = tx + (k & 3), py = ty + (k >> 2);
+        if (px < a.W && py < a.H) {
+          const size_t o = (size_t)view * a.W * a.H + px + (size_t)a.W * py;
+          N = a.nsamples[o];
+          if (N > 0) {
+            ray_dir(a, view, px, py, d);
+            rng = rng_seed((uint32_t)(px + a.W * py), 0u);
+            out_s = b.samples + (ray_slot(a, b, view, px, py) + 64ull * (N - 1u));
+            done = 0;
+            tmin = 0.f;
+            state = kPick;
+          }
+        }
+      }
+    }
+    if (state == kPick) {  // closest reported hit from tmin, ties to the lower slot
+      int hit = -1;
+      float best = 0.f, bt0 = 0.f, bt1 = 0.f;
+      for (int kb = 0; kb < a.nb; ++kb) {
+        const DevBrick &db = a.bricks[a.brick_ids[kb]];  // kb is wave-uniform: scalar loads
+        float s0, s1, rep;
+        if (!slab(org, d, tmin, 1e10f, db.space_lo, db.space_hi, s0, s1)) continue;
+        if (s0 > tmin) rep = s0;
+        else if (s1 > tmin) rep = s1;
+        else continue;
+        if (hit < 0 || rep < best) {
+          hit = a.brick_ids[kb];
+          best = rep;
+          bt0 = s0;
+          bt1 = s1;
+        }
+      }
+      if (hit < 0) {
+        state = kIdle;  // (only with sample counts that are not the forward's)
+      } else {
+        j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
+        const float segs = floorf((bt0 - dt * j) / dt);
+        const float tfirst = dt * (j + segs);
+        k0 = tfirst < bt0 ? segs + 1.f : segs;
+        t1 = bt1;
+        kk = 0;
+        const DevBrick &db = a.bricks[hit];
+        bv = brick_view(db);
+        lo0 = db.vox_lo[0];
+        lo1 = db.vox_lo[1];
+        lo2 = db.vox_lo[2];
+        tmin = bt1 + 1e-20f;
+        state = kMarch;
+      }
+    }
+    // kMarch: lattice samples kk .. kk + G - 1 of the segment; the lanes that
+    // take one are a prefix of the group (t grows with the lane)
+    const bool go = state == kMarch;
+    const float t = dt * (j + (k0 + (float)(kk + gl)));
+    const bool take = go && t < t1 && done + (uint32_t)gl < N;
+    if (take) {
+      const float x = (org[0] + t * d[0]) - lo0;
+      const float y = (org[1] + t * d[1]) - lo1;
+      const float z = (org[2] + t * d[2]) - lo2;
+      Sample sm;
+      if (!sample_fast(bv, x, y, z, sm)) sm = sample_exact<false>(bv, nullptr, a.bricks[0], x, y, z);
+      out_s[-64 * (ptrdiff_t)(done + gl)] = make_float4(sm.val, sm.gx, sm.gy, sm.gz);
+    }
+    const uint64_t took = __ballot(take);
+    if (go) {
+      done += (uint32_t)__popcll((took >> gbase) & ((1ull << G) - 1ull));
+      kk += G;
+      if (done >= N) state = kIdle;
+      else if (!(dt * (j + (k0 + (float)kk)) < t1)) state = kPick;
+    }
+  }
+}
+
+// Exclusive prefix of the bin counts (one workgroup): off[]; off[nbins] is
+// the sample total.
 __global__ __launch_bounds__(1024) void diff_scan_kernel(const DiffBins b) {
   __shared__ uint32_t part[1024];
   const int t = threadIdx.x, n = b.nbins;
@@ -608,29 +673,96 @@ __global__ __launch_bounds__(1024) void diff_scan_kernel(const DiffBins b) {
   uint32_t run = t ? part[t - 1] : 0u;
   for (int i = i0; i < i1; ++i) {
     b.off[i] = run;
-    b.cursor[i] = run;
     run += b.count[i];
   }
   if (t == 1023) b.off[n] = part[1023];
 }
 
+// Runs into their bins: each head takes the next places of its bin for its
+// samples and raises the bin's bounds.  Heads of one wave in the same bin
+// (neighbouring rays entering it at the same backward step) are served by
+// one atomic per bin: the leader adds their lengths, each head's rank is the
+// leader's result plus the lengths of the heads before it.  The order of the
+// runs inside a bin depends on the atomics; the bin's sums do not.
+__global__ __launch_bounds__(kBlock) void diff_run_rank_kernel(const DiffBins b) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int bin = s < b.total ? b.run_bin[s] : -1;
+  if (bin >= b.nbins) bin = -1;
+  float4 info = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (bin >= 0) info = b.run_info[s];
+  const uint32_t len = bin >= 0 ? __float_as_uint(info.x) : 0u;
+  // bounds are >= 0 or NaN: as unsigned bits NaN is above every finite bound
+  const uint32_t bd = __float_as_uint(info.y), bt = __float_as_uint(info.z);
+  bool pending = bin >= 0;
+  uint32_t rank = 0;
+  while (true) {  // wave-uniform: one trip per distinct bin of the wave's heads
+    const uint64_t pm = __ballot(pending);
+    if (!pm) break;
+    const int leader = __ffsll((unsigned long long)pm) - 1;
+    const int lb = __shfl(bin, leader);
+    const bool mine = pending && bin == lb;
+    // inclusive scan of the lengths of this bin's heads, and their bounds' maxima
+    uint32_t v = mine ? len : 0u, md = mine ? bd : 0u, mt = mine ? bt : 0u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = (uint32_t)__shfl_up((int)v, o);
+      if (lane >= o) v += u;
+      md = max(md, (uint32_t)__shfl_xor((int)md, o));
+      mt = max(mt, (uint32_t)__shfl_xor((int)mt, o));
+    }
+    const uint32_t sum = (uint32_t)__shfl((int)v, 63);
+    uint32_t base = 0;
+    if (lane == leader) {
+      base = atomicAdd(&b.count[lb], sum);
+      if (md) atomicMax(&b.maxd[lb], md);
+      if (mt) atomicMax(&b.maxt[lb], mt);
+    }
+    base = (uint32_t)__shfl((int)base, leader);
+    if (mine) {
+      rank = base + v - len;
+      pending = false;
+    }
+  }
+  if (bin >= 0) {
+    info.w = __uint_as_float(rank);
+    b.run_info[s] = info;
+  }
+}
+
+// The bins' slot lists: run (head s, length L, rank r) of bin c fills
+// rec_list[off[c] + r ..] with s, s - 64, .., s - 64 (L - 1).
+__global__ __launch_bounds__(kBlock) void diff_run_list_kernel(const DiffBins b) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= b.total) return;
+  const int bin = b.run_bin[s];
+  if (bin < 0 || bin >= b.nbins) return;
+  const float4 info = b.run_info[s];
+  const uint32_t len = __float_as_uint(info.x), base = b.off[bin] + __float_as_uint(info.w);
+  const uint32_t end = min(base + len, b.off[bin + 1]);
+  for (uint32_t i = base; i < end; ++i) b.rec_list[i] = s - 64u * (i - base);
+}
+
 // The adjoint march: per pixel, the forward's samples in reverse (T_k =
 // T_{k+1}/(1-a_k)), back-propagated through compositing, shading and the TF
-// lookup.  Each sample becomes a record in its bin: its position, the
+// lookup.  Each sample becomes a record in its slot: its position, the
 // derivative of the loss with respect to its value tap and to its gradient
 // (through N = g/|g+1e-4|), its value (which fixes the two TF entries it
 // read) and the derivative with respect to the TF colour and opacity it got;
-// diff_bin_kernel scatters them.
+// diff_bin_kernel scatters them.  A sample's value tap and gradient come
+// from the resample pass (16 bytes, read one sample ahead), so the walk
+// issues no voxel gathers; its bin runs are marked at their head slots (the
+// bin, the length and the largest record bounds), so it needs no atomics
+// either -- diff_run_rank_kernel and diff_run_list_kernel place the runs.
 //
-// One lane per ray, 8x8 pixels per wave.  Rays are short at these
-// opacities (~13 samples per ray that reaches the volume), so the per-ray
-// set-up weighs as much as the samples.  Measured on 256^3 x 8 views of
-// 512^2: 1.36 ms; a lane group per ray, which repeats the set-up on every
-// lane of the group, 1.53 ms (4 lanes); persistent lanes taking pixels of a
-// 16x16 region per wave as their rays end, whose loop trips mix set-up and
-// sample code and which leave a quarter of the waves, 2.5 ms.  Records of
-// one bin run (consecutive samples of a visit in one bin) are reserved with
-// one atomic.
+// One lane per ray, 8x8 pixels per wave.  Measured on 256^3 x 8 views of
+// 512^2: 1.36 ms while the walk evaluated the taps itself and reserved
+// records in bin order (an atomic with return at each bin run, whose
+// length it counted by stepping through the run first): each loop trip
+// waited for some lane's atomic and run count; without the run count 0.67
+// ms, without the atomics as well 0.20 ms.  A lane group per ray (repeating
+// the set-up on every lane of the group) 1.53 ms (4 lanes); persistent lanes
+// taking pixels of a 16x16 region per wave as their rays end, 2.5 ms.
 __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a, const DiffBins b) {
   extern __shared__ float4 s_tf[];
   load_tf(a, s_tf);
@@ -663,18 +795,23 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
     }
     acc += cnt;
   }
+  // the ray's slots: backward step i at slot0 + 64 i
+  const uint32_t slot0 = (uint32_t)ray_slot(a, b, view, px, py);
+  const float4 *smp = b.samples + slot0;
+  uint32_t idx = N;  // samples not yet walked back
+  float4 nxt = smp[0];
   float T = 1.f - C.w;  // transmittance after the last sample
   const float TN = T;
   float Sr = 0.f, Sg = 0.f, Sb = 0.f;  // suffix sums of later contributions
-  uint32_t rec = 0, rec_end = 0, run_left = 0;  // records of the current bin run
-  // the run's bin and the largest bounds of its records (diff_bin_kernel's
-  // fixed-point scales), raised into the bin's maxima once per run
+  // the current bin run (its lowest slot so far is its head) and the
+  // largest bounds of its records
   int run_bin = -1;
+  uint32_t run_len = 0, head = 0;
   float rmd = 0.f, rmt = 0.f;
-  auto flush_run = [&]() {
-    if (run_bin >= 0 && (rmd != 0.f || rmt != 0.f)) {
-      atomicMax(&b.maxd[run_bin], __float_as_uint(rmd));
-      atomicMax(&b.maxt[run_bin], __float_as_uint(rmt));
+  auto close_run = [&]() {
+    if (run_bin >= 0) {
+      b.run_bin[head] = run_bin;
+      b.run_info[head] = make_float4(__uint_as_float(run_len), rmd, rmt, 0.f);
     }
   };
   const int n_tf = a.tf_n;
@@ -683,9 +820,16 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
     visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
     const uint32_t cnt = vi == last ? N - before_last : visit_count(a, v, 0xFFFFFFFFu);
     const DevBrick db = a.bricks[v.bid];
-    for (int k = (int)cnt - 1; k >= 0; --k) {
+    for (int k = (int)cnt - 1; k >= 0 && idx > 0; --k) {
       const float t = lattice_t(a, v, k);
-      const Shaded r = shade_sample(a, s_tf, db, org, d, t);
+      Sample sv;
+      sv.val = nxt.x;
+      sv.gx = nxt.y;
+      sv.gy = nxt.z;
+      sv.gz = nxt.w;
+      --idx;
+      if (idx > 0) nxt = smp[64 * (N - idx)];
+      const Shaded r = shade_of(a, s_tf, sv, d);
       const float am = r.tf.w;
       const float one_m = 1.f - am;
       const float inv_om = 1.f / one_m;
@@ -722,26 +866,18 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
       for (int c = 0; c < 3; ++c) dg[c] = dN[c] * il - dNg * (gv[c] + 1e-4f) * il3;
       float x, y, z;
       sample_pos(org, d, db, t, x, y, z);
-      if (run_left == 0) {
-        // a new run: this sample and the earlier ones of the visit in its bin
-        // get consecutive records of the bin
-        const int bin = sample_bin(b, v.slot, db, x, y, z);
-        uint32_t len = 1;
-        for (int j = k - 1; j >= 0; --j) {
-          float xj, yj, zj;
-          sample_pos(org, d, db, lattice_t(a, v, j), xj, yj, zj);
-          if (sample_bin(b, v.slot, db, xj, yj, zj) != bin) break;
-          ++len;
-        }
-        rec = atomicAdd(&b.cursor[bin], len);
-        rec_end = min(b.off[bin + 1], b.capacity);
-        run_left = len;
-        flush_run();
+      const int bin = sample_bin(b, v.slot, db, x, y, z);
+      if (bin != run_bin) {  // the run walked so far ended at its head
+        close_run();
         run_bin = bin;
+        run_len = 0;
         rmd = rmt = 0.f;
       }
-      if (rec < rec_end) {
-        float4 *o = b.records + (size_t)kRecWords * rec;
+      const uint32_t sl = slot0 + 64u * (N - 1u - idx);
+      head = sl;
+      ++run_len;
+      {
+        float4 *o = b.records + (size_t)kRecWords * sl;
         o[0] = make_float4(x, y, z, dv);
         o[1] = make_float4(dg[0], dg[1], dg[2], r.s.val);
         o[2] = make_float4(dcr, dcg, dcb, da);
@@ -758,11 +894,9 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
         rmd = (bd > rmd || bd != bd) ? bd : rmd;
         rmt = (bt > rmt || bt != bt) ? bt : rmt;
       }
-      ++rec;
-      --run_left;
     }
   }
-  flush_run();
+  close_run();
 }
 
 // Scale 2^S of a bin's fixed-point sums: at most `count` contributions of at
@@ -802,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
   unsigned long long *tile = s_fix;               // kTileVox
   unsigned long long *tfs = s_fix + kTileVox;     // 4 x min(tf_n, kLdsTfChunk)
   const int bin = blockIdx.x;
-  const uint32_t r0 = b.off[bin], r1 = min(b.off[bin + 1], b.capacity);
+  const uint32_t r0 = b.off[bin], r1 = b.off[bin + 1];
   if (r0 >= r1) return;  // block-uniform
   int slot = 0;  // the brick slot holding this bin
   while (b.base[slot + 1] <= bin) ++slot;
@@ -810,9 +944,10 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
   const bool dens = a.dvox[slot] != nullptr;
   const bool want_tf = a.dtf != nullptr;
   const int n = a.tf_n;
-  // a bin's records come in runs of one ray's consecutive samples (half a
+  // a bin's slots come in runs of one ray's consecutive samples (half a
   // voxel apart, footprints overlapping): each thread takes a contiguous
-  // chunk, so the lanes of one instruction rarely add to the same word
+  // chunk of the list, so the lanes of one instruction rarely add to the
+  // same word
   const uint32_t per = (r1 - r0 + kBlock - 1) / kBlock;
   const uint32_t c0 = r0 + per * threadIdx.x, c1 = min(r1, c0 + per);
   // the largest bounds of the bin's records (raised by the adjoint march)
@@ -834,7 +969,9 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
     int ri = INT_MIN;  // the TF entry pair of the current run, its sums
     long long q0[4] = {0, 0, 0, 0}, q1[4] = {0, 0, 0, 0};
     for (uint32_t r = c0; r < c1; ++r) {
-      const float4 *p = b.records + (size_t)kRecWords * r;
+      const uint32_t sl = b.rec_list[r];
+      if (sl >= b.total) continue;  // (only with sample counts that are not the forward's)
+      const float4 *p = b.records + (size_t)kRecWords * sl;
       const float4 p0 = p[0], p1 = p[1];
       if (do_dens) {
         const float gd[3] = {p1.x, p1.y, p1.z};
@@ -904,7 +1041,8 @@ __global__ __launch_bounds__(kBlock) void diff_tf_group_kernel(const DiffArgs a,
   if (i >= m) return;
   const int b0 = g * kTfGroupBins, b1 = min(b.nbins, b0 + kTfGroupBins);
   double s = 0.0;
-  for (int bin = b0; bin < b1; ++bin)
+#pragma unroll 8
+  for (int bin = b0; bin < b1; ++bin)  // (loads issued ahead, added in bin order)
     if (b.off[bin] != b.off[bin + 1]) s += b.tf_part[(size_t)bin * m + i];
   groups[(size_t)g * m + i] = s;
 }
@@ -915,24 +1053,28 @@ __global__ __launch_bounds__(kBlock) void diff_tf_reduce_kernel(const DiffArgs a
   const int m = 4 * a.tf_n;
   if (i >= m) return;
   double s = 0.0;
-  for (int g = 0; g < ngroups; ++g) s += groups[(size_t)g * m + i];
+#pragma unroll 8
+  for (int g = 0; g < ngroups; ++g) s += groups[(size_t)g * m + i];  // (in group order)
   a.dtf[i] += (float)s;
 }
 
 // Tiles into the gradient grid: voxel v of an axis lies in the tile of block
 // (v+1)/16 (at (v+1)%16) and, when (v+1)%16 < 2, also in the previous
-// block's (at (v+1)%16 + 16): up to 8 tiles, added in a fixed order.
+// block's (at (v+1)%16 + 16): up to 8 tiles, added in a fixed order.  Each
+// thread folds kFoldZ voxels of one column (few, fat workgroups: the work
+// per voxel is small).
+constexpr int kFoldZ = 8;
 __global__ __launch_bounds__(kBlock) void diff_fold_kernel(const DiffArgs a, const DiffBins b,
-                                                           int nz_max) {
-  // grid: x = 64-voxel row pieces, y = rows of 4, z = slot * nz_max + z
-  const int slot = blockIdx.z / nz_max, z = blockIdx.z - slot * nz_max;
+                                                           int nzb) {
+  // grid: x = 64-voxel row pieces, y = rows of 4, z = slot * nzb + z / kFoldZ
+  const int slot = blockIdx.z / nzb, z0 = (blockIdx.z - slot * nzb) * kFoldZ;
   float *grad = a.dvox[slot];
   if (!grad) return;
   const DevBrick db = a.bricks[a.brick_ids[slot]];
   const int nx = db.n[0], ny = db.n[1], nz = db.n[2];
   const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= nx || y >= ny || z >= nz) return;
-  int bxs[2], lxs[2], bys[2], lys[2], bzs[2], lzs[2], cx = 0, cy = 0, cz = 0;
+  if (x >= nx || y >= ny) return;
+  int bxs[2], lxs[2], bys[2], lys[2], cx = 0, cy = 0;
   auto cands = [](int v, int *bs, int *ls, int &c) {
     const int u = v + 1, bh = u / kBinEdge, lh = u % kBinEdge;
     if (lh < 2 && bh >= 1) {
@@ -944,18 +1086,31 @@ __global__ __launch_bounds__(kBlock) void diff_fold_kernel(const DiffArgs a, con
   };
   cands(x, bxs, lxs, cx);
   cands(y, bys, lys, cy);
-  cands(z, bzs, lzs, cz);
   const int nbx = b.nbx[slot], nby = b.nby[slot], base = b.base[slot];
-  float sum = 0.f;
-  for (int kz = 0; kz < cz; ++kz)
-    for (int ky = 0; ky < cy; ++ky)
-      for (int kx = 0; kx < cx; ++kx) {
-        const int bin = base + (bzs[kz] * nby + bys[ky]) * nbx + bxs[kx];
-        if (b.off[bin] == b.off[bin + 1]) continue;  // no samples: tile not written
-        sum += b.tiles[(size_t)bin * kTileVox + (lzs[kz] * kTileEdge + lys[ky]) * kTileEdge +
-                       lxs[kx]];
-      }
-  if (sum != 0.f) grad[((size_t)z * ny + y) * nx + x] += sum;
+  // the column's sums and gradient values are all loaded before any store
+  float sum[kFoldZ], g[kFoldZ];
+#pragma unroll
+  for (int zz = 0; zz < kFoldZ; ++zz) {
+    const int z = z0 + zz;
+    sum[zz] = 0.f;
+    if (z >= nz) continue;
+    int bzs[2], lzs[2], cz = 0;
+    cands(z, bzs, lzs, cz);
+    for (int kz = 0; kz < cz; ++kz)
+      for (int ky = 0; ky < cy; ++ky)
+        for (int kx = 0; kx < cx; ++kx) {
+          const int bin = base + (bzs[kz] * nby + bys[ky]) * nbx + bxs[kx];
+          if (b.count[bin] == 0) continue;  // no samples: tile not written
+          sum[zz] += b.tiles[(size_t)bin * kTileVox + (lzs[kz] * kTileEdge + lys[ky]) * kTileEdge +
+                             lxs[kx]];
+        }
+  }
+#pragma unroll
+  for (int zz = 0; zz < kFoldZ; ++zz)
+    g[zz] = sum[zz] != 0.f ? grad[((size_t)(z0 + zz) * ny + y) * nx + x] : 0.f;
+#pragma unroll
+  for (int zz = 0; zz < kFoldZ; ++zz)
+    if (sum[zz] != 0.f) grad[((size_t)(z0 + zz) * ny + y) * nx + x] = g[zz] + sum[zz];
 }
 
 }  // namespace
@@ -967,18 +1122,24 @@ hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_diff_count(const DiffArgs &a, const DiffBins &b, hipStream_t s) {
+hipError_t launch_diff_rays(const DiffArgs &a, const DiffBins &b, hipStream_t s) {
   const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
-  if (b.nbins > 0) {
-    diff_count_kernel<<<grid, kBlock, 0, s>>>(a, b);
-    diff_scan_kernel<<<1, 1024, 0, s>>>(b);
-  }
+  diff_wave_slots_kernel<<<grid, kBlock, 0, s>>>(a, b);
   return hipGetLastError();
 }
 
 hipError_t launch_diff_backward(const DiffArgs &a, const DiffBins &b, hipStream_t s) {
   const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
+  diff_resample_kernel<4><<<dim3((a.W + 7) / 8, (a.H + 7) / 8, a.V), kBlock, 0, s>>>(a, b);
   diff_backward_kernel<<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a, b);
+  if (b.total > 0) {
+    const unsigned blocks = (unsigned)((b.total + kBlock - 1) / kBlock);
+    diff_run_rank_kernel<<<blocks, kBlock, 0, s>>>(b);
+    diff_scan_kernel<<<1, 1024, 0, s>>>(b);
+    diff_run_list_kernel<<<blocks, kBlock, 0, s>>>(b);
+  } else {
+    diff_scan_kernel<<<1, 1024, 0, s>>>(b);
+  }
   return hipGetLastError();
 }
 
@@ -989,9 +1150,10 @@ hipError_t launch_diff_bins(const DiffArgs &a, const DiffBins &b, const int max_
       sizeof(unsigned long long) * ((size_t)kTileVox + 4 * (size_t)std::min(a.tf_n, kLdsTfChunk));
   diff_bin_kernel<<<b.nbins, kBlock, lds, s>>>(a, b);
   if (max_dims[0] > 0) {
+    const int nzb = (max_dims[2] + kFoldZ - 1) / kFoldZ;
     const dim3 grid((unsigned)((max_dims[0] + 63) / 64), (unsigned)((max_dims[1] + 3) / 4),
-                    (unsigned)(max_dims[2] * a.nb));
-    diff_fold_kernel<<<grid, kBlock, 0, s>>>(a, b, max_dims[2]);
+                    (unsigned)(nzb * a.nb));
+    diff_fold_kernel<<<grid, kBlock, 0, s>>>(a, b, nzb);
   }
   if (a.dtf) {
     // the group sums go behind the bins' partials in b.tf_part

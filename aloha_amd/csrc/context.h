@@ -54,17 +54,20 @@ struct aloha_ctx {
   // address could not be had: scratch_err says why)
   void *scratch = nullptr;
   std::string scratch_err;
-  // workspace of aloha_diff_backward, grown on demand: bin counts, offsets
-  // and cursors; sample records; bin tiles; the record total read back
+  // workspace of aloha_diff_backward, grown on demand: bin counts, bounds
+  // and offsets; ray offsets; the sample slots (resampled values, run
+  // marks, records, bin lists); bin tiles; the sample total read back
   uint32_t *diff_bins = nullptr;
   size_t diff_bins_cap = 0;
-  float4 *diff_records = nullptr;
-  size_t diff_records_cap = 0;  // records (2 float4 each)
+  void *diff_rays = nullptr;
+  size_t diff_rays_cap = 0;   // bytes
+  float4 *diff_slots = nullptr;
+  size_t diff_slots_cap = 0;  // slots
   float *diff_tiles = nullptr;
   size_t diff_tiles_cap = 0;  // floats
   double *diff_tf_part = nullptr;
   size_t diff_tf_part_cap = 0;  // doubles
-  uint32_t *h_total = nullptr;  // pinned host word
+  uint64_t *h_total = nullptr;  // pinned host words (the adjoint's slot counters)
   float4 *frame_ws = nullptr;   // compact rank images of aloha_render_frame without images
   size_t frame_ws_cap = 0;      // float4
   std::string err;

@@ -256,7 +256,8 @@ extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
   if (ctx->d_bricks) hipFree(ctx->d_bricks);
   if (ctx->d_tf) hipFree(ctx->d_tf);
   if (ctx->diff_bins) hipFree(ctx->diff_bins);
-  if (ctx->diff_records) hipFree(ctx->diff_records);
+  if (ctx->diff_rays) hipFree(ctx->diff_rays);
+  if (ctx->diff_slots) hipFree(ctx->diff_slots);
   if (ctx->diff_tiles) hipFree(ctx->diff_tiles);
   if (ctx->diff_tf_part) hipFree(ctx->diff_tf_part);
   if (ctx->frame_ws) hipFree(ctx->frame_ws);
@@ -1078,9 +1079,9 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
   if (nbins > kMaxDiffBins)
     return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %lld gradient bins", (long long)nbins);
   if (nbins > 0) {
-    // counts, per-bin maxima, scan, then the record total (read back: it
-    // sizes the records)
-    const size_t words = 5 * (size_t)nbins + 1;
+    // slots of the waves (their total, read back, sizes the slot arrays);
+    // bin counts and maxima start at zero
+    const size_t words = 4 * (size_t)nbins + 1;
     if (words > ctx->diff_bins_cap) {
       if (ctx->diff_bins) HIP_CHECK(ctx, hipFree(ctx->diff_bins));
       ctx->diff_bins = nullptr;
@@ -1088,27 +1089,49 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_bins, words * sizeof(uint32_t)));
       ctx->diff_bins_cap = words;
     }
-    if (!ctx->h_total) HIP_CHECK(ctx, host_malloc((void **)&ctx->h_total, sizeof(uint32_t)));
-    b.count = ctx->diff_bins;  // count | maxd | maxt | off (n + 1) | cursor
+    // first slot of each wave (an 8x8 tile of a view), then the slot counters
+    const size_t nwaves = (size_t)V * ((H + 15) / 16) * ((W + 15) / 16) * 4;
+    const size_t ray_bytes = 8 * (nwaves + kSlotCounters);
+    if (ray_bytes > ctx->diff_rays_cap) {
+      if (ctx->diff_rays) HIP_CHECK(ctx, hipFree(ctx->diff_rays));
+      ctx->diff_rays = nullptr;
+      ctx->diff_rays_cap = 0;
+      HIP_CHECK(ctx, dev_malloc(&ctx->diff_rays, ray_bytes));
+      ctx->diff_rays_cap = ray_bytes;
+    }
+    if (!ctx->h_total)
+      HIP_CHECK(ctx, host_malloc((void **)&ctx->h_total, sizeof(uint64_t) * kSlotCounters));
+    b.count = ctx->diff_bins;  // count | maxd | maxt | off (n + 1)
     b.maxd = b.count + nbins;
     b.maxt = b.maxd + nbins;
     b.off = b.maxt + nbins;
-    b.cursor = b.off + nbins + 1;
+    b.nwaves = (int32_t)nwaves;
+    b.wave_off = (unsigned long long *)ctx->diff_rays;
+    b.slot_ctr = b.wave_off + nwaves;
     HIP_CHECK(ctx, hipMemsetAsync(b.count, 0, sizeof(uint32_t) * 3 * (size_t)nbins, ctx->stream));
-    HIP_CHECK(ctx, launch_diff_count(a, b, ctx->stream));
-    HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_total, b.off + nbins, sizeof(uint32_t),
+    HIP_CHECK(ctx, hipMemsetAsync(b.slot_ctr, 0, sizeof(uint64_t) * kSlotCounters, ctx->stream));
+    HIP_CHECK(ctx, launch_diff_rays(a, b, ctx->stream));
+    HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_total, b.slot_ctr, sizeof(uint64_t) * kSlotCounters,
                                   hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const size_t total = *ctx->h_total;
+    // the counters' regions back to back
+    size_t total = 0;
+    for (int c = 0; c < kSlotCounters; ++c) {
+      b.region[c] = total;
+      total += (size_t)ctx->h_total[c];
+    }
     if (total > ((size_t)1 << 31))
-      return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %zu samples", total);
-    if (total > ctx->diff_records_cap) {
-      if (ctx->diff_records) HIP_CHECK(ctx, hipFree(ctx->diff_records));
-      ctx->diff_records = nullptr;
-      ctx->diff_records_cap = 0;
+      return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %zu sample slots", total);
+    // per slot: resampled value (1 float4), run info (1), record (kRecWords),
+    // run mark and list entry (one word each)
+    constexpr size_t kSlotBytes = (2 + kRecWords) * sizeof(float4) + 2 * sizeof(uint32_t);
+    if (total > ctx->diff_slots_cap) {
+      if (ctx->diff_slots) HIP_CHECK(ctx, hipFree(ctx->diff_slots));
+      ctx->diff_slots = nullptr;
+      ctx->diff_slots_cap = 0;
       const size_t cap = total + total / 4 + 1024;
-      HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_records, cap * kRecWords * sizeof(float4)));
-      ctx->diff_records_cap = cap;
+      HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_slots, cap * kSlotBytes));
+      ctx->diff_slots_cap = cap;
     }
     const size_t tile_floats = max_voxels > 0 ? (size_t)nbins * kTileVox : 0;
     if (tile_floats > ctx->diff_tiles_cap) {
@@ -1128,13 +1151,21 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_tf_part, part * sizeof(double)));
       ctx->diff_tf_part_cap = part;
     }
-    b.records = ctx->diff_records;
-    b.capacity = (uint32_t)std::min<size_t>(ctx->diff_records_cap, 0xFFFFFFFFu);
+    const size_t cap = ctx->diff_slots_cap;
+    b.total = (uint32_t)total;
+    b.cap = (uint32_t)cap;
+    b.samples = ctx->diff_slots;
+    b.run_info = b.samples + cap;
+    b.records = b.run_info + cap;
+    b.run_bin = (int32_t *)(b.records + kRecWords * cap);
+    b.rec_list = (uint32_t *)(b.run_bin + cap);
+    // no run marks but the heads the reverse walk writes
+    HIP_CHECK(ctx, hipMemsetAsync(b.run_bin, 0xFF, sizeof(int32_t) * total, ctx->stream));
     b.tiles = ctx->diff_tiles;
     b.tf_part = ctx->diff_tf_part;
+    HIP_CHECK(ctx, launch_diff_backward(a, b, ctx->stream));
+    HIP_CHECK(ctx, launch_diff_bins(a, b, max_dims, ctx->stream));
   }
-  HIP_CHECK(ctx, launch_diff_backward(a, b, ctx->stream));
-  HIP_CHECK(ctx, launch_diff_bins(a, b, max_dims, ctx->stream));
   return ALOHA_OK;
 }
 

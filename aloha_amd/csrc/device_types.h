@@ -133,17 +133,36 @@ constexpr int kTileVox = kTileEdge * kTileEdge * kTileEdge;
 constexpr int kRecWords = 3;
 constexpr int kLdsTfChunk = 1024;  // TF entries whose gradient a bin sums in LDS at once
 constexpr int kTfGroupBins = 32;   // bins per group of the TF-gradient reduction
+constexpr int kSlotCounters = 8;  // counters the waves take their slots from
+// Sample slots, interleaved by wave: the 64 rays of a wave's 8x8 pixel tile
+// share 64 x (their longest sample count) slots from wave_off[w]; backward
+// step i of the ray in tile place l (step 0 = its last sample) is slot
+// wave_off[w] + 64 i + l, so the 64 lanes of the reverse walk read and write
+// one contiguous run of slots per step.  A sample's resampled value and
+// gradient, its record and its run mark share its slot.  A run is a maximal
+// set of consecutive samples of a ray in one bin; its first sample's slot
+// (the head) holds the run's bin in run_bin (-1 elsewhere) and (length,
+// density bound, TF bound, rank in the bin) in run_info; its other samples
+// sit 64, 128, ... slots below the head.
 struct DiffBins {
   int32_t base[kMaxDiffBricks + 1];  // first bin of each brick slot
   int32_t nbx[kMaxDiffBricks], nby[kMaxDiffBricks], nbz[kMaxDiffBricks];
   int32_t nbins;
-  uint32_t *count;     // samples per bin (count pass)
+  uint32_t *count;     // samples per bin (summed over its runs)
   uint32_t *maxd;      // per bin: bits of the largest record density bound (>= 0 or NaN)
   uint32_t *maxt;      // per bin: bits of the largest record TF bound
   uint32_t *off;       // exclusive prefix of count, nbins + 1 entries
-  uint32_t *cursor;    // next free record of each bin (starts at off)
-  float4 *records;     // kRecWords float4 per sample
-  uint32_t capacity;   // records allocated
+  uint32_t total;      // slots in use
+  uint32_t cap;        // slots allocated
+  int32_t nwaves;      // V x ceil(H / 16) x ceil(W / 16) x 4
+  unsigned long long *wave_off;  // per wave: counter << 56 | first slot in the counter's region
+  unsigned long long *slot_ctr;  // kSlotCounters slot counters
+  unsigned long long region[kSlotCounters];  // first slot of each counter's region
+  float4 *samples;     // per slot: (value, gx, gy, gz)
+  float4 *records;     // per slot: kRecWords float4
+  int32_t *run_bin;    // per slot
+  float4 *run_info;    // per head slot: (length bits, bound d, bound t, rank bits)
+  uint32_t *rec_list;  // per bin (at off[bin]): the slots of its samples
   float *tiles;        // nbins x kTileVox density-gradient partial sums
   double *tf_part;     // nbins x 4 tf_n TF-gradient partial sums
 };
@@ -218,7 +237,7 @@ hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
 // backward in stages: count samples per bin and scan, then the march that
 // writes records and the per-bin bounds, then bins into tiles and tiles into
 // the gradient grids
-hipError_t launch_diff_count(const DiffArgs &a, const DiffBins &b, hipStream_t s);
+hipError_t launch_diff_rays(const DiffArgs &a, const DiffBins &b, hipStream_t s);
 hipError_t launch_diff_backward(const DiffArgs &a, const DiffBins &b, hipStream_t s);
 // max_dims: the largest extent per axis over the bricks with a gradient grid
 hipError_t launch_diff_bins(const DiffArgs &a, const DiffBins &b, const int max_dims[3],

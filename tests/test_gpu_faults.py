@@ -265,7 +265,8 @@ def test_alloc_fault_footprint():
 
 def test_alloc_fault_adjoint_workspace():
     torch = _torch()
-    _case("allocation faults: adjoint workspace (bins, pinned total, records, tiles, TF partials)")
+    _case("allocation faults: adjoint workspace (bins, ray offsets, pinned total, resampled "
+          "values, records, tiles, TF partials)")
     from aloha_amd.autograd import DifferentiableVolume
     wl = WL.diff256(size=40, views=2, W=64, H=56)
 
