@@ -718,19 +718,27 @@ class Batch:
     def set_schedule(self, lanes_per_ray: int, persistent_blocks: int) -> None:
         self.ctx._check(lib().aloha_batch_set_schedule(self._h, lanes_per_ray, persistent_blocks))
 
+    def set_tile(self, tile_w: int) -> None:
+        """Tiles of 64 pixels: 8 (8x8), 16 (16x4) or 32 (32x2) wide."""
+        self.ctx._check(lib().aloha_batch_set_tile(self._h, tile_w))
+
     def schedule(self):
-        t, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        t, b, c, w = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         self.ctx._check(lib().aloha_batch_get_schedule(self._h, C.byref(t), C.byref(b), C.byref(c)))
-        return {"lanes_per_ray": t.value, "persistent_blocks": b.value, "captured": bool(c.value)}
+        self.ctx._check(lib().aloha_batch_get_tile(self._h, C.byref(w)))
+        return {"lanes_per_ray": t.value, "persistent_blocks": b.value, "tile_w": w.value,
+                "captured": bool(c.value)}
 
     def autotune(self, reps: int = 3):
+        """Times every schedule; returns (best ms, [(tile_w, lanes, blocks, ms), ...])."""
         best = C.c_float()
-        table = (C.c_float * (3 * 64))()
+        cap = 64
+        table = (C.c_float * (4 * cap))()
         n = C.c_int32()
-        self.ctx._check(lib().aloha_batch_autotune(self._h, reps, C.byref(best), table, 64,
+        self.ctx._check(lib().aloha_batch_autotune(self._h, reps, C.byref(best), table, cap,
                                                    C.byref(n)))
-        rows = [(int(table[3 * i]), int(table[3 * i + 1]), float(table[3 * i + 2]))
-                for i in range(n.value)]
+        rows = [(int(table[4 * i]), int(table[4 * i + 1]), int(table[4 * i + 2]),
+                 float(table[4 * i + 3])) for i in range(min(n.value, cap))]
         return best.value, rows
 
     def capture(self) -> None:

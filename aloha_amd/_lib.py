@@ -118,6 +118,8 @@ _SIGS = [
     ("aloha_batch_destroy", C.c_int, [_P]),
     ("aloha_batch_set_schedule", C.c_int, [_P, _I, _I]),
     ("aloha_batch_get_schedule", C.c_int, [_P, _IP, _IP, _IP]),
+    ("aloha_batch_set_tile", C.c_int, [_P, _I]),
+    ("aloha_batch_get_tile", C.c_int, [_P, _IP]),
     ("aloha_batch_autotune", C.c_int, [_P, _I, _FP, _FP, _I, _IP]),
     ("aloha_batch_capture", C.c_int, [_P]),
     ("aloha_batch_launch", C.c_int, [_P]),

@@ -1184,6 +1184,8 @@ struct aloha_batch {
   int lanes = 2;
   int persistent_blocks = 0;
   int cus = 0;
+  int P = 0;
+  std::vector<int32_t> rects;  // V x P x 4 (the tile tables are made from them)
   hipStream_t stream = nullptr;  // capture / replay stream (not the null stream)
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -1221,6 +1223,53 @@ void batch_release(aloha_batch *bt) {
 // batch's launch arguments were made: take the current ones and drop a
 // captured graph, which holds the old ones (capture again if there was one).
 int batch_refresh(aloha_batch *bt);
+
+// The work items of tiles (1 << wl) x (64 >> wl): per view the tile
+// bounding box of its rectangles and the prefix of the active items, and the
+// persistent schedule's per-XCD parts (part x of view v is its items
+// [n x / kXcds, n (x + 1) / kXcds), n = the view's item count); copied to
+// the batch's device tables once the batch stream is idle.
+int batch_tables(aloha_batch *bt, int wl) {
+  aloha_ctx *ctx = bt->ctx;
+  const int V = bt->b.V, P = bt->P, W = bt->b.m.W, H = bt->b.m.H;
+  const int tw = 1 << wl, th = 64 >> wl;
+  std::vector<int32_t> boxes((size_t)V * 4, 0), box_off((size_t)V + 1, 0);
+  for (int v = 0; v < V; ++v) {
+    int x0 = W, y0 = H, x1 = 0, y1 = 0;
+    for (int s = 0; s < P; ++s) {
+      const int32_t *rc = &bt->rects[((size_t)v * P + s) * 4];
+      if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
+      x0 = std::min(x0, (int)rc[0]), y0 = std::min(y0, (int)rc[1]);
+      x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
+    }
+    int32_t *bx = &boxes[(size_t)v * 4];
+    if (x0 < x1 && y0 < y1) {
+      bx[0] = x0 / tw, bx[1] = y0 / th;
+      bx[2] = (x1 + tw - 1) / tw - bx[0];
+      bx[3] = (y1 + th - 1) / th - bx[1];
+    }
+    const int64_t next = (int64_t)box_off[v] + (int64_t)bx[2] * bx[3];
+    if (next > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "batch: too many tiles");
+    box_off[v + 1] = (int32_t)next;
+  }
+  std::vector<int32_t> xoff((size_t)kXcds * (V + 1), 0);
+  for (int x = 0; x < kXcds; ++x)
+    for (int v = 0; v < V; ++v) {
+      const int64_t n = (int64_t)box_off[v + 1] - box_off[v];
+      const int64_t part = n * (x + 1) / kXcds - n * x / kXcds;
+      xoff[(size_t)x * (V + 1) + v + 1] = xoff[(size_t)x * (V + 1) + v] + (int32_t)part;
+    }
+  if (bt->stream) HIP_CHECK(ctx, hipStreamSynchronize(bt->stream));
+  HIP_CHECK(ctx, hipMemcpy(bt->d_boxes, boxes.data(), sizeof(int32_t) * boxes.size(),
+                           hipMemcpyHostToDevice));
+  HIP_CHECK(ctx, hipMemcpy(bt->d_box_off, box_off.data(), sizeof(int32_t) * box_off.size(),
+                           hipMemcpyHostToDevice));
+  HIP_CHECK(ctx, hipMemcpy(bt->d_xoff, xoff.data(), sizeof(int32_t) * xoff.size(),
+                           hipMemcpyHostToDevice));
+  bt->b.total_active = box_off[V];
+  bt->b.tile_wl = wl;
+  return ALOHA_OK;
+}
 }  // namespace
 
 namespace aloha {
@@ -1272,37 +1321,7 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
       rank_rect(cams + 12 * v, ctx->bricks, brick_ids + rank_offsets[s],
                 rank_offsets[s + 1] - rank_offsets[s], W, H, &rects[((size_t)v * P + s) * 4]);
   }
-  // per view: bounding box of the rectangles in fused tiles, and the prefix
-  // of the active work items
-  std::vector<int32_t> boxes((size_t)V * 4, 0), box_off((size_t)V + 1, 0);
-  for (int v = 0; v < V; ++v) {
-    int x0 = W, y0 = H, x1 = 0, y1 = 0;
-    for (int s = 0; s < P; ++s) {
-      const int32_t *rc = &rects[((size_t)v * P + s) * 4];
-      if (rc[0] >= rc[2] || rc[1] >= rc[3]) continue;
-      x0 = std::min(x0, (int)rc[0]), y0 = std::min(y0, (int)rc[1]);
-      x1 = std::max(x1, (int)rc[2]), y1 = std::max(y1, (int)rc[3]);
-    }
-    int32_t *bx = &boxes[(size_t)v * 4];
-    if (x0 < x1 && y0 < y1) {
-      bx[0] = x0 / kFusedTileW, bx[1] = y0 / kFusedTileH;
-      bx[2] = (x1 + kFusedTileW - 1) / kFusedTileW - bx[0];
-      bx[3] = (y1 + kFusedTileH - 1) / kFusedTileH - bx[1];
-    }
-    const int64_t next = (int64_t)box_off[v] + (int64_t)bx[2] * bx[3];
-    if (next > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many tiles");
-    box_off[v + 1] = (int32_t)next;
-  }
   if ((int64_t)V * H > INT32_MAX) return fail(ctx, ALOHA_ERR_LIMIT, "aloha_batch_create: too many rows");
-  // the persistent schedule's per-XCD lists: part x of view v is its items
-  // [n x / kXcds, n (x + 1) / kXcds), n = the view's item count
-  std::vector<int32_t> xoff((size_t)kXcds * (V + 1), 0);
-  for (int x = 0; x < kXcds; ++x)
-    for (int v = 0; v < V; ++v) {
-      const int64_t n = (int64_t)box_off[v + 1] - box_off[v];
-      const int64_t part = n * (x + 1) / kXcds - n * x / kXcds;
-      xoff[(size_t)x * (V + 1) + v + 1] = xoff[(size_t)x * (V + 1) + v] + (int32_t)part;
-    }
   auto *bt = new aloha_batch;
   bt->ctx = ctx;
   std::memset(&bt->b, 0, sizeof bt->b);
@@ -1317,16 +1336,13 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   if (e == hipSuccess) e = dev_malloc((void **)&bt->d_cams, sizeof(float) * 12 * (size_t)V);
   if (e == hipSuccess) e = dev_malloc((void **)&bt->d_orders, sizeof(int32_t) * (size_t)V * P);
   if (e == hipSuccess) e = dev_malloc((void **)&bt->d_rects, sizeof(int32_t) * rects.size());
-  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_boxes, sizeof(int32_t) * boxes.size());
-  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_box_off, sizeof(int32_t) * box_off.size());
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_boxes, sizeof(int32_t) * 4 * (size_t)V);
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_box_off, sizeof(int32_t) * ((size_t)V + 1));
   if (e == hipSuccess) e = dev_malloc((void **)&bt->d_counter, sizeof(uint32_t) * kXcds);
-  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_xoff, sizeof(int32_t) * xoff.size());
-  if (e == hipSuccess) e = hipMemcpy(bt->d_xoff, xoff.data(), sizeof(int32_t) * xoff.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = dev_malloc((void **)&bt->d_xoff, sizeof(int32_t) * kXcds * ((size_t)V + 1));
   if (e == hipSuccess) e = hipMemcpy(bt->d_cams, cams, sizeof(float) * 12 * (size_t)V, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_orders, orders, sizeof(int32_t) * (size_t)V * P, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(bt->d_rects, rects.data(), sizeof(int32_t) * rects.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(bt->d_boxes, boxes.data(), sizeof(int32_t) * boxes.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(bt->d_box_off, box_off.data(), sizeof(int32_t) * box_off.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming);
@@ -1342,8 +1358,13 @@ extern "C" int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *par
   bt->b.box_off = bt->d_box_off;
   bt->b.counter = bt->d_counter;
   bt->b.xoff = bt->d_xoff;
-  bt->b.total_active = box_off[V];
   bt->b.fill_blocks = (int)std::min<int64_t>((int64_t)V * H, 1024);
+  bt->P = P;
+  bt->rects = std::move(rects);
+  if (int rc = batch_tables(bt, 4)) {  // 16x4 tiles (the single fused frame's)
+    aloha_batch_destroy(bt);
+    return rc;
+  }
   bt->lanes = ctx->fused_lanes;
   bt->gen = ctx->gen;
   ctx->batches.push_back(bt);
@@ -1382,6 +1403,25 @@ extern "C" int aloha_batch_get_schedule(const aloha_batch *bt, int32_t *lanes_pe
   if (lanes_per_ray) *lanes_per_ray = bt->lanes;
   if (persistent_blocks) *persistent_blocks = bt->persistent_blocks;
   if (captured) *captured = bt->exec != nullptr;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_batch_set_tile(aloha_batch *bt, int32_t tile_w) {
+  if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_set_tile: NULL");
+  if (!bt->ctx) return fail(nullptr, ALOHA_ERR_STATE, "aloha_batch_set_tile: context destroyed");
+  if (tile_w != 8 && tile_w != 16 && tile_w != 32)
+    return fail(bt->ctx, ALOHA_ERR_ARG, "aloha_batch_set_tile: tile width 8, 16 or 32");
+  const int wl = tile_w == 8 ? 3 : tile_w == 16 ? 4 : 5;
+  if (wl == bt->b.tile_wl) return ALOHA_OK;
+  HIP_CHECK(bt->ctx, hipSetDevice(bt->ctx->device));
+  if (bt->stream) HIP_CHECK(bt->ctx, hipStreamSynchronize(bt->stream));
+  batch_drop_graph(bt);
+  return batch_tables(bt, wl);
+}
+
+extern "C" int aloha_batch_get_tile(const aloha_batch *bt, int32_t *tile_w) {
+  if (!bt || !tile_w) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_get_tile: NULL");
+  *tile_w = 1 << bt->b.tile_wl;
   return ALOHA_OK;
 }
 
@@ -1449,11 +1489,10 @@ int batch_refresh(aloha_batch *bt) {
 }
 }  // namespace
 
-// Times every schedule (1/2/4/8 lanes per ray x {one block per item,
-// persistent with 2/4/8 blocks per CU}) over `reps` launches each and keeps
-// the fastest.  The
-// schedule never changes the samples or their arithmetic, so the images do
-// not depend on the choice.  Synchronous.
+// Times every schedule (tiles 8x8, 16x4, 32x2 x 1/2/4/8 lanes per ray x {one
+// block per item, persistent with 2/4/8 blocks per CU}) over `reps` launches
+// each and keeps the fastest.  The schedule never changes the samples or
+// their arithmetic, so the images do not depend on the choice.  Synchronous.
 extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_ms,
                                     float *table, int32_t table_cap, int32_t *table_n) {
   if (!bt) return fail(nullptr, ALOHA_ERR_ARG, "aloha_batch_autotune: NULL");
@@ -1468,10 +1507,16 @@ extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_m
   HIP_CHECK(ctx, hipEventCreate(&e0));
   HIP_CHECK(ctx, hipEventCreate(&e1));
   float best = 1e30f;
-  int best_lanes = bt->lanes, best_blocks = 0, n = 0;
+  int best_lanes = bt->lanes, best_blocks = 0, best_wl = bt->b.tile_wl, n = 0;
   const int blocks_opts[4] = {0, 2, 4, 8};
   const int lane_opts[4] = {1, 2, 4, 8};
-  for (int lanes : lane_opts)
+  for (int wl = 3; wl <= 5; ++wl) {
+    if (int rc = batch_tables(bt, wl)) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return rc;
+    }
+    for (int lanes : lane_opts)
     for (int bo : blocks_opts) {
       bt->lanes = lanes;
       bt->persistent_blocks = bo ? bo * bt->cus : 0;
@@ -1490,21 +1535,25 @@ extern "C" int aloha_batch_autotune(aloha_batch *bt, int32_t reps, float *best_m
       }
       ms /= (float)reps;
       if (table && n < table_cap) {
-        table[3 * n] = (float)lanes;
-        table[3 * n + 1] = (float)bt->persistent_blocks;
-        table[3 * n + 2] = ms;
+        table[4 * n] = (float)(1 << wl);
+        table[4 * n + 1] = (float)lanes;
+        table[4 * n + 2] = (float)bt->persistent_blocks;
+        table[4 * n + 3] = ms;
       }
       ++n;
       if (ms < best) {
         best = ms;
         best_lanes = lanes;
         best_blocks = bt->persistent_blocks;
+        best_wl = wl;
       }
     }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   bt->lanes = best_lanes;
   bt->persistent_blocks = best_blocks;
+  if (int rc = batch_tables(bt, best_wl)) return rc;
   if (best_ms) *best_ms = best;
   if (table_n) *table_n = n;
   return ALOHA_OK;

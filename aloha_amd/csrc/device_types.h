@@ -90,7 +90,7 @@ struct BatchArgs {
   const float *cams;        // V x 12 (org, dir_00, dir_du, dir_dv)
   const int32_t *orders;    // V x P rank slots, front to back
   const int32_t *rects;     // V x P x 4 pixel rectangles
-  const int32_t *boxes;     // V x 4: fused-tile bounding box of the view's rectangles (x0, y0, nx, ny)
+  const int32_t *boxes;     // V x 4: tile bounding box of the view's rectangles (x0, y0, nx, ny)
   const int32_t *box_off;   // V + 1: prefix sums of nx * ny (active work items)
   // persistent schedule: each view's work items cut into kXcds contiguous
   // parts; xoff[x * (V + 1) + v] = the items of parts x of views < v, and
@@ -99,6 +99,7 @@ struct BatchArgs {
   uint32_t *counter;        // kXcds counters
   int32_t V, total_active;  // views, active work items (box_off[V])
   int32_t fill_blocks;      // non-persistent schedule: blocks that write the rows outside the boxes
+  int32_t tile_wl;          // tiles of 64 pixels, (1 << tile_wl) x (64 >> tile_wl): 3, 4 or 5
 };
 
 // Differentiable form: one rank marching its bricks, seen from V views.

@@ -796,11 +796,21 @@ constexpr int kFusedChunk = 6;
 constexpr int kFW = kFusedTileW, kFH = kFusedTileH, kFPix = kFW * kFH;
 static_assert(kBlock % kFPix == 0 && kFPix % 64 == 0, "items of one batch share a rank");
 
-__device__ __forceinline__ void fused_pixel(int idx, int &lx, int &ly) {
-  lx = (idx >> 5) * 8 + (idx & 7);
-  ly = (idx >> 3) & 3;
-}
-__device__ __forceinline__ int fused_index(int lx, int ly) { return (lx >> 3) * 32 + ly * 8 + (lx & 7); }
+// A fused tile of 64 pixels, 8 << (3 - hl) wide and 1 << hl high (8x8, 16x4
+// or 32x2): item idx is pixel (lx, ly) of its 8-wide column block, row-major
+// inside the block, so the 64 items of a wave are an 8 x (1 << hl) patch.
+struct TileShape {
+  int wl, hl;  // log2 of width and height
+  __device__ int w() const { return 1 << wl; }
+  __device__ int h() const { return 1 << hl; }
+  __device__ void pixel(int idx, int &lx, int &ly) const {
+    lx = (idx >> (3 + hl)) * 8 + (idx & 7);
+    ly = (idx >> 3) & ((1 << hl) - 1);
+  }
+  __device__ int index(int lx, int ly) const { return ((lx >> 3) << (3 + hl)) | (ly << 3) | (lx & 7); }
+};
+constexpr int kFusedWl = 4, kFusedHl = 2;  // the single fused frame's 16x4
+static_assert((1 << kFusedWl) == kFW && (1 << kFusedHl) == kFH, "fused tile shape");
 
 struct FusedStage {
   float4 col[kFusedChunk][kFPix];
@@ -856,14 +866,15 @@ __device__ __forceinline__ void fused_clear(const MarchArgs &a, long long pixel)
 // repeatedly on the same `fs` when a barrier separates the calls.
 template <int G, class Src>
 __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, FusedStage &fs,
-                                           const MarchTables &tb, int bx, int by) {
+                                           const MarchTables &tb, TileShape ts, int bx, int by) {
+  const int tw = ts.w(), th = ts.h();
   const int t = threadIdx.x;
   if (t == 0) {
     int n = 0;
     for (int k = 0; k < a.P; ++k) {
       const int s = src.order(k);
       const int32_t *rc = src.rect(s);
-      if (bx < rc[2] && bx + kFW > rc[0] && by < rc[3] && by + kFH > rc[1]) {
+      if (bx < rc[2] && bx + tw > rc[0] && by < rc[3] && by + th > rc[1]) {
         for (int i = 0; i < 4; ++i) fs.rc[n][i] = rc[i];
         fs.ranks[n++] = s;
       }
@@ -875,7 +886,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
   const int nranks = fs.nranks;  // block-uniform
   float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
   const float inv_spp = a.inv_spp;
-  const int own = fused_index(t % kFW, (t / kFW) % kFH);  // pixel of thread t < kFPix
+  const int own = ts.index(t & (tw - 1), (t >> ts.wl) & (th - 1));  // pixel of thread t < kFPix
   for (int c0 = 0; c0 < nranks; c0 += kFusedChunk) {
     const int nc = min(kFusedChunk, nranks - c0);
     __syncthreads();  // previous chunk blended
@@ -895,7 +906,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
       const int r = item / kFPix, idx = item - r * kFPix;
       const int slot = fs.ranks[c0 + r];
       int lx, ly;
-      fused_pixel(idx, lx, ly);
+      ts.pixel(idx, lx, ly);
       const int px = bx + lx, py = by + ly;
       if (px >= a.W || py >= a.H || !in_rc(fs.rc[c0 + r], px, py) || !island_pixel(a, px, py))
         return;
@@ -919,7 +930,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
   }
   __syncthreads();
   if (t >= kFPix) return;
-  const int px = bx + t % kFW, py = by + t / kFW;
+  const int px = bx + (t & (tw - 1)), py = by + (t >> ts.wl);
   if (px >= a.W || py >= a.H) return;
   const long long pixel = src.out_base() + px + (long long)a.W * py;
   if (a.rgba) a.rgba[pixel] = C;
@@ -950,7 +961,8 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
   }
   const int item = (int)blockIdx.x;  // row-major tiles of the box
   const MarchTables tb = load_tables(a, s_tf);
-  fused_tile<G>(a, src, fs, tb, (u.x + item % u.z) * kFW, (u.y + item / u.z) * kFH);
+  fused_tile<G>(a, src, fs, tb, TileShape{kFusedWl, kFusedHl}, (u.x + item % u.z) * kFW,
+                (u.y + item / u.z) * kFH);
 }
 
 // Batched form: V views of the same ranks in one launch.  The active work
@@ -971,8 +983,8 @@ __device__ __forceinline__ void batch_active(const BatchArgs &b, int item, int &
   view = lo;
   const int k = item - b.box_off[view];
   const int32_t *bx4 = b.boxes + 4 * view;
-  bx = (bx4[0] + k % bx4[2]) * kFW;
-  by = (bx4[1] + k / bx4[2]) * kFH;
+  bx = (bx4[0] + k % bx4[2]) << b.tile_wl;
+  by = (bx4[1] + k / bx4[2]) << (6 - b.tile_wl);
 }
 
 __device__ __forceinline__ void batch_fill(const BatchArgs &b, int fb, int nfb) {
@@ -980,8 +992,9 @@ __device__ __forceinline__ void batch_fill(const BatchArgs &b, int fb, int nfb) 
       b.m.W, b.m.H, b.V, fb, nfb,
       [&](int view, int &x0, int &y0, int &x1, int &y1) {
         const int32_t *bx4 = b.boxes + 4 * view;
-        x0 = bx4[0] * kFW, y0 = bx4[1] * kFH;
-        x1 = (bx4[0] + bx4[2]) * kFW, y1 = (bx4[1] + bx4[3]) * kFH;
+        const int wl = b.tile_wl, hl = 6 - b.tile_wl;
+        x0 = bx4[0] << wl, y0 = bx4[1] << hl;
+        x1 = (bx4[0] + bx4[2]) << wl, y1 = (bx4[1] + bx4[3]) << hl;
       },
       [&](int view, int px, int py) {
         fused_clear(b.m, (long long)view * b.m.W * b.m.H + px + (long long)b.m.W * py);
@@ -1002,7 +1015,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
     const MarchTables tb = load_tables(b.m, s_tf);
     int view, bx, by;
     batch_active(b, item, view, bx, by);
-    fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
+    fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
     return;
   }
   // XCD x works through part x of every view (a contiguous band of each
@@ -1030,7 +1043,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
       const int item = b.box_off[lo] + (int)((long long)n * x / kXcds) + (i - xo[lo]);
       int view, bx, by;
       batch_active(b, item, view, bx, by);
-      fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, bx, by);
+      fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
     }
   }
   batch_fill(b, (int)blockIdx.x, (int)gridDim.x);

@@ -627,8 +627,8 @@ def batch_bench(args, timed):
            "speedup_vs_per_view_graph": round(res_seq["per_view"]["ms_graph"] / ms_batch, 2),
            "speedup_vs_per_brick_graph": round(res_seq["per_brick"]["ms_graph"] / ms_batch, 2),
            "autotune": {"chosen": sched, "best_ms": round(best, 4),
-                        "table": [{"lanes_per_ray": t, "persistent_blocks": pb, "ms": round(m, 4)}
-                                  for t, pb, m in table]}}
+                        "table": [{"tile_w": tw, "lanes_per_ray": t, "persistent_blocks": pb,
+                                   "ms": round(m, 4)} for tw, t, pb, m in table]}}
     if not args.no_cpu:
         res["fp64"] = fp64_batched(bw, out)
     b.close()

@@ -341,13 +341,19 @@ int aloha_batch_create(aloha_ctx *ctx, const aloha_render_params *params, int32_
 int aloha_batch_destroy(aloha_batch *batch);
 /* Schedule: lanes_per_ray 1, 2, 4 or 8 (the fused frame's lane groups,
  * aloha_set_march_schedule); persistent_blocks 0 = one workgroup per active
- * 16x4 tile, else that many workgroups pulling (view, tile) items from a
- * counter.  Images do not depend on it.  Drops a captured graph. */
+ * tile, else that many workgroups pulling (view, tile) items from
+ * per-XCD counters.  Images do not depend on it.  Drops a captured graph. */
 int aloha_batch_set_schedule(aloha_batch *batch, int32_t lanes_per_ray, int32_t persistent_blocks);
 int aloha_batch_get_schedule(const aloha_batch *batch, int32_t *lanes_per_ray,
                              int32_t *persistent_blocks, int32_t *captured);
-/* Time every schedule (4 x 4) over reps launches, keep the fastest; table
- * (optional) receives {lanes_per_ray, persistent_blocks, ms} per candidate.
+/* Tile of 64 pixels a workgroup composites: tile_w 8 (8x8), 16 (16x4, the
+ * default and the single fused frame's) or 32 (32x2).  Images do not depend
+ * on it.  Synchronous; drops a captured graph. */
+int aloha_batch_set_tile(aloha_batch *batch, int32_t tile_w);
+int aloha_batch_get_tile(const aloha_batch *batch, int32_t *tile_w);
+/* Time every schedule (3 tiles x 4 lanes x 4 block counts) over reps
+ * launches, keep the fastest; table (optional, cap rows) receives
+ * {tile_w, lanes_per_ray, persistent_blocks, ms} per candidate.
  * Synchronous. */
 int aloha_batch_autotune(aloha_batch *batch, int32_t reps, float *best_ms, float *table,
                          int32_t cap, int32_t *n);

@@ -44,15 +44,21 @@ def test_every_schedule_and_graph_replay_bit_exact(wl):
     V = len(wl.cams)
     out = torch.empty((V, wl.H, wl.W, 4), dtype=torch.float32, device="cuda")
     b = A.Batch(ctx, wl.params(), wl.W, wl.H, wl.cams, ranks, wl.orders, out_rgba=out)
-    for lanes in (1, 2, 4, 8):
-        for blocks in (0, 8, 64):
-            out.fill_(-1.0)
-            b.set_schedule(lanes, blocks)
-            b.launch()
-            torch.cuda.synchronize()
-            assert torch.equal(out, ref), (lanes, blocks)
+    for tile_w in (8, 16, 32):
+        b.set_tile(tile_w)
+        for lanes in (1, 2, 4, 8):
+            for blocks in (0, 8, 64):
+                out.fill_(-1.0)
+                b.set_schedule(lanes, blocks)
+                b.launch()
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref), (tile_w, lanes, blocks)
+    with pytest.raises(A.AlohaError):
+        b.set_tile(12)
     best, table = b.autotune(reps=2)
-    assert len(table) == 16 and best > 0
+    assert len(table) == 48 and best > 0
+    assert {r[0] for r in table} == {8, 16, 32}
+    assert b.schedule()["tile_w"] in (8, 16, 32)
     b.capture()
     assert b.schedule()["captured"]
     for rep in range(3):
@@ -106,7 +112,8 @@ def test_batch_every_output_spp_islands(wl):
     got = {k: torch.empty_like(t) for k, t in ref.items()}
     b = A.Batch(ctx, prm, W, H, wl.cams, ranks, wl.orders, out_rgba=got["rgba"],
                 out_rgba8=got["rgba8"], nsamples=got["ns"])
-    for lanes in (1, 2, 4, 8):
+    for tile_w, lanes in ((16, 1), (16, 2), (8, 4), (32, 8), (8, 1), (32, 4)):
+        b.set_tile(tile_w)
         for blocks in (0, 64):
             for t in got.values():
                 t.fill_(-1)
@@ -114,7 +121,7 @@ def test_batch_every_output_spp_islands(wl):
             b.launch()
             torch.cuda.synchronize()
             for k in ref:
-                assert torch.equal(got[k], ref[k]), (k, lanes, blocks)
+                assert torch.equal(got[k], ref[k]), (k, tile_w, lanes, blocks)
     b.close()
     ctx.close()
 
@@ -141,9 +148,10 @@ def test_batch_multi_brick_and_empty_ranks(wl):
     out = torch.empty_like(ref)
     ns = torch.empty_like(ref_ns)
     b = A.Batch(ctx, wl.params(), W, H, wl.cams, ranks, orders, out_rgba=out, nsamples=ns)
-    for lanes, blocks in ((1, 0), (4, 0), (4, 64), (8, 8)):
+    for tile_w, lanes, blocks in ((16, 1, 0), (8, 4, 0), (32, 4, 64), (8, 8, 8)):
         out.fill_(float("nan"))
         ns.fill_(-1)
+        b.set_tile(tile_w)
         b.set_schedule(lanes, blocks)
         b.launch()
         torch.cuda.synchronize()
