@@ -375,6 +375,34 @@ class Context:
         self._check(lib().aloha_brick_bytes(self._h, C.byref(v)))
         return v.value
 
+    def set_u8_storage(self, keep_u8: bool) -> None:
+        """Storage of u8 bricks uploaded from now on: bytes (True; decoded
+        at sample time to the same floats, a quarter of the memory) or
+        float32 (False, the default)."""
+        self._check(lib().aloha_set_u8_storage(self._h, 1 if keep_u8 else 0))
+
+    def brick_update(self, brick_id: int, voxels) -> None:
+        """Replace an uploaded brick's voxels (same extent) from a cuda
+        tensor (uint8, int16 as u16 bits, or float32), asynchronously on the
+        context's stream; a brick kept as u8 takes uint8 only."""
+        import torch
+        if not (hasattr(voxels, "data_ptr") and getattr(voxels, "is_cuda", False)):
+            raise ValueError("voxels: a cuda tensor")
+        t = voxels.contiguous()
+        code = {torch.uint8: U8, torch.int16: U16, torch.float32: F32}[t.dtype]
+        n = C.c_int32()
+        self._check(lib().aloha_brick_count(self._h, C.byref(n)))
+        if not 0 <= brick_id < n.value:
+            raise ValueError(f"brick_id {brick_id} not uploaded")
+        self._check(lib().aloha_brick_update(self._h, brick_id, C.c_void_p(t.data_ptr()), code))
+        self._keep_update = t  # alive until the stream has read it
+
+    def brick_dtype(self, brick_id: int) -> int:
+        """How the brick is stored in HBM: U8 or F32."""
+        v = C.c_int32()
+        self._check(lib().aloha_brick_dtype(self._h, brick_id, C.byref(v)))
+        return v.value
+
     def set_tf(self, tf: np.ndarray) -> None:
         tf = np.ascontiguousarray(tf, dtype=np.float32)
         if tf.ndim != 2 or tf.shape[1] != 4:
@@ -469,6 +497,17 @@ class Context:
         self._check(lib().aloha_render_footprint(self._h, C.byref(params), W, H, len(ranks), offs,
                                                  ids, C.byref(v)))
         return v.value
+
+    def footprint_bricks(self, params: RenderParams, W: int, H: int,
+                         ranks: Sequence[Sequence[int]]) -> np.ndarray:
+        """Distinct voxels read per brick id (all uploaded bricks)."""
+        offs, ids = _rank_table(ranks)
+        n = C.c_int32()
+        self._check(lib().aloha_brick_count(self._h, C.byref(n)))
+        out = (C.c_uint64 * max(1, n.value))()
+        self._check(lib().aloha_render_footprint_bricks(self._h, C.byref(params), W, H, len(ranks),
+                                                        offs, ids, out, n.value))
+        return np.frombuffer(out, dtype=np.uint64)[:n.value].copy()
 
     def composite_over(self, rank_rgba, order: Sequence[int], W: int, H: int, out_rgba=None,
                        out_rgba8=None, rects=None) -> None:

@@ -101,6 +101,8 @@ _SIGS = [
     ("aloha_brick_upload", C.c_int, [_P, C.POINTER(BrickDesc), _P, _I, _I, _IP]),
     ("aloha_brick_count", C.c_int, [_P, _IP]),
     ("aloha_brick_bytes", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("aloha_set_u8_storage", C.c_int, [_P, _I]),
+    ("aloha_brick_dtype", C.c_int, [_P, _I, _IP]),
     ("aloha_tf_set", C.c_int, [_P, _P, _I]),
     ("aloha_camera_set", C.c_int, [_P, _FP, _FP, _FP, _FP]),
     ("aloha_render_ranks", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, C.POINTER(Images)]),
@@ -129,6 +131,8 @@ _SIGS = [
                                        C.POINTER(Images), _P, _P, _I, _FP, _IP]),
     ("aloha_render_footprint", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP,
                                          C.POINTER(C.c_uint64)]),
+    ("aloha_render_footprint_bricks", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP,
+                                                _IP, C.POINTER(C.c_uint64), _I]),
     ("aloha_composite_over", C.c_int, [_P, _P, _I, _IP, _I, _I, _P, _P]),
     ("aloha_composite_over_sparse", C.c_int, [_P, _P, _I, _IP, _IP, _I, _I, _P, _P]),
     ("aloha_rank_rects", C.c_int, [_P, C.POINTER(RenderParams), _I, _I, _I, _IP, _IP, _IP]),

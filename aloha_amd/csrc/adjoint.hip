@@ -10,21 +10,29 @@
 //     t += dt, so the adjoint can visit them in reverse order exactly;
 //   * the per-sample opacity is clamped to 0.9999 so transmittance can be
 //     recovered backward by division.
-// The adjoint stores nothing per sample: from the forward's final colour
-// and alpha (and per-pixel sample count) it re-derives each brick visit,
+// The forward stores no per-sample state: from its final colour and alpha
+// (and per-pixel sample count) the adjoint re-derives each brick visit,
 // walks its samples from last to first, recovers T_k = T_{k+1}/(1-a_k)
 // (Weiss & Westermann, "Differentiable Direct Volume Rendering", IEEE VIS
 // 2021, the inversion method) and back-propagates through compositing,
 // shading (N = g/|g+1e-4|), the TF lookup and all seven trilinear taps.
-// dTF is summed in registers over each run of a ray's samples between the
-// same two TF entries, then in LDS per workgroup, and flushed with one global
-// atomic per touched entry; dDensity with global float atomics on the brick's
-// unpadded voxel grid (clamped corners receive the weight of the clamped-away
-// neighbour, exactly as the forward reads them), summed per voxel of the
-// shared 20-voxel footprint and then per workgroup in an LDS hash before the
-// atomics.  The forward marches a ray with a lane group (4 lanes, 4 lattice
-// samples a step); the adjoint keeps one lane per ray (its reverse walk is a
-// sequential recurrence).
+// The adjoint runs in stages:
+//   * diff_wave_slots_kernel gives each wave of pixels its sample slots;
+//   * diff_resample_kernel evaluates every sample the forward took once more
+//     (value tap and gradient, 16 bytes per slot; lane groups of 4);
+//   * diff_backward_kernel, one lane per ray, walks the samples in reverse
+//     and writes one record per sample (position, d/d value tap, d/d gradient,
+//     value, d/d TF colour and opacity), marking each run of samples that
+//     falls into one 16^3 spatial bin at its head slot;
+//   * diff_run_rank_kernel / diff_scan_kernel / diff_run_list_kernel place
+//     the runs into per-bin slot lists (integer atomics, one per distinct bin
+//     of a wave);
+//   * diff_bin_kernel scatters each bin's records onto an 18^3 tile and the
+//     TF entries as 64-bit fixed-point LDS sums (clamped corners receive the
+//     weight of the clamped-away neighbour, exactly as the forward reads
+//     them); diff_fold_kernel folds the tiles into dDensity and the TF
+//     kernels reduce the bins' partials, both in a fixed order.
+// No float atomics anywhere, so the gradients are bit-reproducible.
 #include <algorithm>
 #include <climits>
 #include <hip/hip_runtime.h>
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void diff_forward_group_kernel(const DiffAr
   uint32_t ns = 0, rng = 0;
   float tmin = 0.f, t1 = 0.f, j = 0.f, k0 = 0.f;
   int kk = 0;  // the segment's next lattice step
-  BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
+  BrickView bv{nullptr, 0u, 0u, 0, 0, 0, nullptr, false};
   float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
   while (__any(state != kOver)) {
     // a group without a ray takes the tile's next pixel
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void diff_forward_group_kernel(const DiffAr
         t1 = bt1;
         kk = 0;
         const DevBrick &db = a.bricks[hit];
-        bv = brick_view(db);
+        bv = brick_view(db, nullptr);  // f32 bricks only (checked by the host)
         lo0 = db.vox_lo[0];
         lo1 = db.vox_lo[1];
         lo2 = db.vox_lo[2];
@@ -256,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void diff_forward_group_kernel(const DiffAr
       const float y = (org[1] + t * d[1]) - lo1;
       const float z = (org[2] + t * d[2]) - lo2;
       Sample sm;
-      if (!sample_fast(bv, x, y, z, sm)) sm = sample_exact<false>(bv, nullptr, a.bricks[0], x, y, z);
+      if (!sample_fast<false>(bv, x, y, z, sm))
+        sm = sample_exact<false, false>(bv, nullptr, a.bricks[0], x, y, z);
       const float ex = sm.gx + 1e-4f, ey = sm.gy + 1e-4f, ez = sm.gz + 1e-4f;
       const float len = sqrtf(ex * ex + ey * ey + ez * ez);
       const float nx = sm.gx / len, ny = sm.gy / len, nz = sm.gz / len;
@@ -567,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void diff_resample_kernel(const DiffArgs a,
   float4 *out_s = b.samples;  // the slot of the ray's last sample
   float tmin = 0.f, t1 = 0.f, j = 0.f, k0 = 0.f;
   int kk = 0;
-  BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
+  BrickView bv{nullptr, 0u, 0u, 0, 0, 0, nullptr, false};
   float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
   while (__any(state != kOver)) {
     // a group without a ray takes the tile's next pixel
@@ -622,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void diff_resample_kernel(const DiffArgs a,
         t1 = bt1;
         kk = 0;
         const DevBrick &db = a.bricks[hit];
-        bv = brick_view(db);
+        bv = brick_view(db, nullptr);  // f32 bricks only (checked by the host)
         lo0 = db.vox_lo[0];
         lo1 = db.vox_lo[1];
         lo2 = db.vox_lo[2];
@@ -640,7 +649,8 @@ __global__ __launch_bounds__(kBlock) void diff_resample_kernel(const DiffArgs a,
       const float y = (org[1] + t * d[1]) - lo1;
       const float z = (org[2] + t * d[2]) - lo2;
       Sample sm;
-      if (!sample_fast(bv, x, y, z, sm)) sm = sample_exact<false>(bv, nullptr, a.bricks[0], x, y, z);
+      if (!sample_fast<false>(bv, x, y, z, sm))
+        sm = sample_exact<false, false>(bv, nullptr, a.bricks[0], x, y, z);
       out_s[-64 * (ptrdiff_t)(done + gl)] = make_float4(sm.val, sm.gx, sm.gy, sm.gz);
     }
     const uint64_t took = __ballot(take);

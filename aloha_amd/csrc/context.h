@@ -27,7 +27,9 @@ struct aloha_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::vector<aloha::DevBrick> bricks;
-  std::vector<float *> brick_mem;
+  std::vector<void *> brick_mem;
+  // u8 voxels uploaded from now on are kept as bytes (else widened to f32)
+  bool u8_storage = false;
   std::vector<aloha_brick_desc> descs;
   aloha::DevBrick *d_bricks = nullptr;
   size_t d_bricks_cap = 0;
@@ -40,10 +42,10 @@ struct aloha_ctx {
   std::vector<TriGeom> geoms;
   // lanes per ray of the per-rank and fused march launches (1, 2, 4, 8)
   int march_lanes = 1, fused_lanes = 1;
-  // persistent per-rank launch: waves per lanes-per-ray option (and the
-  // dynamic LDS size they were computed for)
-  int q_waves[4] = {0, 0, 0, 0};
-  size_t q_waves_lds[4] = {0, 0, 0, 0};
+  // persistent per-rank launch: waves per lanes-per-ray option, without and
+  // with the u8 decode (and the dynamic LDS size they were computed for)
+  int q_waves[8] = {};
+  size_t q_waves_lds[8] = {};
   // bumped whenever the device address of the brick table or the TF, or the
   // TF's entry count, changes: a batch launch compares it with the value its
   // launch arguments were made with

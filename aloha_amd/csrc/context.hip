@@ -201,7 +201,10 @@ int build_march(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t
   a.bricks = ctx->d_bricks;
   a.tf = ctx->d_tf;
   for (int s = 0; s <= P; ++s) a.rank_off[s] = rank_offsets[s];
-  for (int k = 0; k < nb; ++k) a.rank_bricks[k] = brick_ids[k];
+  for (int k = 0; k < nb; ++k) {
+    a.rank_bricks[k] = brick_ids[k];
+    if (ctx->bricks[brick_ids[k]].dtype == ALOHA_U8) a.u8_lut = 1;
+  }
   for (int s = 0; s < P; ++s)
     rank_rect(ctx->cam, ctx->bricks, brick_ids + rank_offsets[s],
               rank_offsets[s + 1] - rank_offsets[s], W, H, a.rect[s]);
@@ -252,7 +255,7 @@ extern "C" int aloha_ctx_destroy(aloha_ctx *ctx) {
   for (aloha_batch *bt : ctx->batches) batch_detach(bt);
   ctx->batches.clear();
   hipStreamSynchronize(ctx->stream);
-  for (float *m : ctx->brick_mem) hipFree(m);
+  for (void *m : ctx->brick_mem) hipFree(m);
   if (ctx->d_bricks) hipFree(ctx->d_bricks);
   if (ctx->d_tf) hipFree(ctx->d_tf);
   if (ctx->diff_bins) hipFree(ctx->diff_bins);
@@ -324,9 +327,14 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
                 "aloha_brick_upload: brick of %lld padded voxels (slice %lld) exceeds 2^31 (2^24)",
                 (long long)padded, (long long)(px * py));
   const size_t src_bytes = (size_t)n[0] * n[1] * n[2] * (size_t)dtype;
+  // u8 voxels stay bytes in HBM when the context keeps them (a quarter of
+  // the f32 footprint; the kernels decode through u/255.f), else f32
+  const int store = dtype == ALOHA_U8 && ctx->u8_storage ? ALOHA_U8 : ALOHA_F32;
+  const uint64_t mem_bytes =
+      store == ALOHA_U8 ? (uint64_t)padded + kU8Slack : (uint64_t)padded * sizeof(float);
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  float *mem = nullptr;
-  HIP_CHECK(ctx, dev_malloc((void **)&mem, (size_t)padded * sizeof(float)));
+  void *mem = nullptr;
+  HIP_CHECK(ctx, dev_malloc(&mem, mem_bytes));
   const void *src = voxels;
   void *staging = nullptr;
   if (!src_on_device) {
@@ -340,7 +348,8 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
     }
     src = staging;
   }
-  hipError_t e = launch_pad_brick(src, dtype, n, mem, (int32_t)px, (int32_t)py, (int32_t)pz, ctx->stream);
+  hipError_t e = launch_pad_brick(src, dtype, n, mem, store, (int32_t)px, (int32_t)py, (int32_t)pz,
+                                  ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (staging) hipFree(staging);
   if (e != hipSuccess) {
@@ -356,12 +365,12 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
   }
   b.px = (int32_t)px;
   b.pxy = (int32_t)(px * py);
-  b.dtype = ALOHA_F32;
+  b.dtype = store;
   b.origin = mem;  // padded element (-1,-1,-1)
   ctx->bricks.push_back(b);
   ctx->brick_mem.push_back(mem);
   ctx->descs.push_back(*desc);
-  ctx->brick_bytes += (uint64_t)padded * sizeof(float);
+  ctx->brick_bytes += mem_bytes;
   ctx->bricks_dirty = true;
   // the device brick table follows at once, so render calls find it current
   // (and aloha_render_frame_noalloc never has to grow it); if it cannot,
@@ -370,7 +379,7 @@ extern "C" int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc,
     ctx->bricks.pop_back();
     ctx->brick_mem.pop_back();
     ctx->descs.pop_back();
-    ctx->brick_bytes -= (uint64_t)padded * sizeof(float);
+    ctx->brick_bytes -= mem_bytes;
     hipFree(mem);
     return rc;
   }
@@ -387,6 +396,21 @@ extern "C" int aloha_brick_count(const aloha_ctx *ctx, int32_t *count) {
 extern "C" int aloha_brick_bytes(const aloha_ctx *ctx, uint64_t *bytes) {
   if (!ctx || !bytes) return fail(nullptr, ALOHA_ERR_ARG, "aloha_brick_bytes: NULL");
   *bytes = ctx->brick_bytes;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_set_u8_storage(aloha_ctx *ctx, int32_t keep_u8) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_set_u8_storage: ctx is NULL");
+  ctx->u8_storage = keep_u8 != 0;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_brick_dtype(const aloha_ctx *ctx, int32_t brick_id, int32_t *dtype) {
+  if (!ctx || !dtype) return fail(nullptr, ALOHA_ERR_ARG, "aloha_brick_dtype: NULL");
+  if (brick_id < 0 || brick_id >= (int)ctx->bricks.size())
+    return fail(const_cast<aloha_ctx *>(ctx), ALOHA_ERR_ARG, "aloha_brick_dtype: bad brick id %d",
+                brick_id);
+  *dtype = ctx->bricks[brick_id].dtype;
   return ALOHA_OK;
 }
 
@@ -447,11 +471,11 @@ int prepare_ranks(aloha_ctx *ctx, const MarchArgs &a, int lanes, RanksArgs &q, i
   q.box = make_int4(0, 0, 0, 0);
   q.compact = 0;
   // persistent grid: as many one-wave blocks as the GPU holds at once
-  const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
+  const int li = (lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3) + (a.u8_lut ? 4 : 0);
   const size_t lds = ranks_lds_bytes(a);
   if (ctx->q_waves[li] == 0 || ctx->q_waves_lds[li] != lds) {
     int per_cu = 0, cus = 0;
-    HIP_CHECK(ctx, ranks_occupancy(lanes, lds, &per_cu));
+    HIP_CHECK(ctx, ranks_occupancy(lanes, a.u8_lut != 0, lds, &per_cu));
     HIP_CHECK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     ctx->q_waves[li] = std::max(1, per_cu) * std::max(1, cus);
     ctx->q_waves_lds[li] = lds;
@@ -794,12 +818,12 @@ extern "C" int aloha_autotune_march(aloha_ctx *ctx, const aloha_render_params *p
   return ALOHA_OK;
 }
 
-extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params,
-                                      int32_t W, int32_t H, int32_t P,
-                                      const int32_t *rank_offsets, const int32_t *brick_ids,
-                                      uint64_t *touched_voxels) {
-  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_footprint: ctx is NULL");
-  if (!touched_voxels) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_footprint: NULL output");
+namespace {
+// The footprint pass: per brick id, the distinct voxels the march of the
+// launch reads (bitset per brick, counted on the host).
+int footprint(aloha_ctx *ctx, const char *name, const aloha_render_params *params, int32_t W,
+              int32_t H, int32_t P, const int32_t *rank_offsets, const int32_t *brick_ids,
+              std::vector<uint64_t> &per_brick) {
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
   MarchArgs a;
   if (int rc = build_march(ctx, params, W, H, P, rank_offsets, brick_ids, a)) return rc;
@@ -808,7 +832,7 @@ extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params 
   for (auto &b : ctx->bricks) {
     const uint64_t nv = (uint64_t)b.n[0] * b.n[1] * b.n[2];
     if (words + (nv + 31) / 32 > 0x7fffffffull)
-      return fail(ctx, ALOHA_ERR_LIMIT, "aloha_render_footprint: bitset too large");
+      return fail(ctx, ALOHA_ERR_LIMIT, "%s: bitset too large", name);
     b.touched_word = (int32_t)words;
     words += (nv + 31) / 32;
   }
@@ -824,10 +848,48 @@ extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params 
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipMemcpy(host.data(), bits, words * 4, hipMemcpyDeviceToHost);
   hipFree(bits);
-  if (e != hipSuccess) return fail(ctx, ALOHA_ERR_HIP, "aloha_render_footprint: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(ctx, ALOHA_ERR_HIP, "%s: %s", name, hipGetErrorString(e));
+  per_brick.assign(ctx->bricks.size(), 0);
+  for (size_t k = 0; k < ctx->bricks.size(); ++k) {
+    const auto &b = ctx->bricks[k];
+    const uint64_t w0 = (uint64_t)b.touched_word;
+    const uint64_t nw = ((uint64_t)b.n[0] * b.n[1] * b.n[2] + 31) / 32;
+    for (uint64_t w = w0; w < w0 + nw; ++w) per_brick[k] += (uint64_t)__builtin_popcount(host[w]);
+  }
+  return ALOHA_OK;
+}
+}  // namespace
+
+extern "C" int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params,
+                                      int32_t W, int32_t H, int32_t P,
+                                      const int32_t *rank_offsets, const int32_t *brick_ids,
+                                      uint64_t *touched_voxels) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_footprint: ctx is NULL");
+  if (!touched_voxels) return fail(ctx, ALOHA_ERR_ARG, "aloha_render_footprint: NULL output");
+  std::vector<uint64_t> per;
+  if (int rc = footprint(ctx, "aloha_render_footprint", params, W, H, P, rank_offsets, brick_ids,
+                         per))
+    return rc;
   uint64_t cnt = 0;
-  for (uint32_t w : host) cnt += (uint64_t)__builtin_popcount(w);
+  for (uint64_t v : per) cnt += v;
   *touched_voxels = cnt;
+  return ALOHA_OK;
+}
+
+extern "C" int aloha_render_footprint_bricks(aloha_ctx *ctx, const aloha_render_params *params,
+                                             int32_t W, int32_t H, int32_t P,
+                                             const int32_t *rank_offsets,
+                                             const int32_t *brick_ids, uint64_t *touched,
+                                             int32_t cap) {
+  if (!ctx) return fail(nullptr, ALOHA_ERR_ARG, "aloha_render_footprint_bricks: ctx is NULL");
+  if (!touched || cap < (int32_t)ctx->bricks.size())
+    return fail(ctx, ALOHA_ERR_ARG, "aloha_render_footprint_bricks: need %d outputs",
+                (int)ctx->bricks.size());
+  std::vector<uint64_t> per;
+  if (int rc = footprint(ctx, "aloha_render_footprint_bricks", params, W, H, P, rank_offsets,
+                         brick_ids, per))
+    return rc;
+  for (size_t k = 0; k < per.size(); ++k) touched[k] = per[k];
   return ALOHA_OK;
 }
 
@@ -955,8 +1017,11 @@ extern "C" int aloha_brick_update(aloha_ctx *ctx, int32_t brick_id, const void *
   if (dtype != ALOHA_U8 && dtype != ALOHA_U16 && dtype != ALOHA_F32)
     return fail(ctx, ALOHA_ERR_ARG, "aloha_brick_update: unknown dtype %d", dtype);
   const DevBrick &b = ctx->bricks[brick_id];
+  if (b.dtype == ALOHA_U8 && dtype != ALOHA_U8)
+    return fail(ctx, ALOHA_ERR_ARG,
+                "aloha_brick_update: brick %d keeps u8 voxels; its update must be u8 too", brick_id);
   HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  HIP_CHECK(ctx, launch_pad_brick(d_voxels, dtype, b.n, ctx->brick_mem[brick_id], b.px,
+  HIP_CHECK(ctx, launch_pad_brick(d_voxels, dtype, b.n, ctx->brick_mem[brick_id], b.dtype, b.px,
                                   b.pxy / b.px, b.n[2] + kPadLo + kPadHi, ctx->stream));
   return ALOHA_OK;
 }
@@ -980,9 +1045,14 @@ int build_diff(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t 
     return fail(ctx, ALOHA_ERR_LIMIT, "diff: V=%d outside [1,%d]", V, kMaxViews);
   if (nb < 1 || nb > kMaxDiffBricks || !brick_ids)
     return fail(ctx, ALOHA_ERR_LIMIT, "diff: nb=%d outside [1,%d]", nb, kMaxDiffBricks);
-  for (int k = 0; k < nb; ++k)
+  for (int k = 0; k < nb; ++k) {
     if (brick_ids[k] < 0 || brick_ids[k] >= (int)ctx->bricks.size())
       return fail(ctx, ALOHA_ERR_ARG, "diff: brick id %d not uploaded", brick_ids[k]);
+    if (ctx->bricks[brick_ids[k]].dtype != ALOHA_F32)
+      return fail(ctx, ALOHA_ERR_ARG,
+                  "diff: brick %d keeps u8 voxels; the differentiable form needs float32 bricks "
+                  "(upload it after aloha_set_u8_storage(ctx, 0))", brick_ids[k]);
+  }
   for (int i = 0; i < 12 * V; ++i)
     if (!std::isfinite(cams[i])) return fail(ctx, ALOHA_ERR_ARG, "diff: non-finite camera");
   for (int v = 0; v < V; ++v)

@@ -9,13 +9,19 @@
 
 namespace aloha {
 
-// Brick storage in HBM: f32 voxels with a replicated border of 1 voxel below
-// and 3 above on every axis, so that every index a trilinear tap or the
-// 3-wide gradient footprint can form (-1 .. n+2) reads the clamp-to-edge
-// value without a per-load clamp.  `origin` points at padded voxel (-1,-1,-1)
-// (the allocation start), so tap offsets are non-negative 24-bit products.
+// Brick storage in HBM: f32 voxels, or u8 voxels kept as bytes (decoded at
+// sample time through a 256-entry table of u/255.f, the floats Volume::get
+// makes), with a replicated border of 1 voxel below and 3 above on every
+// axis, so that every index a trilinear tap or the 3-wide gradient footprint
+// can form (-1 .. n+2) reads the clamp-to-edge value without a per-load
+// clamp.  `origin` points at padded voxel (-1,-1,-1) (the allocation start),
+// so tap offsets are non-negative 24-bit products.
 constexpr int kPadLo = 1;
 constexpr int kPadHi = 3;
+// A u8 brick's 3-voxel rows are read 4 bytes at a time: the last row of the
+// last slice reads one byte past the padded grid, so its allocation carries
+// this many bytes more.
+constexpr int kU8Slack = 16;
 
 struct DevBrick {
   float space_lo[3], space_hi[3];  // spaceRange (ray clipping)
@@ -23,9 +29,9 @@ struct DevBrick {
   int32_t n[3];                    // voxel extent voxelRange.size()
   int32_t px;                      // padded x pitch, elements
   int32_t pxy;                     // padded slice pitch, elements
-  int32_t dtype;                   // storage encoding (ALOHA_F32)
+  int32_t dtype;                   // storage encoding: ALOHA_F32 or ALOHA_U8
   int32_t touched_word;            // footprint pass: first 32-bit word of this brick's bitset
-  const float *origin;
+  const void *origin;              // float or uint8_t voxels, by dtype
 };
 
 // Kernel arguments of one render launch (all virtual ranks at once).
@@ -37,7 +43,7 @@ struct MarchArgs {
   int32_t island_id, num_islands, jitter_mode, flags;
   int32_t P, tf_n;
   uint32_t heat_clock;  // clock delta of every pixel under ALOHA_FLAG_HEAT_CLOCK
-  int32_t pad0;
+  int32_t u8_lut;       // nonzero: a brick of the launch stores u8 voxels (the decode table is staged)
   const DevBrick *bricks;
   const float4 *tf;
   float4 *rgba;
@@ -213,11 +219,14 @@ hipError_t launch_march_ranks(const MarchArgs &a, const RanksArgs &q, int group,
                               hipStream_t s);
 // Dynamic LDS of the march launches: the TF with its zero borders
 // (tf_n + 4 float4), a 64-byte brick record per rank_bricks entry,
-// rank_off[0..P] and the P pixel rectangles.
+// rank_off[0..P], the P pixel rectangles and, when a brick of the launch
+// stores u8 voxels, the 256-entry decode table.
 constexpr size_t kLdsBrickBytes = 64;
+constexpr int kU8Lut = 256;
 __host__ __device__ inline size_t march_tables_bytes(const MarchArgs &a) {
   return sizeof(float4) * (size_t)(a.tf_n + 4) + kLdsBrickBytes * (size_t)a.rank_off[a.P] +
-         sizeof(int32_t) * (size_t)(a.P + 1) + sizeof(int32_t) * 4 * (size_t)a.P;
+         sizeof(int32_t) * (size_t)(a.P + 1) + sizeof(int32_t) * 4 * (size_t)a.P +
+         (a.u8_lut ? sizeof(float) * kU8Lut : 0);
 }
 // The persistent per-rank launch adds a list of 64 ray records (48 B each)
 // after the tables, at the next 16-byte boundary past them, then the run
@@ -228,12 +237,13 @@ __host__ __device__ inline size_t ranks_lds_bytes(const MarchArgs &a) {
          sizeof(int32_t) * (size_t)(2 * a.P + 1);
 }
 // resident one-wave blocks per CU of the persistent per-rank kernel
-hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu);
+hipError_t ranks_occupancy(int group, bool u8, size_t lds, int *blocks_per_cu);
 hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s);
 // lanes: lanes per ray (1, 2, 4, 8); persistent_blocks 0: one block per active tile
 hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s);
-hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
-                            int32_t px, int32_t py, int32_t pz, hipStream_t s);
+// decode (to f32 storage) or copy (u8 to u8 storage) and pad one brick
+hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], void *dst_base,
+                            int store_dtype, int32_t px, int32_t py, int32_t pz, hipStream_t s);
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s);
 // backward in stages: count samples per bin and scan, then the march that
 // writes records and the per-bin bounds, then bins into tiles and tiles into

@@ -44,21 +44,22 @@ constexpr int kTileW = 16, kTileH = 16;
 // the brick selection's slab test, so the caller passes that result.
 template <bool kFoot>
 __device__ __forceinline__ void integrate(const MarchArgs &a, const float4 *s_tf,
-                                          const DevBrick &db, const float o[3],
+                                          const float *lut, const DevBrick &db, const float o[3],
                                           const float d[3], float t0, float t1, float jitter,
                                           float4 &c, uint32_t &ns) {
   const float dt = a.dt;
   const float segs = floorf((t0 - dt * jitter) / dt);
   float t = dt * (jitter + segs);
   if (t < t0) t += dt;
-  const BrickView b = brick_view(db);
+  const BrickView b = brick_view(db, lut);
   for (; t < t1 && c.w < .99f; t += dt) {
     ++ns;
     const float x = (o[0] + t * d[0]) - db.vox_lo[0];
     const float y = (o[1] + t * d[1]) - db.vox_lo[1];
     const float z = (o[2] + t * d[2]) - db.vox_lo[2];
     Sample s;
-    if (kFoot || !sample_fast(b, x, y, z, s)) s = sample_exact<kFoot>(b, a.touched, db, x, y, z);
+    if (kFoot || !sample_fast<true>(b, x, y, z, s))
+      s = sample_exact<kFoot, true>(b, a.touched, db, x, y, z);
     const float ex = s.gx + 1e-4f, ey = s.gy + 1e-4f, ez = s.gz + 1e-4f;
     const float len = sqrtf(ex * ex + ey * ey + ez * ez);
     const float nx = s.gx / len, ny = s.gy / len, nz = s.gz / len;
@@ -83,22 +84,25 @@ __device__ __forceinline__ void load_tf(const MarchArgs &a, float4 *s_tf) {
 struct LdsBrick {
   float lo[3], hi[3];  // spaceRange
   float vlo[3];        // voxelRange.lower
-  uint32_t px, pxy;
+  uint32_t px;         // padded x pitch | kLdsU8 for u8 voxels
+  uint32_t pxy;
   int32_t n[3];
-  const float *origin;
+  const void *origin;
 };
+constexpr uint32_t kLdsU8 = 0x80000000u;  // pitches stay below 2^24
 static_assert(sizeof(LdsBrick) == kLdsBrickBytes, "LDS brick record");
 
 // The launch's dynamic LDS: the TF with its zero borders (tf_n + 4 float4),
-// then one LdsBrick per rank_bricks entry, rank_off[0..P] and the ranks'
-// pixel rectangles -- every table a lane indexes with a per-lane value, so
-// that such reads are LDS reads, not vector loads from the kernel
-// arguments.
+// then one LdsBrick per rank_bricks entry, rank_off[0..P], the ranks' pixel
+// rectangles and (a.u8_lut) the u8 decode table -- every table a lane
+// indexes with a per-lane value, so that such reads are LDS reads, not
+// vector loads from the kernel arguments.
 struct MarchTables {
   const float4 *tf;
   const LdsBrick *brick;  // indexed like rank_bricks
   const int32_t *roff;    // rank_off[0..P]
   const int32_t *rect;    // rect[0..P) x 4
+  const float *lut;       // kU8Lut entries (null when no brick stores u8)
 };
 
 // Stage the TF, the rank table and the ranks' brick records in the dynamic
@@ -117,7 +121,7 @@ __device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *s
       r.vlo[k] = db.vox_lo[k];
       r.n[k] = db.n[k];
     }
-    r.px = (uint32_t)db.px;
+    r.px = (uint32_t)db.px | (db.dtype == ALOHA_U8 ? kLdsU8 : 0u);
     r.pxy = (uint32_t)db.pxy;
     r.origin = db.origin;
     sb[i] = r;
@@ -125,7 +129,12 @@ __device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *s
   int32_t *rect = roff + a.P + 1;
   for (int i = threadIdx.x; i <= a.P; i += blockDim.x) roff[i] = a.rank_off[i];
   for (int i = threadIdx.x; i < 4 * a.P; i += blockDim.x) rect[i] = a.rect[i / 4][i % 4];
-  return MarchTables{smem, sb, roff, rect};
+  float *lut = nullptr;
+  if (a.u8_lut) {
+    lut = reinterpret_cast<float *>(rect + 4 * a.P);
+    load_u8_lut(lut);
+  }
+  return MarchTables{smem, sb, roff, rect, lut};
 }
 
 // One ray of a work list: rank slot (-1: the ray can reach no brick, its
@@ -317,7 +326,7 @@ __device__ __forceinline__ void rec_adopt(RayState &s, const RayRec &r) {
 // gives list entry k: a ListRay (kRec false: the group sets the ray up) or a
 // RayRec (kRec true: set up already).  store(ray, acc, z, ns, clock_begin) is
 // called on the group's first lane when its ray is done.
-template <int G, bool kRec, class Fetch, class Entry, class Store>
+template <int G, bool kRec, bool kU8, class Fetch, class Entry, class Store>
 __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables &tb,
                                            const float org[3], const float dir00[3],
                                            const float du[3], const float dv[3],
@@ -328,7 +337,7 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
   const float dt = a.dt;
   RayState s;
   ray_state_init(s);
-  BrickView bv{nullptr, 0u, 0u, 0, 0, 0};
+  BrickView bv{nullptr, 0u, 0u, 0, 0, 0, nullptr, false};
   float lo0 = 0.f, lo1 = 0.f, lo2 = 0.f;
   bool fresh = false;  // a new segment: its brick view is still to be read
   // One trip: a group whose segment ended in a rank of several bricks
@@ -356,7 +365,10 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
     const bool go = s.state == kStep;
     if (go && fresh) {
       const LdsBrick &db = tb.brick[s.kb];
-      bv = BrickView{(const gfloat *)db.origin, db.px, db.pxy, db.n[0], db.n[1], db.n[2]};
+      // kU8 false: no brick of the launch stores u8, and the decode path
+      // is not compiled in
+      bv = BrickView{(const gfloat *)db.origin, db.px & ~kLdsU8, db.pxy, db.n[0], db.n[1], db.n[2],
+                     kU8 ? tb.lut : nullptr, kU8 && (db.px & kLdsU8)};
       lo0 = db.vlo[0];
       lo1 = db.vlo[1];
       lo2 = db.vlo[2];
@@ -381,8 +393,8 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
       const float y = (org[1] + tk * d[1]) - lo1;
       const float zz = (org[2] + tk * d[2]) - lo2;
       Sample sm;
-      if (!sample_fast(bv, x, y, zz, sm))
-        sm = sample_exact<false>(bv, nullptr, a.bricks[0], x, y, zz);  // no footprint: db unread
+      if (!sample_fast<kU8>(bv, x, y, zz, sm))
+        sm = sample_exact<false, kU8>(bv, nullptr, a.bricks[0], x, y, zz);  // no footprint: db unread
       const float ex = sm.gx + 1e-4f, ey = sm.gy + 1e-4f, ez = sm.gz + 1e-4f;
       const float len = sqrtf(ex * ex + ey * ey + ez * ez);
       const float nx = sm.gx / len, ny = sm.gy / len, nz = sm.gz / len;
@@ -419,7 +431,8 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
 // brick do/while (deviceCode.cpp:362-393) over the bricks of rank `slot`.
 // The RNG is seeded per (pixel, accumID) as the reference seeds it per launch.
 template <bool kFoot>
-__device__ __forceinline__ float4 rank_pixel(const MarchArgs &a, const float4 *s_tf, int slot,
+__device__ __forceinline__ float4 rank_pixel(const MarchArgs &a, const float4 *s_tf,
+                                             const float *lut, int slot,
                                              const float org[3], const float d[3],
                                              uint32_t pixel, float &z, uint32_t &ns) {
   const int b0 = a.rank_off[slot], b1 = a.rank_off[slot + 1];
@@ -459,7 +472,7 @@ __device__ __forceinline__ float4 rank_pixel(const MarchArgs &a, const float4 *s
         bnear = bnear + tmin;
         const float j = a.jitter_mode == ALOHA_JITTER_FIXED ? a.jitter_value : rng_next(rng);
         const DevBrick db = a.bricks[hit];
-        integrate<kFoot>(a, s_tf, db, org, d, bt0, bfar, j, c, ns);
+        integrate<kFoot>(a, s_tf, lut, db, org, d, bt0, bfar, j, c, ns);
         z = fminf(z, bnear / a.max_depth);
         tmin = bfar + 1e-20f;
       }
@@ -502,8 +515,12 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   const int slot = blockIdx.z;
   const int bx = blockIdx.x * kTileW, by = blockIdx.y * kTileH;
   const bool active = block_meets(a, slot, bx, by);  // block-uniform
+  // the u8 decode table, staged whatever the bricks' storage: the samplers
+  // compiled with the decode read it on every lane
+  float *lut = reinterpret_cast<float *>(s_tf + a.tf_n + 4);
   if (active) {
     load_tf(a, s_tf);
+    load_u8_lut(lut);
     __syncthreads();
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -515,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs a) {
   if (active && in_rect(a, slot, px, py) && island_pixel(a, px, py)) {
     float d[3];
     pixel_ray(a.dir00, a.du, a.dv, px, py, d);
-    (void)rank_pixel<kFoot>(a, s_tf, slot, a.org, d, (uint32_t)(px + a.W * py), z, ns);
+    (void)rank_pixel<kFoot>(a, s_tf, lut, slot, a.org, d, (uint32_t)(px + a.W * py), z, ns);
   }
 }
 
@@ -653,7 +670,7 @@ __device__ __forceinline__ long long rank_index(const MarchArgs &a, int slot, in
   return (long long)slot * a.W * a.H + px + (long long)a.W * py;
 }
 
-template <int G>
+template <int G, bool kU8>
 __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, const RanksArgs q) {
   static_assert(64 % kRankRun == 0, "a set-up pass covers whole runs");
   constexpr int kRunsPerPass = 64 / kRankRun;
@@ -730,7 +747,7 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
     next += __popcll(m);
     return k < nrec ? k : pass < passes ? -1 : INT_MAX;
   };
-  march_list<G, true>(a, tb, a.org, a.dir00, a.du, a.dv, fetch, INT_MAX,
+  march_list<G, true, kU8>(a, tb, a.org, a.dir00, a.du, a.dv, fetch, INT_MAX,
                       [&](int k) { return recs[k]; }, store);
 
   // fill: the miss result of every pixel outside the rectangles, rows
@@ -864,7 +881,7 @@ __device__ __forceinline__ void fused_clear(const MarchArgs &a, long long pixel)
 // marched by lane groups, in-order blend, output.  The TF must be staged in
 // s_tf before the call; every thread of the block calls it.  Safe to call
 // repeatedly on the same `fs` when a barrier separates the calls.
-template <int G, class Src>
+template <int G, bool kU8, class Src>
 __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, FusedStage &fs,
                                            const MarchTables &tb, TileShape ts, int bx, int by) {
   const int tw = ts.w(), th = ts.h();
@@ -913,7 +930,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
       begin_ray(a, tb, src.org(), src.dir00(), src.du(), src.dv(), st, ListRay{slot, px, py, item},
                 true, store);
     });
-    march_list<G, true>(
+    march_list<G, true, kU8>(
         a, tb, src.org(), src.dir00(), src.du(), src.dv(),
         [&](bool want) { return want ? atomicAdd(&fs.next, 1) : 0; }, count,
         [&](int k) { return fs.recs[k]; }, store);
@@ -942,7 +959,7 @@ __device__ __forceinline__ void fused_tile(const MarchArgs &a, const Src &src, F
 // Single fused frame.  Grid: the tiles of the rectangles' bounding box
 // (u = tile range x0, y0, nx, ny) first, then `fill_blocks` blocks striding
 // over the other tiles of the frame.
-template <int G>
+template <int G, bool kU8>
 __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const MarchArgs a,
                                                                        int4 u,
                                                                        int fill_blocks) {
@@ -961,7 +978,7 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
   }
   const int item = (int)blockIdx.x;  // row-major tiles of the box
   const MarchTables tb = load_tables(a, s_tf);
-  fused_tile<G>(a, src, fs, tb, TileShape{kFusedWl, kFusedHl}, (u.x + item % u.z) * kFW,
+  fused_tile<G, kU8>(a, src, fs, tb, TileShape{kFusedWl, kFusedHl}, (u.x + item % u.z) * kFW,
                 (u.y + item / u.z) * kFH);
 }
 
@@ -1001,7 +1018,7 @@ __device__ __forceinline__ void batch_fill(const BatchArgs &b, int fb, int nfb) 
       });
 }
 
-template <int G, bool kPersistent>
+template <int G, bool kPersistent, bool kU8>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
   extern __shared__ float4 s_tf[];
   __shared__ FusedStage fs;
@@ -1015,7 +1032,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
     const MarchTables tb = load_tables(b.m, s_tf);
     int view, bx, by;
     batch_active(b, item, view, bx, by);
-    fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
+    fused_tile<G, kU8>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
     return;
   }
   // XCD x works through part x of every view (a contiguous band of each
@@ -1043,7 +1060,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
       const int item = b.box_off[lo] + (int)((long long)n * x / kXcds) + (i - xo[lo]);
       int view, bx, by;
       batch_active(b, item, view, bx, by);
-      fused_tile<G>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
+      fused_tile<G, kU8>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
     }
   }
   batch_fill(b, (int)blockIdx.x, (int)gridDim.x);
@@ -1053,10 +1070,12 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
 __global__ void reset_counter_kernel(uint32_t *c) { c[threadIdx.x] = 0u; }
 
 // Decode + pad one brick: dst element (i,j,k), i in [-1, n+2], reads the
-// clamp-to-edge source voxel.  u8/u16 decode as Volume::get (u/255.f).
+// clamp-to-edge source voxel.  f32 storage: u8/u16 decode as Volume::get
+// (u/255.f, u/65535.f); u8 storage (u8 source only): the bytes as they are.
 __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dtype, int nx,
-                                                        int ny, int nz, float *dst, int px,
-                                                        int py, long long total) {
+                                                        int ny, int nz, void *dst_base,
+                                                        int store_dtype, int px, int py,
+                                                        long long total) {
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(e % px) - kPadLo;
@@ -1064,11 +1083,15 @@ __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dty
     const int k = (int)(e / ((long long)px * py)) - kPadLo;
     const long long s = ((long long)clampi(k, 0, nz - 1) * ny + clampi(j, 0, ny - 1)) * nx +
                         clampi(i, 0, nx - 1);
+    if (store_dtype == ALOHA_U8) {
+      ((uint8_t *)dst_base)[e] = ((const uint8_t *)src)[s];
+      continue;
+    }
     float v;
     if (dtype == ALOHA_U8) v = (float)((const uint8_t *)src)[s] / 255.f;
     else if (dtype == ALOHA_U16) v = (float)((const uint16_t *)src)[s] / 65535.f;
     else v = ((const float *)src)[s];
-    dst[e] = v;
+    ((float *)dst_base)[e] = v;
   }
 }
 
@@ -1077,6 +1100,21 @@ __global__ __launch_bounds__(256) void pad_brick_kernel(const void *src, int dty
 // fill blocks of the per-rank and fused launches: a few per CU, each
 // striding over tiles
 constexpr int kFillBlocks = 1024;
+
+// f(std::integral_constant<int, G>, std::bool_constant<kU8>) for G = lanes
+// (1, 2, 4 or 8 lanes per ray) and kU8 = u8 (the u8 decode compiled in: a
+// brick of the launch keeps u8 voxels)
+template <class F>
+hipError_t with_schedule(int lanes, bool u8, const F &f) {
+  auto go = [&](auto g) { return u8 ? f(g, std::true_type{}) : f(g, std::false_type{}); };
+  switch (lanes) {
+    case 1: return go(std::integral_constant<int, 1>{});
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) {
   // bounding box of the ranks' rectangles, in fused tiles
@@ -1095,14 +1133,10 @@ hipError_t launch_march_composite(const MarchArgs &a, int group, hipStream_t s) 
   const int fill_blocks = std::min<int>(a.H, kFillBlocks);
   const dim3 grid((unsigned)(u.z * u.w + fill_blocks));
   const size_t lds = march_tables_bytes(a);
-  switch (group) {
-    case 1: march_composite_group_kernel<1><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
-    case 2: march_composite_group_kernel<2><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
-    case 4: march_composite_group_kernel<4><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
-    case 8: march_composite_group_kernel<8><<<grid, kBlock, lds, s>>>(a, u, fill_blocks); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return with_schedule(group, a.u8_lut != 0, [&](auto g, auto u8) {
+    march_composite_group_kernel<g(), u8()><<<grid, kBlock, lds, s>>>(a, u, fill_blocks);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hipStream_t s) {
@@ -1110,61 +1144,49 @@ hipError_t launch_batch(const BatchArgs &b, int lanes, int persistent_blocks, hi
   if (persistent_blocks > 0) {
     reset_counter_kernel<<<1, kXcds, 0, s>>>(b.counter);
     const dim3 grid(persistent_blocks);
-    switch (lanes) {
-      case 1: batch_kernel<1, true><<<grid, kBlock, lds, s>>>(b); break;
-      case 2: batch_kernel<2, true><<<grid, kBlock, lds, s>>>(b); break;
-      case 4: batch_kernel<4, true><<<grid, kBlock, lds, s>>>(b); break;
-      case 8: batch_kernel<8, true><<<grid, kBlock, lds, s>>>(b); break;
-      default: return hipErrorInvalidValue;
-    }
-  } else {
-    const dim3 grid((unsigned)(b.total_active + b.fill_blocks));
-    switch (lanes) {
-      case 1: batch_kernel<1, false><<<grid, kBlock, lds, s>>>(b); break;
-      case 2: batch_kernel<2, false><<<grid, kBlock, lds, s>>>(b); break;
-      case 4: batch_kernel<4, false><<<grid, kBlock, lds, s>>>(b); break;
-      case 8: batch_kernel<8, false><<<grid, kBlock, lds, s>>>(b); break;
-      default: return hipErrorInvalidValue;
-    }
+    return with_schedule(lanes, b.m.u8_lut != 0, [&](auto g, auto u8) {
+      batch_kernel<g(), true, u8()><<<grid, kBlock, lds, s>>>(b);
+      return hipGetLastError();
+    });
   }
-  return hipGetLastError();
+  const dim3 grid((unsigned)(b.total_active + b.fill_blocks));
+  return with_schedule(lanes, b.m.u8_lut != 0, [&](auto g, auto u8) {
+    batch_kernel<g(), false, u8()><<<grid, kBlock, lds, s>>>(b);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_march_footprint(const MarchArgs &a, hipStream_t s) {
   const dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileH - 1) / kTileH, a.P);
-  march_kernel<true><<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
+  const size_t lds = sizeof(float4) * (size_t)(a.tf_n + 4) + sizeof(float) * kU8Lut;
+  march_kernel<true><<<grid, kBlock, lds, s>>>(a);
   return hipGetLastError();
 }
 
-hipError_t ranks_occupancy(int group, size_t lds, int *blocks_per_cu) {
-  switch (group) {
-    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<1>, 64, lds);
-    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<2>, 64, lds);
-    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<4>, 64, lds);
-    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, march_persist_kernel<8>, 64, lds);
-    default: return hipErrorInvalidValue;
-  }
+hipError_t ranks_occupancy(int group, bool u8, size_t lds, int *blocks_per_cu) {
+  return with_schedule(group, u8, [&](auto g, auto k) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        march_persist_kernel<g(), k()>, 64, lds);
+  });
 }
 
 hipError_t launch_march_ranks(const MarchArgs &a, const RanksArgs &q, int group, int waves,
                               hipStream_t s) {
   const size_t lds = ranks_lds_bytes(a);
-  switch (group) {
-    case 1: march_persist_kernel<1><<<waves, 64, lds, s>>>(a, q); break;
-    case 2: march_persist_kernel<2><<<waves, 64, lds, s>>>(a, q); break;
-    case 4: march_persist_kernel<4><<<waves, 64, lds, s>>>(a, q); break;
-    case 8: march_persist_kernel<8><<<waves, 64, lds, s>>>(a, q); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return with_schedule(group, a.u8_lut != 0, [&](auto g, auto u8) {
+    march_persist_kernel<g(), u8()><<<waves, 64, lds, s>>>(a, q);
+    return hipGetLastError();
+  });
 }
 
-hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], float *dst_base,
-                            int32_t px, int32_t py, int32_t pz, hipStream_t s) {
+hipError_t launch_pad_brick(const void *src, int dtype, const int32_t n[3], void *dst_base,
+                            int store_dtype, int32_t px, int32_t py, int32_t pz, hipStream_t s) {
+  if (store_dtype == ALOHA_U8 && dtype != ALOHA_U8) return hipErrorInvalidValue;
   const long long total = (long long)px * py * pz;
   const long long blocks = (total + 255) / 256;
   const int grid = (int)(blocks < 65536 ? blocks : 65536);
-  pad_brick_kernel<<<grid, 256, 0, s>>>(src, dtype, n[0], n[1], n[2], dst_base, px, py, total);
+  pad_brick_kernel<<<grid, 256, 0, s>>>(src, dtype, n[0], n[1], n[2], dst_base, store_dtype, px,
+                                        py, total);
   return hipGetLastError();
 }
 

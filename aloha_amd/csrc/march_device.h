@@ -47,18 +47,48 @@ __device__ __forceinline__ float rng_next(uint32_t &s) {
 // which also count against the LDS counter and so would serialize with the
 // TF lookups and lane shuffles.
 typedef const float __attribute__((address_space(1))) gfloat;
+typedef const uint8_t __attribute__((address_space(1))) gbyte;
 
 // o points at padded voxel (-1,-1,-1), so every index the taps form is
-// non-negative after +1 and all offsets are 24-bit products.
+// non-negative after +1 and all offsets are 24-bit products.  u8: the brick
+// stores u8 voxels, decoded through the LDS table lut[u] = u/255.f (the
+// float Volume::get gives, volume.h:78, bit for bit); lut is the launch's
+// table (null in kernels compiled without the u8 decode).
 struct BrickView {
-  const gfloat *o;
+  const gfloat *o;  // f32 voxels, or the u8 voxels' bytes
   uint32_t px, pxy;
   int n0, n1, n2;
+  const float *lut;
+  bool u8;
 };
 
-__device__ __forceinline__ BrickView brick_view(const DevBrick &db) {
+__device__ __forceinline__ BrickView brick_view(const DevBrick &db, const float *lut) {
   return BrickView{(const gfloat *)db.origin, (uint32_t)db.px, (uint32_t)db.pxy, db.n[0], db.n[1],
-                   db.n[2]};
+                   db.n[2], lut, lut != nullptr && db.dtype == ALOHA_U8};
+}
+
+// N consecutive voxels of one x-row from element e, in kernels that handle
+// u8 bricks too: the same instructions for both storages (no branch, so no
+// second gather holds registers): N dwords from byte e of a u8 brick (an unaligned load; the allocation's slack covers the bytes past
+// the grid) or from element e of an f32 brick, then per voxel the decoded
+// byte or the float, by the brick's storage.
+template <int N, bool kU8>
+__device__ __forceinline__ void read_row(const BrickView &b, uint32_t e, float *v) {
+  static_assert(kU8, "f32-only kernels read their rows directly");
+  const gbyte *p = (const gbyte *)b.o + (b.u8 ? e : 4u * e);
+  uint32_t w[N];
+  __builtin_memcpy(w, (const void *)p, 4 * N);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const float dec = b.lut[(w[0] >> (8 * i)) & 0xFFu];
+    v[i] = b.u8 ? dec : __uint_as_float(w[i]);
+  }
+}
+
+
+// The decode table: lut[u] = u/255.f (IEEE division), filled by the block.
+__device__ __forceinline__ void load_u8_lut(float *lut) {
+  for (int i = threadIdx.x; i < kU8Lut; i += blockDim.x) lut[i] = (float)i / 255.f;
 }
 
 __device__ __forceinline__ uint32_t voff(const BrickView &b, int i, int j, int k) {
@@ -75,8 +105,10 @@ __device__ __forceinline__ void mark(uint32_t *touched, const DevBrick &db, int 
   atomicOr(&touched[db.touched_word + (idx >> 5)], 1u << (idx & 31));
 }
 
-// One trilinear fetch, exactly as the restated texture unit does it.
-template <bool kFoot>
+// One trilinear fetch, exactly as the restated texture unit does it.  kU8:
+// the decode of u8 voxels is compiled in (sample_exact's and sample_fast's
+// kU8 likewise); without it the brick must hold f32 voxels.
+template <bool kFoot, bool kU8>
 __device__ __forceinline__ float tap_exact(const BrickView &b, uint32_t *touched,
                                            const DevBrick &db, float x, float y, float z) {
   const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
@@ -95,10 +127,20 @@ __device__ __forceinline__ float tap_exact(const BrickView &b, uint32_t *touched
   // live across a caller's loop for this (rarely taken) path
   const uint32_t i00 = voff(b, ix, iy, iz), i10 = i00 + b.px, i01 = i00 + b.pxy,
                  i11 = i10 + b.pxy;
-  const float c00 = lerp_(b.o[i00], b.o[i00 + 1u], wx);
-  const float c10 = lerp_(b.o[i10], b.o[i10 + 1u], wx);
-  const float c01 = lerp_(b.o[i01], b.o[i01 + 1u], wx);
-  const float c11 = lerp_(b.o[i11], b.o[i11 + 1u], wx);
+  float v[8];
+  if constexpr (kU8) {
+    read_row<2, kU8>(b, i00, v);
+    read_row<2, kU8>(b, i10, v + 2);
+    read_row<2, kU8>(b, i01, v + 4);
+    read_row<2, kU8>(b, i11, v + 6);
+  } else {
+    v[0] = b.o[i00], v[1] = b.o[i00 + 1u], v[2] = b.o[i10], v[3] = b.o[i10 + 1u];
+    v[4] = b.o[i01], v[5] = b.o[i01 + 1u], v[6] = b.o[i11], v[7] = b.o[i11 + 1u];
+  }
+  const float c00 = lerp_(v[0], v[1], wx);
+  const float c10 = lerp_(v[2], v[3], wx);
+  const float c01 = lerp_(v[4], v[5], wx);
+  const float c11 = lerp_(v[6], v[7], wx);
   const float c0 = lerp_(c00, c10, wy);
   const float c1 = lerp_(c01, c11, wy);
   return lerp_(c0, c1, wz);
@@ -108,26 +150,26 @@ struct Sample {
   float val, gx, gy, gz;
 };
 
-template <bool kFoot>
+template <bool kFoot, bool kU8>
 __device__ __forceinline__ Sample sample_exact(const BrickView &b, uint32_t *touched,
                                                const DevBrick &db, float x, float y, float z) {
   // one tap at a time (scheduling barriers between them): this path is
   // rare, and keeping its 56 loads in flight at once would raise the
   // register count of every kernel that inlines it
   Sample s;
-  const float xp = tap_exact<kFoot>(b, touched, db, x + .5f, y, z);
+  const float xp = tap_exact<kFoot, kU8>(b, touched, db, x + .5f, y, z);
   __builtin_amdgcn_sched_barrier(0);
-  const float xm = tap_exact<kFoot>(b, touched, db, x - .5f, y, z);
+  const float xm = tap_exact<kFoot, kU8>(b, touched, db, x - .5f, y, z);
   __builtin_amdgcn_sched_barrier(0);
-  const float yp = tap_exact<kFoot>(b, touched, db, x, y + .5f, z);
+  const float yp = tap_exact<kFoot, kU8>(b, touched, db, x, y + .5f, z);
   __builtin_amdgcn_sched_barrier(0);
-  const float ym = tap_exact<kFoot>(b, touched, db, x, y - .5f, z);
+  const float ym = tap_exact<kFoot, kU8>(b, touched, db, x, y - .5f, z);
   __builtin_amdgcn_sched_barrier(0);
-  const float zp = tap_exact<kFoot>(b, touched, db, x, y, z + .5f);
+  const float zp = tap_exact<kFoot, kU8>(b, touched, db, x, y, z + .5f);
   __builtin_amdgcn_sched_barrier(0);
-  const float zm = tap_exact<kFoot>(b, touched, db, x, y, z - .5f);
+  const float zm = tap_exact<kFoot, kU8>(b, touched, db, x, y, z - .5f);
   __builtin_amdgcn_sched_barrier(0);
-  s.val = tap_exact<kFoot>(b, touched, db, x, y, z);
+  s.val = tap_exact<kFoot, kU8>(b, touched, db, x, y, z);
   s.gx = xp - xm;
   s.gy = yp - ym;
   s.gz = zp - zm;
@@ -136,6 +178,7 @@ __device__ __forceinline__ Sample sample_exact(const BrickView &b, uint32_t *tou
 
 // Shared 20-voxel footprint.  Returns false (and computes nothing) when the
 // lane's taps do not fit the window; the caller then uses sample_exact.
+template <bool kU8>
 __device__ __forceinline__ bool sample_fast(const BrickView &b, float x, float y, float z,
                                             Sample &s) {
   const float xm = x - .5f, xp = x + .5f;
@@ -157,33 +200,67 @@ __device__ __forceinline__ bool sample_fast(const BrickView &b, float x, float y
   const float wzm = zm - fzm, wzc = z - fzc, wzp = zp - fzp;
   const uint32_t px = b.px, pxy = b.pxy;
 
-  // R[jj][kk][i] = V(ax+i, cy+jj, cz+kk): four x-rows of 3 voxels, one row
-  // pointer each so the x offsets are immediates
-  const gfloat *r00 = b.o + voff(b, ax, cy, cz);
-  const gfloat *r01 = r00 + pxy;
-  const gfloat *row[2][2] = {{r00, r01}, {r00 + px, r01 + px}};
-  float R[2][2][3];
-#pragma unroll
-  for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-    for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) R[jj][kk][i] = row[jj][kk][i];
+  // R[jj][kk][i] = V(ax+i, cy+jj, cz+kk): four x-rows of 3 voxels; YN / ZN:
   // the y row outside the value tap's pair, and the z row likewise
   const int yn = scy ? ay : ay + 2;
   const int zn = scz ? az : az + 2;
-  const gfloat *ry = b.o + voff(b, cx, yn, cz);
-  const gfloat *rz = b.o + voff(b, cx, cy, zn);
-  const gfloat *ry1 = ry + pxy, *rz1 = rz + px;
-  float YN[2][2], ZN[2][2];
-  YN[0][0] = ry[0];
-  YN[0][1] = ry[1];
-  YN[1][0] = ry1[0];
-  YN[1][1] = ry1[1];
-  ZN[0][0] = rz[0];
-  ZN[0][1] = rz[1];
-  ZN[1][0] = rz1[0];
-  ZN[1][1] = rz1[1];
+  float R[2][2][3], YN[2][2], ZN[2][2];
+  if (kU8 && b.u8) {
+    // u8 voxels: each row's bytes in one unaligned load (4 bytes for the
+    // 3-voxel rows, 2 for the pairs; the allocation has slack past its
+    // last row), decoded through the table
+    const gbyte *o8 = (const gbyte *)b.o;
+    const float *lut = b.lut;
+    auto ld4 = [&](uint32_t e) {
+      uint32_t w;
+      __builtin_memcpy(&w, (const void *)(o8 + e), 4);
+      return w;
+    };
+    auto ld2 = [&](uint32_t e) {
+      uint16_t w;
+      __builtin_memcpy(&w, (const void *)(o8 + e), 2);
+      return (uint32_t)w;
+    };
+    const uint32_t e00 = voff(b, ax, cy, cz);
+    const uint32_t wr[2][2] = {{ld4(e00), ld4(e00 + pxy)}, {ld4(e00 + px), ld4(e00 + px + pxy)}};
+    const uint32_t ey = voff(b, cx, yn, cz), ez = voff(b, cx, cy, zn);
+    const uint32_t wy0 = ld2(ey), wy1 = ld2(ey + pxy), wz0 = ld2(ez), wz1 = ld2(ez + px);
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) R[jj][kk][i] = lut[(wr[jj][kk] >> (8 * i)) & 0xFFu];
+    YN[0][0] = lut[wy0 & 0xFFu];
+    YN[0][1] = lut[wy0 >> 8];
+    YN[1][0] = lut[wy1 & 0xFFu];
+    YN[1][1] = lut[wy1 >> 8];
+    ZN[0][0] = lut[wz0 & 0xFFu];
+    ZN[0][1] = lut[wz0 >> 8];
+    ZN[1][0] = lut[wz1 & 0xFFu];
+    ZN[1][1] = lut[wz1 >> 8];
+  } else {  // f32 voxels: one row pointer per row, the x offsets immediates
+    const gfloat *r00 = b.o + voff(b, ax, cy, cz);
+    const gfloat *r01 = r00 + pxy;
+    const gfloat *row[2][2] = {{r00, r01}, {r00 + px, r01 + px}};
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) R[jj][kk][i] = row[jj][kk][i];
+    const gfloat *ry = b.o + voff(b, cx, yn, cz);
+    const gfloat *rz = b.o + voff(b, cx, cy, zn);
+    const gfloat *ry1 = ry + pxy, *rz1 = rz + px;
+    YN[0][0] = ry[0];
+    YN[0][1] = ry[1];
+    YN[1][0] = ry1[0];
+    YN[1][1] = ry1[1];
+    ZN[0][0] = rz[0];
+    ZN[0][1] = rz[1];
+    ZN[1][0] = rz1[0];
+    ZN[1][1] = rz1[1];
+  }
 
   // value tap: x-lerps on the (cx, cx+1) pair
   float cX[2][2];

@@ -629,8 +629,43 @@ def batch_bench(args, timed):
            "autotune": {"chosen": sched, "best_ms": round(best, 4),
                         "table": [{"tile_w": tw, "lanes_per_ray": t, "persistent_blocks": pb,
                                    "ms": round(m, 4)} for tw, t, pb, m in table]}}
+    res["u8_storage"] = batch_u8_storage(bw, out, ms_batch, ctx.brick_bytes(), timed)
     if not args.no_cpu:
         res["fp64"] = fp64_batched(bw, out)
+    b.close()
+    ctx.close()
+    return res
+
+
+def batch_u8_storage(bw, ref_out, ms_f32, f32_bytes, timed):
+    """The same batch with the u8 bricks kept as bytes in HBM (decoded at
+    sample time through u/255.f): the launch time, the brick memory and the
+    frames' equality with the float32-storage batch."""
+    import torch
+    import aloha_amd as A
+    side = torch.cuda.Stream()
+    ctx = A.Context(0, stream=side)
+    ctx.set_u8_storage(True)
+    ids = [ctx.upload_brick(d, v) for d, v in zip(bw.bricks, bw.voxels)]
+    ctx.set_tf(bw.tf)
+    ranks = [[ids[i] for i in r] for r in bw.rank_bricks]
+    V, W, H = len(bw.cams), bw.W, bw.H
+    out = torch.empty((V, H, W, 4), dtype=torch.float32, device="cuda")
+    b = A.Batch(ctx, bw.params(), W, H, bw.cams, ranks, bw.orders, out)
+    best, _ = b.autotune(reps=3)
+    b.capture()
+    sched = b.schedule()
+    with torch.cuda.stream(side):
+        ms = timed(b.launch, 10)
+    torch.cuda.synchronize()
+    res = {"ms_graph_launch": round(ms, 4), "schedule": sched,
+           "brick_bytes": ctx.brick_bytes(), "brick_bytes_f32_storage": f32_bytes,
+           "u8_bricks": sum(1 for v in bw.voxels if v.dtype == np.uint8),
+           "time_vs_f32_storage": round(ms / ms_f32, 3),
+           "bit_identical_to_f32_storage": bool(torch.equal(out, ref_out))}
+    log(f"[bench] batched with u8 bricks kept as bytes: {ms:.3f} ms ({sched}), "
+        f"{res['brick_bytes'] / 1e6:.0f} MB of bricks against {f32_bytes / 1e6:.0f} MB, "
+        f"equal {res['bit_identical_to_f32_storage']}")
     b.close()
     ctx.close()
     return res

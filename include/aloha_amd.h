@@ -195,9 +195,21 @@ int aloha_sync(aloha_ctx *ctx);
 /* Brick upload (cudaMalloc3DArray+cudaMemcpy3D+texture, :697-777).  voxels
  * is x-fastest with the desc's voxel_range extent, encoding dtype
  * (ALOHA_U8/U16/F32), on the device if src_on_device.  Values are decoded
- * as Volume::get does (u8/255.f, u16/65535.f).  *brick_id receives its id. */
+ * as Volume::get does (u8/255.f, u16/65535.f) and stored as float32, or,
+ * after aloha_set_u8_storage(ctx, 1), u8 bricks are kept as bytes (a
+ * quarter of the HBM) and decoded at sample time through a table of the
+ * same floats -- the images are the same bit for bit either way.
+ * *brick_id receives its id. */
 int aloha_brick_upload(aloha_ctx *ctx, const aloha_brick_desc *desc, const void *voxels,
                        int32_t dtype, int32_t src_on_device, int32_t *brick_id);
+/* Storage of u8 bricks uploaded from now on (no reference counterpart: the
+ * reference's float textures hold every brick as float32): keep_u8 != 0
+ * keeps them as bytes (for volumes that would not fit as float32; the
+ * march kernels that decode them run at fewer waves per SIMD), 0 (the
+ * default) widens them to float32, which the differentiable form needs. */
+int aloha_set_u8_storage(aloha_ctx *ctx, int32_t keep_u8);
+/* *dtype = how brick_id is stored in HBM: ALOHA_U8 or ALOHA_F32. */
+int aloha_brick_dtype(const aloha_ctx *ctx, int32_t brick_id, int32_t *dtype);
 int aloha_brick_count(const aloha_ctx *ctx, int32_t *count);
 /* device bytes held by bricks (padded layout) */
 int aloha_brick_bytes(const aloha_ctx *ctx, uint64_t *bytes);
@@ -282,6 +294,11 @@ int aloha_autotune_march(aloha_ctx *ctx, const aloha_render_params *params, int3
 int aloha_render_footprint(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
                            int32_t H, int32_t P, const int32_t *rank_offsets,
                            const int32_t *brick_ids, uint64_t *touched_voxels);
+/* The same count per brick id: touched[id] for every uploaded brick (cap >=
+ * aloha_brick_count), so the compulsory bytes follow each brick's storage. */
+int aloha_render_footprint_bricks(aloha_ctx *ctx, const aloha_render_params *params, int32_t W,
+                                  int32_t H, int32_t P, const int32_t *rank_offsets,
+                                  const int32_t *brick_ids, uint64_t *touched, int32_t cap);
 
 /* Sort-last ordered 'over' of P premultiplied float RGBA rank images, front
  * to back in order[] (IceT ORDERED BLEND, :911-916, :1043-1077), onto
@@ -368,7 +385,8 @@ int aloha_batch_launch(aloha_batch *batch);
 /* TF from n device-resident RGBA float entries; asynchronous on the stream. */
 int aloha_tf_set_device(aloha_ctx *ctx, const float *d_rgba, int32_t n);
 /* Replace the voxels of an uploaded brick (same extent) from d_voxels
- * (x-fastest, encoding dtype); asynchronous on the stream. */
+ * (x-fastest, encoding dtype; a brick kept as u8 takes u8 only);
+ * asynchronous on the stream. */
 int aloha_brick_update(aloha_ctx *ctx, int32_t brick_id, const void *d_voxels, int32_t dtype);
 
 /* ---------------------------------------------------------------------- */
