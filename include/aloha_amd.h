@@ -406,7 +406,11 @@ int aloha_diff_forward(aloha_ctx *ctx, const aloha_render_params *params, int32_
  * float4), ADD dLoss/dTF into d_dtf (n x 4 floats) and dLoss/dvoxel into
  * d_dvoxels[i] (device, brick i's unpadded voxel count, x-fastest; a NULL
  * entry skips that brick).  Transmittance is recovered backward from the
- * stored final alpha; nothing per sample is stored. */
+ * forward's final alpha (the forward keeps nothing per sample); the adjoint
+ * re-evaluates the samples into a context workspace (88 B per sample slot,
+ * grown on demand) and sums every gradient in 64-bit fixed point in a fixed
+ * order, so two calls give the same bits.  Waits for the stream once (the
+ * sample total sizes the workspace). */
 int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *params, int32_t W, int32_t H,
                         int32_t V, const float *cams, int32_t nb, const int32_t *brick_ids,
                         const float *d_rgba, const uint32_t *d_nsamples,
