@@ -936,12 +936,17 @@ __device__ __forceinline__ void tf_run_flush(unsigned long long *tfs, int i, int
   }
 }
 
+// Threads of a bin's workgroup: its LDS (the 18^3 tile and the TF sums,
+// ~54 KB) allows two workgroups per CU, so a 512-thread workgroup keeps 4
+// waves on each SIMD (256 threads: 2).
+constexpr int kBinBlock = 512;
+
 // One bin: its records' density gradient scattered onto an 18^3 tile (each
 // sample's 20 footprint voxels, or the 56 corners of its seven taps near a
 // brick face) and their TF gradient onto the TF entries, both as 64-bit
 // fixed-point LDS sums; then the tile goes out as floats to b.tiles and the
 // TF sums as doubles to b.tf_part.  Order-independent sums: bit-reproducible.
-__global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, const DiffBins b) {
+__global__ __launch_bounds__(kBinBlock) void diff_bin_kernel(const DiffArgs a, const DiffBins b) {
   extern __shared__ unsigned long long s_fix[];
   unsigned long long *tile = s_fix;               // kTileVox
   unsigned long long *tfs = s_fix + kTileVox;     // 4 x min(tf_n, kLdsTfChunk)
@@ -958,7 +963,7 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
   // voxel apart, footprints overlapping): each thread takes a contiguous
   // chunk of the list, so the lanes of one instruction rarely add to the
   // same word
-  const uint32_t per = (r1 - r0 + kBlock - 1) / kBlock;
+  const uint32_t per = (r1 - r0 + kBinBlock - 1) / kBinBlock;
   const uint32_t c0 = r0 + per * threadIdx.x, c1 = min(r1, c0 + per);
   // the largest bounds of the bin's records (raised by the adjoint march)
   const float md = __uint_as_float(b.maxd[bin]), mt = __uint_as_float(b.maxt[bin]);
@@ -973,16 +978,37 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
     const int e0 = ch * kLdsTfChunk, e1 = min(n, e0 + kLdsTfChunk);
     const bool do_dens = ch == 0 && sd > 0.0;
     if (do_dens)
-      for (int i = threadIdx.x; i < kTileVox; i += kBlock) tile[i] = 0ull;
-    for (int i = threadIdx.x; i < 4 * (e1 - e0); i += kBlock) tfs[i] = 0ull;
+      for (int i = threadIdx.x; i < kTileVox; i += kBinBlock) tile[i] = 0ull;
+    for (int i = threadIdx.x; i < 4 * (e1 - e0); i += kBinBlock) tfs[i] = 0ull;
     __syncthreads();
     int ri = INT_MIN;  // the TF entry pair of the current run, its sums
     long long q0[4] = {0, 0, 0, 0}, q1[4] = {0, 0, 0, 0};
+    // The thread's records are loaded one ahead of their scatter and its
+    // list entries two ahead, so the loads of the next record are in flight
+    // while this one's LDS sums run (each record is its own cache lines:
+    // consecutive records of a run lie 64 slots apart).  A slot at or past
+    // b.total (only with sample counts that are not the forward's) is
+    // skipped.
+    const bool want_t = st > 0.0;
+    float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
+    uint32_t nsl = c0 < c1 ? b.rec_list[c0] : b.total;
+    uint32_t nsl2 = c0 + 1 < c1 ? b.rec_list[c0 + 1] : b.total;
+    auto fetch = [&](uint32_t sl) {
+      if (sl < b.total) {
+        const float4 *p = b.records + (size_t)kRecWords * sl;
+        n0 = p[0];
+        n1 = p[1];
+        if (want_t) n2 = p[2];
+      }
+    };
+    fetch(nsl);
     for (uint32_t r = c0; r < c1; ++r) {
-      const uint32_t sl = b.rec_list[r];
-      if (sl >= b.total) continue;  // (only with sample counts that are not the forward's)
-      const float4 *p = b.records + (size_t)kRecWords * sl;
-      const float4 p0 = p[0], p1 = p[1];
+      const uint32_t sl = nsl;
+      const float4 p0 = n0, p1 = n1, p2 = n2;
+      nsl = nsl2;
+      if (r + 2 < c1) nsl2 = b.rec_list[r + 2];
+      if (r + 1 < c1) fetch(nsl);
+      if (sl >= b.total) continue;
       if (do_dens) {
         const float gd[3] = {p1.x, p1.y, p1.z};
         if (p0.w != 0.f || gd[0] != 0.f || gd[1] != 0.f || gd[2] != 0.f) {
@@ -1002,7 +1028,7 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
         // consecutive records of a thread are neighbouring samples of one
         // ray and mostly read the same entry pair: their fixed-point values
         // are summed in registers and added to LDS once per run
-        const float4 g4 = p[2];
+        const float4 g4 = p2;
         const float xt = p1.w * (float)n - 0.5f;
         const float fi = floorf(xt);
         const float w = xt - fi;
@@ -1026,14 +1052,14 @@ __global__ __launch_bounds__(kBlock) void diff_bin_kernel(const DiffArgs a, cons
     if (ch == 0 && dens) {
       float *dst = b.tiles + (size_t)bin * kTileVox;
       const double inv = sd > 0.0 ? 1.0 / sd : 0.0;
-      for (int i = threadIdx.x; i < kTileVox; i += kBlock)
+      for (int i = threadIdx.x; i < kTileVox; i += kBinBlock)
         dst[i] = sd > 0.0 ? (float)((double)(long long)tile[i] * inv)
                           : (sd < 0.0 ? __int_as_float(0x7fc00000) : 0.f);
     }
     if (want_tf) {
       double *dst = b.tf_part + (size_t)bin * 4 * n;
       const double inv = st > 0.0 ? 1.0 / st : 0.0;
-      for (int i = 4 * e0 + threadIdx.x; i < 4 * e1; i += kBlock)
+      for (int i = 4 * e0 + threadIdx.x; i < 4 * e1; i += kBinBlock)
         dst[i] = st > 0.0 ? (double)(long long)tfs[i - 4 * e0] * inv
                           : (st < 0.0 ? __longlong_as_double(0x7ff8000000000000ll) : 0.0);
     }
@@ -1158,7 +1184,7 @@ hipError_t launch_diff_bins(const DiffArgs &a, const DiffBins &b, const int max_
   if (b.nbins == 0) return hipSuccess;
   const size_t lds =
       sizeof(unsigned long long) * ((size_t)kTileVox + 4 * (size_t)std::min(a.tf_n, kLdsTfChunk));
-  diff_bin_kernel<<<b.nbins, kBlock, lds, s>>>(a, b);
+  diff_bin_kernel<<<b.nbins, kBinBlock, lds, s>>>(a, b);
   if (max_dims[0] > 0) {
     const int nzb = (max_dims[2] + kFoldZ - 1) / kFoldZ;
     const dim3 grid((unsigned)((max_dims[0] + 63) / 64), (unsigned)((max_dims[1] + 3) / 4),
