@@ -577,9 +577,10 @@ class Context:
         return {"ntris": nt.value, "nnodes": nn.value, "depth": dp.value, "bytes": nb.value}
 
     def tris_export(self, geom: int):
-        """(nodes (n,16) raw 4-byte words as uint32, tris (ntris,12) float32, root)."""
+        """(nodes (n,32) raw 4-byte words as uint32 (4-wide BVH nodes), tris
+        (ntris,12) float32, root)."""
         info = self.tris_info(geom)
-        nodes = np.zeros((max(info["nnodes"], 1), 16), dtype=np.uint32)
+        nodes = np.zeros((max(info["nnodes"], 1), 32), dtype=np.uint32)
         tris = np.zeros((max(info["ntris"], 1), 12), dtype=np.float32)
         root = C.c_int32()
         self._check(lib().aloha_tris_export(self._h, geom, nodes.ctypes.data, tris.ctypes.data,
@@ -694,7 +695,8 @@ class Mesh:
 
     def build_bvh(self, cluster_ids: Sequence[int], colors: Sequence[int] | None = None):
         """Host-only build of one rank's device layout (aloha_tris_build_host):
-        (nodes (n,16) uint32 words, tris (m,12) float32, root, depth)."""
+        (nodes (n,32) uint32 words of 4-wide nodes, tris (m,12) float32, root,
+        depth of the 4-wide tree)."""
         ids = (C.c_int32 * max(1, len(cluster_ids)))(*cluster_ids)
         col = np.ascontiguousarray(colors if colors is not None else np.zeros(len(cluster_ids)),
                                    dtype=np.uint32)
@@ -704,7 +706,7 @@ class Mesh:
         check(lib().aloha_tris_build_host(self._h, len(cluster_ids), ids, col.ctypes.data,
                                           C.byref(nt), C.byref(nn), C.byref(dp), None, None,
                                           C.byref(root)))
-        nodes = np.zeros((max(nn.value, 1), 16), np.uint32)
+        nodes = np.zeros((max(nn.value, 1), 32), np.uint32)
         tris = np.zeros((max(nt.value, 1), 12), np.float32)
         check(lib().aloha_tris_build_host(self._h, len(cluster_ids), ids, col.ctypes.data,
                                           C.byref(nt), C.byref(nn), C.byref(dp), nodes.ctypes.data,

@@ -20,7 +20,9 @@ using namespace aloha;
 namespace {
 
 constexpr int32_t kEmptyRoot = INT32_MIN;  // the kernels' marker of a rank with no triangles
-static_assert(kTriStack == kMaxBvhDepth, "traversal stack must cover the deepest leaf");
+// a walk pushes at most 3 children per 4-wide level, and a 4-wide leaf is no
+// deeper than its binary one
+static_assert(kTriStack == 3 * kMaxBvhDepth, "traversal stack must cover the deepest leaf");
 
 int geom_of(const aloha_ctx *ctx, int32_t id) {
   return id >= 0 && id < (int32_t)ctx->geoms.size() ? id : -1;
@@ -83,8 +85,9 @@ extern "C" int aloha_tris_upload(aloha_ctx *ctx, const aloha_mesh *mesh, int32_t
       g.root = kEmptyRoot;
     } else {
       const TriBvh &b = rb[r].bvh;
-      if (b.depth > kTriStack)
-        return fail(ctx, ALOHA_ERR_LIMIT, "aloha_tris_upload: BVH depth %d exceeds %d", b.depth, kTriStack);
+      if (3 * b.depth > kTriStack)
+        return fail(ctx, ALOHA_ERR_LIMIT, "aloha_tris_upload: BVH depth %d exceeds %d", b.depth,
+                    kTriStack / 3);
       g.root = b.root;
       g.depth = b.depth;
       for (int k = 0; k < 3; ++k) {
@@ -92,12 +95,13 @@ extern "C" int aloha_tris_upload(aloha_ctx *ctx, const aloha_mesh *mesh, int32_t
         g.box.upper[k] = b.hi[k] + b.pad;
       }
       g.nnodes = (int64_t)b.nodes.size();
-      const size_t nb = std::max<size_t>(1, b.nodes.size()) * sizeof(TriNode);
+      const size_t nb = std::max<size_t>(1, b.nodes.size()) * sizeof(TriNode4);
       const size_t tb = rb[r].dev_tris.size() * sizeof(float);
       hipError_t e = dev_malloc((void **)&g.d_nodes, nb);
       if (e == hipSuccess) e = dev_malloc((void **)&g.d_tris, tb);
       if (e == hipSuccess && !b.nodes.empty())
-        e = hipMemcpy(g.d_nodes, b.nodes.data(), b.nodes.size() * sizeof(TriNode), hipMemcpyHostToDevice);
+        e = hipMemcpy(g.d_nodes, b.nodes.data(), b.nodes.size() * sizeof(TriNode4),
+                      hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMemcpy(g.d_tris, rb[r].dev_tris.data(), tb, hipMemcpyHostToDevice);
       if (e != hipSuccess) {
         if (g.d_nodes) hipFree(g.d_nodes);
@@ -143,7 +147,7 @@ extern "C" int aloha_tris_export(const aloha_ctx *ctx, int32_t geom, void *nodes
   HIP_CHECK(c, hipStreamSynchronize(ctx->stream));
   if (root) *root = t.root;
   if (nodes && t.nnodes)
-    HIP_CHECK(c, hipMemcpy(nodes, t.d_nodes, t.nnodes * sizeof(TriNode), hipMemcpyDeviceToHost));
+    HIP_CHECK(c, hipMemcpy(nodes, t.d_nodes, t.nnodes * sizeof(TriNode4), hipMemcpyDeviceToHost));
   if (tris && t.ntris)
     HIP_CHECK(c, hipMemcpy(tris, t.d_tris, t.ntris * 48, hipMemcpyDeviceToHost));
   return ALOHA_OK;

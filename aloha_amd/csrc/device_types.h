@@ -185,10 +185,10 @@ struct CompositeArgs {
   int32_t cbase[ALOHA_MAX_RANKS];
 };
 
-// Surface path: one rank's triangle geometry (BVH nodes of 4 float4, see
-// TriNode; triangles of 3 float4 in leaf order, w = primitive id / colour id
-// bits) and the launch arguments of all ranks.
-constexpr int kTriStack = 32;  // = kMaxBvhDepth: deepest leaf bounds the stack
+// Surface path: one rank's triangle geometry (4-wide BVH nodes of 8 float4,
+// see TriNode4; triangles of 3 float4 in leaf order, w = primitive id /
+// colour id bits) and the launch arguments of all ranks.
+constexpr int kTriStack = 96;  // 3 entries per level of the deepest (32) leaf
 struct DevTris {
   const float4 *nodes;
   const float4 *tris;

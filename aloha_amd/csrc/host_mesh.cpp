@@ -378,6 +378,74 @@ int need_levels(int64_t n) {
   return k;
 }
 
+// Node of the binary build: both children's boxes and refs.
+struct BinNode {
+  float lo[2][3], hi[2][3];
+  int32_t ref[2];
+};
+
+// One child of a 4-wide node while it is being collapsed: a binary-build ref
+// (node index >= 0 or leaf) and its padded box.
+struct Kid {
+  int32_t ref;
+  float lo[3], hi[3];
+};
+
+// Collapse the binary subtree under `b` into 4-wide node `at` (already
+// allocated): the children are b's two, and while fewer than four, the inner
+// child with the largest box (half area) is replaced by its own two; then
+// each inner child becomes a 4-wide node in turn (depth-first after `at`).
+void collapse(const std::vector<BinNode> &bin, int32_t b, int32_t at, int depth, TriBvh &out) {
+  std::vector<Kid> kids;
+  for (int s = 0; s < 2; ++s) {
+    Kid k{bin[b].ref[s], {}, {}};
+    for (int a = 0; a < 3; ++a) k.lo[a] = bin[b].lo[s][a], k.hi[a] = bin[b].hi[s][a];
+    kids.push_back(k);
+  }
+  while (kids.size() < 4) {
+    int pick = -1;
+    double best = -1.0;
+    for (int i = 0; i < (int)kids.size(); ++i) {
+      if (kids[i].ref < 0) continue;
+      aloha_box3f bb;
+      for (int a = 0; a < 3; ++a) bb.lower[a] = kids[i].lo[a], bb.upper[a] = kids[i].hi[a];
+      const double ha = half_area(bb);
+      if (ha > best) best = ha, pick = i;
+    }
+    if (pick < 0) break;
+    const int32_t c = kids[pick].ref;
+    kids.erase(kids.begin() + pick);
+    for (int s = 0; s < 2; ++s) {
+      Kid k{bin[c].ref[s], {}, {}};
+      for (int a = 0; a < 3; ++a) k.lo[a] = bin[c].lo[s][a], k.hi[a] = bin[c].hi[s][a];
+      kids.insert(kids.begin() + pick + s, k);
+    }
+  }
+  int32_t refs[4];
+  for (int i = 0; i < 4; ++i) {
+    TriNode4 &n = out.nodes[at];
+    if (i >= (int)kids.size()) {
+      n.lo_x[i] = n.lo_y[i] = n.lo_z[i] = FLT_MAX;
+      n.hi_x[i] = n.hi_y[i] = n.hi_z[i] = -FLT_MAX;
+      refs[i] = kEmptyChild;
+      continue;
+    }
+    const Kid &k = kids[i];
+    n.lo_x[i] = k.lo[0], n.hi_x[i] = k.hi[0];
+    n.lo_y[i] = k.lo[1], n.hi_y[i] = k.hi[1];
+    n.lo_z[i] = k.lo[2], n.hi_z[i] = k.hi[2];
+    if (k.ref < 0) {
+      refs[i] = k.ref;
+      out.depth = std::max(out.depth, depth + 1);
+    } else {
+      refs[i] = (int32_t)out.nodes.size();
+      out.nodes.push_back(TriNode4{});
+      collapse(bin, k.ref, refs[i], depth + 1, out);
+    }
+  }
+  for (int i = 0; i < 4; ++i) out.nodes[at].ref[i] = refs[i];
+}
+
 }  // namespace
 
 bool build_tri_bvh(const float *tri9, int64_t ntris, float pad, TriBvh &out, std::string &err) {
@@ -412,6 +480,8 @@ bool build_tri_bvh(const float *tri9, int64_t ntris, float pad, TriBvh &out, std
     int32_t parent, side, depth;
     int64_t first, last;
   };
+  std::vector<BinNode> bin;  // the binary build, collapsed below
+  int32_t broot = 0;
   std::vector<Task> stack{{-1, 0, 0, 0, ntris}};
   while (!stack.empty()) {
     const Task t = stack.back();
@@ -483,7 +553,7 @@ bool build_tri_bvh(const float *tri9, int64_t ntris, float pad, TriBvh &out, std
     int64_t mid = 0;
     if (axis < 0 && !median) {
       ref = ~(int32_t)((t.first << kLeafBits) | (n - 1));
-      out.depth = std::max(out.depth, t.depth);
+      out.depth2 = std::max(out.depth2, t.depth);
     } else {
       if (median) {
         int a = 0;
@@ -503,29 +573,36 @@ bool build_tri_bvh(const float *tri9, int64_t ntris, float pad, TriBvh &out, std
         });
         mid = it - order.begin();
       }
-      ref = (int32_t)out.nodes.size();
-      out.nodes.push_back(TriNode{});
+      ref = (int32_t)bin.size();
+      bin.push_back(BinNode{});
       // right first: the left subtree is built next, right after its parent
       stack.push_back({ref, 1, t.depth + 1, mid, t.last});
       stack.push_back({ref, 0, t.depth + 1, t.first, mid});
     }
     if (t.parent < 0) {
-      out.root = ref;
+      broot = ref;
       for (int a = 0; a < 3; ++a) {
         out.lo[a] = b.lower[a];
         out.hi[a] = b.upper[a];
       }
       continue;
     }
-    TriNode &pn = out.nodes[t.parent];
-    float *xy = t.side ? pn.r_xy : pn.l_xy;
-    xy[0] = b.lower[0] - pad;
-    xy[1] = b.upper[0] + pad;
-    xy[2] = b.lower[1] - pad;
-    xy[3] = b.upper[1] + pad;
-    pn.lr_z[2 * t.side] = b.lower[2] - pad;
-    pn.lr_z[2 * t.side + 1] = b.upper[2] + pad;
+    BinNode &pn = bin[t.parent];
+    for (int a = 0; a < 3; ++a) {
+      pn.lo[t.side][a] = b.lower[a] - pad;
+      pn.hi[t.side][a] = b.upper[a] + pad;
+    }
     pn.ref[t.side] = ref;
+  }
+  // the 4-wide tree: a leaf root stays the root; else node 0 collapses the
+  // binary root
+  if (broot < 0) {
+    out.root = broot;
+  } else {
+    out.root = 0;
+    out.nodes.reserve(bin.size() / 2 + 1);
+    out.nodes.push_back(TriNode4{});
+    collapse(bin, broot, 0, 0, out);
   }
   return true;
 }
@@ -606,7 +683,7 @@ extern "C" int aloha_tris_build_host(const aloha_mesh *mesh, int32_t n, const in
   if (depth) *depth = rb.bvh.depth;
   if (root) *root = nt ? rb.bvh.root : INT32_MIN;
   if (nodes && !rb.bvh.nodes.empty())
-    std::memcpy(nodes, rb.bvh.nodes.data(), rb.bvh.nodes.size() * sizeof(TriNode));
+    std::memcpy(nodes, rb.bvh.nodes.data(), rb.bvh.nodes.size() * sizeof(TriNode4));
   if (tris && nt) std::memcpy(tris, rb.dev_tris.data(), rb.dev_tris.size() * sizeof(float));
   return ALOHA_OK;
 }

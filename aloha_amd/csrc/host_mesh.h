@@ -23,28 +23,31 @@ struct aloha_mesh {
 
 namespace aloha {
 
-// Binary BVH node as the traversal kernel reads it: 64 bytes, four 16-byte
-// words holding both children's boxes, so one node fetch tests two boxes.
-//   word 0: left  {lo.x, hi.x, lo.y, hi.y}
-//   word 1: right {lo.x, hi.x, lo.y, hi.y}
-//   word 2: {left lo.z, left hi.z, right lo.z, right hi.z}
-//   word 3: {left ref, right ref, 0, 0}
-// A child ref >= 0 is an internal node index; a leaf is
-// ~(first << kLeafBits | (count - 1)) over the BVH-ordered triangle array.
+// The BVH is built binary (binned SAH) and collapsed to four children per
+// node, the layout the traversal kernel reads: 128 bytes, so one node fetch
+// (eight 16-byte words) tests four boxes.
+//   words 0-5: lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4]
+//   word 6:    ref[4];  word 7: zero
+// A child ref >= 0 is a node index; a leaf is ~(first << kLeafBits |
+// (count - 1)) over the BVH-ordered triangle array; an unused child slot is
+// kEmptyChild (its box empty: lo = +FLT_MAX, hi = -FLT_MAX).
 constexpr int kLeafBits = 3;
 constexpr int kMaxLeaf = 1 << kLeafBits;  // triangles per leaf, at most
-constexpr int kMaxBvhDepth = 32;          // leaf depth bound = traversal stack size
-struct TriNode {
-  float l_xy[4], r_xy[4], lr_z[4];
+constexpr int kMaxBvhDepth = 32;          // leaf depth bound of the binary build
+constexpr int32_t kEmptyChild = 0x7fffffff;
+struct TriNode4 {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
   int32_t ref[4];
+  int32_t zero[4];
 };
-static_assert(sizeof(TriNode) == 64, "BVH node layout");
+static_assert(sizeof(TriNode4) == 128, "BVH node layout");
 
 struct TriBvh {
-  std::vector<TriNode> nodes;   // internal nodes, depth-first (left child next)
+  std::vector<TriNode4> nodes;  // nodes, depth-first (a node's children after it)
   std::vector<uint32_t> order;  // BVH triangle slot -> input triangle
   int32_t root = 0;             // node 0, or a leaf ref when there is one leaf
-  int32_t depth = 0;            // deepest leaf (root = 0)
+  int32_t depth = 0;            // deepest leaf of the 4-wide tree (root = 0)
+  int32_t depth2 = 0;           // deepest leaf of the binary build
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // the triangles' bounds
   float pad = 0.f;                              // the node boxes' widening
 };

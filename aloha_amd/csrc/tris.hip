@@ -1,15 +1,16 @@
 // tris.hip -- the surface path's raygen: renderFrame with traceRay(world) and
 // the ModelCH closest-hit program (deviceCode.cpp:340-416, :176-188), with
-// OptiX's hardware BVH replaced by a binary binned-SAH BVH walked in software.
+// OptiX's hardware BVH replaced by a binned-SAH BVH collapsed to four children
+// per node, walked in software.
 //
 // Layout: one 64-lane wavefront per 8x8 pixel tile, 2x2 tiles per 256-thread
 // workgroup, grid.z = rank slot (per-rank images).  Each lane keeps its
 // traversal stack in LDS (4-byte entries, lane-interleaved so that the 64
 // lanes of a push or pop hit 64 different banks; the per-rank launch keeps
-// the first 16 there and deeper ones in private memory); the BVH depth bound
-// of the builder makes 32 entries (and a sentinel) always enough.  A node
-// fetch is 64 bytes
-// (both children's boxes) and a triangle 48 bytes.  Pixels outside the
+// the first 16 there and deeper ones in private memory); a walk pushes at
+// most 3 children per level, so the builder's depth bound makes 96 entries
+// (and a sentinel) always enough.  A node fetch is 128 bytes (four
+// children's boxes and refs) and a triangle 48 bytes.  Pixels outside the
 // rectangle that a rank's geometry can project to (aloha_project_boxes of its
 // root's child boxes) take the miss result without a walk.
 //
@@ -37,14 +38,14 @@ constexpr int kBlock = 256;
 constexpr int kTileW = 16, kTileH = 16;
 constexpr int kLeafBits = 3;
 constexpr int32_t kEmptyRoot = INT32_MIN;
+constexpr int32_t kEmptyChild = 0x7fffffff;  // an unused child slot of a node
 constexpr float kTMax = 1e10f;
 // LDS entries of the per-rank launch's traversal stack: 16 (16 KB per
-// workgroup) lets 8 waves share a SIMD instead of 5 with all 32 in LDS
-// (2.95 vs 3.42 ms on the 9.75 M-triangle scene); the rare deeper entries go
-// to private memory.  The fused launch keeps all 32 in LDS (measured faster
-// there).
+// workgroup) lets 8 waves share a SIMD instead of 5 with 32 in LDS (2.95 vs
+// 3.42 ms on the 9.75 M-triangle scene with the binary BVH); the rare deeper
+// entries go to private memory.  The fused launch keeps 32 in LDS.
 constexpr int kRankStackLds = 16;
-constexpr int kFusedStackLds = kTriStack;
+constexpr int kFusedStackLds = 32;
 // slack on the far end of each slab interval: covers the rounding of
 // (plane - org) * (1/dir) (a few ulps) so that culling stays conservative
 constexpr float kSlabSlack = 1.000001f;
@@ -165,21 +166,41 @@ __device__ __forceinline__ int trace(const DevTris &g, const Ray &r, float &best
   while (true) {
     bool searching = true;
     while ((uint32_t)node < (uint32_t)kSentinel) {  // an inner node
-      const float4 *n = g.nodes + 4 * (int64_t)node;
-      const float4 lxy = n[0], rxy = n[1], z = n[2];
-      const int4 kids = *reinterpret_cast<const int4 *>(n + 3);
+      const float4 *n = g.nodes + 8 * (int64_t)node;
+      const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+      const int4 kids = *reinterpret_cast<const int4 *>(n + 6);
       if (kStats) ++nn;
-      float tl, tr;
-      const bool hl = slab(r, lxy.x, lxy.y, lxy.z, lxy.w, z.x, z.y, best_t, tl);
-      const bool hr = slab(r, rxy.x, rxy.y, rxy.z, rxy.w, z.z, z.w, best_t, tr);
-      if (hl && hr) {
-        const bool left_first = tl <= tr;
-        push(left_first ? kids.y : kids.x);
-        node = left_first ? kids.x : kids.y;
-      } else if (hl || hr) {
-        node = hl ? kids.x : kids.y;
-      } else {
+      // the children the ray meets, nearest entry first (a miss sorts last:
+      // +inf); an unused slot (kEmptyChild) is a miss
+      float t0, t1, t2, t3;
+      const bool h0 = slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, best_t, t0) && kids.x != kEmptyChild;
+      const bool h1 = slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, best_t, t1) && kids.y != kEmptyChild;
+      const bool h2 = slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best_t, t2) && kids.z != kEmptyChild;
+      const bool h3 = slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best_t, t3) && kids.w != kEmptyChild;
+      float k[4] = {h0 ? t0 : INFINITY, h1 ? t1 : INFINITY, h2 ? t2 : INFINITY, h3 ? t3 : INFINITY};
+      int32_t c[4] = {kids.x, kids.y, kids.z, kids.w};
+      const int hits = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+      auto order = [&](int i, int j) {  // compare-exchange on (k, c)
+        const bool sw = k[j] < k[i];
+        const float tk = sw ? k[j] : k[i];
+        k[j] = sw ? k[i] : k[j];
+        k[i] = tk;
+        const int32_t tc = sw ? c[j] : c[i];
+        c[j] = sw ? c[i] : c[j];
+        c[i] = tc;
+      };
+      order(0, 1);
+      order(2, 3);
+      order(0, 2);
+      order(1, 3);
+      order(1, 2);
+      if (hits == 0) {
         node = pop();
+      } else {
+        if (hits > 3) push(c[3]);
+        if (hits > 2) push(c[2]);
+        if (hits > 1) push(c[1]);
+        node = c[0];
       }
       if (node < 0 && leaf >= 0) {  // the first leaf: keep it, walk on
         searching = false;

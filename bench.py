@@ -764,9 +764,9 @@ def surface_bench(args, timed):
     ftot = fst.to(torch.int64).sum(dim=(0, 1)).tolist()
     rays = P * W * H
     hit_frac = float((outz < 1).float().mean())
-    # algorithmic bytes of one per-rank launch: nodes x 64 + triangle tests x 48
+    # algorithmic bytes of one per-rank launch: nodes x 128 + triangle tests x 48
     # + the RGBA8 and depth images written
-    bytes_render = tot[0] * 64 + tot[1] * 48 + rays * 8
+    bytes_render = tot[0] * 128 + tot[1] * 48 + rays * 8
     log(f"[bench] surface: frame {ms_frame:.3f} ms (render {ms_render:.3f}, zmin {ms_zmin:.3f}), "
         f"fused {ms_fused:.3f} ms (equal {equal}), {tot[0] / rays:.1f} nodes + {tot[1] / rays:.1f} "
         f"tris per ray")
@@ -786,7 +786,7 @@ def surface_bench(args, timed):
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(bytes_render / (ms_render * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "algorithmic_bytes": int(bytes_render),
-                        "note": "bytes of the traversal this BVH needs (64 B per node visit, 48 B "
+                        "note": "bytes of the traversal this BVH needs (128 B per 4-wide node visit, 48 B "
                                 "per triangle test, not deduplicated) + the images written"},
            "fused_ms": round(ms_fused, 4), "fused_Mrays/s": round(rays / (ms_fused * 1e-3) / 1e6, 1),
            "fused_nodes_per_pixel": round(ftot[0] / (W * H), 2),

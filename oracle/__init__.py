@@ -421,9 +421,9 @@ def tri_render_bvh(cam, nodes: np.ndarray, tris: np.ndarray, root: int, W: int, 
                    max_depth=1.0, spp=1, island_id=0, num_islands=1, crosshair=False, rows=None,
                    threads=0, heat_clock=None, heat_scale=1e-5):
     """The same image through a BVH in the library's device layout (nodes
-    (n,16) 4-byte words, tris (m,12) float32), for meshes too large for brute
-    force.  prim holds primitive ids."""
-    nodes = np.ascontiguousarray(nodes).view(np.uint32).reshape(-1, 16)
+    (n,32) 4-byte words, 4-wide, tris (m,12) float32), for meshes too large
+    for brute force.  prim holds primitive ids."""
+    nodes = np.ascontiguousarray(nodes).view(np.uint32).reshape(-1, 32)
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
     p = _tri_params(cam, W, H, max_depth, spp, island_id, num_islands, crosshair, heat_clock,
                     heat_scale)

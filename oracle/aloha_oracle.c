@@ -1317,11 +1317,12 @@ int orc_tri_render(const orc_params *p, const float *tri9, const uint32_t *color
 }
 
 /* The same image by walking a BVH in the library's device layout
- * (nodes: 16 x 4-byte words each -- left box xy, right box xy, both z,
- * child refs; tris: 12 floats each, vertex w = primitive id / colour id
- * bits).  Used where brute force is too slow; the traversal is ordered
- * and culled differently from the kernel's but returns the same closest
- * hit (smallest t, then smallest primitive id). */
+ * (nodes: 32 x 4-byte words each -- lo.x[4], hi.x[4], lo.y[4], hi.y[4],
+ * lo.z[4], hi.z[4], child refs[4] (0x7fffffff: unused), 4 zero words;
+ * tris: 12 floats each, vertex w = primitive id / colour id bits).  Used
+ * where brute force is too slow; the traversal is ordered and culled
+ * differently from the kernel's but returns the same closest hit (smallest
+ * t, then smallest primitive id). */
 int orc_tri_render_bvh(const orc_params *p, const void *nodes, const float *tris, int32_t root,
                        int row0, int row1, float *rgba, uint32_t *rgba8, float *depth,
                        int32_t *hit_prim, float *hit_t, int nthreads) {
@@ -1347,17 +1348,18 @@ int orc_tri_render_bvh(const orc_params *p, const void *nodes, const float *tris
         tri_ray_setup(p->org, d, &r);
         /* double-precision slab tests widened by 1e-6 relative: culling
          * only has to be conservative */
-        int32_t stack[64];
+        int32_t stack[160];
         int sp = 0;
         stack[sp++] = root;
         while (sp > 0) {
           const int32_t ref = stack[--sp];
           if (ref >= 0) {
-            const float *n = nf + 16 * (size_t)ref;
-            for (int side = 1; side >= 0; --side) {
-              const float *xy = n + 4 * side;
-              const float lo[3] = {xy[0], xy[2], n[8 + 2 * side]};
-              const float hi[3] = {xy[1], xy[3], n[9 + 2 * side]};
+            const float *n = nf + 32 * (size_t)ref;
+            for (int k = 3; k >= 0; --k) {
+              const int32_t kid = ni[32 * (size_t)ref + 24 + k];
+              if (kid == 0x7fffffff) continue; /* unused slot */
+              const float lo[3] = {n[k], n[8 + k], n[16 + k]};
+              const float hi[3] = {n[4 + k], n[12 + k], n[20 + k]};
               double t0 = 0.0, t1 = (double)best * (1.0 + 1e-6);
               int ok = 1;
               for (int a = 0; a < 3 && ok; ++a) {
@@ -1371,8 +1373,8 @@ int orc_tri_render_bvh(const orc_params *p, const void *nodes, const float *tris
                 ok = t0 <= t1;
               }
               if (ok) {
-                if (sp < 64)
-                  stack[sp++] = ni[16 * (size_t)ref + 12 + side];
+                if (sp < 160)
+                  stack[sp++] = kid;
                 else
                   overflow = 1;
               }

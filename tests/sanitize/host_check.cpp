@@ -174,7 +174,7 @@ static int meshes(const std::string &dir) {
   int32_t depth = 0, root = 0;
   CHECK(aloha_tris_build_host(m, 2, ids, cols, &ntr, &nno, &depth, nullptr, nullptr, &root) ==
         ALOHA_OK);
-  std::vector<unsigned char> nodes(64 * (size_t)nno);
+  std::vector<unsigned char> nodes(128 * (size_t)nno);
   std::vector<float> dtris(12 * (size_t)ntr);
   CHECK(aloha_tris_build_host(m, 2, ids, cols, &ntr, &nno, &depth, nodes.data(), dtris.data(),
                               &root) == ALOHA_OK);

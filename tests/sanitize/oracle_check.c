@@ -121,9 +121,9 @@ static int surface(const char *dir) {
   float cam[12];
   CHECK(fread(&nt, 8, 1, f) == 1 && fread(&nn, 8, 1, f) == 1 && fread(&root, 4, 1, f) == 1);
   CHECK(fread(cam, 4, 12, f) == 12 && nt > 0 && nt < (1 << 24) && nn >= 0 && nn < (1 << 24));
-  float *nodes = (float *)malloc(64 * (size_t)(nn > 0 ? nn : 1));
+  float *nodes = (float *)malloc(128 * (size_t)(nn > 0 ? nn : 1));
   float *tris = (float *)malloc(48 * (size_t)nt);
-  CHECK(fread(nodes, 64, (size_t)nn, f) == (size_t)nn && fread(tris, 48, (size_t)nt, f) == (size_t)nt);
+  CHECK(fread(nodes, 128, (size_t)nn, f) == (size_t)nn && fread(tris, 48, (size_t)nt, f) == (size_t)nt);
   fclose(f);
   /* the same triangles by primitive id for the brute force */
   float *tri9 = (float *)malloc(36 * (size_t)nt);

@@ -163,21 +163,30 @@ def test_random_color_closed_form():
 
 def _check_bvh(nodes, tris, root, tri9):
     """Every triangle in exactly one leaf; every child box contains its
-    subtree's triangles; leaves hold 1..8 triangles."""
+    subtree's triangles; leaves hold 1..8 triangles; unused child slots
+    (0x7fffffff) hold empty boxes; every node is reached once; depth <= 32."""
     seen = np.zeros(len(tri9), np.int32)
+    reached = np.zeros(len(nodes), np.int32)
     nf = nodes.view(np.float32)
     ni = nodes.view(np.int32)
 
     def walk(ref, lo, hi, depth):
         if ref >= 0:
-            for side in (0, 1):
-                xy = nf[ref, 4 * side:4 * side + 4]
-                z = nf[ref, 8 + 2 * side:10 + 2 * side]
-                clo = np.array([xy[0], xy[2], z[0]])
-                chi = np.array([xy[1], xy[3], z[1]])
+            reached[ref] += 1
+            used = 0
+            for k in range(4):
+                kid = int(ni[ref, 24 + k])
+                clo = np.array([nf[ref, k], nf[ref, 8 + k], nf[ref, 16 + k]])
+                chi = np.array([nf[ref, 4 + k], nf[ref, 12 + k], nf[ref, 20 + k]])
+                if kid == 0x7FFFFFFF:
+                    assert (clo > chi).all()
+                    continue
+                used += 1
                 if lo is not None:
                     assert (clo >= lo - 1e-6).all() and (chi <= hi + 1e-6).all()
-                walk(int(ni[ref, 12 + side]), clo, chi, depth + 1)
+                walk(kid, clo, chi, depth + 1)
+            assert used >= 2
+            assert (ni[ref, 28:32] == 0).all()
         else:
             leaf = (~ref) & 0xFFFFFFFF
             first, cnt = leaf >> 3, (leaf & 7) + 1
@@ -191,6 +200,7 @@ def _check_bvh(nodes, tris, root, tri9):
             assert depth <= 32
     walk(root, None, None, 0)
     assert (seen == 1).all()
+    assert (reached == 1).all()
 
 
 @pytest.mark.parametrize("seed", [1, 2])
