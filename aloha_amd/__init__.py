@@ -395,7 +395,10 @@ class Context:
         if not 0 <= brick_id < n.value:
             raise ValueError(f"brick_id {brick_id} not uploaded")
         self._check(lib().aloha_brick_update(self._h, brick_id, C.c_void_p(t.data_ptr()), code))
-        self._keep_update = t  # alive until the stream has read it
+        if t.data_ptr() != voxels.data_ptr():
+            # a contiguous copy made here: the context's stream reads it
+            # before it goes out of scope
+            self._check(lib().aloha_sync(self._h))
 
     def brick_dtype(self, brick_id: int) -> int:
         """How the brick is stored in HBM: U8 or F32."""
