@@ -443,6 +443,9 @@ def adjoint_bench(args, timed):
     ms_f, ms_fb, ms_b, samples = measure()
     log(f"[bench] adjoint: forward {ms_f:.3f} ms, forward+adjoint {ms_fb:.3f} ms, "
         f"adjoint launches {ms_b:.3f} ms, {samples} samples")
+    plain = plain_march_views(vol, dw, tfp, denp, timed, n)
+    log(f"[bench] the same grid, TF and views through the plain per-brick march "
+        f"(render_ranks per view): {plain['ms']:.3f} ms, {plain['samples']} samples")
     # compulsory voxel reads: the footprint of the same views' march of this
     # grid (the differentiable march samples the same lattice)
     touched = 0
@@ -486,6 +489,7 @@ def adjoint_bench(args, timed):
            "samples_per_forward": samples, "touched_voxels": touched,
            "fwd_ms": round(ms_f, 4), "fwd_adjoint_ms": round(ms_fb, 4),
            "adjoint_launches_ms": round(ms_b, 4), "ratio": round(ms_fb / ms_f, 3),
+           "plain_per_brick_march": plain,
            "roofline_forward": roof(fwd_bytes, ms_f),
            "roofline_adjoint": roof(bwd_bytes, ms_b),
            "adam_steps": args.adam_steps, "adam_s": round(adam_s, 2),
@@ -495,6 +499,34 @@ def adjoint_bench(args, timed):
     if not args.no_cpu:
         res["fp64"] = fp64_adjoint_forward(dw, vol, tf, den)
     return res
+
+
+def plain_march_views(vol, dw, tf, den, timed, n):
+    """The plain per-brick form on the differentiable workload: the same
+    256^3 grid (one brick, one rank), TF and 8 views rendered by render_ranks
+    one view after another -- renderFrame's semantics (repeated t += dt, no
+    opacity clamp), so its images differ from the differentiable forward's by
+    those two rules, not by accuracy."""
+    import torch
+    import aloha_amd as A
+    from aloha_amd import RankImages
+    vol.forward_only(tf, den)  # the context holds these exact inputs
+    p = A.render_params(dt=dw.dt, jitter_mode=1, jitter_value=0.5)
+    V, H, W = len(dw.cams), dw.H, dw.W
+    imgs = [RankImages.alloc(1, W, H, rgba=True, rgba8=False, depth=False, nsamples=True)
+            for _ in range(V)]
+
+    def views():
+        for cam, im in zip(dw.cams, imgs):
+            vol.ctx.set_camera(cam["org"], cam["dir_00"], cam["dir_du"], cam["dir_dv"])
+            vol.ctx.render_ranks(p, W, H, [[vol.brick]], im)
+
+    views()
+    ms = timed(views, n)
+    torch.cuda.synchronize()
+    samples = int(sum(int(im.nsamples.to(torch.int64).sum()) for im in imgs))
+    return {"ms": round(ms, 4), "samples": samples,
+            "note": "render_ranks per view (8 launches) on the same grid, TF and views"}
 
 
 def fp64_adjoint_forward(dw, vol, tf, den):
