@@ -106,7 +106,9 @@ struct MarchTables {
 };
 
 // Stage the TF, the rank table and the ranks' brick records in the dynamic
-// LDS `smem`; the caller synchronizes before use.
+// LDS `smem`; the caller synchronizes before use.  kU8 false: no brick of
+// the launch stores u8 (a.u8_lut is 0), nothing of the decode compiled in.
+template <bool kU8>
 __device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *smem) {
   load_tf(a, smem);
   LdsBrick *sb = reinterpret_cast<LdsBrick *>(smem + a.tf_n + 4);
@@ -121,7 +123,8 @@ __device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *s
       r.vlo[k] = db.vox_lo[k];
       r.n[k] = db.n[k];
     }
-    r.px = (uint32_t)db.px | (db.dtype == ALOHA_U8 ? kLdsU8 : 0u);
+    r.px = (uint32_t)db.px;
+    if constexpr (kU8) r.px |= db.dtype == ALOHA_U8 ? kLdsU8 : 0u;
     r.pxy = (uint32_t)db.pxy;
     r.origin = db.origin;
     sb[i] = r;
@@ -130,7 +133,7 @@ __device__ __forceinline__ MarchTables load_tables(const MarchArgs &a, float4 *s
   for (int i = threadIdx.x; i <= a.P; i += blockDim.x) roff[i] = a.rank_off[i];
   for (int i = threadIdx.x; i < 4 * a.P; i += blockDim.x) rect[i] = a.rect[i / 4][i % 4];
   float *lut = nullptr;
-  if (a.u8_lut) {
+  if constexpr (kU8) {
     lut = reinterpret_cast<float *>(rect + 4 * a.P);
     load_u8_lut(lut);
   }
@@ -367,8 +370,12 @@ __device__ __forceinline__ void march_list(const MarchArgs &a, const MarchTables
       const LdsBrick &db = tb.brick[s.kb];
       // kU8 false: no brick of the launch stores u8, and the decode path
       // is not compiled in
-      bv = BrickView{(const gfloat *)db.origin, db.px & ~kLdsU8, db.pxy, db.n[0], db.n[1], db.n[2],
-                     kU8 ? tb.lut : nullptr, kU8 && (db.px & kLdsU8)};
+      if constexpr (kU8)
+        bv = BrickView{(const gfloat *)db.origin, db.px & ~kLdsU8, db.pxy, db.n[0], db.n[1],
+                       db.n[2], tb.lut, (db.px & kLdsU8) != 0u};
+      else  // no u8 brick in the launch: no flag in the pitch
+        bv = BrickView{(const gfloat *)db.origin, db.px, db.pxy, db.n[0], db.n[1], db.n[2],
+                       nullptr, false};
       lo0 = db.vlo[0];
       lo1 = db.vlo[1];
       lo2 = db.vlo[2];
@@ -675,7 +682,7 @@ __global__ __launch_bounds__(64) void march_persist_kernel(const MarchArgs a, co
   static_assert(64 % kRankRun == 0, "a set-up pass covers whole runs");
   constexpr int kRunsPerPass = 64 / kRankRun;
   extern __shared__ float4 smem[];
-  const MarchTables tb = load_tables(a, smem);
+  const MarchTables tb = load_tables<kU8>(a, smem);
   RayRec *recs = reinterpret_cast<RayRec *>(smem + march_tables_bytes(a) / sizeof(float4) + 1);
   int32_t *roff = reinterpret_cast<int32_t *>(recs + 64);
   int32_t *cbase = roff + a.P + 1;
@@ -977,7 +984,7 @@ __global__ __launch_bounds__(kBlock) void march_composite_group_kernel(const Mar
     return;
   }
   const int item = (int)blockIdx.x;  // row-major tiles of the box
-  const MarchTables tb = load_tables(a, s_tf);
+  const MarchTables tb = load_tables<kU8>(a, s_tf);
   fused_tile<G, kU8>(a, src, fs, tb, TileShape{kFusedWl, kFusedHl}, (u.x + item % u.z) * kFW,
                 (u.y + item / u.z) * kFH);
 }
@@ -1029,7 +1036,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
       return;
     }
     const int item = (int)blockIdx.x;
-    const MarchTables tb = load_tables(b.m, s_tf);
+    const MarchTables tb = load_tables<kU8>(b.m, s_tf);
     int view, bx, by;
     batch_active(b, item, view, bx, by);
     fused_tile<G, kU8>(b.m, ViewSrc{b, view}, fs, tb, TileShape{b.tile_wl, 6 - b.tile_wl}, bx, by);
@@ -1038,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
   // XCD x works through part x of every view (a contiguous band of each
   // view's tiles, so its L2 holds the bricks of that band), then helps the
   // other XCDs with what is left of theirs
-  const MarchTables tb = load_tables(b.m, s_tf);
+  const MarchTables tb = load_tables<kU8>(b.m, s_tf);
   const int x0 = (int)(blockIdx.x % kXcds);
   for (int q = 0; q < kXcds; ++q) {
     const int x = (x0 + q) % kXcds;
