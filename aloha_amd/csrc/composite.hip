@@ -59,34 +59,90 @@ __device__ __forceinline__ void store_nt(float4 *p, float4 v) {
   __builtin_nontemporal_store(u, reinterpret_cast<f4 *>(p));
 }
 
+// Compact images: each thread issues the loads of up to 8 ranks for both of
+// its pixels before it blends them (front to back, as for full-size
+// images), so the loads are in flight together instead of one rank's
+// latency after the other's (measured on 8 ranks at 1024^2: 5.8 against
+// 7.1 us).  Full-size images (W x H x 16 bytes apart) go pixel by pixel,
+// rank by rank: loading them in batches measured slower (10.7 against 8.2
+// us for back-to-back composites of the same images).
 template <bool kCompact>
 __global__ __launch_bounds__(kBlock) void over_sparse_kernel(const float4 *__restrict__ imgs,
                                                              const CompositeArgs c, int W,
                                                              int x0, int y0, int x1,
                                                              float4 *__restrict__ out,
                                                              uint32_t *__restrict__ out8) {
+  constexpr int kH = kSparsePix / kBlock;  // pixels per thread
+  constexpr int kRankBatch = 8;
   const int py = y0 + (int)blockIdx.y;
+  if constexpr (!kCompact) {
 #pragma unroll
-  for (int h = 0; h < kSparsePix / kBlock; ++h) {
-    const int px = x0 + (int)(blockIdx.x * kSparsePix + h * kBlock + threadIdx.x);
-    if (px >= x1) return;
-    const int64_t i = px + (int64_t)W * py;
-    float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int k = 0; k < c.P; ++k) {
-      const int s = c.order[k];  // k is uniform: scalar loads
-      const int32_t *rc = c.rect[s];
-      if (px < rc[0] || px >= rc[2] || py < rc[1] || py >= rc[3]) continue;
-      const int64_t src = kCompact ? c.cbase[s] + (int64_t)(py - rc[1]) * (rc[2] - rc[0]) + (px - rc[0])
-                                   : (int64_t)s * c.npix + i;
-      const float4 v = imgs[src];
-      const float w = 1.f - C.w;
-      C.x = C.x + w * v.x;
-      C.y = C.y + w * v.y;
-      C.z = C.z + w * v.z;
-      C.w = C.w + w * v.w;
+    for (int h = 0; h < kH; ++h) {
+      const int px = x0 + (int)(blockIdx.x * kSparsePix + h * kBlock + threadIdx.x);
+      if (px >= x1) return;
+      const int64_t i = px + (int64_t)W * py;
+      float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int k = 0; k < c.P; ++k) {
+        const int s = c.order[k];  // k is uniform: scalar loads
+        const int32_t *rc = c.rect[s];
+        if (px < rc[0] || px >= rc[2] || py < rc[1] || py >= rc[3]) continue;
+        const float4 v = imgs[(int64_t)s * c.npix + i];
+        const float w = 1.f - C.w;
+        C.x = C.x + w * v.x;
+        C.y = C.y + w * v.y;
+        C.z = C.z + w * v.z;
+        C.w = C.w + w * v.w;
+      }
+      if (out) store_nt(out + i, C);
+      if (out8) __builtin_nontemporal_store(pack_rgba(C), out8 + i);
     }
-    if (out) store_nt(out + i, C);
-    if (out8) __builtin_nontemporal_store(pack_rgba(C), out8 + i);
+    return;
+  }
+  int px[kH];
+  bool live[kH];
+  float4 C[kH];
+#pragma unroll
+  for (int h = 0; h < kH; ++h) {
+    px[h] = x0 + (int)(blockIdx.x * kSparsePix + h * kBlock + threadIdx.x);
+    live[h] = px[h] < x1;
+    C[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  for (int k0 = 0; k0 < c.P; k0 += kRankBatch) {
+    float4 v[kH][kRankBatch];
+    bool in[kH][kRankBatch];
+#pragma unroll
+    for (int j = 0; j < kRankBatch; ++j) {
+      const bool rank = k0 + j < c.P;  // uniform: scalar loads of the rank's table rows
+      const int s = rank ? c.order[k0 + j] : 0;
+      const int32_t *rc = c.rect[s];
+#pragma unroll
+      for (int h = 0; h < kH; ++h) {
+        in[h][j] = rank && live[h] && px[h] >= rc[0] && px[h] < rc[2] && py >= rc[1] &&
+                   py < rc[3];
+        v[h][j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (in[h][j]) {
+          v[h][j] = imgs[c.cbase[s] + (int64_t)(py - rc[1]) * (rc[2] - rc[0]) + (px[h] - rc[0])];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kRankBatch; ++j)
+#pragma unroll
+      for (int h = 0; h < kH; ++h)
+        if (in[h][j]) {  // outside its rectangle a rank is transparent black: C unchanged
+          const float w = 1.f - C[h].w;
+          C[h].x = C[h].x + w * v[h][j].x;
+          C[h].y = C[h].y + w * v[h][j].y;
+          C[h].z = C[h].z + w * v[h][j].z;
+          C[h].w = C[h].w + w * v[h][j].w;
+        }
+  }
+#pragma unroll
+  for (int h = 0; h < kH; ++h) {
+    if (!live[h]) continue;
+    const int64_t i = px[h] + (int64_t)W * py;
+    if (out) store_nt(out + i, C[h]);
+    if (out8) __builtin_nontemporal_store(pack_rgba(C[h]), out8 + i);
   }
 }
 
