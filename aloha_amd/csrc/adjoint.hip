@@ -17,7 +17,11 @@
 // 2021, the inversion method) and back-propagates through compositing,
 // shading (N = g/|g+1e-4|), the TF lookup and all seven trilinear taps.
 // The adjoint runs in stages:
-//   * diff_wave_slots_kernel gives each wave of pixels its sample slots;
+//   * diff_wave_slots_kernel gives each wave of pixels (an 8x8 tile) its
+//     sample slots and lists the waves with samples: the next two stages
+//     launch over those only (on 256^3 x 8 views of 512^2, 13 % of the
+//     tiles; the other workgroups held up the dispatch of the working ones
+//     for most of the kernels' time);
 //   * diff_resample_kernel evaluates every sample the forward took once more
 //     (value tap and gradient, 16 bytes per slot; lane groups of 4);
 //   * diff_backward_kernel, one lane per ray, walks the samples in reverse
@@ -148,14 +152,20 @@ __device__ void load_tf(const DiffArgs &a, float4 *s_tf) {
     s_tf[i] = (i >= 2 && i < a.tf_n + 2) ? a.tf[i - 2] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// The bin of a sample at brick-texel position (x, y, z) of brick slot `slot`.
-__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, int n0, int n1, int n2,
-                                          float x, float y, float z) {
+// The bin of a sample at brick-texel position (x, y, z) of a brick of n0 x
+// n1 x n2 voxels whose bins start at `base`, nbx x nby of them per slice.
+__device__ __forceinline__ int bin_of(int base, int nbx, int nby, int n0, int n1, int n2, float x,
+                                      float y, float z) {
   const int ax = clampi((int)floorf(x - .5f), -1, n0);
   const int ay = clampi((int)floorf(y - .5f), -1, n1);
   const int az = clampi((int)floorf(z - .5f), -1, n2);
   const int bx = (ax + 1) / kBinEdge, by = (ay + 1) / kBinEdge, bz = (az + 1) / kBinEdge;
-  return b.base[slot] + (bz * b.nby[slot] + by) * b.nbx[slot] + bx;
+  return base + (bz * nby + by) * nbx + bx;
+}
+// The same for brick slot `slot`.
+__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, int n0, int n1, int n2,
+                                          float x, float y, float z) {
+  return bin_of(b.base[slot], b.nbx[slot], b.nby[slot], n0, n1, n2, x, y, z);
 }
 __device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const DevBrick &db, float x,
                                           float y, float z) {
@@ -515,6 +525,17 @@ __device__ __forceinline__ int wave_id(const DiffArgs &a, int view, int bx, int 
   const int gx = (a.W + kTile - 1) / kTile, gy = (a.H + kTile - 1) / kTile;
   return ((view * gy + by) * gx + bx) * 4 + wave;
 }
+// Inverse of wave_id: the view, 16x16 block and wave of wave w.
+__device__ __forceinline__ void wave_of(const DiffArgs &a, int w, int &view, int &bx, int &by,
+                                        int &wave) {
+  const int gx = (a.W + kTile - 1) / kTile, gy = (a.H + kTile - 1) / kTile;
+  wave = w & 3;
+  int t = w >> 2;
+  bx = t % gx;
+  t /= gx;
+  by = t % gy;
+  view = t / gy;
+}
 // The first slot of pixel (px, py)'s ray (its place in its wave's tile added).
 __device__ __forceinline__ unsigned long long ray_slot(const DiffArgs &a, const DiffBins &b,
                                                       int view, int px, int py) {
@@ -539,12 +560,30 @@ __global__ __launch_bounds__(kBlock) void diff_wave_slots_kernel(const DiffArgs 
     const unsigned long long total = s_size[0] + s_size[1] + s_size[2] + s_size[3];
     const int c = (int)((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) %
                         kSlotCounters);
-    unsigned long long off = total ? atomicAdd(&b.slot_ctr[c], total) : 0ull;
+    // the walks launch over the waves with samples only: most tiles of a
+    // view miss the bricks, and workgroups that find nothing to do still
+    // hold up the dispatch of the ones behind them.  The same atomic takes
+    // the block's slots and its places in counter c's list of such waves.
+    const unsigned long long nl = (s_size[0] > 0) + (s_size[1] > 0) + (s_size[2] > 0) +
+                                  (s_size[3] > 0);
+    const unsigned long long old =
+        total ? atomicAdd(&b.slot_ctr[c], total | nl << kLiveShift) : 0ull;
+    unsigned long long off = old & ((1ull << kLiveShift) - 1ull);
+    uint32_t li = (uint32_t)(old >> kLiveShift);
     for (int w = 0; w < 4; ++w) {
-      b.wave_off[wave_id(a, view, blockIdx.x, blockIdx.y, w)] = (unsigned long long)c << 56 | off;
+      const int id = wave_id(a, view, blockIdx.x, blockIdx.y, w);
+      b.wave_off[id] = (unsigned long long)c << 56 | off;
       off += s_size[w];
+      if (s_size[w]) b.live[(size_t)c * b.live_cap + li++] = id;
     }
   }
+}
+
+// Wave i of the waves with samples.
+__device__ __forceinline__ int live_wave(const DiffBins &b, uint32_t i) {
+  int c = 0;
+  while (i >= b.live_pre[c + 1]) ++c;
+  return b.live[(size_t)c * b.live_cap + (i - b.live_pre[c])];
 }
 
 // Resample pass of the adjoint: every sample the forward took (the first
@@ -564,10 +603,13 @@ __global__ __launch_bounds__(kBlock) void diff_wave_slots_kernel(const DiffArgs 
 template <int G>
 __global__ __launch_bounds__(kBlock) void diff_resample_kernel(const DiffArgs a, const DiffBins b) {
   enum : int { kIdle = 0, kPick = 1, kMarch = 2, kOver = 3 };
-  const int view = blockIdx.z;
+  // block i: the 8x8 tile of live wave i
+  int view, bx, by, tw;
+  wave_of(a, live_wave(b, blockIdx.x), view, bx, by, tw);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gl = lane % G, gbase = lane - gl;
-  const int tx = blockIdx.x * 8 + (wave & 1) * 4, ty = blockIdx.y * 8 + (wave >> 1) * 4;
+  const int tx = bx * kTile + (tw & 1) * 8 + (wave & 1) * 4;
+  const int ty = by * kTile + (tw >> 1) * 8 + (wave >> 1) * 4;
   const float *org = a.cam[view];
   const float dt = a.dt;
   int state = kIdle, next = 0;  // next: wave-uniform
@@ -778,10 +820,14 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
   load_tf(a, s_tf);
   __syncthreads();
 
-  const int view = blockIdx.z;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
-  const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+  // wave i of the launch walks live wave i
+  const int lane = threadIdx.x & 63;
+  const uint32_t li = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (li >= b.live_pre[kSlotCounters]) return;
+  int view, bx, by, wave;
+  wave_of(a, live_wave(b, li), view, bx, by, wave);
+  const int px = bx * kTile + (wave & 1) * 8 + (lane & 7);
+  const int py = by * kTile + (wave >> 1) * 8 + (lane >> 3);
   if (px >= a.W || py >= a.H) return;
   const size_t out = (size_t)view * a.W * a.H + (px + a.W * py);
   const uint32_t N = a.nsamples[out];
@@ -829,7 +875,19 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
     Visit v;
     visits(a, org, d, (uint32_t)pixel, vi + 1, &v, vi);
     const uint32_t cnt = vi == last ? N - before_last : visit_count(a, v, 0xFFFFFFFFu);
-    const DevBrick db = a.bricks[v.bid];
+    // The visit's brick and bin table, read here and waited for once: the
+    // vector memory counter also counts stores, so a wait for these inside
+    // the loop below (where the compiler cannot tell that they arrived in an
+    // earlier step) would make every step wait for the previous step's
+    // record stores and for the sample it has just requested.  The empty asm
+    // takes them as its operands, so the loop uses its results instead of
+    // the loads.
+    const DevBrick &db = a.bricks[v.bid];
+    float lo0 = db.vox_lo[0], lo1 = db.vox_lo[1], lo2 = db.vox_lo[2];
+    int n0 = db.n[0], n1 = db.n[1], n2 = db.n[2];
+    int bbase = b.base[v.slot], bnx = b.nbx[v.slot], bny = b.nby[v.slot];
+    asm volatile("" : "+v"(lo0), "+v"(lo1), "+v"(lo2), "+v"(n0), "+v"(n1), "+v"(n2), "+v"(bbase),
+                 "+v"(bnx), "+v"(bny));
     for (int k = (int)cnt - 1; k >= 0 && idx > 0; --k) {
       const float t = lattice_t(a, v, k);
       Sample sv;
@@ -874,9 +932,10 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
       float dg[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) dg[c] = dN[c] * il - dNg * (gv[c] + 1e-4f) * il3;
-      float x, y, z;
-      sample_pos(org, d, db, t, x, y, z);
-      const int bin = sample_bin(b, v.slot, db, x, y, z);
+      const float x = (org[0] + t * d[0]) - lo0;  // sample_pos
+      const float y = (org[1] + t * d[1]) - lo1;
+      const float z = (org[2] + t * d[2]) - lo2;
+      const int bin = bin_of(bbase, bnx, bny, n0, n1, n2, x, y, z);
       if (bin != run_bin) {  // the run walked so far ended at its head
         close_run();
         run_bin = bin;
@@ -1165,9 +1224,12 @@ hipError_t launch_diff_rays(const DiffArgs &a, const DiffBins &b, hipStream_t s)
 }
 
 hipError_t launch_diff_backward(const DiffArgs &a, const DiffBins &b, hipStream_t s) {
-  const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
-  diff_resample_kernel<4><<<dim3((a.W + 7) / 8, (a.H + 7) / 8, a.V), kBlock, 0, s>>>(a, b);
-  diff_backward_kernel<<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a, b);
+  const uint32_t nlive = b.live_pre[kSlotCounters];
+  if (nlive > 0) {  // (no live wave: no slots either)
+    diff_resample_kernel<4><<<nlive, kBlock, 0, s>>>(a, b);
+    diff_backward_kernel<<<(nlive + kBlock / 64 - 1) / (kBlock / 64), kBlock,
+                           sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a, b);
+  }
   if (b.total > 0) {
     const unsigned blocks = (unsigned)((b.total + kBlock - 1) / kBlock);
     diff_run_rank_kernel<<<blocks, kBlock, 0, s>>>(b);

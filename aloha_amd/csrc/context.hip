@@ -1159,9 +1159,12 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
       HIP_CHECK(ctx, dev_malloc((void **)&ctx->diff_bins, words * sizeof(uint32_t)));
       ctx->diff_bins_cap = words;
     }
-    // first slot of each wave (an 8x8 tile of a view), then the slot counters
+    // first slot of each wave (an 8x8 tile of a view) and the slot counters
+    // (8 bytes each), then each counter's list of waves with samples (the
+    // waves of every kSlotCounters-th block of 4)
     const size_t nwaves = (size_t)V * ((H + 15) / 16) * ((W + 15) / 16) * 4;
-    const size_t ray_bytes = 8 * (nwaves + kSlotCounters);
+    const size_t live_cap = (nwaves / 4 + kSlotCounters - 1) / kSlotCounters * 4;
+    const size_t ray_bytes = 8 * (nwaves + kSlotCounters) + 4 * kSlotCounters * live_cap;
     if (ray_bytes > ctx->diff_rays_cap) {
       if (ctx->diff_rays) HIP_CHECK(ctx, hipFree(ctx->diff_rays));
       ctx->diff_rays = nullptr;
@@ -1178,19 +1181,26 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
     b.nwaves = (int32_t)nwaves;
     b.wave_off = (unsigned long long *)ctx->diff_rays;
     b.slot_ctr = b.wave_off + nwaves;
+    b.live = (int32_t *)(b.slot_ctr + kSlotCounters);
+    b.live_cap = (uint32_t)live_cap;
     HIP_CHECK(ctx, hipMemsetAsync(b.count, 0, sizeof(uint32_t) * 3 * (size_t)nbins, ctx->stream));
     HIP_CHECK(ctx, hipMemsetAsync(b.slot_ctr, 0, sizeof(uint64_t) * kSlotCounters, ctx->stream));
     HIP_CHECK(ctx, launch_diff_rays(a, b, ctx->stream));
     HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_total, b.slot_ctr, sizeof(uint64_t) * kSlotCounters,
                                   hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    b.live_pre[0] = 0;
+    for (int c = 0; c < kSlotCounters; ++c)
+      b.live_pre[c + 1] = b.live_pre[c] + (uint32_t)(ctx->h_total[c] >> kLiveShift);
     // the counters' regions back to back
     size_t total = 0;
     for (int c = 0; c < kSlotCounters; ++c) {
       b.region[c] = total;
-      total += (size_t)ctx->h_total[c];
+      total += (size_t)(ctx->h_total[c] & ((1ull << kLiveShift) - 1ull));
     }
-    if (total > ((size_t)1 << 31))
+    // (a counter's slots past 2^kLiveShift would spill into its wave count:
+    // the counts then exceed the waves)
+    if (total > ((size_t)1 << 31) || b.live_pre[kSlotCounters] > nwaves)
       return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %zu sample slots", total);
     // per slot: resampled value (1 float4), run info (1), record (kRecWords),
     // run mark and list entry (one word each)

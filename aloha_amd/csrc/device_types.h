@@ -141,6 +141,7 @@ constexpr int kRecWords = 3;
 constexpr int kLdsTfChunk = 1024;  // TF entries whose gradient a bin sums in LDS at once
 constexpr int kTfGroupBins = 32;   // bins per group of the TF-gradient reduction
 constexpr int kSlotCounters = 8;  // counters the waves take their slots from
+constexpr int kLiveShift = 40;    // slot counter bits below its live-wave count
 // Sample slots, interleaved by wave: the 64 rays of a wave's 8x8 pixel tile
 // share 64 x (their longest sample count) slots from wave_off[w]; backward
 // step i of the ray in tile place l (step 0 = its last sample) is slot
@@ -163,7 +164,15 @@ struct DiffBins {
   uint32_t cap;        // slots allocated
   int32_t nwaves;      // V x ceil(H / 16) x ceil(W / 16) x 4
   unsigned long long *wave_off;  // per wave: counter << 56 | first slot in the counter's region
-  unsigned long long *slot_ctr;  // kSlotCounters slot counters
+  // kSlotCounters counters, each: the slots taken from it (low kLiveShift
+  // bits) and the waves with samples among them (high bits)
+  unsigned long long *slot_ctr;
+  // the waves with samples: counter c's at live[c * live_cap ..], in the
+  // order their atomics took; wave i of all of them is list c's entry
+  // i - live_pre[c] for live_pre[c] <= i < live_pre[c + 1]
+  int32_t *live;
+  uint32_t live_cap;
+  uint32_t live_pre[kSlotCounters + 1];
   unsigned long long region[kSlotCounters];  // first slot of each counter's region
   float4 *samples;     // per slot: (value, gx, gy, gz)
   float4 *records;     // per slot: kRecWords float4
