@@ -181,16 +181,47 @@ __device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const Dev
 // the 16x16 block; a group takes the tile's next pixel as soon as its ray is
 // done.  States: kIdle (no ray), kPick (closest brick hit from tmin next),
 // kMarch (stepping the segment), kOver (the tile is done).
+// The miss result (transparent, no samples) of every pixel outside the
+// views' boxes: fill block fb of nfb takes rows fb, fb + nfb, ... of the
+// V x H rows.
+__device__ __forceinline__ void diff_fill(const DiffArgs &a, int fb, int nfb) {
+  for (int r = fb; r < a.V * a.H; r += nfb) {
+    const int v = r / a.H, y = r - v * a.H;
+    const int x0 = a.fbox[v][0] * kTile, x1 = (a.fbox[v][0] + a.fbox[v][2]) * kTile;
+    const int y0 = a.fbox[v][1] * kTile, y1 = (a.fbox[v][1] + a.fbox[v][3]) * kTile;
+    const bool band = y >= y0 && y < y1;
+    for (int x = threadIdx.x; x < a.W; x += blockDim.x) {
+      if (band && x >= x0 && x < x1) continue;  // a box pixel: marched
+      const size_t o = (size_t)v * a.W * a.H + x + (size_t)a.W * y;
+      a.rgba[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+      a.nsamples[o] = 0u;
+    }
+  }
+}
+
+// Grid: the 16x16 blocks of the views' boxes (a.block_off), then blocks
+// that fill in the miss result outside them -- launching only the blocks
+// that can reach a brick (on 256^3 seen from 2.5 diagonals, a quarter of a
+// view's blocks), so that empty blocks do not hold up the dispatch of the
+// others.
 template <int G>
 __global__ __launch_bounds__(kBlock) void diff_forward_group_kernel(const DiffArgs a) {
   enum : int { kIdle = 0, kPick = 1, kMarch = 2, kOver = 3 };
+  const int nbox = a.block_off[a.V];
+  if ((int)blockIdx.x >= nbox) {  // block-uniform
+    diff_fill(a, (int)blockIdx.x - nbox, (int)gridDim.x - nbox);
+    return;
+  }
   extern __shared__ float4 s_tf[];
   load_tf(a, s_tf);
   __syncthreads();
-  const int view = blockIdx.z;
+  int view = 0;
+  while ((int)blockIdx.x >= a.block_off[view + 1]) ++view;
+  const int bl = (int)blockIdx.x - a.block_off[view], bnx = a.fbox[view][2];
+  const int bx = a.fbox[view][0] + bl % bnx, by = a.fbox[view][1] + bl / bnx;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gl = lane % G, gbase = lane - gl;
-  const int tx = blockIdx.x * kTile + (wave & 1) * 8, ty = blockIdx.y * kTile + (wave >> 1) * 8;
+  const int tx = bx * kTile + (wave & 1) * 8, ty = by * kTile + (wave >> 1) * 8;
   const float *org = a.cam[view];
   const float dt = a.dt;
   int state = kIdle, next = 0, px = 0, py = 0;  // next: wave-uniform
@@ -1212,8 +1243,9 @@ __global__ __launch_bounds__(kBlock) void diff_fold_kernel(const DiffArgs a, con
 
 hipError_t launch_diff(const DiffArgs &a, bool backward, hipStream_t s) {
   if (backward) return hipErrorInvalidValue;  // the backward runs in stages (launch_diff_*)
-  const dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.V);
-  diff_forward_group_kernel<4><<<grid, kBlock, sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
+  const int fill_blocks = std::min(a.V * a.H, 1024);
+  diff_forward_group_kernel<4><<<a.block_off[a.V] + fill_blocks, kBlock,
+                                 sizeof(float4) * (size_t)(a.tf_n + 4), s>>>(a);
   return hipGetLastError();
 }
 

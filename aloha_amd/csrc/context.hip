@@ -1073,6 +1073,19 @@ int build_diff(aloha_ctx *ctx, const aloha_render_params *p, int32_t W, int32_t 
   for (int k = 0; k < nb; ++k) a.brick_ids[k] = brick_ids[k];
   a.bricks = ctx->d_bricks;
   a.tf = ctx->d_tf;
+  // the forward's 16x16 blocks: those of each view's brick rectangle
+  a.block_off[0] = 0;
+  for (int v = 0; v < V; ++v) {
+    int32_t r[4];
+    rank_rect(cams + 12 * v, ctx->bricks, brick_ids, nb, W, H, r);
+    int32_t *fb = a.fbox[v];
+    fb[0] = r[0] / 16;
+    fb[1] = r[1] / 16;
+    fb[2] = r[2] > r[0] ? (r[2] + 15) / 16 - fb[0] : 0;
+    fb[3] = r[3] > r[1] ? (r[3] + 15) / 16 - fb[1] : 0;
+    if (fb[2] == 0 || fb[3] == 0) fb[2] = fb[3] = 0;
+    a.block_off[v + 1] = a.block_off[v] + fb[2] * fb[3];
+  }
   return ALOHA_OK;
 }
 }  // namespace

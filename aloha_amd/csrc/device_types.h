@@ -125,6 +125,12 @@ struct DiffArgs {
   uint32_t *nsamples;           // V x H x W
   const float4 *dldc;           // backward: dLoss/dRGBA, V x H x W
   float *dtf;                   // backward: dLoss/dTF, tf_n x 4 (accumulated)
+  // forward launch: per view the 16x16 pixel blocks [x0, x0 + nx) x
+  // [y0, y0 + ny) (fbox = x0, y0, nx, ny) covering the rectangle its bricks
+  // project to -- no ray of a pixel outside reaches a brick -- and
+  // block_off[v] = the blocks of the views before v
+  int32_t fbox[kMaxViews][4];
+  int32_t block_off[kMaxViews + 1];
 };
 
 // Backward of the differentiable form: the density gradient goes through
