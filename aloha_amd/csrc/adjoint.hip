@@ -162,15 +162,6 @@ __device__ __forceinline__ int bin_of(int base, int nbx, int nby, int n0, int n1
   const int bx = (ax + 1) / kBinEdge, by = (ay + 1) / kBinEdge, bz = (az + 1) / kBinEdge;
   return base + (bz * nby + by) * nbx + bx;
 }
-// The same for brick slot `slot`.
-__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, int n0, int n1, int n2,
-                                          float x, float y, float z) {
-  return bin_of(b.base[slot], b.nbx[slot], b.nby[slot], n0, n1, n2, x, y, z);
-}
-__device__ __forceinline__ int sample_bin(const DiffBins &b, int slot, const DevBrick &db, float x,
-                                          float y, float z) {
-  return sample_bin(b, slot, db.n[0], db.n[1], db.n[2], x, y, z);
-}
 
 // The forward with lane groups: G lanes per ray evaluate G lattice samples
 // of its segment at once (t_k depends on k alone, so the lanes need no
@@ -375,9 +366,12 @@ struct FixedTile {
     // out of bounds
     if ((unsigned)lx < (unsigned)kTileEdge && (unsigned)ly < (unsigned)kTileEdge &&
         (unsigned)lz < (unsigned)kTileEdge)
-      __hip_atomic_fetch_add((lds_u64 *)tile + (lz * kTileEdge + ly) * kTileEdge + lx,
-                             (unsigned long long)to_fixed(v, scale), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      add_at((lz * kTileEdge + ly) * kTileEdge + lx, v);
+  }
+  // tile voxel i (in range: the caller checked it)
+  __device__ __forceinline__ void add_at(int i, float v) const {
+    __hip_atomic_fetch_add((lds_u64 *)tile + i, (unsigned long long)to_fixed(v, scale),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 };
 
@@ -420,13 +414,6 @@ __device__ __forceinline__ void add_x(float row[3], int scx, float v) {
   }
 }
 
-template <class Sink>
-__device__ __forceinline__ void add_voxel(const Sink &sink, const DevBrick &db, int x, int y, int z,
-                                          float v) {
-  if (v == 0.f) return;
-  sink.add(clampi(x, 0, db.n[0] - 1), clampi(y, 0, db.n[1] - 1), clampi(z, 0, db.n[2] - 1), v);
-}
-
 // Scatter of all seven taps of one sample through the shared 20-voxel
 // footprint (sample_fast's frame: R = 3 x-columns of the value tap's y/z
 // pair, one extra y row, one extra z plane).  Each tap's eight corner
@@ -452,6 +439,26 @@ __device__ __forceinline__ bool scatter_fast(const Sink &sink, const DevBrick &d
                   (bx == ax + 1) & (by == ay + 1) & (bz == az + 1) & (ax >= -1) &
                   (ax <= db.n[0]) & (ay >= -1) & (ay <= db.n[1]) & (az >= -1) & (az <= db.n[2]);
   if (!ok) return false;
+  // The footprint's clamped coordinates as tile offsets: x columns ax ..
+  // ax + 2 (cx, cx + 1 among them), y rows cy, cy + 1 and yn (the row
+  // outside the value tap's pair), z planes cz, cz + 1 and zn.  Every one
+  // lies in the bin's tile (the bin is keyed by the sample's lowest corner);
+  // checked once here, and a footprint that left it would take the per-tap
+  // scatter, which drops what falls outside.
+  const int yn = scy ? ay : ay + 2;
+  const int zn = scz ? az : az + 2;
+  const int n0 = db.n[0] - 1, n1 = db.n[1] - 1, n2 = db.n[2] - 1;
+  const int ox0 = clampi(ax, 0, n0) - sink.x0, ox1 = clampi(ax + 1, 0, n0) - sink.x0,
+            ox2 = clampi(ax + 2, 0, n0) - sink.x0;
+  const int oy0 = clampi(cy, 0, n1) - sink.y0, oy1 = clampi(cy + 1, 0, n1) - sink.y0,
+            oyn = clampi(yn, 0, n1) - sink.y0;
+  const int oz0 = clampi(cz, 0, n2) - sink.z0, oz1 = clampi(cz + 1, 0, n2) - sink.z0,
+            ozn = clampi(zn, 0, n2) - sink.z0;
+  constexpr unsigned kE = (unsigned)kTileEdge;
+  if (!(((unsigned)ox0 < kE) & ((unsigned)ox1 < kE) & ((unsigned)ox2 < kE) &
+        ((unsigned)oy0 < kE) & ((unsigned)oy1 < kE) & ((unsigned)oyn < kE) &
+        ((unsigned)oz0 < kE) & ((unsigned)oz1 < kE) & ((unsigned)ozn < kE)))
+    return false;
   const float wxm = xm - fxm, wxc = x - fxc, wxp = xp - fxp;
   const float wym = ym - fym, wyc = y - fyc, wyp = yp - fyp;
   const float wzm = zm - fzm, wzc = z - fzc, wzp = zp - fzp;
@@ -512,22 +519,34 @@ __device__ __forceinline__ bool scatter_fast(const Sink &sink, const DevBrick &d
       }
     }
   }
-  const int yn = scy ? ay : ay + 2;
-  const int zn = scz ? az : az + 2;
+  // tile rows (z plane, y row) of the footprint, then the x columns
+  const int ozs[2] = {oz0 * kTileEdge, oz1 * kTileEdge};
+  const int oys[2] = {oy0, oy1};
+  const int ox[3] = {ox0, ox1, ox2};
+  const int oxc[2] = {scx ? ox1 : ox0, scx ? ox2 : ox1};  // columns cx, cx + 1
+  auto put = [&](int i, float v) {
+    if (v != 0.f) sink.add_at(i, v);
+  };
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-    for (int jj = 0; jj < 2; ++jj)
+    for (int jj = 0; jj < 2; ++jj) {
+      const int row = (ozs[kk] + oys[jj]) * kTileEdge;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) add_voxel(sink, db, ax + i, cy + jj, cz + kk, A[kk][jj][i]);
+      for (int i = 0; i < 3; ++i) put(row + ox[i], A[kk][jj][i]);
+    }
 #pragma unroll
-  for (int kk = 0; kk < 2; ++kk)
+  for (int kk = 0; kk < 2; ++kk) {
+    const int row = (ozs[kk] + oyn) * kTileEdge;
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii) add_voxel(sink, db, cx + ii, yn, cz + kk, YN[kk][ii]);
+    for (int ii = 0; ii < 2; ++ii) put(row + oxc[ii], YN[kk][ii]);
+  }
 #pragma unroll
-  for (int jj = 0; jj < 2; ++jj)
+  for (int jj = 0; jj < 2; ++jj) {
+    const int row = (ozn * kTileEdge + oys[jj]) * kTileEdge;
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii) add_voxel(sink, db, cx + ii, cy + jj, zn, ZN[jj][ii]);
+    for (int ii = 0; ii < 2; ++ii) put(row + oxc[ii], ZN[jj][ii]);
+  }
   return true;
 }
 
@@ -536,13 +555,6 @@ __device__ __forceinline__ uint32_t visit_count(const DiffArgs &a, const Visit &
   uint32_t cnt = 0;
   while (cnt < limit && lattice_t(a, v, (int)cnt) < v.t1) ++cnt;
   return cnt;
-}
-
-__device__ __forceinline__ void sample_pos(const float org[3], const float d[3], const DevBrick &db,
-                                           float t, float &x, float &y, float &z) {
-  x = (org[0] + t * d[0]) - db.vox_lo[0];
-  y = (org[1] + t * d[1]) - db.vox_lo[1];
-  z = (org[2] + t * d[2]) - db.vox_lo[2];
 }
 
 // Slots of the waves: each wave of the 16x16-pixel blocks (an 8x8 tile of
@@ -777,9 +789,16 @@ __global__ __launch_bounds__(kBlock) void diff_run_rank_kernel(const DiffBins b)
   const uint32_t len = bin >= 0 ? __float_as_uint(info.x) : 0u;
   // bounds are >= 0 or NaN: as unsigned bits NaN is above every finite bound
   const uint32_t bd = __float_as_uint(info.y), bt = __float_as_uint(info.z);
+  // Group the heads by bin (one trip per distinct bin, registers only),
+  // then every group's first lane takes the group's places with one atomic
+  // -- all of them in the same instruction, so a row waits for one atomic
+  // round trip however many bins its heads fall into.
   bool pending = bin >= 0;
-  uint32_t rank = 0;
-  while (true) {  // wave-uniform: one trip per distinct bin of the wave's heads
+  int leader_of = 0;     // the first lane of this head's group
+  uint32_t before = 0;   // the lengths of the group's heads before this one
+  bool leads = false;    // this lane is its group's first
+  uint32_t gsum = 0, gmd = 0, gmt = 0;  // (leaders) the group's length and bounds
+  while (true) {  // wave-uniform
     const uint64_t pm = __ballot(pending);
     if (!pm) break;
     const int leader = __ffsll((unsigned long long)pm) - 1;
@@ -795,20 +814,27 @@ __global__ __launch_bounds__(kBlock) void diff_run_rank_kernel(const DiffBins b)
       mt = max(mt, (uint32_t)__shfl_xor((int)mt, o));
     }
     const uint32_t sum = (uint32_t)__shfl((int)v, 63);
-    uint32_t base = 0;
     if (lane == leader) {
-      base = atomicAdd(&b.count[lb], sum);
-      if (md) atomicMax(&b.maxd[lb], md);
-      if (mt) atomicMax(&b.maxt[lb], mt);
+      leads = true;
+      gsum = sum;
+      gmd = md;
+      gmt = mt;
     }
-    base = (uint32_t)__shfl((int)base, leader);
     if (mine) {
-      rank = base + v - len;
+      leader_of = leader;
+      before = v - len;
       pending = false;
     }
   }
+  uint32_t base = 0;
+  if (leads) {
+    base = atomicAdd(&b.count[bin], gsum);
+    if (gmd) atomicMax(&b.maxd[bin], gmd);
+    if (gmt) atomicMax(&b.maxt[bin], gmt);
+  }
+  base = (uint32_t)__shfl((int)base, leader_of);
   if (bin >= 0) {
-    info.w = __uint_as_float(rank);
+    info.w = __uint_as_float(base + before);
     b.run_info[s] = info;
   }
 }
@@ -963,7 +989,7 @@ __global__ __launch_bounds__(kBlock) void diff_backward_kernel(const DiffArgs a,
       float dg[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) dg[c] = dN[c] * il - dNg * (gv[c] + 1e-4f) * il3;
-      const float x = (org[0] + t * d[0]) - lo0;  // sample_pos
+      const float x = (org[0] + t * d[0]) - lo0;  // the sample's brick-texel position
       const float y = (org[1] + t * d[1]) - lo1;
       const float z = (org[2] + t * d[2]) - lo2;
       const int bin = bin_of(bbase, bnx, bny, n0, n1, n2, x, y, z);
