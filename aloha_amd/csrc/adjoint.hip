@@ -1210,60 +1210,63 @@ __global__ __launch_bounds__(kBlock) void diff_tf_reduce_kernel(const DiffArgs a
   a.dtf[i] += (float)s;
 }
 
-// Tiles into the gradient grid: voxel v of an axis lies in the tile of block
-// (v+1)/16 (at (v+1)%16) and, when (v+1)%16 < 2, also in the previous
-// block's (at (v+1)%16 + 16): up to 8 tiles, added in a fixed order.  Each
-// thread folds kFoldZ voxels of one column (few, fat workgroups: the work
-// per voxel is small).
-constexpr int kFoldZ = 8;
-__global__ __launch_bounds__(kBlock) void diff_fold_kernel(const DiffArgs a, const DiffBins b,
-                                                           int nzb) {
-  // grid: x = 64-voxel row pieces, y = rows of 4, z = slot * nzb + z / kFoldZ
-  const int slot = blockIdx.z / nzb, z0 = (blockIdx.z - slot * nzb) * kFoldZ;
+// Tiles into the gradient grid, one workgroup per bin: voxel v of an axis
+// lies in the tile of block (v + 1) / 16 (at l = (v + 1) % 16) and, when
+// l < 2, also in the previous block's (at l + 16): up to 8 tiles, added in a
+// fixed order (per axis the previous block's first).  The workgroup of
+// bin b folds the 16^3 voxels with (v + 1) / 16 == b per axis; the 8 tiles
+// that can reach them are known per workgroup, and a tile of a bin without
+// samples was not written and is skipped.  Each thread folds one z column
+// of 16 voxels: the tiles' and the grid's x rows are read contiguously.
+__global__ __launch_bounds__(kBlock) void diff_fold_kernel(const DiffArgs a, const DiffBins b) {
+  static_assert(kBlock == kBinEdge * kBinEdge, "a thread per (x, y) of the bin");
+  const int bin = blockIdx.x;
+  int slot = 0;  // the brick slot holding this bin (block-uniform)
+  while (b.base[slot + 1] <= bin) ++slot;
   float *grad = a.dvox[slot];
   if (!grad) return;
-  const DevBrick db = a.bricks[a.brick_ids[slot]];
+  const int nbx = b.nbx[slot], nby = b.nby[slot];
+  const int l = bin - b.base[slot];
+  const int bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
+  // has[dz][dy][dx]: the tile of block (bz - dz, by - dy, bx - dx) exists and was written
+  bool has[2][2][2], any = false;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const bool in = bz >= dz && by >= dy && bx >= dx;
+        has[dz][dy][dx] = in && b.count[bin - (dz * nby + dy) * nbx - dx] != 0;
+        any |= has[dz][dy][dx];
+      }
+  if (!any) return;
+  const DevBrick &db = a.bricks[a.brick_ids[slot]];
   const int nx = db.n[0], ny = db.n[1], nz = db.n[2];
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= nx || y >= ny) return;
-  int bxs[2], lxs[2], bys[2], lys[2], cx = 0, cy = 0;
-  auto cands = [](int v, int *bs, int *ls, int &c) {
-    const int u = v + 1, bh = u / kBinEdge, lh = u % kBinEdge;
-    if (lh < 2 && bh >= 1) {
-      bs[c] = bh - 1;
-      ls[c++] = lh + kBinEdge;
-    }
-    bs[c] = bh;
-    ls[c++] = lh;
-  };
-  cands(x, bxs, lxs, cx);
-  cands(y, bys, lys, cy);
-  const int nbx = b.nbx[slot], nby = b.nby[slot], base = b.base[slot];
-  // the column's sums and gradient values are all loaded before any store
-  float sum[kFoldZ], g[kFoldZ];
-#pragma unroll
-  for (int zz = 0; zz < kFoldZ; ++zz) {
-    const int z = z0 + zz;
-    sum[zz] = 0.f;
-    if (z >= nz) continue;
-    int bzs[2], lzs[2], cz = 0;
-    cands(z, bzs, lzs, cz);
-    for (int kz = 0; kz < cz; ++kz)
-      for (int ky = 0; ky < cy; ++ky)
-        for (int kx = 0; kx < cx; ++kx) {
-          const int bin = base + (bzs[kz] * nby + bys[ky]) * nbx + bxs[kx];
-          if (b.count[bin] == 0) continue;  // no samples: tile not written
-          sum[zz] += b.tiles[(size_t)bin * kTileVox + (lzs[kz] * kTileEdge + lys[ky]) * kTileEdge +
-                             lxs[kx]];
+  const int lx = threadIdx.x % kBinEdge, ly = threadIdx.x / kBinEdge;
+  const int x = bx * kBinEdge - 1 + lx, y = by * kBinEdge - 1 + ly;
+  if (x < 0 || x >= nx || y < 0 || y >= ny) return;
+  const int dxs = lx < 2 ? 1 : 0, dys = ly < 2 ? 1 : 0;  // the previous block's tile reaches x / y
+  for (int lz = 0; lz < kBinEdge; ++lz) {
+    const int z = bz * kBinEdge - 1 + lz;
+    if (z < 0 || z >= nz) continue;
+    float sum = 0.f;
+    for (int dz = lz < 2 ? 1 : 0; dz >= 0; --dz)
+      for (int dy = dys; dy >= 0; --dy)
+        for (int dx = dxs; dx >= 0; --dx) {
+          if (!has[dz][dy][dx]) continue;
+          const int tb = bin - (dz * nby + dy) * nbx - dx;
+          sum += b.tiles[(size_t)tb * kTileVox +
+                         ((lz + kBinEdge * dz) * kTileEdge + (ly + kBinEdge * dy)) * kTileEdge +
+                         (lx + kBinEdge * dx)];
         }
+    if (sum != 0.f) {
+      float *gp = grad + ((size_t)z * ny + y) * nx + x;
+      *gp = *gp + sum;
+    }
   }
-#pragma unroll
-  for (int zz = 0; zz < kFoldZ; ++zz)
-    g[zz] = sum[zz] != 0.f ? grad[((size_t)(z0 + zz) * ny + y) * nx + x] : 0.f;
-#pragma unroll
-  for (int zz = 0; zz < kFoldZ; ++zz)
-    if (sum[zz] != 0.f) grad[((size_t)(z0 + zz) * ny + y) * nx + x] = g[zz] + sum[zz];
 }
+
 
 }  // namespace
 
@@ -1305,12 +1308,7 @@ hipError_t launch_diff_bins(const DiffArgs &a, const DiffBins &b, const int max_
   const size_t lds =
       sizeof(unsigned long long) * ((size_t)kTileVox + 4 * (size_t)std::min(a.tf_n, kLdsTfChunk));
   diff_bin_kernel<<<b.nbins, kBinBlock, lds, s>>>(a, b);
-  if (max_dims[0] > 0) {
-    const int nzb = (max_dims[2] + kFoldZ - 1) / kFoldZ;
-    const dim3 grid((unsigned)((max_dims[0] + 63) / 64), (unsigned)((max_dims[1] + 3) / 4),
-                    (unsigned)(nzb * a.nb));
-    diff_fold_kernel<<<grid, kBlock, 0, s>>>(a, b, nzb);
-  }
+  if (max_dims[0] > 0) diff_fold_kernel<<<b.nbins, kBlock, 0, s>>>(a, b);
   if (a.dtf) {
     // the group sums go behind the bins' partials in b.tf_part
     const int m = 4 * a.tf_n, ngroups = (b.nbins + kTfGroupBins - 1) / kTfGroupBins;

@@ -1157,8 +1157,6 @@ extern "C" int aloha_diff_backward(aloha_ctx *ctx, const aloha_render_params *pa
   int max_dims[3];
   int64_t max_voxels = 0;
   const int64_t nbins = diff_layout(ctx, nb, brick_ids, a.dvox, b, max_dims, max_voxels);
-  if ((int64_t)max_dims[2] * nb > 65535)  // the fold's grid.z
-    return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %d bricks of depth %d", nb, max_dims[2]);
   if (nbins > kMaxDiffBins)
     return fail(ctx, ALOHA_ERR_LIMIT, "aloha_diff_backward: %lld gradient bins", (long long)nbins);
   if (nbins > 0) {
